@@ -1,0 +1,751 @@
+// elem_kernels.hip -- per-element residual / stiffness / geometric-stiffness
+// kernels for the 4-node director shell, one wavefront per element.
+//
+// Formulation.  The reference (TACSShellElement.h:303-672) builds R_e and
+// K_e by chaining first/second derivatives of the strain energy through
+// (u0x, u1x, tying strains, drilling strain) back to the element dofs.  Here
+// the same energy is differentiated in strain-Jacobian form, which maps onto
+// a wavefront as a small GEMM:
+//
+//   e_qp(q)  = 9 strains at quadrature point qp (membrane 3, bending 3,
+//              transverse shear 2 from MITC tying, drilling 1)
+//   B_qp(q)  = de_qp/dq                                       (9 x 24)
+//   sig_qp   = w detXd C (e_qp - e_thermal)                   (9)
+//   R_e      = sum_qp B_qp^T sig_qp
+//   K_e      = sum_qp B_qp^T (w detXd C) B_qp + sum_qp sum_i sig_i d2e_i/dq2
+//
+// The strains of both reference models are at most quadratic in q, so
+// B_qp(q) = B0 + (terms linear in q) and the reference's geometric stiffness
+// -- a central difference of the nonlinear tangent along the state
+// (TACSShellElement.h:705-766) -- equals, up to round-off, the exact
+// directional derivative computed here:
+//
+//   G_e = sum_qp [B0^T (wC) B1 + B1^T (wC) B0] + sum_qp sum_i sig_i(u) d2e_i/dq2
+//
+// with B1 = dB/d(eps) along u and sig(u) = w detXd C (B0 u - e_thermal).
+//
+// Work split inside the wavefront (64 lanes): geometry per node / quadrature
+// point / tying point on lanes 0-16, strain-Jacobian columns on lanes 0-23
+// (and 32-55 for B1), and the 24 x 24 product as one 3 x 3 register tile per
+// lane.  Elements of one colour share no node, so each wavefront adds its 16
+// blocks into the BSR values without atomics.
+#include "elem_common.h"
+#include "shellgpu_internal.h"
+
+namespace shellgpu {
+
+struct ElemSmem {
+  double X[12];
+  double s[24];
+  double fn[12];
+  double Tn[36], XTn[36];
+  double T[36], XT[36], XzT[36], wdet[4];
+  double tXxi[54], tn0[27];
+  double tcU[54], tcd[27];    // tying derivative coefficients (B or B0)
+  double tcU1[54], tcd1[27];  // tying derivative coefficients of B1
+  double ety[9];
+  double W[180];  // [qp][5][9] tying value -> strain rows (0,1,2,6,7)
+  double u0x[36], u1x[36];
+  double drot[96];  // [node][24] d(drill strain at node)/dq
+  double etn[4];
+  double Jty[216], Jty1[216];  // [tying point][24]
+  double sig[36];              // [qp][9]
+  double dety[9];
+  double B[216], Y[216], B1[216], Y1[216];  // [strain][24] for one qp
+};
+
+// rows of the 9-strain vector fed by the tying strains
+__device__ __forceinline__ int tying_row(int r) {
+  return r < 3 ? r : (r == 3 ? 6 : 7);
+}
+
+// alpha/beta/gamma factors of dof k at a quadrature point:
+// d(u0x)/dq_k = alpha beta^T, d(u1x)/dq_k = alpha gamma^T
+// (from TacsShellComputeDispGrad, TACSShellUtilities.h:361-422, with the
+// linearized-rotation director d = q x n, TACSDirector.h:215-242)
+__device__ __forceinline__ void dof_abg(const ElemSmem &sm, int iq, int k,
+                                        const double N[4], const double Na[4],
+                                        const double Nb[4], double al[3],
+                                        double be[3], double ga[3]) {
+  const int n = k / 6, r = k % 6;
+  const double *T = &sm.T[9 * iq], *XT = &sm.XT[9 * iq],
+               *XzT = &sm.XzT[9 * iq];
+  if (r < 3) {
+    al[0] = T[3 * r];
+    al[1] = T[3 * r + 1];
+    al[2] = T[3 * r + 2];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      be[b] = Na[n] * XT[b] + Nb[n] * XT[3 + b];
+      ga[b] = Na[n] * XzT[b] + Nb[n] * XzT[3 + b];
+    }
+  } else {
+    const double *f = &sm.fn[3 * n];
+    double c[3];
+    const int rr = r - 3;
+    // c = e_rr x f
+    c[0] = rr == 0 ? 0.0 : (rr == 1 ? f[2] : -f[1]);
+    c[1] = rr == 0 ? -f[2] : (rr == 1 ? 0.0 : f[0]);
+    c[2] = rr == 0 ? f[1] : (rr == 1 ? -f[0] : 0.0);
+#pragma unroll
+    for (int a = 0; a < 3; a++) al[a] = T[a] * c[0] + T[3 + a] * c[1] + T[6 + a] * c[2];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      be[b] = N[n] * XT[6 + b];
+      ga[b] = Na[n] * XT[b] + Nb[n] * XT[3 + b] + N[n] * XzT[6 + b];
+    }
+  }
+}
+
+enum BVariant { kBLin = 0, kBFull = 1, kBQuad = 2 };
+
+// one column of B_qp (9 strains) for dof k
+template <int VAR>
+__device__ __forceinline__ void build_bcol(const ElemSmem &sm, int iq, int k,
+                                           const double *J, double out[9]) {
+  double u, v, N[4], Na[4], Nb[4];
+  quad_point(iq, u, v);
+  shape4(u, v, N, Na, Nb);
+  const double *Wq = &sm.W[45 * iq];
+#pragma unroll
+  for (int r = 0; r < 5; r++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < 9; t++) acc += Wq[9 * r + t] * J[24 * t + k];
+    out[tying_row(r)] = acc;
+  }
+  double al[3], be[3], ga[3];
+  dof_abg(sm, iq, k, N, Na, Nb, al, be, ga);
+  double e3 = 0.0, e4 = 0.0, e5 = 0.0;
+  if (VAR != kBQuad) {
+    e3 = al[0] * ga[0];
+    e4 = al[1] * ga[1];
+    e5 = al[0] * ga[1] + al[1] * ga[0];
+  }
+  if (VAR != kBLin) {
+    const double *U0 = &sm.u0x[9 * iq], *U1 = &sm.u1x[9 * iq];
+    const double p00 = al[0] * U0[0] + al[1] * U0[3] + al[2] * U0[6];
+    const double p01 = al[0] * U0[1] + al[1] * U0[4] + al[2] * U0[7];
+    const double p10 = al[0] * U1[0] + al[1] * U1[3] + al[2] * U1[6];
+    const double p11 = al[0] * U1[1] + al[1] * U1[4] + al[2] * U1[7];
+    e3 += be[0] * p10 + ga[0] * p00;
+    e4 += be[1] * p11 + ga[1] * p01;
+    e5 += be[0] * p11 + ga[1] * p00 + ga[0] * p01 + be[1] * p10;
+  }
+  out[3] = e3;
+  out[4] = e4;
+  out[5] = e5;
+  double e8 = 0.0;
+  if (VAR != kBQuad) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) e8 += N[m] * sm.drot[24 * m + k];
+  }
+  out[8] = e8;
+}
+
+// Y = C B for one column (C: A membrane, D bending, As shear, drill)
+__device__ __forceinline__ void apply_c(const Section &c, double w,
+                                        const double b[9], double y[9]) {
+  y[0] = w * (c.A[0] * b[0] + c.A[1] * b[1] + c.A[2] * b[2]);
+  y[1] = w * (c.A[1] * b[0] + c.A[3] * b[1] + c.A[4] * b[2]);
+  y[2] = w * (c.A[2] * b[0] + c.A[4] * b[1] + c.A[5] * b[2]);
+  y[3] = w * (c.D[0] * b[3] + c.D[1] * b[4] + c.D[2] * b[5]);
+  y[4] = w * (c.D[1] * b[3] + c.D[3] * b[4] + c.D[4] * b[5]);
+  y[5] = w * (c.D[2] * b[3] + c.D[4] * b[4] + c.D[5] * b[5]);
+  y[6] = w * (c.As[0] * b[6] + c.As[1] * b[7]);
+  y[7] = w * (c.As[1] * b[6] + c.As[2] * b[7]);
+  y[8] = w * (c.drill * b[8]);
+}
+
+// second derivative of the tying strains summed with weights dety
+// (the dety * d2(ety)/dq2 terms of addComputeTyingStrainHessian,
+// TACSShellElementModel.h:782-1031)
+__device__ __forceinline__ double tying_geo(const ElemSmem &sm, int k, int l) {
+  const int n = k / 6, r = k % 6, m = l / 6, rp = l % 6;
+  if (r >= 3 && rp >= 3) return 0.0;
+  if (r < 3 && rp < 3) {
+    if (r != rp) return 0.0;
+    double h = 0.0;
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      double u, v, N[4], Na[4], Nb[4];
+      tying_point(t, u, v);
+      shape4(u, v, N, Na, Nb);
+      if (t < 2) {
+        h += sm.dety[t] * (Na[n] * Na[m]);
+      } else if (t < 4) {
+        h += sm.dety[t] * (Nb[n] * Nb[m]);
+      } else {
+        h += sm.dety[t] * (0.5 * (Na[n] * Nb[m] + Nb[n] * Na[m]));
+      }
+    }
+    return h;
+  }
+  // one displacement dof (node nd, comp cd) and one rotation dof (node nr, comp cr)
+  const int nd = r < 3 ? n : m, cd = r < 3 ? r : rp;
+  const int nr = r < 3 ? m : n, cr = (r < 3 ? rp : r) - 3;
+  const double *f = &sm.fn[3 * nr];
+  double c[3];
+  c[0] = cr == 0 ? 0.0 : (cr == 1 ? f[2] : -f[1]);
+  c[1] = cr == 0 ? -f[2] : (cr == 1 ? 0.0 : f[0]);
+  c[2] = cr == 0 ? f[1] : (cr == 1 ? -f[0] : 0.0);
+  double h = 0.0;
+#pragma unroll
+  for (int t = 5; t < 9; t++) {
+    double u, v, N[4], Na[4], Nb[4];
+    tying_point(t, u, v);
+    shape4(u, v, N, Na, Nb);
+    const double g = (t < 7) ? Nb[nd] : Na[nd];
+    h += sm.dety[t] * (0.5 * N[nr] * c[cd] * g);
+  }
+  return h;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
+  __shared__ ElemSmem sm;
+  const int lane = threadIdx.x;
+  const int e = a.elems[blockIdx.x];
+  const long design = blockIdx.y;
+  constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
+  constexpr bool kState = (MODE != kModeKLin);
+  constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl);
+  constexpr bool kRes = (MODE == kModeRLin || MODE == kModeRNl);
+  constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG);
+
+  const double thick = a.thick[design * a.nelems + e];
+  double temp = a.temp ? a.temp[design * a.nelems + e] : 0.0;
+  double alpha = a.alpha;
+  const Section sec = make_section(a.mat, thick);
+  const int *cn = a.conn + 4 * e;
+
+  // ---- P0: gather coordinates and state ----
+  if (lane < 12) sm.X[lane] = a.Xpts[3 * cn[lane / 3] + lane % 3];
+  if (lane < 24) {
+    sm.s[lane] = (kState && a.state)
+                     ? a.state[design * a.state_stride + 6 * cn[lane / 6] + lane % 6]
+                     : 0.0;
+  }
+  __syncthreads();
+
+  // central-difference step of getMatType(GEOMETRIC) (TACSShellElement.h:
+  // 705-766): state h u/n, temperature T (1 + h/n), scale n/(2h)
+  if (MODE == kModeKNl && a.fd_sign != 0) {
+    const double dh = 1e-4;
+    double nrm = 0.0;
+#pragma unroll
+    for (int i = 0; i < 24; i++) nrm += sm.s[i] * sm.s[i];
+    nrm += temp * temp;
+    nrm = (nrm == 0.0) ? 1.0 : sqrt(nrm);
+    __syncthreads();
+    if (lane < 24) sm.s[lane] = (a.fd_sign * dh) * sm.s[lane] / nrm;
+    temp = temp + (a.fd_sign * dh) * temp / nrm;
+    alpha = a.fd_sign * (0.5 * nrm / dh);
+    __syncthreads();
+  }
+
+  // ---- P1: node normals + nodal drilling frames; tying-point X,xi ----
+  // (TacsShellComputeNodeNormals, TACSShellUtilities.h:301-343)
+  if (lane < 4) {
+    const int n = lane;
+    double u, v, N[4], Na[4], Nb[4];
+    node_point(n, u, v);
+    shape4(u, v, N, Na, Nb);
+    double Xxi[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Xxi[2 * d] += Na[m] * sm.X[3 * m + d];
+        Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
+      }
+    double a3[3] = {Xxi[0], Xxi[2], Xxi[4]}, b3[3] = {Xxi[1], Xxi[3], Xxi[5]};
+    double f[3];
+    cross3(a3, b3, f);
+    const double nrm = sqrt(dot3(f, f));
+    if (nrm != 0.0) {
+      const double inv = 1.0 / nrm;
+      f[0] *= inv;
+      f[1] *= inv;
+      f[2] *= inv;
+    }
+    sm.fn[3 * n] = f[0];
+    sm.fn[3 * n + 1] = f[1];
+    sm.fn[3 * n + 2] = f[2];
+    double Tn[9], Xd[9], Xdi[9], XTn[9];
+    natural_transform(Xxi, f, Tn);
+    frame3(Xxi, f, Xd);
+    inv3(Xd, Xdi);
+    mm3(Xdi, Tn, XTn);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      sm.Tn[9 * n + i] = Tn[i];
+      sm.XTn[9 * n + i] = XTn[i];
+    }
+  } else if (lane >= 8 && lane < 17) {
+    const int t = lane - 8;
+    double u, v, N[4], Na[4], Nb[4];
+    tying_point(t, u, v);
+    shape4(u, v, N, Na, Nb);
+    double Xxi[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Xxi[2 * d] += Na[m] * sm.X[3 * m + d];
+        Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
+      }
+#pragma unroll
+    for (int i = 0; i < 6; i++) sm.tXxi[6 * t + i] = Xxi[i];
+  }
+  __syncthreads();
+
+  // ---- P2: drilling rows; quadrature-point frames + tying weights; tying
+  // point state and derivative coefficients ----
+  if (lane < 4) {
+    // TacsShellAddDrillStrainSens map (TACSShellUtilities.h:787-835) with
+    // TACSLinearizedRotation: et = 0.5 (Ct[3] + u0x[3] - Ct[1] - u0x[1])
+    const int n = lane;
+    const double *Tn = &sm.Tn[9 * n], *XTn = &sm.XTn[9 * n];
+    double row[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) row[i] = 0.0;
+    const double dA[9] = {0, -0.5, 0, 0.5, 0, 0, 0, 0, 0};
+    double tmp[9], dC[9], du0d[9];
+    // dC = Tn dA Tn^T ; du0d = Tn dA XTn^T
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        tmp[3 * i + j] = dA[3 * i] * Tn[3 * j] + dA[3 * i + 1] * Tn[3 * j + 1] +
+                         dA[3 * i + 2] * Tn[3 * j + 2];
+      }
+    mm3(Tn, tmp, dC);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        tmp[3 * i + j] = dA[3 * i] * XTn[3 * j] + dA[3 * i + 1] * XTn[3 * j + 1] +
+                         dA[3 * i + 2] * XTn[3 * j + 2];
+      }
+    mm3(Tn, tmp, du0d);
+    row[6 * n + 3] += -(dC[7] - dC[5]);
+    row[6 * n + 4] += -(dC[2] - dC[6]);
+    row[6 * n + 5] += -(dC[3] - dC[1]);
+    double u, v, N[4], Na[4], Nb[4];
+    node_point(n, u, v);
+    shape4(u, v, N, Na, Nb);
+    const double g[6] = {du0d[0], du0d[1], du0d[3], du0d[4], du0d[6], du0d[7]};
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) row[6 * m + d] += Na[m] * g[2 * d] + Nb[m] * g[2 * d + 1];
+    double et = 0.0;
+#pragma unroll
+    for (int i = 0; i < 24; i++) {
+      sm.drot[24 * n + i] = row[i];
+      et += row[i] * sm.s[i];
+    }
+    sm.etn[n] = et;
+  } else if (lane < 8) {
+    // quadrature point kinematics (TacsShellComputeDispGrad,
+    // TACSShellUtilities.h:361-422)
+    const int iq = lane - 4;
+    double u, v, N[4], Na[4], Nb[4];
+    quad_point(iq, u, v);
+    shape4(u, v, N, Na, Nb);
+    double Xxi[6] = {0, 0, 0, 0, 0, 0}, nxi[6] = {0, 0, 0, 0, 0, 0}, n0[3] = {0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Xxi[2 * d] += Na[m] * sm.X[3 * m + d];
+        Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
+        nxi[2 * d] += Na[m] * sm.fn[3 * m + d];
+        nxi[2 * d + 1] += Nb[m] * sm.fn[3 * m + d];
+        n0[d] += N[m] * sm.fn[3 * m + d];
+      }
+    double T[9], Xd[9], Xdz[9], Xdi[9], XT[9], nXdz[9], XzT[9];
+    natural_transform(Xxi, n0, T);
+    frame3(Xxi, n0, Xd);
+    frame3(nxi, nullptr, Xdz);
+    const double det = inv3(Xd, Xdi);
+    mm3(Xdi, Xdz, nXdz);
+#pragma unroll
+    for (int i = 0; i < 9; i++) nXdz[i] = -nXdz[i];
+    mm3(Xdi, T, XT);
+    mm3(nXdz, XT, XzT);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      sm.T[9 * iq + i] = T[i];
+      sm.XT[9 * iq + i] = XT[i];
+      sm.XzT[9 * iq + i] = XzT[i];
+    }
+    sm.wdet[iq] = det;  // Gauss weight 1
+
+    // tying value -> strain weights: e0ty = XT^T gty XT, gty from the tying
+    // interpolation (mat3x3SymmTransformTranspose, TACSElementAlgebra.h:1094)
+    double Nt[9];
+    tying_interp(u, v, Nt);
+    const int kk[5] = {0, 3, 1, 4, 2};
+    const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
+    const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+      const int ka = pa[kk[r]], kb = pb[kk[r]];
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int sl = field_slot(tying_field(t));
+        const int c = pa[sl], d = pb[sl];
+        const double M = (c == d) ? XT[3 * c + ka] * XT[3 * c + kb]
+                                  : XT[3 * c + ka] * XT[3 * d + kb] + XT[3 * d + ka] * XT[3 * c + kb];
+        sm.W[45 * iq + 9 * r + t] = ff[r] * M * Nt[t];
+      }
+    }
+    if (kState) {
+      // displacement-gradient state at the point: u0x, u1x
+      double d0[3] = {0, 0, 0}, d0xi[6] = {0, 0, 0, 0, 0, 0}, u0xi[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        double dm[3];
+        cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          d0[d] += N[m] * dm[d];
+          d0xi[2 * d] += Na[m] * dm[d];
+          d0xi[2 * d + 1] += Nb[m] * dm[d];
+          u0xi[2 * d] += Na[m] * sm.s[6 * m + d];
+          u0xi[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
+        }
+      }
+      double u0d[9], u1d[9], tmp[9], tmp2[9], o0[9], o1[9];
+      frame3(u0xi, d0, u0d);
+      frame3(d0xi, nullptr, u1d);
+      mm3(u1d, XT, tmp);
+      mm3(u0d, XzT, tmp2);
+#pragma unroll
+      for (int i = 0; i < 9; i++) tmp[i] += tmp2[i];
+      mtm3(T, tmp, o1);
+      mm3(u0d, XT, tmp);
+      mtm3(T, tmp, o0);
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        sm.u0x[9 * iq + i] = o0[i];
+        sm.u1x[9 * iq + i] = o1[i];
+      }
+    }
+  } else if (lane >= 8 && lane < 17) {
+    const int t = lane - 8;
+    double u, v, N[4], Na[4], Nb[4];
+    tying_point(t, u, v);
+    shape4(u, v, N, Na, Nb);
+    double n0[3] = {0, 0, 0}, U[6] = {0, 0, 0, 0, 0, 0}, d0[3] = {0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      double dm[3];
+      cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        n0[d] += N[m] * sm.fn[3 * m + d];
+        d0[d] += N[m] * dm[d];
+        U[2 * d] += Na[m] * sm.s[6 * m + d];
+        U[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
+      }
+    }
+    const double *X = &sm.tXxi[6 * t];
+    const int f = tying_field(t);
+    // tying strain value (TACSShellElementModel.h:32-77 and 643-697)
+    double lin = 0.0, quad = 0.0;
+    if (f == kG11) {
+      lin = U[0] * X[0] + U[2] * X[2] + U[4] * X[4];
+      quad = 0.5 * (U[0] * U[0] + U[2] * U[2] + U[4] * U[4]);
+    } else if (f == kG22) {
+      lin = U[1] * X[1] + U[3] * X[3] + U[5] * X[5];
+      quad = 0.5 * (U[1] * U[1] + U[3] * U[3] + U[5] * U[5]);
+    } else if (f == kG12) {
+      lin = 0.5 * (U[0] * X[1] + U[2] * X[3] + U[4] * X[5] + U[1] * X[0] + U[3] * X[2] + U[5] * X[4]);
+      quad = 0.5 * (U[0] * U[1] + U[2] * U[3] + U[4] * U[5]);
+    } else if (f == kG23) {
+      lin = 0.5 * (X[1] * d0[0] + X[3] * d0[1] + X[5] * d0[2] + n0[0] * U[1] + n0[1] * U[3] + n0[2] * U[5]);
+      quad = 0.5 * (d0[0] * U[1] + d0[1] * U[3] + d0[2] * U[5]);
+    } else {
+      lin = 0.5 * (X[0] * d0[0] + X[2] * d0[1] + X[4] * d0[2] + n0[0] * U[0] + n0[1] * U[2] + n0[2] * U[4]);
+      quad = 0.5 * (d0[0] * U[0] + d0[1] * U[2] + d0[2] * U[4]);
+    }
+    sm.ety[t] = kNonlin ? lin + quad : lin;
+    // derivative coefficients with a = xfac X + ufac U, nd = xfac n0 + ufac d0
+    for (int pass = 0; pass < 2; pass++) {
+      if (pass == 1 && MODE != kModeG) break;
+      const double xf = (pass == 0) ? 1.0 : 0.0;
+      const double uf = (pass == 0) ? (kNonlin ? 1.0 : 0.0) : 1.0;
+      double av[6], nd[3], dU[6] = {0, 0, 0, 0, 0, 0}, dd[3] = {0, 0, 0};
+#pragma unroll
+      for (int i = 0; i < 6; i++) av[i] = xf * X[i] + uf * U[i];
+#pragma unroll
+      for (int i = 0; i < 3; i++) nd[i] = xf * n0[i] + uf * d0[i];
+      if (f == kG11) {
+        dU[0] = av[0];
+        dU[2] = av[2];
+        dU[4] = av[4];
+      } else if (f == kG22) {
+        dU[1] = av[1];
+        dU[3] = av[3];
+        dU[5] = av[5];
+      } else if (f == kG12) {
+        dU[0] = 0.5 * av[1];
+        dU[1] = 0.5 * av[0];
+        dU[2] = 0.5 * av[3];
+        dU[3] = 0.5 * av[2];
+        dU[4] = 0.5 * av[5];
+        dU[5] = 0.5 * av[4];
+      } else if (f == kG23) {
+        dU[1] = 0.5 * nd[0];
+        dU[3] = 0.5 * nd[1];
+        dU[5] = 0.5 * nd[2];
+        dd[0] = 0.5 * av[1];
+        dd[1] = 0.5 * av[3];
+        dd[2] = 0.5 * av[5];
+      } else {
+        dU[0] = 0.5 * nd[0];
+        dU[2] = 0.5 * nd[1];
+        dU[4] = 0.5 * nd[2];
+        dd[0] = 0.5 * av[0];
+        dd[1] = 0.5 * av[2];
+        dd[2] = 0.5 * av[4];
+      }
+      double *cU = pass == 0 ? sm.tcU : sm.tcU1;
+      double *cd = pass == 0 ? sm.tcd : sm.tcd1;
+#pragma unroll
+      for (int i = 0; i < 6; i++) cU[6 * t + i] = dU[i];
+#pragma unroll
+      for (int i = 0; i < 3; i++) cd[3 * t + i] = dd[i];
+    }
+  }
+  __syncthreads();
+
+  // ---- P3: tying-strain Jacobian rows; strains and stresses at the
+  // quadrature points ----
+  {
+    const int pass = (lane < 24) ? 0 : ((MODE == kModeG && lane >= 32 && lane < 56) ? 1 : -1);
+    if (pass >= 0) {
+      const int k = (pass == 0) ? lane : lane - 32;
+      const int n = k / 6, r = k % 6;
+      const double *cU = pass == 0 ? sm.tcU : sm.tcU1;
+      const double *cd = pass == 0 ? sm.tcd : sm.tcd1;
+      double *J = pass == 0 ? sm.Jty : sm.Jty1;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        double u, v, N[4], Na[4], Nb[4];
+        tying_point(t, u, v);
+        shape4(u, v, N, Na, Nb);
+        double val;
+        if (r < 3) {
+          val = cU[6 * t + 2 * r] * Na[n] + cU[6 * t + 2 * r + 1] * Nb[n];
+        } else {
+          double fx[3];
+          cross3(&sm.fn[3 * n], &cd[3 * t], fx);  // n x dd ; d(d.dd)/dq_r
+          val = N[n] * fx[r - 3];
+        }
+        J[24 * t + k] = val;
+      }
+    }
+    if ((kRes || kGeo) && lane >= 56 && lane < 60) {
+      const int iq = lane - 56;
+      double u, v, N[4], Na[4], Nb[4];
+      quad_point(iq, u, v);
+      shape4(u, v, N, Na, Nb);
+      double e[9];
+      const double *Wq = &sm.W[45 * iq];
+#pragma unroll
+      for (int r = 0; r < 5; r++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < 9; t++) acc += Wq[9 * r + t] * sm.ety[t];
+        e[tying_row(r)] = acc;
+      }
+      const double *U0 = &sm.u0x[9 * iq], *U1 = &sm.u1x[9 * iq];
+      e[3] = U1[0];
+      e[4] = U1[4];
+      e[5] = U1[1] + U1[3];
+      if (kNonlin) {
+        e[3] += U0[0] * U1[0] + U0[3] * U1[3] + U0[6] * U1[6];
+        e[4] += U0[1] * U1[1] + U0[4] * U1[4] + U0[7] * U1[7];
+        e[5] += U0[0] * U1[1] + U0[3] * U1[4] + U0[6] * U1[7] + U1[0] * U0[1] + U1[3] * U0[4] +
+                U1[6] * U0[7];
+      }
+      e[8] = N[0] * sm.etn[0] + N[1] * sm.etn[1] + N[2] * sm.etn[2] + N[3] * sm.etn[3];
+      // mechanical strain (TACSIsoShellConstitutive.cpp:438-456)
+      e[0] -= a.mat.cte * temp;
+      e[1] -= a.mat.cte * temp;
+      double s9[9];
+      apply_c(sec, sm.wdet[iq], e, s9);
+#pragma unroll
+      for (int i = 0; i < 9; i++) sm.sig[9 * iq + i] = s9[i];
+    }
+  }
+  __syncthreads();
+
+  // ---- P4: stress resultants at the tying points ----
+  if (kGeo && lane >= 8 && lane < 17) {
+    const int t = lane - 8;
+    double acc = 0.0;
+#pragma unroll
+    for (int iq = 0; iq < 4; iq++)
+#pragma unroll
+      for (int r = 0; r < 5; r++) acc += sm.sig[9 * iq + tying_row(r)] * sm.W[45 * iq + 9 * r + t];
+    sm.dety[t] = acc;
+  }
+
+  // ---- P5: quadrature loop: B columns, residual, B^T C B tile ----
+  const int row0 = 3 * (lane >> 3), col0 = 3 * (lane & 7);
+  double acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) acc[i] = 0.0;
+  double racc = 0.0;
+
+  for (int iq = 0; iq < 4; iq++) {
+    if (lane < 24) {
+      double b[9], y[9];
+      if (kNonlin) {
+        build_bcol<kBFull>(sm, iq, lane, sm.Jty, b);
+      } else {
+        build_bcol<kBLin>(sm, iq, lane, sm.Jty, b);
+      }
+      if (kRes) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) racc += b[i] * sm.sig[9 * iq + i];
+      }
+      if (kMat) {
+        apply_c(sec, sm.wdet[iq], b, y);
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+          sm.B[24 * i + lane] = b[i];
+          sm.Y[24 * i + lane] = y[i];
+        }
+      }
+    } else if (MODE == kModeG && lane >= 32 && lane < 56) {
+      const int k = lane - 32;
+      double b[9], y[9];
+      build_bcol<kBQuad>(sm, iq, k, sm.Jty1, b);
+      apply_c(sec, sm.wdet[iq], b, y);
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        sm.B1[24 * i + k] = b[i];
+        sm.Y1[24 * i + k] = y[i];
+      }
+    }
+    if (kMat) {
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 9; s++) {
+        const double b0 = sm.B[24 * s + row0], b1 = sm.B[24 * s + row0 + 1],
+                     b2 = sm.B[24 * s + row0 + 2];
+        if (MODE == kModeG) {
+          const double y0 = sm.Y1[24 * s + col0], y1 = sm.Y1[24 * s + col0 + 1],
+                       y2 = sm.Y1[24 * s + col0 + 2];
+          const double c0 = sm.B1[24 * s + row0], c1 = sm.B1[24 * s + row0 + 1],
+                       c2 = sm.B1[24 * s + row0 + 2];
+          const double z0 = sm.Y[24 * s + col0], z1 = sm.Y[24 * s + col0 + 1],
+                       z2 = sm.Y[24 * s + col0 + 2];
+          acc[0] += b0 * y0 + c0 * z0;
+          acc[1] += b0 * y1 + c0 * z1;
+          acc[2] += b0 * y2 + c0 * z2;
+          acc[3] += b1 * y0 + c1 * z0;
+          acc[4] += b1 * y1 + c1 * z1;
+          acc[5] += b1 * y2 + c1 * z2;
+          acc[6] += b2 * y0 + c2 * z0;
+          acc[7] += b2 * y1 + c2 * z1;
+          acc[8] += b2 * y2 + c2 * z2;
+        } else {
+          const double y0 = sm.Y[24 * s + col0], y1 = sm.Y[24 * s + col0 + 1],
+                       y2 = sm.Y[24 * s + col0 + 2];
+          acc[0] += b0 * y0;
+          acc[1] += b0 * y1;
+          acc[2] += b0 * y2;
+          acc[3] += b1 * y0;
+          acc[4] += b1 * y1;
+          acc[5] += b1 * y2;
+          acc[6] += b2 * y0;
+          acc[7] += b2 * y1;
+          acc[8] += b2 * y2;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- P6: geometric (stress) terms ----
+  if (kGeo) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) acc[3 * i + j] += tying_geo(sm, row0 + i, col0 + j);
+    for (int iq = 0; iq < 4; iq++) {
+      double u, v, N[4], Na[4], Nb[4];
+      quad_point(iq, u, v);
+      shape4(u, v, N, Na, Nb);
+      const double S00 = sm.sig[9 * iq + 3], S11 = sm.sig[9 * iq + 4], S01 = sm.sig[9 * iq + 5];
+      double ar[3][3], br[3][3], gr[3][3], ac[3][3], bc[3][3], gc[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        dof_abg(sm, iq, row0 + i, N, Na, Nb, ar[i], br[i], gr[i]);
+        dof_abg(sm, iq, col0 + i, N, Na, Nb, ac[i], bc[i], gc[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const double aa = ar[i][0] * ac[j][0] + ar[i][1] * ac[j][1] + ar[i][2] * ac[j][2];
+          const double h = S00 * (br[i][0] * gc[j][0] + bc[j][0] * gr[i][0]) +
+                           S11 * (br[i][1] * gc[j][1] + bc[j][1] * gr[i][1]) +
+                           S01 * (br[i][0] * gc[j][1] + bc[j][0] * gr[i][1] + br[i][1] * gc[j][0] +
+                                  bc[j][1] * gr[i][0]);
+          acc[3 * i + j] += aa * h;
+        }
+    }
+  }
+
+  // ---- P7: scatter (elements of one colour share no node) ----
+  if (kMat) {
+    const int ni = row0 / 6, nj = col0 / 6;
+    const int rr = row0 % 6, cc = col0 % 6;
+    const long blk = a.blk[16 * e + 4 * ni + nj];
+    double *dst = a.vals + design * a.vals_stride + 36 * blk;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) dst[6 * (rr + i) + cc + j] += alpha * acc[3 * i + j];
+  }
+  if (kRes && lane < 24) {
+    double *dst = a.res + design * a.res_stride;
+    dst[6 * cn[lane / 6] + lane % 6] += alpha * racc;
+  }
+}
+
+// explicit instantiation + launch wrappers
+template <int MODE>
+static hipError_t launch_mode(const ElemArgs &args, int nblocks, int nbatch, hipStream_t stream) {
+  if (nblocks <= 0 || nbatch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(elem_kernel<MODE>, dim3(nblocks, nbatch), dim3(64), 0, stream, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int nbatch,
+                              hipStream_t stream) {
+  switch (mode) {
+    case kModeKLin:
+      return launch_mode<kModeKLin>(args, nblocks, nbatch, stream);
+    case kModeKNl:
+      return launch_mode<kModeKNl>(args, nblocks, nbatch, stream);
+    case kModeG:
+      return launch_mode<kModeG>(args, nblocks, nbatch, stream);
+    case kModeRLin:
+      return launch_mode<kModeRLin>(args, nblocks, nbatch, stream);
+    case kModeRNl:
+      return launch_mode<kModeRNl>(args, nblocks, nbatch, stream);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace shellgpu

@@ -1,0 +1,553 @@
+// shellgpu.hip -- C-ABI layer: context, mesh / pattern / colouring set-up,
+// matrix objects, assembly launches, boundary conditions and the 6x6-block
+// SpMV.  Declarations and reference citations are in include/shellgpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "shellgpu.h"
+#include "shellgpu_internal.h"
+
+using namespace shellgpu;
+
+struct shellgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int nnodes = 0, nelems = 0, nnzb = 0, ncolors = 0, nbatch = 0;
+  std::vector<int> h_conn, h_rowp, h_cols, h_color, h_color_ptr, h_color_elems, h_blk;
+  int *d_conn = nullptr, *d_rowp = nullptr, *d_cols = nullptr, *d_blk = nullptr;
+  int *d_color_elems = nullptr;
+  double *d_X = nullptr;
+
+  Material mat{70e9, 0.3, 2700.0, 0.0, 10.0};
+
+  int nbc = 0;
+  int *d_bc_nodes = nullptr, *d_bc_mask = nullptr;
+  double *d_bc_vals = nullptr;
+
+  double *d_thick = nullptr, *d_temp = nullptr;
+  double *d_tmp = nullptr;  // scratch state vector for the FD geometric path
+  size_t tmp_bytes = 0;
+};
+
+struct shellgpu_mat {
+  shellgpu_ctx *ctx = nullptr;
+  int nbatch = 0;
+  double *vals = nullptr;
+};
+
+namespace {
+
+int set_err(shellgpu_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCALL(ctx, call)                                                        \
+  do {                                                                            \
+    hipError_t _e = (call);                                                       \
+    if (_e != hipSuccess)                                                         \
+      return set_err(ctx, 2, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <typename T>
+int upload(shellgpu_ctx *ctx, T **dptr, const T *h, size_t n) {
+  if (*dptr) {
+    hipFree(*dptr);
+    *dptr = nullptr;
+  }
+  if (n == 0) return 0;
+  HIPCALL(ctx, hipMalloc((void **)dptr, n * sizeof(T)));
+  HIPCALL(ctx, hipMemcpyAsync(*dptr, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// BCSRMat::zeroRow for every BC node (BCSRMat.cpp:2005-2034)
+__global__ void bsr_bc_kernel(const int *rowp, const int *cols, double *vals, long vstride,
+                              const int *bc_nodes, const int *bc_mask, double diag) {
+  const int b = blockIdx.x;
+  vals += (long)blockIdx.y * vstride;
+  const int row = bc_nodes[b], mask = bc_mask[b];
+  const int p0 = rowp[row], p1 = rowp[row + 1];
+  for (int idx = threadIdx.x; idx < 36 * (p1 - p0); idx += blockDim.x) {
+    const int p = p0 + idx / 36, e = idx % 36, r = e / 6, c = e % 6;
+    if (mask & (1 << r)) vals[36L * p + e] = (cols[p] == row && r == c) ? diag : 0.0;
+  }
+}
+
+// TACSBVec::applyBCs / setBCs (TACSBVec.cpp:546-625)
+// mode 0: x[bc] = 0; 1: x[bc] = ubar; 2: x[bc] = u[bc] - ubar (u may be null)
+__global__ void vec_bc_kernel(double *x, const double *u, long stride, int nbc,
+                              const int *bc_nodes, const int *bc_mask, const double *bc_vals,
+                              int mode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * nbc) return;
+  const int b = i / 6, k = i % 6;
+  if (!(bc_mask[b] & (1 << k))) return;
+  const long off = (long)blockIdx.y * stride + 6L * bc_nodes[b] + k;
+  const double ub = bc_vals ? bc_vals[6 * b + k] : 0.0;
+  if (mode == 0) {
+    x[off] = 0.0;
+  } else if (mode == 1) {
+    x[off] = ub;
+  } else {
+    x[off] = (u ? u[off] : 0.0) - ub;
+  }
+}
+
+// y = A x, 6x6 blocks; one lane per scalar row (BCSRMatVecMult6,
+// BCSRMatMult6.cpp:82-125)
+__global__ __launch_bounds__(192) void bsr_spmv6_kernel(int nrows, const int *__restrict__ rowp,
+                                                        const int *__restrict__ cols,
+                                                        const double *__restrict__ vals, long vstride,
+                                                        const double *__restrict__ x,
+                                                        double *__restrict__ y, long xstride) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(tid / 6), r = (int)(tid % 6);
+  if (row >= nrows) return;
+  vals += (long)blockIdx.y * vstride;
+  x += (long)blockIdx.y * xstride;
+  y += (long)blockIdx.y * xstride;
+  double acc = 0.0;
+  for (int p = rowp[row]; p < rowp[row + 1]; p++) {
+    const double *a = vals + 36L * p + 6 * r;
+    const double *xx = x + 6L * cols[p];
+    acc += a[0] * xx[0] + a[1] * xx[1] + a[2] * xx[2] + a[3] * xx[3] + a[4] * xx[4] + a[5] * xx[5];
+  }
+  y[6L * row + r] = acc;
+}
+
+__global__ void axpy_kernel(long n, double alpha, const double *x, double *y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
+int launch_grid(long n) { return (int)std::min<long>((n + 255) / 256, 65535L * 4); }
+
+// greedy first-fit colouring of the element dual graph in element order
+// (the scheme of TacsComputeSerialMultiColor, src/utils/TacsUtilities.cpp:
+// 430-487, applied to elements that share a node)
+void color_elements(int nnodes, int nelems, const std::vector<int> &conn, std::vector<int> &color,
+                    int &ncolors) {
+  std::vector<int> nep(nnodes + 1, 0);
+  for (int e = 0; e < nelems; e++)
+    for (int k = 0; k < 4; k++) nep[conn[4 * e + k] + 1]++;
+  for (int i = 0; i < nnodes; i++) nep[i + 1] += nep[i];
+  std::vector<int> n2e(nep[nnodes]), fill(nep.begin(), nep.end() - 1);
+  for (int e = 0; e < nelems; e++)
+    for (int k = 0; k < 4; k++) n2e[fill[conn[4 * e + k]]++] = e;
+  color.assign(nelems, -1);
+  ncolors = 0;
+  std::vector<int> mark;
+  for (int e = 0; e < nelems; e++) {
+    mark.assign(ncolors + 1, 0);
+    for (int k = 0; k < 4; k++) {
+      const int n = conn[4 * e + k];
+      for (int p = nep[n]; p < nep[n + 1]; p++) {
+        const int c = color[n2e[p]];
+        if (c >= 0) mark[c] = 1;
+      }
+    }
+    int c = 0;
+    while (mark[c]) c++;
+    color[e] = c;
+    ncolors = std::max(ncolors, c + 1);
+  }
+}
+
+int check_batch(shellgpu_ctx *ctx, const shellgpu_mat *A) {
+  if (!ctx->d_thick) return set_err(ctx, 3, "designs not set (shellgpu_set_designs)");
+  if (A && A->nbatch != ctx->nbatch)
+    return set_err(ctx, 3, "matrix batch size does not match the context's designs");
+  return 0;
+}
+
+ElemArgs base_args(shellgpu_ctx *ctx, const double *u) {
+  ElemArgs a{};
+  a.conn = ctx->d_conn;
+  a.blk = ctx->d_blk;
+  a.Xpts = ctx->d_X;
+  a.state = u;
+  a.state_stride = 6L * ctx->nnodes;
+  a.thick = ctx->d_thick;
+  a.temp = ctx->d_temp;
+  a.nelems = ctx->nelems;
+  a.mat = ctx->mat;
+  a.alpha = 1.0;
+  a.vals_stride = 36L * ctx->nnzb;
+  a.res_stride = 6L * ctx->nnodes;
+  return a;
+}
+
+int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
+  for (int c = 0; c < ctx->ncolors; c++) {
+    a.elems = ctx->d_color_elems + ctx->h_color_ptr[c];
+    const int nb = ctx->h_color_ptr[c + 1] - ctx->h_color_ptr[c];
+    HIPCALL(ctx, launch_elem_kernel(mode, a, nb, ctx->nbatch, ctx->stream));
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int shellgpu_create(shellgpu_ctx **out, int device) {
+  if (!out) return 1;
+  *out = nullptr;
+  shellgpu_ctx *ctx = new shellgpu_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+  if (e != hipSuccess) {
+    fprintf(stderr, "shellgpu_create: %s\n", hipGetErrorString(e));
+    delete ctx;
+    return 2;
+  }
+  *out = ctx;
+  return 0;
+}
+
+void shellgpu_destroy(shellgpu_ctx *ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  void *bufs[] = {ctx->d_conn, ctx->d_rowp,     ctx->d_cols,    ctx->d_blk,
+                  ctx->d_color_elems, ctx->d_X, ctx->d_bc_nodes, ctx->d_bc_mask,
+                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp,     ctx->d_tmp};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *shellgpu_last_error(const shellgpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int shellgpu_synchronize(shellgpu_ctx *ctx) {
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+void *shellgpu_get_stream(shellgpu_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int shellgpu_malloc(shellgpu_ctx *ctx, void **dptr, size_t bytes) {
+  HIPCALL(ctx, hipMalloc(dptr, bytes));
+  return 0;
+}
+int shellgpu_free(shellgpu_ctx *ctx, void *dptr) {
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCALL(ctx, hipFree(dptr));
+  return 0;
+}
+int shellgpu_memcpy_h2d(shellgpu_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  HIPCALL(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+int shellgpu_memcpy_d2h(shellgpu_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  HIPCALL(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+int shellgpu_memcpy_d2d(shellgpu_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  HIPCALL(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return 0;
+}
+int shellgpu_memset(shellgpu_ctx *ctx, void *dst, int value, size_t bytes) {
+  HIPCALL(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return 0;
+}
+
+int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn, const double *Xpts) {
+  if (!ctx || nnodes <= 0 || nelems <= 0 || !conn || !Xpts) return set_err(ctx, 1, "invalid mesh arguments");
+  for (long i = 0; i < 4L * nelems; i++)
+    if (conn[i] < 0 || conn[i] >= nnodes) return set_err(ctx, 1, "connectivity entry out of range");
+  ctx->nnodes = nnodes;
+  ctx->nelems = nelems;
+  ctx->h_conn.assign(conn, conn + 4L * nelems);
+
+  // node-to-node pattern: sorted, unique, diagonal kept
+  // (TACSAssembler::computeLocalNodeToNodeCSR + TacsSortAndUniquifyCSR)
+  std::vector<int> nep(nnodes + 1, 0);
+  for (int e = 0; e < nelems; e++)
+    for (int k = 0; k < 4; k++) nep[conn[4 * e + k] + 1]++;
+  for (int i = 0; i < nnodes; i++) nep[i + 1] += nep[i];
+  std::vector<int> n2e(nep[nnodes]), fill(nep.begin(), nep.end() - 1);
+  for (int e = 0; e < nelems; e++)
+    for (int k = 0; k < 4; k++) n2e[fill[conn[4 * e + k]]++] = e;
+  ctx->h_rowp.assign(nnodes + 1, 0);
+  ctx->h_cols.clear();
+  std::vector<int> row;
+  for (int i = 0; i < nnodes; i++) {
+    row.clear();
+    for (int p = nep[i]; p < nep[i + 1]; p++)
+      for (int k = 0; k < 4; k++) row.push_back(conn[4 * n2e[p] + k]);
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    ctx->h_cols.insert(ctx->h_cols.end(), row.begin(), row.end());
+    ctx->h_rowp[i + 1] = (int)ctx->h_cols.size();
+  }
+  ctx->nnzb = (int)ctx->h_cols.size();
+
+  // element -> BSR block table
+  ctx->h_blk.assign(16L * nelems, 0);
+  for (int e = 0; e < nelems; e++)
+    for (int i = 0; i < 4; i++) {
+      const int r = conn[4 * e + i];
+      const int *c0 = &ctx->h_cols[ctx->h_rowp[r]], *c1 = &ctx->h_cols[ctx->h_rowp[r + 1]];
+      for (int j = 0; j < 4; j++) {
+        const int *it = std::lower_bound(c0, c1, conn[4 * e + j]);
+        ctx->h_blk[16L * e + 4 * i + j] = (int)(it - ctx->h_cols.data());
+      }
+    }
+
+  // colouring
+  color_elements(nnodes, nelems, ctx->h_conn, ctx->h_color, ctx->ncolors);
+  ctx->h_color_ptr.assign(ctx->ncolors + 1, 0);
+  for (int e = 0; e < nelems; e++) ctx->h_color_ptr[ctx->h_color[e] + 1]++;
+  for (int c = 0; c < ctx->ncolors; c++) ctx->h_color_ptr[c + 1] += ctx->h_color_ptr[c];
+  ctx->h_color_elems.assign(nelems, 0);
+  {
+    std::vector<int> pos(ctx->h_color_ptr.begin(), ctx->h_color_ptr.end() - 1);
+    for (int e = 0; e < nelems; e++) ctx->h_color_elems[pos[ctx->h_color[e]]++] = e;
+  }
+
+  int rc = 0;
+  if ((rc = upload(ctx, &ctx->d_conn, conn, 4L * nelems))) return rc;
+  if ((rc = upload(ctx, &ctx->d_X, Xpts, 3L * nnodes))) return rc;
+  if ((rc = upload(ctx, &ctx->d_rowp, ctx->h_rowp.data(), ctx->h_rowp.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_cols, ctx->h_cols.data(), ctx->h_cols.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_blk, ctx->h_blk.data(), ctx->h_blk.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_color_elems, ctx->h_color_elems.data(), ctx->h_color_elems.size())))
+    return rc;
+  return 0;
+}
+
+int shellgpu_set_material(shellgpu_ctx *ctx, double E, double nu, double rho, double cte, double k_drill) {
+  if (!ctx) return 1;
+  if (!(E > 0.0) || !(nu > -1.0 && nu < 0.5)) return set_err(ctx, 1, "invalid material");
+  ctx->mat = Material{E, nu, rho, cte, k_drill};
+  return 0;
+}
+
+int shellgpu_set_bcs(shellgpu_ctx *ctx, int nbc, const int *nodes, const int *mask, const double *vals) {
+  if (!ctx || nbc < 0) return 1;
+  for (int i = 0; i < nbc; i++)
+    if (nodes[i] < 0 || nodes[i] >= ctx->nnodes) return set_err(ctx, 1, "BC node out of range");
+  ctx->nbc = nbc;
+  int rc = 0;
+  if ((rc = upload(ctx, &ctx->d_bc_nodes, nodes, (size_t)nbc))) return rc;
+  if ((rc = upload(ctx, &ctx->d_bc_mask, mask, (size_t)nbc))) return rc;
+  std::vector<double> v(6L * nbc, 0.0);
+  if (vals) std::copy(vals, vals + 6L * nbc, v.begin());
+  if ((rc = upload(ctx, &ctx->d_bc_vals, v.data(), v.size()))) return rc;
+  return 0;
+}
+
+int shellgpu_set_designs(shellgpu_ctx *ctx, int nbatch, const double *thick, const double *temp) {
+  if (!ctx || nbatch <= 0 || !thick) return set_err(ctx, 1, "invalid design arguments");
+  if (ctx->nelems <= 0) return set_err(ctx, 1, "set the mesh before the designs");
+  ctx->nbatch = nbatch;
+  int rc = 0;
+  if ((rc = upload(ctx, &ctx->d_thick, thick, (size_t)nbatch * ctx->nelems))) return rc;
+  if (temp) {
+    if ((rc = upload(ctx, &ctx->d_temp, temp, (size_t)nbatch * ctx->nelems))) return rc;
+  } else if (ctx->d_temp) {
+    hipFree(ctx->d_temp);
+    ctx->d_temp = nullptr;
+  }
+  return 0;
+}
+
+int shellgpu_get_sizes(shellgpu_ctx *ctx, int *nnodes, int *nelems, int *nnzb, int *ncolors, int *nbatch) {
+  if (!ctx) return 1;
+  if (nnodes) *nnodes = ctx->nnodes;
+  if (nelems) *nelems = ctx->nelems;
+  if (nnzb) *nnzb = ctx->nnzb;
+  if (ncolors) *ncolors = ctx->ncolors;
+  if (nbatch) *nbatch = ctx->nbatch;
+  return 0;
+}
+
+int shellgpu_get_pattern(shellgpu_ctx *ctx, int *rowp, int *cols) {
+  if (!ctx) return 1;
+  if (rowp) std::copy(ctx->h_rowp.begin(), ctx->h_rowp.end(), rowp);
+  if (cols) std::copy(ctx->h_cols.begin(), ctx->h_cols.end(), cols);
+  return 0;
+}
+
+int shellgpu_get_colors(shellgpu_ctx *ctx, int *elem_color) {
+  if (!ctx) return 1;
+  std::copy(ctx->h_color.begin(), ctx->h_color.end(), elem_color);
+  return 0;
+}
+
+int shellgpu_mat_create(shellgpu_ctx *ctx, int nbatch, shellgpu_mat **A) {
+  if (!ctx || !A || nbatch <= 0) return set_err(ctx, 1, "invalid matrix arguments");
+  if (ctx->nnzb <= 0) return set_err(ctx, 1, "set the mesh before creating matrices");
+  shellgpu_mat *m = new shellgpu_mat();
+  m->ctx = ctx;
+  m->nbatch = nbatch;
+  hipError_t e = hipMalloc((void **)&m->vals, sizeof(double) * 36L * ctx->nnzb * nbatch);
+  if (e != hipSuccess) {
+    delete m;
+    return set_err(ctx, 2, std::string("hipMalloc matrix: ") + hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(m->vals, 0, sizeof(double) * 36L * ctx->nnzb * nbatch, ctx->stream);
+  if (e != hipSuccess) {
+    hipFree(m->vals);
+    delete m;
+    return set_err(ctx, 2, "hipMemset matrix");
+  }
+  *A = m;
+  return 0;
+}
+
+void shellgpu_mat_destroy(shellgpu_mat *A) {
+  if (!A) return;
+  hipStreamSynchronize(A->ctx->stream);
+  hipFree(A->vals);
+  delete A;
+}
+
+double *shellgpu_mat_values(shellgpu_mat *A) { return A ? A->vals : nullptr; }
+
+int shellgpu_mat_get_values(shellgpu_mat *A, double *h) {
+  shellgpu_ctx *ctx = A->ctx;
+  HIPCALL(ctx, hipMemcpyAsync(h, A->vals, sizeof(double) * 36L * ctx->nnzb * A->nbatch, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int shellgpu_mat_set_values(shellgpu_mat *A, const double *h) {
+  shellgpu_ctx *ctx = A->ctx;
+  HIPCALL(ctx, hipMemcpyAsync(A->vals, h, sizeof(double) * 36L * ctx->nnzb * A->nbatch, hipMemcpyHostToDevice,
+                              ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int shellgpu_mat_zero(shellgpu_mat *A) {
+  shellgpu_ctx *ctx = A->ctx;
+  HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  return 0;
+}
+
+int shellgpu_mat_copy(shellgpu_mat *dst, const shellgpu_mat *src) {
+  shellgpu_ctx *ctx = dst->ctx;
+  if (dst->nbatch != src->nbatch || dst->ctx != src->ctx) return set_err(ctx, 1, "matrix shapes differ");
+  HIPCALL(ctx, hipMemcpyAsync(dst->vals, src->vals, sizeof(double) * 36L * ctx->nnzb * src->nbatch,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  return 0;
+}
+
+int shellgpu_mat_axpy(shellgpu_mat *Y, double alpha, const shellgpu_mat *X) {
+  shellgpu_ctx *ctx = Y->ctx;
+  if (Y->nbatch != X->nbatch || Y->ctx != X->ctx) return set_err(ctx, 1, "matrix shapes differ");
+  const long n = 36L * ctx->nnzb * Y->nbatch;
+  hipLaunchKernelGGL(axpy_kernel, dim3(launch_grid(n)), dim3(256), 0, ctx->stream, n, alpha, X->vals, Y->vals);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_mat_apply_bcs(shellgpu_mat *A, double diag) {
+  shellgpu_ctx *ctx = A->ctx;
+  if (ctx->nbc == 0) return 0;
+  hipLaunchKernelGGL(bsr_bc_kernel, dim3(ctx->nbc, A->nbatch), dim3(64), 0, ctx->stream, ctx->d_rowp, ctx->d_cols,
+                     A->vals, 36L * ctx->nnzb, ctx->d_bc_nodes, ctx->d_bc_mask, diag);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y) {
+  shellgpu_ctx *ctx = A->ctx;
+  const long nthreads = 6L * ctx->nnodes;
+  const int blocks = (int)((nthreads + 191) / 192);
+  hipLaunchKernelGGL(bsr_spmv6_kernel, dim3(blocks, A->nbatch), dim3(192), 0, ctx->stream, ctx->nnodes,
+                     ctx->d_rowp, ctx->d_cols, A->vals, 36L * ctx->nnzb, x, y, 6L * ctx->nnodes);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, shellgpu_mat *A) {
+  int rc = check_batch(ctx, A);
+  if (rc) return rc;
+  HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  ElemArgs a = base_args(ctx, u);
+  a.vals = A->vals;
+  if (mattype == SHELLGPU_STIFFNESS) {
+    if ((rc = run_colored(ctx, kModeKLin, a))) return rc;
+  } else if (mattype == SHELLGPU_GEOMETRIC) {
+    if ((rc = run_colored(ctx, kModeG, a))) return rc;
+  } else if (mattype == SHELLGPU_GEOMETRIC_FD) {
+    a.fd_sign = 1;
+    if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
+    a.fd_sign = -1;
+    if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
+  } else if (mattype == SHELLGPU_TANGENT_NONLINEAR) {
+    if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
+  } else {
+    return set_err(ctx, 1, "unknown matrix type");
+  }
+  return shellgpu_mat_apply_bcs(A, 1.0);
+}
+
+int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double *R) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  HIPCALL(ctx, hipMemsetAsync(R, 0, sizeof(double) * 6L * ctx->nnodes * ctx->nbatch, ctx->stream));
+  ElemArgs a = base_args(ctx, u);
+  a.res = R;
+  if ((rc = run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeRNl : kModeRLin, a))) return rc;
+  if (ctx->nbc > 0) {
+    hipLaunchKernelGGL(vec_bc_kernel, dim3((6 * ctx->nbc + 255) / 256, ctx->nbatch), dim3(256), 0, ctx->stream, R,
+                       u, 6L * ctx->nnodes, ctx->nbc, ctx->d_bc_nodes, ctx->d_bc_mask, ctx->d_bc_vals, 2);
+    HIPCALL(ctx, hipGetLastError());
+  }
+  return 0;
+}
+
+int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u, double *R, shellgpu_mat *A) {
+  int rc = check_batch(ctx, A);
+  if (rc) return rc;
+  if (R && (rc = shellgpu_assemble_res(ctx, model, u, R))) return rc;
+  HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  ElemArgs a = base_args(ctx, u);
+  a.vals = A->vals;
+  if ((rc = run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeKNl : kModeKLin, a))) return rc;
+  return shellgpu_mat_apply_bcs(A, 1.0);
+}
+
+int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x) {
+  if (ctx->nbc == 0) return 0;
+  hipLaunchKernelGGL(vec_bc_kernel, dim3((6 * ctx->nbc + 255) / 256, std::max(ctx->nbatch, 1)), dim3(256), 0,
+                     ctx->stream, x, (const double *)nullptr, 6L * ctx->nnodes, ctx->nbc, ctx->d_bc_nodes,
+                     ctx->d_bc_mask, ctx->d_bc_vals, 0);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_vec_set_bcs(shellgpu_ctx *ctx, double *x) {
+  if (ctx->nbc == 0) return 0;
+  hipLaunchKernelGGL(vec_bc_kernel, dim3((6 * ctx->nbc + 255) / 256, std::max(ctx->nbatch, 1)), dim3(256), 0,
+                     ctx->stream, x, (const double *)nullptr, 6L * ctx->nnodes, ctx->nbc, ctx->d_bc_nodes,
+                     ctx->d_bc_mask, ctx->d_bc_vals, 1);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

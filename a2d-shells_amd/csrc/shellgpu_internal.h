@@ -1,0 +1,67 @@
+// shellgpu_internal.h -- types shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace shellgpu {
+
+// Isotropic material (TACSMaterialProperties, TACSMaterialProperties.cpp:24-63)
+struct Material {
+  double E, nu, rho, cte, kdrill;
+};
+
+// Section stiffness of TACSIsoShellConstitutive with tOffset = 0
+// (TACSIsoShellConstitutive.cpp:192-226): A = t Q, B = 0, D = t^3/12 Q,
+// As = 5/6 G t I, drill = 0.5 kdrill (As0 + As2).
+struct Section {
+  double A[6], D[6], As[3], drill;
+};
+
+__host__ __device__ inline Section make_section(const Material &m, double t) {
+  Section s;
+  const double G = 0.5 * m.E / (1.0 + m.nu);
+  const double Dm = m.E / (1.0 - m.nu * m.nu);
+  const double Q[6] = {Dm, m.nu * Dm, 0.0, Dm, 0.0, G};
+  const double I = t * t * t / 12.0;
+  for (int i = 0; i < 6; i++) {
+    s.D[i] = I * Q[i];
+    s.A[i] = Q[i] * t;
+  }
+  s.As[0] = s.As[2] = (5.0 / 6.0) * s.A[5];
+  s.As[1] = 0.0;
+  s.drill = 0.5 * m.kdrill * (s.As[0] + s.As[2]);
+  return s;
+}
+
+enum ElemMode {
+  kModeKLin = 0,  // linear-model stiffness (getMatType STIFFNESS, TACSQuad4Shell)
+  kModeKNl = 1,   // nonlinear-model tangent at the state
+  kModeG = 2,     // geometric stiffness, exact directional derivative
+  kModeRLin = 3,  // linear-model residual
+  kModeRNl = 4,   // nonlinear-model residual
+};
+
+struct ElemArgs {
+  const int *elems;  // element ids of this launch (one colour)
+  const int *conn;   // 4 node ids per element
+  const int *blk;    // 16 BSR block ids per element, (row node i, col node j)
+  const double *Xpts;
+  const double *state;  // 6 per node, may be null
+  long state_stride;    // per design
+  const double *thick;  // per element, stride nelems per design
+  const double *temp;   // per element, may be null
+  int nelems;
+  Material mat;
+  double alpha;
+  int fd_sign;   // kModeKNl only: +-1 selects the reference's central-difference
+                 // geometric stiffness step (TACSShellElement.h:705-766), 0 = off
+  double *vals;  // BSR values
+  long vals_stride;
+  double *res;
+  long res_stride;
+};
+
+hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int nbatch,
+                              hipStream_t stream);
+
+}  // namespace shellgpu

@@ -1,0 +1,292 @@
+"""Python binding of libshellgpu (the MI355X shell assembly / buckling library).
+
+Thin ctypes layer over the C ABI declared in include/shellgpu.h.  The library
+is loaded from ``a2d-shells_amd/lib/libshellgpu.so`` (built in-tree by
+``make -C a2d-shells_amd``); there is no CPU fallback: a missing library or a
+failing call raises :class:`ShellGPUError`.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libshellgpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_ROOT), "include", "shellgpu.h")
+
+STIFFNESS, GEOMETRIC, GEOMETRIC_FD, TANGENT_NONLINEAR = 0, 1, 2, 3
+LINEAR, NONLINEAR = 0, 1
+
+
+class ShellGPUError(RuntimeError):
+    pass
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_VP = C.c_void_p
+
+
+def load_library(path=LIB_PATH):
+    """Load libshellgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ShellGPUError(
+            f"{path} not found: build it with `make -C a2d-shells_amd` "
+            "(or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    sig = {
+        "shellgpu_create": (C.c_int, [C.POINTER(_VP), C.c_int]),
+        "shellgpu_destroy": (None, [_VP]),
+        "shellgpu_last_error": (C.c_char_p, [_VP]),
+        "shellgpu_synchronize": (C.c_int, [_VP]),
+        "shellgpu_get_stream": (_VP, [_VP]),
+        "shellgpu_malloc": (C.c_int, [_VP, C.POINTER(_VP), C.c_size_t]),
+        "shellgpu_free": (C.c_int, [_VP, _VP]),
+        "shellgpu_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+        "shellgpu_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+        "shellgpu_memcpy_d2d": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+        "shellgpu_memset": (C.c_int, [_VP, _VP, C.c_int, C.c_size_t]),
+        "shellgpu_set_mesh": (C.c_int, [_VP, C.c_int, C.c_int, _IP, _DP]),
+        "shellgpu_set_material": (C.c_int, [_VP] + [C.c_double] * 5),
+        "shellgpu_set_bcs": (C.c_int, [_VP, C.c_int, _IP, _IP, _DP]),
+        "shellgpu_set_designs": (C.c_int, [_VP, C.c_int, _DP, _DP]),
+        "shellgpu_get_sizes": (C.c_int, [_VP] + [_IP] * 5),
+        "shellgpu_get_pattern": (C.c_int, [_VP, _IP, _IP]),
+        "shellgpu_get_colors": (C.c_int, [_VP, _IP]),
+        "shellgpu_mat_create": (C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
+        "shellgpu_mat_destroy": (None, [_VP]),
+        "shellgpu_mat_values": (_VP, [_VP]),
+        "shellgpu_mat_get_values": (C.c_int, [_VP, _DP]),
+        "shellgpu_mat_set_values": (C.c_int, [_VP, _DP]),
+        "shellgpu_mat_zero": (C.c_int, [_VP]),
+        "shellgpu_mat_copy": (C.c_int, [_VP, _VP]),
+        "shellgpu_mat_axpy": (C.c_int, [_VP, C.c_double, _VP]),
+        "shellgpu_mat_apply_bcs": (C.c_int, [_VP, C.c_double]),
+        "shellgpu_mat_mult": (C.c_int, [_VP, _VP, _VP]),
+        "shellgpu_assemble_mattype": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+        "shellgpu_assemble_res": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+        "shellgpu_assemble_jacobian": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+        "shellgpu_vec_zero_bcs": (C.c_int, [_VP, _VP]),
+        "shellgpu_vec_set_bcs": (C.c_int, [_VP, _VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+class DeviceArray:
+    """A device buffer owned by a Context (float64 or int32)."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = _VP()
+        ctx._check(ctx.lib.shellgpu_malloc(ctx.h, C.byref(p), max(1, self.n) * self.dtype.itemsize))
+        self.ptr = p.value
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype).ravel()
+        if host.size != self.n:
+            raise ShellGPUError(f"size mismatch {host.size} != {self.n}")
+        self.ctx._check(self.ctx.lib.shellgpu_memcpy_h2d(self.ctx.h, self.ptr, host.ctypes.data, self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        self.ctx._check(self.ctx.lib.shellgpu_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        self.ctx._check(self.ctx.lib.shellgpu_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.shellgpu_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """A batch of BSR matrices sharing the context's pattern (TACSMat)."""
+
+    def __init__(self, ctx, nbatch=None):
+        self.ctx = ctx
+        self.nbatch = nbatch or ctx.nbatch
+        h = _VP()
+        ctx._check(ctx.lib.shellgpu_mat_create(ctx.h, self.nbatch, C.byref(h)))
+        self.h = h.value
+
+    @property
+    def device_values(self):
+        return self.ctx.lib.shellgpu_mat_values(self.h)
+
+    def values(self):
+        out = np.empty(self.nbatch * self.ctx.nnzb * 36)
+        self.ctx._check(self.ctx.lib.shellgpu_mat_get_values(self.h, _dp(out)))
+        return out.reshape(self.nbatch, self.ctx.nnzb, 6, 6)
+
+    def set_values(self, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64).ravel()
+        self.ctx._check(self.ctx.lib.shellgpu_mat_set_values(self.h, _dp(vals)))
+
+    def zero(self):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_zero(self.h))
+
+    def copy_from(self, other):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_copy(self.h, other.h))
+
+    def axpy(self, alpha, other):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_axpy(self.h, float(alpha), other.h))
+
+    def apply_bcs(self, diag=1.0):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_apply_bcs(self.h, float(diag)))
+
+    def mult(self, x, y):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_mult(self.h, x.ptr, y.ptr))
+
+    def destroy(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.shellgpu_mat_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU context: mesh, material, BCs, designs and a HIP stream."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = _VP()
+        rc = self.lib.shellgpu_create(C.byref(h), int(device))
+        if rc != 0:
+            raise ShellGPUError(f"shellgpu_create failed ({rc})")
+        self.h = h.value
+        self.nnodes = self.nelems = self.nnzb = self.ncolors = self.nbatch = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.shellgpu_last_error(self.h)
+            raise ShellGPUError(f"shellgpu error {rc}: {msg.decode() if msg else ''}")
+
+    def _refresh(self):
+        v = [C.c_int() for _ in range(5)]
+        self._check(self.lib.shellgpu_get_sizes(self.h, *[C.byref(x) for x in v]))
+        self.nnodes, self.nelems, self.nnzb, self.ncolors, self.nbatch = [x.value for x in v]
+
+    @property
+    def stream(self):
+        return self.lib.shellgpu_get_stream(self.h)
+
+    def synchronize(self):
+        self._check(self.lib.shellgpu_synchronize(self.h))
+
+    def set_mesh(self, conn, Xpts):
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        Xpts = np.ascontiguousarray(Xpts, dtype=np.float64)
+        self._check(self.lib.shellgpu_set_mesh(self.h, Xpts.size // 3, conn.size // 4, _ip(conn), _dp(Xpts)))
+        self._refresh()
+
+    def set_material(self, E, nu, rho=2700.0, cte=0.0, k_drill=10.0):
+        self._check(self.lib.shellgpu_set_material(self.h, E, nu, rho, cte, k_drill))
+
+    def set_bcs(self, nodes, mask, vals=None):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        vp = None
+        if vals is not None:
+            vals = np.ascontiguousarray(vals, dtype=np.float64)
+            vp = _dp(vals)
+        self._check(self.lib.shellgpu_set_bcs(self.h, nodes.size, _ip(nodes), _ip(mask), vp))
+
+    def set_designs(self, thick, temp=None):
+        thick = np.ascontiguousarray(np.atleast_2d(thick), dtype=np.float64)
+        nb = thick.shape[0]
+        tp = None
+        if temp is not None:
+            temp = np.ascontiguousarray(np.broadcast_to(temp, thick.shape), dtype=np.float64)
+            tp = _dp(temp)
+        self._check(self.lib.shellgpu_set_designs(self.h, nb, _dp(thick), tp))
+        self._refresh()
+
+    def pattern(self):
+        rowp = np.empty(self.nnodes + 1, np.int32)
+        cols = np.empty(self.nnzb, np.int32)
+        self._check(self.lib.shellgpu_get_pattern(self.h, _ip(rowp), _ip(cols)))
+        return rowp, cols
+
+    def colors(self):
+        c = np.empty(self.nelems, np.int32)
+        self._check(self.lib.shellgpu_get_colors(self.h, _ip(c)))
+        return c
+
+    def vector(self, host=None, nbatch=None):
+        n = 6 * self.nnodes * (nbatch or max(self.nbatch, 1))
+        d = DeviceArray(self, n)
+        if host is None:
+            d.zero()
+        else:
+            d.upload(host)
+        return d
+
+    def matrix(self, nbatch=None):
+        return Matrix(self, nbatch)
+
+    def assemble_mattype(self, mattype, A, u=None):
+        self._check(self.lib.shellgpu_assemble_mattype(self.h, int(mattype), u.ptr if u else None, A.h))
+
+    def assemble_res(self, R, u=None, model=LINEAR):
+        self._check(self.lib.shellgpu_assemble_res(self.h, int(model), u.ptr if u else None, R.ptr))
+
+    def assemble_jacobian(self, A, R=None, u=None, model=LINEAR):
+        self._check(self.lib.shellgpu_assemble_jacobian(
+            self.h, int(model), u.ptr if u else None, R.ptr if R else None, A.h))
+
+    def vec_zero_bcs(self, x):
+        self._check(self.lib.shellgpu_vec_zero_bcs(self.h, x.ptr))
+
+    def vec_set_bcs(self, x):
+        self._check(self.lib.shellgpu_vec_set_bcs(self.h, x.ptr))
+
+    def close(self):
+        if self.h:
+            self.lib.shellgpu_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

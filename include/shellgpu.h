@@ -1,0 +1,122 @@
+/*
+ * shellgpu.h -- C ABI of the MI355X (gfx950) shell-assembly and linear
+ * buckling library.
+ *
+ * Plain C types only: device data are passed as raw device pointers, host
+ * set-up data as host pointers that are copied.  Every function returns 0 on
+ * success and a non-zero code on failure; shellgpu_last_error() returns the
+ * message.  A context owns one HIP stream; calls on one context are not
+ * thread-safe and are ordered on that stream.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository sean-engelstad/a2d-shells).
+ */
+#ifndef SHELLGPU_H
+#define SHELLGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shellgpu_ctx shellgpu_ctx;
+typedef struct shellgpu_mat shellgpu_mat;
+
+/* matrix types (ElementMatrixType, src/elements/TACSElementTypes.h) */
+enum {
+  SHELLGPU_STIFFNESS = 0,        /* TACS_STIFFNESS_MATRIX, linear model     */
+  SHELLGPU_GEOMETRIC = 1,        /* TACS_GEOMETRIC_STIFFNESS_MATRIX, exact  */
+  SHELLGPU_GEOMETRIC_FD = 2,     /* same, reference central difference      */
+  SHELLGPU_TANGENT_NONLINEAR = 3 /* addJacobian(alpha=1) of the nonlinear model */
+};
+/* element models (src/elements/shell/TACSShellElementDefs.h:12-29) */
+enum { SHELLGPU_LINEAR = 0, SHELLGPU_NONLINEAR = 1 };
+
+/* ---- context ---------------------------------------------------------- */
+/* TACSAssembler constructor (src/TACSAssembler.cpp:40-120), one rank */
+int shellgpu_create(shellgpu_ctx **ctx, int device);
+void shellgpu_destroy(shellgpu_ctx *ctx);
+const char *shellgpu_last_error(const shellgpu_ctx *ctx);
+int shellgpu_synchronize(shellgpu_ctx *ctx);
+/* the context's hipStream_t */
+void *shellgpu_get_stream(shellgpu_ctx *ctx);
+
+/* device memory helpers (plumbing for callers without a HIP runtime) */
+int shellgpu_malloc(shellgpu_ctx *ctx, void **dptr, size_t bytes);
+int shellgpu_free(shellgpu_ctx *ctx, void *dptr);
+int shellgpu_memcpy_h2d(shellgpu_ctx *ctx, void *dst, const void *src, size_t bytes);
+int shellgpu_memcpy_d2h(shellgpu_ctx *ctx, void *dst, const void *src, size_t bytes);
+int shellgpu_memcpy_d2d(shellgpu_ctx *ctx, void *dst, const void *src, size_t bytes);
+int shellgpu_memset(shellgpu_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---- model set-up (host arrays, copied) -------------------------------- */
+/* TACSAssembler::setElementConnectivity + setNodes (src/TACSAssembler.h:
+ * 120-140): conn[4*nelems] node ids in tensor order (n0,n1,n2,n3) =
+ * (xi,eta) = (-1,-1),(1,-1),(-1,1),(1,1); Xpts[3*nnodes].  Builds the
+ * node-to-node BSR pattern (TACSAssembler.cpp:1839-1992) and the element
+ * colouring. */
+int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn,
+                      const double *Xpts);
+/* TACSMaterialProperties(rho, 0, E, nu, ys, cte, 0) + TACSIsoShellConstitutive
+ * (tOffset = 0) + TACSShellConstitutive::setDrillingRegularization */
+int shellgpu_set_material(shellgpu_ctx *ctx, double E, double nu, double rho,
+                          double cte, double k_drill);
+/* TACSAssembler::addBCs (src/TACSAssembler.cpp, TACSBcMap): mask bits 0-5
+ * of the 6 dofs per node; vals[6*nbc] (may be NULL = 0) */
+int shellgpu_set_bcs(shellgpu_ctx *ctx, int nbc, const int *nodes, const int *mask,
+                     const double *vals);
+/* TACSAssembler::setDesignVars with one thickness per element
+ * (TACSIsoShellConstitutive tNum = element) for nbatch designs;
+ * thick[nbatch*nelems]; temp[nbatch*nelems] or NULL (TACSShellElement::
+ * setTemperature, TACSShellElement.h:76). */
+int shellgpu_set_designs(shellgpu_ctx *ctx, int nbatch, const double *thick,
+                         const double *temp);
+int shellgpu_get_sizes(shellgpu_ctx *ctx, int *nnodes, int *nelems, int *nnzb,
+                       int *ncolors, int *nbatch);
+/* the BSR pattern (rowp[nnodes+1], cols[nnzb]) and per-element colour */
+int shellgpu_get_pattern(shellgpu_ctx *ctx, int *rowp, int *cols);
+int shellgpu_get_colors(shellgpu_ctx *ctx, int *elem_color);
+
+/* ---- matrices (TACSMat, src/bpmat/KSM.h:146-200) ---------------------- */
+/* nbatch matrices sharing the context's pattern; values on the device,
+ * nbatch*nnzb*36 doubles, 6x6 blocks row-major (BCSRMat, BCSRMatImpl.h) */
+int shellgpu_mat_create(shellgpu_ctx *ctx, int nbatch, shellgpu_mat **A);
+void shellgpu_mat_destroy(shellgpu_mat *A);
+double *shellgpu_mat_values(shellgpu_mat *A);
+int shellgpu_mat_get_values(shellgpu_mat *A, double *host_vals);
+int shellgpu_mat_set_values(shellgpu_mat *A, const double *host_vals);
+/* TACSMat::zeroEntries / copyValues / axpy (Y += alpha X) */
+int shellgpu_mat_zero(shellgpu_mat *A);
+int shellgpu_mat_copy(shellgpu_mat *dst, const shellgpu_mat *src);
+int shellgpu_mat_axpy(shellgpu_mat *Y, double alpha, const shellgpu_mat *X);
+/* TACSSchurMat::applyBCs -> BCSRMat::zeroRow (BCSRMat.cpp:2005-2034): masked
+ * rows zeroed, diagonal of masked dofs set to diag (1.0 in the reference) */
+int shellgpu_mat_apply_bcs(shellgpu_mat *A, double diag);
+/* TACSMat::mult -> BCSRMatVecMult6 (BCSRMatMult6.cpp:82-125): y = A x for
+ * every matrix of the batch; x, y device, nbatch*6*nnodes */
+int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y);
+
+/* ---- assembly (device state vectors, nbatch*6*nnodes, NULL = zero) ----- */
+/* TACSAssembler::assembleMatType (src/TACSAssembler.cpp:4186-4250): zero,
+ * element loop, applyBCs (unit diagonal) */
+int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u,
+                              shellgpu_mat *A);
+/* TACSAssembler::assembleRes (src/TACSAssembler.cpp:4000-4070): R = element
+ * residuals, then R[bc] = u[bc] - ubar[bc] (TACSBVec::applyBCs) */
+int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double *R);
+/* TACSAssembler::assembleJacobian (alpha = 1, beta = gamma = 0)
+ * (src/TACSAssembler.cpp:4084-4178): R (may be NULL) and A with BCs */
+int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u,
+                               double *R, shellgpu_mat *A);
+
+/* ---- vector BCs (TACSBVec, src/bpmat/TACSBVec.cpp:546-625) ------------- */
+/* x[bc] = 0 for every design of the batch */
+int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x);
+/* x[bc] = ubar[bc] (TACSBVec::setBCs) */
+int shellgpu_vec_set_bcs(shellgpu_ctx *ctx, double *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

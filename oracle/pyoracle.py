@@ -1,0 +1,161 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU FP64 restatement of the
+reference path.  TEST INFRASTRUCTURE: imported only by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+
+class Material(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
+                ("cte", C.c_double), ("k_drill", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        _lib = C.CDLL(LIB)
+        _lib.or_elem_energy.restype = C.c_double
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _i(a):
+    return a.ctypes.data_as(_IP)
+
+
+def material(E=70e9, nu=0.3, rho=2700.0, cte=0.0, k_drill=10.0):
+    return Material(E, nu, rho, cte, k_drill)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def elem_residual(model, mat, t, temp, X, q):
+    res = np.zeros(24)
+    lib().or_elem_residual(model, C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(q)), _d(res))
+    return res
+
+
+def elem_stiffness(model, mat, t, temp, X, q):
+    K = np.zeros(576)
+    lib().or_elem_stiffness(model, C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(q)), _d(K))
+    return K.reshape(24, 24)
+
+
+def elem_jacobian(model, mat, t, temp, X, q, alpha=1.0):
+    K = np.zeros(576)
+    res = np.zeros(24)
+    lib().or_elem_jacobian(model, C.byref(mat), C.c_double(t), C.c_double(temp), C.c_double(alpha),
+                           _d(_f64(X)), _d(_f64(q)), _d(res), _d(K))
+    return res, K.reshape(24, 24)
+
+
+def elem_geometric_fd(mat, t, temp, X, u):
+    G = np.zeros(576)
+    lib().or_elem_geometric_fd(C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(u)), _d(G))
+    return G.reshape(24, 24)
+
+
+def elem_geometric_step(mat, t, temp, X, u, dh):
+    G = np.zeros(576)
+    lib().or_elem_geometric_step(C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(u)),
+                                 C.c_double(dh), _d(G))
+    return G.reshape(24, 24)
+
+
+def elem_adj_res_dv(model, mat, t, temp, psi, X, q, scale=1.0):
+    out = C.c_double(0.0)
+    lib().or_elem_adj_res_dv(model, C.byref(mat), C.c_double(t), C.c_double(temp), C.c_double(scale),
+                             _d(_f64(psi)), _d(_f64(X)), _d(_f64(q)), C.byref(out))
+    return out.value
+
+
+def elem_energy(model, mat, t, temp, X, q):
+    return lib().or_elem_energy(model, C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(q)))
+
+
+def node_csr(nnodes, conn):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    rp, cp = _IP(), _IP()
+    nnz = lib().or_node_csr(nnodes, conn.size // 4, _i(conn), C.byref(rp), C.byref(cp))
+    rowp = np.ctypeslib.as_array(rp, shape=(nnodes + 1,)).copy()
+    cols = np.ctypeslib.as_array(cp, shape=(nnz,)).copy()
+    libc = C.CDLL(None)
+    libc.free(C.cast(rp, C.c_void_p))
+    libc.free(C.cast(cp, C.c_void_p))
+    return rowp, cols
+
+
+def first_appearance(nnodes, conn):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    out = np.empty(nnodes, np.int32)
+    lib().or_first_appearance(nnodes, conn.size // 4, _i(conn), _i(out))
+    return out
+
+
+KIND_K, KIND_G_FD, KIND_KNL, KIND_G, KIND_G_THERMAL = 0, 1, 2, 3, 4
+
+
+def assemble_mat(kind, model, mat, Xpts, conn, thick, rowp, cols, u=None, temp=None):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    nnodes = Xpts.size // 3
+    nelems = conn.size // 4
+    vals = np.zeros(36 * cols.size)
+    lib().or_assemble_mat(kind, model, C.byref(mat), nnodes, nelems, _i(conn), _d(_f64(Xpts)), _d(_f64(thick)),
+                          _d(_f64(temp)) if temp is not None else None, _d(_f64(u)) if u is not None else None,
+                          _i(np.ascontiguousarray(rowp, np.int32)), _i(np.ascontiguousarray(cols, np.int32)),
+                          _d(vals))
+    return vals.reshape(-1, 6, 6)
+
+
+def assemble_res(model, mat, Xpts, conn, thick, u=None, temp=None):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    nnodes = Xpts.size // 3
+    R = np.zeros(6 * nnodes)
+    lib().or_assemble_res(model, C.byref(mat), nnodes, conn.size // 4, _i(conn), _d(_f64(Xpts)), _d(_f64(thick)),
+                          _d(_f64(temp)) if temp is not None else None, _d(_f64(u)) if u is not None else None, _d(R))
+    return R
+
+
+def bsr_apply_bcs(vals, rowp, cols, bc_nodes, bc_mask, diag=1.0):
+    vals = np.ascontiguousarray(vals, dtype=np.float64).copy()
+    lib().or_bsr_apply_bcs(rowp.size - 1, _i(np.ascontiguousarray(rowp, np.int32)),
+                           _i(np.ascontiguousarray(cols, np.int32)), _d(vals), len(bc_nodes),
+                           _i(np.ascontiguousarray(bc_nodes, np.int32)), _i(np.ascontiguousarray(bc_mask, np.int32)),
+                           C.c_double(diag))
+    return vals
+
+
+def bsr_matvec(rowp, cols, vals, x):
+    y = np.zeros(6 * (rowp.size - 1))
+    lib().or_bsr_matvec(rowp.size - 1, _i(np.ascontiguousarray(rowp, np.int32)),
+                        _i(np.ascontiguousarray(cols, np.int32)), _d(_f64(vals)), _d(_f64(x)), _d(y))
+    return y
+
+
+def bsr_to_dense(rowp, cols, vals):
+    n = rowp.size - 1
+    A = np.zeros((6 * n, 6 * n))
+    vals = np.asarray(vals).reshape(-1, 6, 6)
+    for i in range(n):
+        for p in range(rowp[i], rowp[i + 1]):
+            A[6 * i:6 * i + 6, 6 * cols[p]:6 * cols[p] + 6] = vals[p]
+    return A

@@ -1,0 +1,101 @@
+"""The oracle against the reference's own element code (golden vectors made by
+tests/golden/make_golden.py from the reference compiled from source), plus
+invariants the mathematics fixes."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "element_golden.npz"))
+NREC = GOLD["Xpts"].shape[0]
+
+
+def _rec(i):
+    m = O.material(E=GOLD["E"][i], nu=GOLD["nu"][i], rho=GOLD["rho"][i], cte=GOLD["cte"][i])
+    return (m, GOLD["thickness"][i], GOLD["temperature"][i], GOLD["Xpts"][i], GOLD["vars"][i], GOLD["psi"][i])
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_residual_and_stiffness_match_reference(i):
+    m, t, T, X, q, psi = _rec(i)
+    assert rel(O.elem_residual(0, m, t, T, X, q), GOLD["res_lin"][i]) < 1e-13
+    assert rel(O.elem_residual(1, m, t, T, X, q), GOLD["res_nl"][i]) < 1e-13
+    assert rel(O.elem_stiffness(0, m, t, T, X, q), GOLD["K_lin"][i]) < 1e-13
+    _, Kn = O.elem_jacobian(1, m, t, T, X, q)
+    assert rel(Kn, GOLD["K_nl"][i]) < 1e-13
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_geometric_fd_matches_reference(i):
+    m, t, T, X, q, psi = _rec(i)
+    G = O.elem_geometric_fd(m, t, T, X, q)
+    # both sides are a central difference with step 1e-4: their round-off
+    # differs; with a temperature the step along the state is ~1e-4/|T| and the
+    # difference carries more cancellation.
+    tol = 1e-11 if T == 0.0 else 1e-7
+    assert rel(G, GOLD["G"][i]) < tol
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_thickness_adjoint_matches_reference_derivative(i):
+    m, t, T, X, q, psi = _rec(i)
+    a_lin = O.elem_adj_res_dv(0, m, t, T, psi, X, q)
+    a_nl = O.elem_adj_res_dv(1, m, t, T, psi, X, q)
+    assert abs(a_lin - GOLD["dRdt_lin"][i]) <= 1e-12 * abs(GOLD["dRdt_lin"][i])
+    assert abs(a_nl - GOLD["dRdt_nl"][i]) <= 1e-12 * abs(GOLD["dRdt_nl"][i])
+    if T == 0.0:
+        # the reference's own addAdjResProduct agrees when no thermal strain
+        assert abs(a_lin - GOLD["adj_lin"][i]) <= 1e-12 * abs(GOLD["adj_lin"][i])
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_energy_matches_reference(i):
+    m, t, T, X, q, psi = _rec(i)
+    assert abs(O.elem_energy(0, m, t, T, X, q) - GOLD["energy_lin"][i]) <= 1e-13 * abs(GOLD["energy_lin"][i])
+
+
+def test_linear_model_invariants():
+    """K symmetric, R = K q (T = 0), K annihilates rigid translations, and
+    R = dU/dq by central differences of the energy."""
+    rng = np.random.default_rng(7)
+    m = O.material()
+    for k in range(3):
+        X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float) * 0.4 + 0.05 * rng.standard_normal((4, 3))
+        q = 1e-3 * rng.standard_normal(24)
+        K = O.elem_stiffness(0, m, 0.003, 0.0, X, q)
+        assert np.abs(K - K.T).max() <= 1e-14 * np.abs(K).max()
+        R = O.elem_residual(0, m, 0.003, 0.0, X, q)
+        assert rel(K @ q, R) < 1e-13
+        for d in range(3):
+            r = np.zeros(24)
+            r[d::6][:4] = 1.0
+            assert np.abs(K @ r).max() <= 1e-9 * np.abs(K).max()
+        # energy gradient: U is quadratic in q, central differences are exact
+        h = 1e-6
+        for j in rng.choice(24, 4, replace=False):
+            e = np.zeros(24)
+            e[j] = h
+            dU = (O.elem_energy(0, m, 0.003, 0.0, X, q + e) - O.elem_energy(0, m, 0.003, 0.0, X, q - e)) / (2 * h)
+            assert abs(dU - R[j]) <= 1e-6 * np.abs(R).max()
+
+
+def test_nonlinear_tangent_is_residual_derivative():
+    """K_nl = dR_nl/dq (central differences, O(h^2) exact for the quadratic
+    strain up to round-off)."""
+    rng = np.random.default_rng(8)
+    m = O.material()
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float) * 0.4 + 0.05 * rng.standard_normal((4, 3))
+    q = 1e-2 * rng.standard_normal(24)
+    _, K = O.elem_jacobian(1, m, 0.003, 0.0, X, q)
+    h = 1e-7
+    for j in range(24):
+        e = np.zeros(24)
+        e[j] = h
+        col = (O.elem_residual(1, m, 0.003, 0.0, X, q + e) - O.elem_residual(1, m, 0.003, 0.0, X, q - e)) / (2 * h)
+        assert np.abs(col - K[:, j]).max() <= 1e-6 * np.abs(K).max()
