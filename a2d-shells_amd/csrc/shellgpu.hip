@@ -31,8 +31,10 @@ struct shellgpu_ctx {
   double *d_bc_vals = nullptr;
 
   double *d_thick = nullptr, *d_temp = nullptr;
-  double *d_tmp = nullptr;  // scratch state vector for the FD geometric path
-  size_t tmp_bytes = 0;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
 struct shellgpu_mat {
@@ -58,7 +60,7 @@ int set_err(shellgpu_ctx *ctx, int code, const std::string &msg) {
 template <typename T>
 int upload(shellgpu_ctx *ctx, T **dptr, const T *h, size_t n) {
   if (*dptr) {
-    hipFree(*dptr);
+    (void)hipFree(*dptr);
     *dptr = nullptr;
   }
   if (n == 0) return 0;
@@ -189,11 +191,38 @@ ElemArgs base_args(shellgpu_ctx *ctx, const double *u) {
   return a;
 }
 
+int record_begin(shellgpu_ctx *ctx, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev) {
+  if (!ctx->timing) return 0;
+  hipEvent_t a, b;
+  HIPCALL(ctx, hipEventCreate(&a));
+  HIPCALL(ctx, hipEventCreate(&b));
+  ev.push_back({a, b});
+  HIPCALL(ctx, hipEventRecord(a, ctx->stream));
+  return 0;
+}
+
+int record_end(shellgpu_ctx *ctx, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev) {
+  if (!ctx->timing) return 0;
+  HIPCALL(ctx, hipEventRecord(ev.back().second, ctx->stream));
+  return 0;
+}
+
+void drop_events(std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev) {
+  for (auto &p : ev) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  ev.clear();
+}
+
 int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
+  int rc = 0;
   for (int c = 0; c < ctx->ncolors; c++) {
     a.elems = ctx->d_color_elems + ctx->h_color_ptr[c];
     const int nb = ctx->h_color_ptr[c + 1] - ctx->h_color_ptr[c];
+    if ((rc = record_begin(ctx, ctx->elem_ev))) return rc;
     HIPCALL(ctx, launch_elem_kernel(mode, a, nb, ctx->nbatch, ctx->stream));
+    if ((rc = record_end(ctx, ctx->elem_ev))) return rc;
   }
   return 0;
 }
@@ -223,14 +252,18 @@ int shellgpu_create(shellgpu_ctx **out, int device) {
 
 void shellgpu_destroy(shellgpu_ctx *ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   void *bufs[] = {ctx->d_conn, ctx->d_rowp,     ctx->d_cols,    ctx->d_blk,
                   ctx->d_color_elems, ctx->d_X, ctx->d_bc_nodes, ctx->d_bc_mask,
-                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp,     ctx->d_tmp};
+                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp};
   for (void *b : bufs)
-    if (b) hipFree(b);
-  hipStreamDestroy(ctx->stream);
+    if (b) (void)hipFree(b);
+  drop_events(ctx->elem_ev);
+  drop_events(ctx->spmv_ev);
+  if (ctx->t0) (void)hipEventDestroy(ctx->t0);
+  if (ctx->t1) (void)hipEventDestroy(ctx->t1);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -366,7 +399,7 @@ int shellgpu_set_designs(shellgpu_ctx *ctx, int nbatch, const double *thick, con
   if (temp) {
     if ((rc = upload(ctx, &ctx->d_temp, temp, (size_t)nbatch * ctx->nelems))) return rc;
   } else if (ctx->d_temp) {
-    hipFree(ctx->d_temp);
+    (void)hipFree(ctx->d_temp);
     ctx->d_temp = nullptr;
   }
   return 0;
@@ -408,7 +441,7 @@ int shellgpu_mat_create(shellgpu_ctx *ctx, int nbatch, shellgpu_mat **A) {
   }
   e = hipMemsetAsync(m->vals, 0, sizeof(double) * 36L * ctx->nnzb * nbatch, ctx->stream);
   if (e != hipSuccess) {
-    hipFree(m->vals);
+    (void)hipFree(m->vals);
     delete m;
     return set_err(ctx, 2, "hipMemset matrix");
   }
@@ -418,8 +451,8 @@ int shellgpu_mat_create(shellgpu_ctx *ctx, int nbatch, shellgpu_mat **A) {
 
 void shellgpu_mat_destroy(shellgpu_mat *A) {
   if (!A) return;
-  hipStreamSynchronize(A->ctx->stream);
-  hipFree(A->vals);
+  (void)hipStreamSynchronize(A->ctx->stream);
+  (void)hipFree(A->vals);
   delete A;
 }
 
@@ -477,10 +510,12 @@ int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y) {
   shellgpu_ctx *ctx = A->ctx;
   const long nthreads = 6L * ctx->nnodes;
   const int blocks = (int)((nthreads + 191) / 192);
+  int rc = record_begin(ctx, ctx->spmv_ev);
+  if (rc) return rc;
   hipLaunchKernelGGL(bsr_spmv6_kernel, dim3(blocks, A->nbatch), dim3(192), 0, ctx->stream, ctx->nnodes,
                      ctx->d_rowp, ctx->d_cols, A->vals, 36L * ctx->nnzb, x, y, 6L * ctx->nnodes);
   HIPCALL(ctx, hipGetLastError());
-  return 0;
+  return record_end(ctx, ctx->spmv_ev);
 }
 
 int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, shellgpu_mat *A) {
@@ -547,6 +582,54 @@ int shellgpu_vec_set_bcs(shellgpu_ctx *ctx, double *x) {
                      ctx->stream, x, (const double *)nullptr, 6L * ctx->nnodes, ctx->nbc, ctx->d_bc_nodes,
                      ctx->d_bc_mask, ctx->d_bc_vals, 1);
   HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_set_timing(shellgpu_ctx *ctx, int enable) {
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  drop_events(ctx->elem_ev);
+  drop_events(ctx->spmv_ev);
+  ctx->timing = enable != 0;
+  return 0;
+}
+
+int shellgpu_get_timing(shellgpu_ctx *ctx, double *elem_ms, int *elem_launches, double *spmv_ms,
+                        int *spmv_launches) {
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  double te = 0.0, ts = 0.0;
+  for (auto &p : ctx->elem_ev) {
+    float ms = 0.0f;
+    HIPCALL(ctx, hipEventElapsedTime(&ms, p.first, p.second));
+    te += ms;
+  }
+  for (auto &p : ctx->spmv_ev) {
+    float ms = 0.0f;
+    HIPCALL(ctx, hipEventElapsedTime(&ms, p.first, p.second));
+    ts += ms;
+  }
+  if (elem_ms) *elem_ms = te;
+  if (elem_launches) *elem_launches = (int)ctx->elem_ev.size();
+  if (spmv_ms) *spmv_ms = ts;
+  if (spmv_launches) *spmv_launches = (int)ctx->spmv_ev.size();
+  drop_events(ctx->elem_ev);
+  drop_events(ctx->spmv_ev);
+  return 0;
+}
+
+int shellgpu_timer_start(shellgpu_ctx *ctx) {
+  if (!ctx->t0) HIPCALL(ctx, hipEventCreate(&ctx->t0));
+  if (!ctx->t1) HIPCALL(ctx, hipEventCreate(&ctx->t1));
+  HIPCALL(ctx, hipEventRecord(ctx->t0, ctx->stream));
+  return 0;
+}
+
+int shellgpu_timer_stop(shellgpu_ctx *ctx, double *ms) {
+  if (!ctx->t0) return set_err(ctx, 1, "timer not started");
+  HIPCALL(ctx, hipEventRecord(ctx->t1, ctx->stream));
+  HIPCALL(ctx, hipEventSynchronize(ctx->t1));
+  float f = 0.0f;
+  HIPCALL(ctx, hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+  *ms = f;
   return 0;
 }
 

@@ -73,6 +73,10 @@ def load_library(path=LIB_PATH):
         "shellgpu_assemble_jacobian": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
         "shellgpu_vec_zero_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_vec_set_bcs": (C.c_int, [_VP, _VP]),
+        "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
+        "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
+        "shellgpu_timer_start": (C.c_int, [_VP]),
+        "shellgpu_timer_stop": (C.c_int, [_VP, _DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -279,6 +283,24 @@ class Context:
 
     def vec_set_bcs(self, x):
         self._check(self.lib.shellgpu_vec_set_bcs(self.h, x.ptr))
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.shellgpu_set_timing(self.h, int(bool(enable))))
+
+    def get_timing(self):
+        """(element-kernel ms, launches, spmv ms, launches) since enabling."""
+        em, sm_ = C.c_double(), C.c_double()
+        el, sl = C.c_int(), C.c_int()
+        self._check(self.lib.shellgpu_get_timing(self.h, C.byref(em), C.byref(el), C.byref(sm_), C.byref(sl)))
+        return em.value, el.value, sm_.value, sl.value
+
+    def timer_start(self):
+        self._check(self.lib.shellgpu_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_double()
+        self._check(self.lib.shellgpu_timer_stop(self.h, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if self.h:

@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X shell-assembly / buckling path.
+
+Workload: the 200 x 400 axially compressed cylinder (R = 1, L = 2, 80,000
+4-node shell elements, 80,400 nodes, 482,400 dofs) with seeded per-element
+thickness U[0.0015, 0.0025] (seed 1234) and a seeded radial imperfection of
+amplitude 0.1 * 0.002 (wavenumbers from seed 1235); E = 70 GPa, nu = 0.3.
+
+One step = assembly of the stiffness matrix K (every element Jacobian of the
+linear-model shell, scattered into the 6x6-block BSR matrix, BCs applied).
+`value` = element Jacobians per second over the timed steps (inputs resident in
+HBM).  The line also reports the BSR SpMV bandwidth on the assembled matrix,
+the roofline of the dominant kernel (the element-Jacobian kernel, timed with
+HIP events on the library's stream) and a CPU baseline (the oracle's
+restatement of the reference element loop on a bounded sample).
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "a2d-shells_amd"))
+
+import numpy as np  # noqa: E402
+
+import shellgpu as sg  # noqa: E402
+from shellgpu.problems import Cylinder, seeded_imperfection, seeded_thickness  # noqa: E402
+
+METRIC = "elem Jacobians/s, BSR GB/s; s/design (buckling+grad) vs plain GPU, λ err<1e-8"
+
+# Floating-point operations of one element of the reference algorithm
+# (TACSShellElement addJacobian / getMatType / addResidual), counted by
+# compiling the oracle's operation-by-operation restatement with an
+# operation-counting scalar (oracle/flopcount.cpp; +,-,*,/,sqrt = 1 flop).
+FLOPS_PER_ELEMENT = {
+    "stiffness_linear": 143931,
+    "tangent_nonlinear": 155155,
+    "geometric_fd": 310465,
+    "residual_linear": 13651,
+    "residual_nonlinear": 13831,
+    "adjoint_thickness": 13781,
+}
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBPS = 8000.0    # MI355X HBM3E
+
+
+def spmv_bytes(nnzb, nnodes):
+    """Algorithmic bytes of one 6x6-block SpMV: values (288 B/block), column
+    indices (4 B/block), row pointers, x gathered once and y written once
+    (48 B/node each)."""
+    return 292 * nnzb + 4 * (nnodes + 1) + 96 * nnodes
+
+
+def build_problem(na=200, nc=400):
+    cyl = Cylinder(na, nc, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
+    thick = seeded_thickness(cyl.nelems, 1234)
+    return cyl, thick
+
+
+def cpu_baseline(cyl, thick, seconds):
+    """Oracle (scalar C restatement of the reference element loop) on the
+    host: K of consecutive elements of the same mesh for ~`seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    m = O.material()
+    sample = 2000
+    conn = cyl.conn[:sample]
+    rowp, cols = O.node_csr(cyl.nnodes, conn)
+    n_done, t_used = 0, 0.0
+    while t_used < seconds:
+        t0 = time.perf_counter()
+        O.assemble_mat(O.KIND_K, 0, m, cyl.Xpts, conn, thick[0, :sample], rowp, cols)
+        t_used += time.perf_counter() - t0
+        n_done += sample
+    return {
+        "value": n_done / t_used,
+        "unit": "elem Jacobians/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"K of the first {sample} elements of the 200x400 cylinder, repeated for {t_used:.1f} s "
+                  f"({n_done} element Jacobians) on one host core",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--mode", choices=["assembly", "calibrate"], default="assembly",
+                    help="calibrate: only run a streaming kernel of known traffic (Y += a X over "
+                         "3 x 36 x nnzb doubles of the workload's matrices) for counter calibration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU decomposition is a "
+                                   "design note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    cyl, thick = build_problem()
+    ctx = sg.Context(0)
+    ctx.set_mesh(cyl.conn, cyl.Xpts)
+    ctx.set_material(70e9, 0.3, 2700.0, 0.0, 10.0)
+    ctx.set_bcs(*cyl.bcs())
+    ctx.set_designs(thick)
+    K = ctx.matrix()
+
+    if args.mode == "calibrate":
+        K2 = ctx.matrix()
+        n = 36 * ctx.nnzb
+        for _ in range(3):
+            K2.axpy(0.5, K)  # reads 2n doubles, writes n doubles
+        ctx.synchronize()
+        print(json.dumps({"calibrate": "axpy", "doubles": n, "read_bytes": 16 * n, "write_bytes": 8 * n,
+                          "launches": 3}), flush=True)
+        return
+
+    for _ in range(args.warmup):
+        ctx.assemble_mattype(sg.STIFFNESS, K)
+    ctx.synchronize()
+
+    ctx.set_timing(True)
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.assemble_mattype(sg.STIFFNESS, K)
+    wall_ms = ctx.timer_stop()
+    elem_ms, elem_launches, _, _ = ctx.get_timing()
+    ctx.set_timing(False)
+
+    n_jac = cyl.nelems * args.steps
+    value = n_jac / (wall_ms * 1e-3)
+    flops = FLOPS_PER_ELEMENT["stiffness_linear"] * n_jac
+    achieved = flops / (elem_ms * 1e-3) / 1e12
+
+    # SpMV on the assembled K
+    x = ctx.vector(np.random.default_rng(7).standard_normal(6 * cyl.nnodes))
+    y = ctx.vector()
+    for _ in range(3):
+        K.mult(x, y)
+    ctx.set_timing(True)
+    nspmv = 50
+    for _ in range(nspmv):
+        K.mult(x, y)
+    _, _, spmv_ms, spmv_n = ctx.get_timing()
+    ctx.set_timing(False)
+    spmv_gbps = spmv_bytes(ctx.nnzb, ctx.nnodes) * spmv_n / (spmv_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elem Jacobians/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cylinder mesh, thickness and imperfection)",
+        "config": {"workload": "cylinder 200x400 (80000 quads, 482400 dof): K assembly", "nelems": cyl.nelems,
+                   "nnodes": cyl.nnodes, "nnzb": ctx.nnzb, "colors": ctx.ncolors, "parallelism": "single GPU"},
+        "bsr_spmv_GBps": spmv_gbps,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": "elem_kernel<0> (stiffness, one wavefront per element)",
+            "avg_launch_ms": elem_ms / max(elem_launches, 1),
+            "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cyl, thick, args.cpu_seconds)
+    K.destroy()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

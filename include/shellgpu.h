@@ -116,6 +116,18 @@ int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x);
 /* x[bc] = ubar[bc] (TACSBVec::setBCs) */
 int shellgpu_vec_set_bcs(shellgpu_ctx *ctx, double *x);
 
+/* ---- timing (no reference counterpart: measurement plumbing) ---------- */
+/* With timing enabled, HIP events on the context's stream bracket every
+ * element-kernel launch and every SpMV launch; shellgpu_get_timing waits for
+ * the stream and returns the summed kernel durations (ms) and launch counts
+ * since timing was enabled, then resets them. */
+int shellgpu_set_timing(shellgpu_ctx *ctx, int enable);
+int shellgpu_get_timing(shellgpu_ctx *ctx, double *elem_ms, int *elem_launches,
+                        double *spmv_ms, int *spmv_launches);
+/* wall time between two points on the context's stream (HIP events) */
+int shellgpu_timer_start(shellgpu_ctx *ctx);
+int shellgpu_timer_stop(shellgpu_ctx *ctx, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

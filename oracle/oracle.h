@@ -24,7 +24,7 @@
 #ifndef SHELL_ORACLE_H
 #define SHELL_ORACLE_H
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(OR_NO_EXTERN_C)
 extern "C" {
 #endif
 
@@ -169,7 +169,7 @@ typedef struct {
 int or_linear_buckling(const or_buckling_problem *p, double *lambda,
                        double *phi, double *path, double *resid);
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(OR_NO_EXTERN_C)
 }
 #endif
 #endif
