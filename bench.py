@@ -55,6 +55,19 @@ def spmv_bytes(nnzb, nnodes):
     return 292 * nnzb + 4 * (nnodes + 1) + 96 * nnodes
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration recorded in the same file); None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def build_problem(na=200, nc=400):
     cyl = Cylinder(na, nc, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
     thick = seeded_thickness(cyl.nelems, 1234)
@@ -173,7 +186,7 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": profiled_traffic("elem_stiffness_cylinder200x400.md"),
             "kernel": "elem_kernel<0> (stiffness, one wavefront per element)",
             "avg_launch_ms": elem_ms / max(elem_launches, 1),
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
