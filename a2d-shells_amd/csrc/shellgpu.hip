@@ -8,40 +8,12 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "shellgpu.h"
-#include "shellgpu_internal.h"
+#include "shellgpu_ctx.h"
 
 using namespace shellgpu;
-
-struct shellgpu_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-
-  int nnodes = 0, nelems = 0, nnzb = 0, ncolors = 0, nbatch = 0;
-  std::vector<int> h_conn, h_rowp, h_cols, h_color, h_color_ptr, h_color_elems, h_blk;
-  int *d_conn = nullptr, *d_rowp = nullptr, *d_cols = nullptr, *d_blk = nullptr;
-  int *d_color_elems = nullptr;
-  double *d_X = nullptr;
-
-  Material mat{70e9, 0.3, 2700.0, 0.0, 10.0};
-
-  int nbc = 0;
-  int *d_bc_nodes = nullptr, *d_bc_mask = nullptr;
-  double *d_bc_vals = nullptr;
-
-  double *d_thick = nullptr, *d_temp = nullptr;
-  // timing
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
-  hipEvent_t t0 = nullptr, t1 = nullptr;
-};
-
-struct shellgpu_mat {
-  shellgpu_ctx *ctx = nullptr;
-  int nbatch = 0;
-  double *vals = nullptr;
-};
 
 namespace {
 
@@ -263,6 +235,7 @@ void shellgpu_destroy(shellgpu_ctx *ctx) {
   drop_events(ctx->spmv_ev);
   if (ctx->t0) (void)hipEventDestroy(ctx->t0);
   if (ctx->t1) (void)hipEventDestroy(ctx->t1);
+  if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

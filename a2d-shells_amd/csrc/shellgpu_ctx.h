@@ -1,0 +1,44 @@
+// shellgpu_ctx.h -- the context and matrix objects behind the C ABI
+// (host-side state shared by the translation units of the library).
+#pragma once
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "shellgpu_internal.h"
+
+struct shellgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int nnodes = 0, nelems = 0, nnzb = 0, ncolors = 0, nbatch = 0;
+  std::vector<int> h_conn, h_rowp, h_cols, h_color, h_color_ptr, h_color_elems, h_blk;
+  int *d_conn = nullptr, *d_rowp = nullptr, *d_cols = nullptr, *d_blk = nullptr;
+  int *d_color_elems = nullptr;
+  double *d_X = nullptr;
+
+  shellgpu::Material mat{70e9, 0.3, 2700.0, 0.0, 10.0};
+
+  int nbc = 0;
+  int *d_bc_nodes = nullptr, *d_bc_mask = nullptr;
+  double *d_bc_vals = nullptr;
+
+  double *d_thick = nullptr, *d_temp = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+
+  // solver state
+  std::vector<int> order_seed;  // BFS seed nodes of the level ordering
+  void *blas = nullptr;         // rocblas_handle bound to `stream`
+};
+
+struct shellgpu_mat {
+  shellgpu_ctx *ctx = nullptr;
+  int nbatch = 0;
+  double *vals = nullptr;
+};
+

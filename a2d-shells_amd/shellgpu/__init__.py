@@ -73,6 +73,15 @@ def load_library(path=LIB_PATH):
         "shellgpu_assemble_jacobian": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
         "shellgpu_vec_zero_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_vec_set_bcs": (C.c_int, [_VP, _VP]),
+        "shellgpu_set_ordering_seed": (C.c_int, [_VP, C.c_int, _IP]),
+        "shellgpu_factor_create": (C.c_int, [_VP, C.POINTER(_VP)]),
+        "shellgpu_factor_destroy": (None, [_VP]),
+        "shellgpu_factor_info": (C.c_int, [_VP, _IP, _IP, _DP]),
+        "shellgpu_factor_compute": (C.c_int, [_VP, _VP]),
+        "shellgpu_factor_apply": (C.c_int, [_VP, _VP, _VP]),
+        "shellgpu_gmres": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, _IP, _DP]),
+        "shellgpu_linear_buckling": (C.c_int, [_VP, C.c_double, _VP, C.c_int, C.c_int, C.c_double, _DP, _VP, _VP,
+                                               _IP, _DP]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
         "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
         "shellgpu_timer_start": (C.c_int, [_VP]),
@@ -188,6 +197,38 @@ class Matrix:
             pass
 
 
+class Factor:
+    """Level-set block LU of a batch of matrices (TACSPc)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = _VP()
+        ctx._check(ctx.lib.shellgpu_factor_create(ctx.h, C.byref(h)))
+        self.h = h.value
+
+    def info(self):
+        nl, mx, gib = C.c_int(), C.c_int(), C.c_double()
+        self.ctx._check(self.ctx.lib.shellgpu_factor_info(self.h, C.byref(nl), C.byref(mx), C.byref(gib)))
+        return nl.value, mx.value, gib.value
+
+    def compute(self, A):
+        self.ctx._check(self.ctx.lib.shellgpu_factor_compute(self.h, A.h))
+
+    def apply(self, b, x):
+        self.ctx._check(self.ctx.lib.shellgpu_factor_apply(self.h, b.ptr, x.ptr))
+
+    def destroy(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.shellgpu_factor_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU context: mesh, material, BCs, designs and a HIP stream."""
 
@@ -277,6 +318,40 @@ class Context:
     def assemble_jacobian(self, A, R=None, u=None, model=LINEAR):
         self._check(self.lib.shellgpu_assemble_jacobian(
             self.h, int(model), u.ptr if u else None, R.ptr if R else None, A.h))
+
+    def set_ordering_seed(self, nodes):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        self._check(self.lib.shellgpu_set_ordering_seed(self.h, nodes.size, _ip(nodes)))
+
+    def factor(self):
+        return Factor(self)
+
+    def gmres(self, A, b, x, F=None, msub=10, nrestart=15, rtol=1e-12, atol=1e-30):
+        its, rn = C.c_int(), C.c_double()
+        self._check(self.lib.shellgpu_gmres(self.h, A.h, F.h if F else None, b.ptr, x.ptr, msub, nrestart, rtol,
+                                            atol, C.byref(its), C.byref(rn)))
+        return its.value, rn.value
+
+    def linear_buckling(self, sigma, force, nev=5, max_lanczos=100, tol=1e-12, want_vectors=False, want_path=False):
+        """TACSLinearBuckling::solve for every design: returns lambda
+        [nbatch, nev], Ritz residuals, Lanczos steps, and optionally the
+        eigenvectors [nbatch, nev, 6n] and the path [nbatch, 6n]."""
+        nb, n = self.nbatch, 6 * self.nnodes
+        f = force if isinstance(force, DeviceArray) else self.vector(np.broadcast_to(force, (nb, n)))
+        lam = np.zeros(nb * nev)
+        res = np.zeros(nb * nev)
+        steps = C.c_int()
+        phi = DeviceArray(self, nb * nev * n) if want_vectors else None
+        path = DeviceArray(self, nb * n) if want_path else None
+        self._check(self.lib.shellgpu_linear_buckling(self.h, float(sigma), f.ptr, nev, max_lanczos, tol, _dp(lam),
+                                                      phi.ptr if phi else None, path.ptr if path else None,
+                                                      C.byref(steps), _dp(res)))
+        out = {"lambda": lam.reshape(nb, nev), "resid": res.reshape(nb, nev), "steps": steps.value}
+        if phi is not None:
+            out["phi"] = phi.download().reshape(nb, nev, n)
+        if path is not None:
+            out["path"] = path.download().reshape(nb, n)
+        return out
 
     def vec_zero_bcs(self, x):
         self._check(self.lib.shellgpu_vec_zero_bcs(self.h, x.ptr))

@@ -68,6 +68,15 @@ class Cylinder:
         mask = np.concatenate([np.full(n0.size, 0x3F), np.full(n1.size, 0x3E)]).astype(np.int32)
         return nodes, mask
 
+    def ring_nodes(self, i):
+        """Node ids of axial station i (a seed for the level ordering)."""
+        return np.where(self.axial_index == i)[0].astype(np.int32)
+
+    def classical_load(self, t=0.002, E=E_ALU, nu=NU_ALU):
+        """Classical axial buckling line load E t^2 / (R sqrt(3(1-nu^2)))
+        (N/m), an estimate used to place the shift."""
+        return E * t * t / (self.R * np.sqrt(3.0 * (1.0 - nu * nu)))
+
     def edge_load(self, line_load=1.0):
         f = np.zeros((self.nnodes, 6))
         n1 = np.where(self.axial_index == self.na)[0]

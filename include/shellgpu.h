@@ -22,6 +22,7 @@ extern "C" {
 
 typedef struct shellgpu_ctx shellgpu_ctx;
 typedef struct shellgpu_mat shellgpu_mat;
+typedef struct shellgpu_factor shellgpu_factor;
 
 /* matrix types (ElementMatrixType, src/elements/TACSElementTypes.h) */
 enum {
@@ -115,6 +116,46 @@ int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u,
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x);
 /* x[bc] = ubar[bc] (TACSBVec::setBCs) */
 int shellgpu_vec_set_bcs(shellgpu_ctx *ctx, double *x);
+
+/* ---- direct solver (TACSPc, src/bpmat/KSM.h:209-230) ------------------ */
+/* Replaces TACSSchurPc's AMD-ordered complete block LU (src/bpmat/
+ * TACSSchurMat.cpp:944-1652): nodes are ordered by breadth-first level sets
+ * from a seed set, which makes the matrix block tridiagonal; the level
+ * blocks are factored densely on the GPU for every design of the batch.
+ * Seed nodes (e.g. one boundary ring) set the level structure; without a
+ * seed a pseudo-peripheral node is used. */
+int shellgpu_set_ordering_seed(shellgpu_ctx *ctx, int n, const int *nodes);
+int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **F);
+void shellgpu_factor_destroy(shellgpu_factor *F);
+/* number of levels, largest level (dofs), factor memory (GiB, all designs) */
+int shellgpu_factor_info(shellgpu_factor *F, int *nlevels, int *max_level_dofs, double *gib);
+/* TACSPc::factor (TACSSchurPc::factor, TACSSchurMat.cpp:1430-1510) */
+int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A);
+/* TACSPc::applyFactor: x = A^{-1} b for every design (device vectors) */
+int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x);
+
+/* ---- Krylov (TACSKsm / GMRES, src/bpmat/KSM.cpp:547-955) --------------- */
+/* Right-preconditioned restarted GMRES(msub) with nrestart restarts; stops
+ * when |r| < atol or |r| < rtol |r0| (GMRES::solve, KSM.cpp:785-955); x is
+ * zeroed first (zero_guess = 1).  F may be NULL (no preconditioner).
+ * Returns the iteration count and the worst relative residual over designs. */
+int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b,
+                   double *x, int msub, int nrestart, double rtol, double atol, int *iters,
+                   double *resnorm);
+
+/* ---- linear buckling (TACSLinearBuckling, src/TACSBuckling.h:43-107) ---- */
+/* TACSLinearBuckling::solve(rhs = force, u0 = NULL) (src/TACSBuckling.cpp:
+ * 197-278) for every design: K, static path K u = f, G(u), K + sigma G,
+ * shift-invert Lanczos (SEP::solve, src/bpmat/GSEP.cpp:485-599) with
+ * max_lanczos steps at most, stopping when the nev eigenvalues of smallest
+ * magnitude have relative Ritz residual <= tol.  Outputs: lambda[nbatch*nev]
+ * (host), phi (device, nbatch*nev*6*nnodes, K-normalised; may be NULL), path
+ * (device, nbatch*6*nnodes; may be NULL), Lanczos steps, relative Ritz
+ * residuals resid[nbatch*nev] (host, may be NULL).  force is device,
+ * nbatch*6*nnodes. */
+int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *force, int nev,
+                             int max_lanczos, double tol, double *lambda, double *phi,
+                             double *path, int *nsteps, double *resid);
 
 /* ---- timing (no reference counterpart: measurement plumbing) ---------- */
 /* With timing enabled, HIP events on the context's stream bracket every

@@ -159,3 +159,46 @@ def bsr_to_dense(rowp, cols, vals):
         for p in range(rowp[i], rowp[i + 1]):
             A[6 * i:6 * i + 6, 6 * cols[p]:6 * cols[p] + 6] = vals[p]
     return A
+
+
+class BucklingProblem(C.Structure):
+    _fields_ = [("nnodes", C.c_int), ("nelems", C.c_int), ("conn", _IP), ("Xpts", _DP), ("thick", _DP),
+                ("temp", _DP), ("mat", C.POINTER(Material)), ("nbc", C.c_int), ("bc_nodes", _IP),
+                ("bc_mask", _IP), ("force", _DP), ("sigma", C.c_double), ("max_lanczos", C.c_int),
+                ("nev", C.c_int), ("seed", C.c_ulonglong)]
+
+
+def tridiag_eig(d, e):
+    d = _f64(d)
+    e = _f64(e)
+    n = d.size
+    eig = np.zeros(n)
+    z = np.zeros(n * n)
+    rc = lib().or_tridiag_eig(n, _d(d), _d(e), _d(eig), _d(z))
+    if rc != 0:
+        raise RuntimeError("QL did not converge")
+    return eig, z.reshape(n, n)
+
+
+def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60, mat=None, temp=None, seed=12345):
+    """Oracle linear buckling: returns (lambda[nev], phi[nev, 6n], path[6n],
+    resid[nev], nsteps)."""
+    mat = mat or material()
+    conn = np.ascontiguousarray(conn, np.int32)
+    Xpts = _f64(Xpts)
+    thick = _f64(thick)
+    bn = np.ascontiguousarray(bcs[0], np.int32)
+    bm = np.ascontiguousarray(bcs[1], np.int32)
+    force = _f64(force)
+    tp = _f64(temp) if temp is not None else None
+    nn = Xpts.size // 3
+    p = BucklingProblem(nn, conn.size // 4, _i(conn), _d(Xpts), _d(thick), _d(tp) if tp is not None else None,
+                        C.pointer(mat), bn.size, _i(bn), _i(bm), _d(force), sigma, max_lanczos, nev, seed)
+    lam = np.zeros(nev)
+    phi = np.zeros((nev, 6 * nn))
+    path = np.zeros(6 * nn)
+    resid = np.zeros(nev)
+    ns = lib().or_linear_buckling(C.byref(p), _d(lam), _d(phi), _d(path), _d(resid))
+    if ns < 0:
+        raise RuntimeError(f"oracle buckling failed ({ns})")
+    return lam, phi, path, resid, ns
