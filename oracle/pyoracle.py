@@ -202,3 +202,26 @@ def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60,
     if ns < 0:
         raise RuntimeError(f"oracle buckling failed ({ns})")
     return lam, phi, path, resid, ns
+
+
+def rcm(rowp, cols):
+    rowp = np.ascontiguousarray(rowp, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    perm = np.empty(rowp.size - 1, np.int32)
+    lib().or_rcm(rowp.size - 1, _i(rowp), _i(cols), _i(perm))
+    return perm
+
+
+def band_solve(rowp, cols, vals, b):
+    """Direct solve A x = b with the oracle's RCM-ordered band LU."""
+    rowp = np.ascontiguousarray(rowp, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = _f64(vals)
+    perm = rcm(rowp, cols)
+    f = lib().or_band_factor
+    f.restype = C.c_void_p
+    h = f(rowp.size - 1, _i(rowp), _i(cols), _d(vals), _i(perm))
+    x = np.zeros(6 * (rowp.size - 1))
+    lib().or_band_solve(C.c_void_p(h), _d(_f64(b)), _d(x))
+    lib().or_band_free(C.c_void_p(h))
+    return x

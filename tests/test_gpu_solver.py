@@ -1,0 +1,99 @@
+"""GPU direct solver, GMRES and linear buckling against the oracle: static
+path, eigenvalues within 1e-8 relative, eigenvectors by principal angles of
+each (near-)degenerate cluster within 1e-6 in the K inner product, and
+batched designs against per-design oracle runs."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import shellgpu as sg
+from shellgpu.problems import Cylinder
+
+pytestmark = pytest.mark.gpu
+
+
+def make_ctx(cyl, thick):
+    ctx = sg.Context(0)
+    ctx.set_mesh(cyl.conn, cyl.Xpts)
+    ctx.set_material(70e9, 0.3)
+    ctx.set_bcs(*cyl.bcs())
+    ctx.set_designs(thick)
+    ctx.set_ordering_seed(cyl.ring_nodes(0))
+    return ctx
+
+
+def test_direct_solve_and_gmres():
+    cyl = Cylinder(10, 20)
+    thick = np.random.default_rng(1).uniform(0.0015, 0.0025, cyl.nelems)
+    ctx = make_ctx(cyl, thick)
+    K = ctx.matrix()
+    ctx.assemble_mattype(sg.STIFFNESS, K)
+    F = ctx.factor()
+    nl, mx, _ = F.info()
+    assert nl == cyl.na + 1 and mx == 6 * cyl.nc  # one level per ring
+    F.compute(K)
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(6 * cyl.nnodes)
+    db, dx = ctx.vector(b), ctx.vector()
+    its, rn = ctx.gmres(K, db, dx, F=F, rtol=1e-12)
+    x = dx.download()
+    rowp, cols = ctx.pattern()
+    Kref = O.bsr_apply_bcs(O.assemble_mat(O.KIND_K, 0, O.material(), cyl.Xpts, cyl.conn, thick, rowp, cols),
+                           rowp, cols, *cyl.bcs())
+    r = O.bsr_matvec(rowp, cols, Kref, x) - b
+    # explicit level inverses: the true residual floor is ~cond * eps
+    assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(b)
+    xr = O.band_solve(rowp, cols, Kref, b)
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max()
+
+
+def _subspace_angle(V, W, rowp, cols, Kvals):
+    KW = np.stack([O.bsr_matvec(rowp, cols, Kvals, w) for w in W])
+    M = V @ KW.T
+    s = np.linalg.svd(M, compute_uv=False)
+    return float(np.arccos(min(1.0, s.min())))
+
+
+@pytest.mark.parametrize("na,nc", [(20, 40)])
+def test_buckling_matches_oracle(na, nc):
+    cyl = Cylinder(na, nc)
+    thick = np.full(cyl.nelems, 0.002)
+    sigma = 0.9 * cyl.classical_load()
+    ctx = make_ctx(cyl, thick)
+    nev = 8  # 5 wanted + margin so that a degenerate pair at the boundary is whole
+    out = ctx.linear_buckling(sigma, cyl.edge_load(1.0), nev=nev, max_lanczos=120, tol=1e-12, want_vectors=True,
+                              want_path=True)
+    lam, phi, path, resid, ns = O.linear_buckling(cyl.Xpts, cyl.conn, thick, cyl.bcs(), cyl.edge_load(1.0), sigma,
+                                                  nev=nev, max_lanczos=120)
+    g = out["lambda"][0]
+    print("gpu", g, "oracle", lam, "steps", out["steps"])
+    assert np.abs(g[:5] - lam[:5]).max() <= 1e-8 * np.abs(lam[:5]).max()
+    assert np.abs(out["path"][0] - path).max() <= 1e-8 * np.abs(path).max()
+    rowp, cols = ctx.pattern()
+    Kref = O.bsr_apply_bcs(O.assemble_mat(O.KIND_K, 0, O.material(), cyl.Xpts, cyl.conn, thick, rowp, cols),
+                           rowp, cols, *cyl.bcs())
+    # clusters of (near-)degenerate eigenvalues compared as subspaces; the
+    # clusters covering the first 5 eigenvalues must close before index nev
+    i = 0
+    while i < 5:
+        j = i + 1
+        while j < nev and abs(lam[j] - lam[i]) <= 1e-6 * abs(lam[i]):
+            j += 1
+        assert j < nev, "cluster reaches the end of the computed set"
+        ang = _subspace_angle(out["phi"][0][i:j], phi[i:j], rowp, cols, Kref)
+        print("cluster", i, j, "angle", ang)
+        assert ang <= 1e-6
+        i = j
+
+
+def test_batched_designs_match_single_designs():
+    cyl = Cylinder(8, 16, imperfection=(2e-4, 2, 3))
+    rng = np.random.default_rng(9)
+    thick = rng.uniform(0.0015, 0.0025, (3, cyl.nelems))
+    sigma = 0.8 * cyl.classical_load()
+    ctx = make_ctx(cyl, thick)
+    out = ctx.linear_buckling(sigma, cyl.edge_load(1.0), nev=4, max_lanczos=80, tol=1e-12)
+    for b in range(3):
+        lam, _, _, _, _ = O.linear_buckling(cyl.Xpts, cyl.conn, thick[b], cyl.bcs(), cyl.edge_load(1.0), sigma,
+                                            nev=4, max_lanczos=80)
+        assert np.abs(out["lambda"][b] - lam).max() <= 1e-8 * np.abs(lam).max()
