@@ -354,6 +354,8 @@ int shellgpu_set_bcs(shellgpu_ctx *ctx, int nbc, const int *nodes, const int *ma
   for (int i = 0; i < nbc; i++)
     if (nodes[i] < 0 || nodes[i] >= ctx->nnodes) return set_err(ctx, 1, "BC node out of range");
   ctx->nbc = nbc;
+  ctx->h_bc_nodes.assign(nodes, nodes + nbc);
+  ctx->h_bc_mask.assign(mask, mask + nbc);
   int rc = 0;
   if ((rc = upload(ctx, &ctx->d_bc_nodes, nodes, (size_t)nbc))) return rc;
   if ((rc = upload(ctx, &ctx->d_bc_mask, mask, (size_t)nbc))) return rc;

@@ -22,6 +22,7 @@ struct shellgpu_ctx {
   shellgpu::Material mat{70e9, 0.3, 2700.0, 0.0, 10.0};
 
   int nbc = 0;
+  std::vector<int> h_bc_nodes, h_bc_mask;
   int *d_bc_nodes = nullptr, *d_bc_mask = nullptr;
   double *d_bc_vals = nullptr;
 

@@ -6,16 +6,24 @@
 // breadth-first level sets from a seed set (a whole end ring of the
 // cylinder, or a pseudo-peripheral node), which makes every shell mesh
 // block-tridiagonal in level blocks: a node couples only to nodes of its own
-// and the neighbouring levels.  The block LU then runs level by level on
-// dense blocks with the GPU's dense libraries, strided-batched over designs:
+// and the neighbouring levels.  Boundary conditions are eliminated
+// symmetrically (BC rows and columns zero, unit diagonal), so the stiffness
+// and the shifted operator K + sigma G (sigma below the first buckling
+// load) are symmetric positive definite and the level blocks are factored
+// by block Cholesky, level by level, strided-batched over designs:
 //
-//   S_0 = D_0,  S_l = D_l - L_l W_{l-1},  W_l = S_l^{-1} U_l
+//   S_0 = D_0,  V_l = L_l C_{l-1}^{-T},  S_l = D_l - V_l V_l^T,  S_l = C_l C_l^T
 //
-// with S_l^{-1} formed explicitly (getrf + getri) so that a solve is a chain
-// of bandwidth-bound GEMVs:
+// with the triangular inverses C_l^{-1} formed explicitly (potrf + trtri;
+// their error grows with sqrt(cond), not cond) so that a solve is a chain of
+// bandwidth-bound matrix-vector products:
 //
-//   forward  z_l = S_l^{-1} (b_l - L_l z_{l-1})
-//   backward x_l = z_l - W_l x_{l+1}
+//   forward  y_l = C_l^{-1} (b_l - V_l y_{l-1})
+//   backward x_l = C_l^{-T} (y_l - V_{l+1}^T x_{l+1})
+//
+// A failed Cholesky (non-positive pivot) reports a shift above the first
+// eigenvalue.  The solve is exact for right-hand sides that vanish on BC
+// dofs (all solves of the buckling path); GMRES corrects the remainder.
 //
 // GMRES.  Right-preconditioned restarted GMRES with Givens rotations
 // (GMRES::solve, src/bpmat/KSM.cpp:785-955), FP64, preconditioned by the
@@ -52,18 +60,19 @@ struct shellgpu_factor {
   int n = 0;  // scalar dofs
   std::vector<int> m;        // dofs per level
   std::vector<long> voff;    // offset of level l in a level-ordered vector
-  std::vector<long> S_off, L_off, W_off;
+  std::vector<long> S_off, L_off;  // C_l^{-1} and V_l in a design's buffer
   long per_design = 0;       // doubles per design
   int maxm = 0;
-  long tmp_elems = 0;        // largest m_l * m_{l+1}
+  long tmp_elems = 0;        // largest m_l * m_{l-1}
   double *buf = nullptr;     // nbatch * per_design
   double *tmp = nullptr;     // nbatch * tmp_elems
-  int *ipiv = nullptr;       // nbatch * maxm
   int *info = nullptr;       // nbatch
+  int *d_bcflag = nullptr;   // per level-ordered dof: 1 if a BC dof
   int *d_pos2dof = nullptr;  // level-ordered position -> original dof
   long *d_blk_base = nullptr;  // per BSR block: dense offset of entry (0,0)
   int *d_blk_ld = nullptr;     // per BSR block: leading dimension of its target
-  double *vb = nullptr, *vz = nullptr;  // level-ordered work vectors, nbatch * n
+  double *vb = nullptr, *vz = nullptr, *vx = nullptr;  // level-ordered work vectors
+  double *part = nullptr;    // GEMV partial sums
   bool factored = false;
 };
 
@@ -87,15 +96,81 @@ int ferr(shellgpu_ctx *ctx, int code, const std::string &msg) {
       return ferr(ctx, 4, std::string(#call) + ": " + rocblas_status_to_string(_s)); \
   } while (0)
 
+// BSR values -> dense level blocks (D_l and L_l; blocks above the diagonal
+// level are not stored).  BC columns are zeroed (the BSR rows of BC dofs are
+// already zero with a unit diagonal), giving the symmetric elimination.
 __global__ void scatter_dense_kernel(int nnzb, const double *__restrict__ vals, long vstride,
                                      const long *__restrict__ base, const int *__restrict__ ld,
-                                     double *__restrict__ dst, long dstride) {
+                                     const int *__restrict__ colbc, double *__restrict__ dst, long dstride) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 36L * nnzb) return;
   const int p = (int)(i / 36), e = (int)(i % 36), r = e / 6, c = e % 6;
+  if (base[p] < 0) return;
   vals += (long)blockIdx.y * vstride;
   dst += (long)blockIdx.y * dstride;
-  dst[base[p] + (long)c * ld[p] + r] = vals[i];
+  const int bcmask = colbc[p];  // bits r (row dof BC) and 6 + c (column dof BC), bit 12 diagonal block
+  double v = vals[i];
+  if ((bcmask >> (6 + c)) & 1) v = ((bcmask >> 12) & 1) && r == c ? v : 0.0;
+  dst[base[p] + (long)c * ld[p] + r] = v;
+}
+
+// y = beta y + alpha A x, A column-major m x n (lda); lower != 0 restricts to
+// the lower triangle (col <= row).  Pass 1: 64 rows x 4 column groups per
+// block, columns split over gridDim.y chunks -> partial sums.
+__global__ __launch_bounds__(256) void gemv_n_pass1(int m, int n, const double *__restrict__ A, int lda, long sA,
+                                                    const double *__restrict__ x, long sx, int lower, int chunk,
+                                                    double *__restrict__ part) {
+  __shared__ double red[256];
+  const int b = blockIdx.z;
+  A += b * sA;
+  x += b * sx;
+  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int g = threadIdx.x >> 6;
+  const int c0 = blockIdx.y * chunk;
+  int c1 = min(n, c0 + chunk);
+  double acc = 0.0;
+  if (row < m) {
+    if (lower) c1 = min(c1, row + 1);
+    for (int c = c0 + g; c < c1; c += 4) acc += A[row + (long)c * lda] * x[c];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64 && row < m) {
+    const double s = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+    part[((long)b * gridDim.y + blockIdx.y) * m + row] = s;
+  }
+}
+
+__global__ void gemv_n_pass2(int m, int nsplit, const double *__restrict__ part, double alpha, double beta,
+                             double *__restrict__ y, long sy) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= m) return;
+  const int b = blockIdx.y;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; k++) s += part[((long)b * nsplit + k) * m + row];
+  double *yy = y + b * sy;
+  yy[row] = (beta == 0.0 ? 0.0 : beta * yy[row]) + alpha * s;
+}
+
+// y = beta y + alpha A^T x, A column-major m x n (lda): one wavefront per
+// output entry j (a column of A read coalesced); lower != 0 restricts to
+// rows i >= j.
+__global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double *__restrict__ A, int lda, long sA,
+                                                     const double *__restrict__ x, long sx, int lower, double alpha,
+                                                     double beta, double *__restrict__ y, long sy) {
+  const int b = blockIdx.y;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= n) return;
+  const double *col = A + b * sA + (long)j * lda;
+  const double *xx = x + b * sx;
+  double acc = 0.0;
+  for (int i = (lower ? j : 0) + lane; i < m; i += 64) acc += col[i] * xx[i];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) {
+    double *yy = y + b * sy;
+    yy[j] = (beta == 0.0 ? 0.0 : beta * yy[j]) + alpha * acc;
+  }
 }
 
 __global__ void permute_kernel(int n, const int *__restrict__ pos2dof, const double *__restrict__ src,
@@ -211,50 +286,59 @@ int setup_plan(shellgpu_factor *F) {
   }
   F->S_off.assign(nl, 0);
   F->L_off.assign(nl, -1);
-  F->W_off.assign(nl, -1);
   long off = 0;
-  F->tmp_elems = 0;
+  F->tmp_elems = 1;
   for (int l = 0; l < nl; l++) {
     F->S_off[l] = off;
     off += (long)F->m[l] * F->m[l];
     if (l > 0) {
       F->L_off[l] = off;
       off += (long)F->m[l] * F->m[l - 1];
-    }
-    if (l < nl - 1) {
-      F->W_off[l] = off;
-      off += (long)F->m[l] * F->m[l + 1];
-      F->tmp_elems = std::max(F->tmp_elems, (long)F->m[l] * F->m[l + 1]);
+      F->tmp_elems = std::max(F->tmp_elems, (long)F->m[l] * F->m[l - 1]);
     }
   }
   F->per_design = off;
 
+  // BC dofs (symmetric elimination)
+  std::vector<int> isbc(F->n, 0);
+  for (size_t b = 0; b < ctx->h_bc_nodes.size(); b++)
+    for (int r = 0; r < 6; r++)
+      if (ctx->h_bc_mask[b] & (1 << r)) isbc[6 * ctx->h_bc_nodes[b] + r] = 1;
+
   // dof permutation and block scatter map
-  std::vector<int> pos2dof(F->n);
+  std::vector<int> pos2dof(F->n), bcpos(F->n);
   for (int i = 0; i < nn; i++)
-    for (int d = 0; d < 6; d++) pos2dof[6L * node_pos[i] + d] = 6 * i + d;
+    for (int d = 0; d < 6; d++) {
+      pos2dof[6L * node_pos[i] + d] = 6 * i + d;
+      bcpos[6L * node_pos[i] + d] = isbc[6 * i + d];
+    }
   std::vector<long> base(ctx->nnzb);
-  std::vector<int> ld(ctx->nnzb);
+  std::vector<int> ld(ctx->nnzb), colbc(ctx->nnzb);
   for (int i = 0; i < nn; i++) {
     const int li = level[i];
     const long rl = 6L * (node_pos[i] - noff[li]);
     for (int p = rowp[i]; p < rowp[i + 1]; p++) {
       const int j = cols[p], lj = level[j];
       const long cl = 6L * (node_pos[j] - noff[lj]);
-      long tgt;
+      int mask = 0;
+      for (int d = 0; d < 6; d++) mask |= (isbc[6 * i + d] << d) | (isbc[6 * j + d] << (6 + d));
+      if (i == j) mask |= 1 << 12;
+      colbc[p] = mask;
+      ld[p] = F->m[li];
       if (lj == li) {
-        tgt = F->S_off[li];
+        base[p] = F->S_off[li] + cl * F->m[li] + rl;
       } else if (lj == li - 1) {
-        tgt = F->L_off[li];
+        base[p] = F->L_off[li] + cl * F->m[li] + rl;
       } else if (lj == li + 1) {
-        tgt = F->W_off[li];
+        base[p] = -1;  // upper block: the transpose of the next level's L
       } else {
         return ferr(ctx, 1, "level ordering is not block tridiagonal");
       }
-      ld[p] = F->m[li];
-      base[p] = tgt + cl * F->m[li] + rl;
     }
   }
+  HIPCALL(ctx, hipMalloc((void **)&F->d_bcflag, sizeof(int) * ctx->nnzb));
+  HIPCALL(ctx, hipMemcpyAsync(F->d_bcflag, colbc.data(), sizeof(int) * ctx->nnzb, hipMemcpyHostToDevice,
+                              ctx->stream));
   HIPCALL(ctx, hipMalloc((void **)&F->d_pos2dof, sizeof(int) * F->n));
   HIPCALL(ctx, hipMalloc((void **)&F->d_blk_base, sizeof(long) * ctx->nnzb));
   HIPCALL(ctx, hipMalloc((void **)&F->d_blk_ld, sizeof(int) * ctx->nnzb));
@@ -356,10 +440,11 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
   const size_t bytes = sizeof(double) * (size_t)F->per_design * F->nbatch;
   hipError_t e = hipMalloc((void **)&F->buf, bytes);
   if (e == hipSuccess) e = hipMalloc((void **)&F->tmp, sizeof(double) * std::max(1L, F->tmp_elems) * F->nbatch);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->ipiv, sizeof(int) * (size_t)F->maxm * F->nbatch);
   if (e == hipSuccess) e = hipMalloc((void **)&F->info, sizeof(int) * F->nbatch);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vb, sizeof(double) * (size_t)F->n * F->nbatch);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vz, sizeof(double) * (size_t)F->n * F->nbatch);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->vx, sizeof(double) * (size_t)F->n * F->nbatch);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->part, sizeof(double) * (size_t)F->maxm * 32 * F->nbatch);
   if (e != hipSuccess) {
     shellgpu_factor_destroy(F);
     return ferr(ctx, 2, std::string("factor allocation (") + std::to_string(bytes >> 20) +
@@ -372,7 +457,8 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
 void shellgpu_factor_destroy(shellgpu_factor *F) {
   if (!F) return;
   (void)hipStreamSynchronize(F->ctx->stream);
-  void *bufs[] = {F->buf, F->tmp, F->ipiv, F->info, F->d_pos2dof, F->d_blk_base, F->d_blk_ld, F->vb, F->vz};
+  void *bufs[] = {F->buf, F->tmp, F->info, F->d_bcflag, F->d_pos2dof, F->d_blk_base, F->d_blk_ld,
+                  F->vb,  F->vz,  F->vx,   F->part};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete F;
@@ -394,75 +480,99 @@ int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   rocblas_handle h = (rocblas_handle)ctx->blas;
   const int nb = F->nbatch;
   const long ps = F->per_design;
+  F->factored = false;
   HIPCALL(ctx, hipMemsetAsync(F->buf, 0, sizeof(double) * (size_t)ps * nb, ctx->stream));
   {
     const long nthreads = 36L * ctx->nnzb;
     hipLaunchKernelGGL(scatter_dense_kernel, dim3((unsigned)((nthreads + 255) / 256), nb), dim3(256), 0, ctx->stream,
-                       ctx->nnzb, shellgpu_mat_values(A), 36L * ctx->nnzb, F->d_blk_base, F->d_blk_ld, F->buf, ps);
+                       ctx->nnzb, shellgpu_mat_values(A), 36L * ctx->nnzb, F->d_blk_base, F->d_blk_ld, F->d_bcflag,
+                       F->buf, ps);
     HIPCALL(ctx, hipGetLastError());
   }
   HIPCALL(ctx, hipMemsetAsync(F->info, 0, sizeof(int) * nb, ctx->stream));
-  const double one = 1.0, mone = -1.0, zero = 0.0;
+  const double one = 1.0, mone = -1.0;
   for (int l = 0; l < F->nlevels; l++) {
     const int ml = F->m[l];
     double *S = F->buf + F->S_off[l];
     if (l > 0) {
       const int mp = F->m[l - 1];
-      BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, ml, ml, mp,
-                                                  &mone, F->buf + F->L_off[l], ml, ps, F->buf + F->W_off[l - 1], mp,
+      double *V = F->buf + F->L_off[l];
+      // V_l = L_l C_{l-1}^{-T}
+      BLASCALL(ctx, rocblas_dtrmm_strided_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                                  rocblas_diagonal_non_unit, ml, mp, &one, F->buf + F->S_off[l - 1],
+                                                  mp, ps, V, ml, ps, F->tmp, ml, (long)ml * mp, nb));
+      HIPCALL(ctx, hipMemcpy2DAsync(V, sizeof(double) * ps, F->tmp, sizeof(double) * (size_t)ml * mp,
+                                    sizeof(double) * (size_t)ml * mp, nb, hipMemcpyDeviceToDevice, ctx->stream));
+      // S_l = D_l - V_l V_l^T (lower triangle)
+      BLASCALL(ctx, rocblas_dsyrk_strided_batched(h, rocblas_fill_lower, rocblas_operation_none, ml, mp, &mone, V, ml,
                                                   ps, &one, S, ml, ps, nb));
     }
-    BLASCALL(ctx, rocsolver_dgetrf_strided_batched(h, ml, ml, S, ml, ps, F->ipiv, F->maxm, F->info, nb));
-    BLASCALL(ctx, rocsolver_dgetri_strided_batched(h, ml, S, ml, ps, F->ipiv, F->maxm, F->info, nb));
-    if (l < F->nlevels - 1) {
-      const int mn = F->m[l + 1];
-      double *W = F->buf + F->W_off[l];
-      BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, ml, mn, ml,
-                                                  &one, S, ml, ps, W, ml, ps, &zero, F->tmp, ml, (long)ml * mn, nb));
-      HIPCALL(ctx, hipMemcpy2DAsync(W, sizeof(double) * ps, F->tmp, sizeof(double) * (size_t)ml * mn,
-                                    sizeof(double) * (size_t)ml * mn, nb, hipMemcpyDeviceToDevice, ctx->stream));
-    }
+    BLASCALL(ctx, rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, ml, S, ml, ps, F->info, nb));
+    BLASCALL(ctx, rocsolver_dtrtri_strided_batched(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ml, S, ml, ps,
+                                                   F->info, nb));
   }
   std::vector<int> info(nb);
   HIPCALL(ctx, hipMemcpyAsync(info.data(), F->info, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
   HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   for (int b = 0; b < nb; b++)
-    if (info[b] != 0) return ferr(ctx, 5, "singular pivot block in the level factorization (design " +
-                                              std::to_string(b) + ")");
+    if (info[b] != 0)
+      return ferr(ctx, 5, "matrix not positive definite in the level Cholesky (design " + std::to_string(b) +
+                              "): for K + sigma G the shift is above the first buckling load");
   F->factored = true;
+  return 0;
+}
+
+// y = beta y + alpha op(A) x over the batch (device, strides per design)
+static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, int lda, const double *x, double alpha,
+                double beta, double *y, int lower) {
+  shellgpu_ctx *ctx = F->ctx;
+  const int nb = F->nbatch;
+  const long ps = F->per_design, sv = F->n;
+  if (!trans) {
+    const int nsplit = std::max(1, std::min(32, n / 64));
+    const int chunk = (n + nsplit - 1) / nsplit;
+    hipLaunchKernelGGL(gemv_n_pass1, dim3((m + 63) / 64, nsplit, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x,
+                       sv, lower, chunk, F->part);
+    HIPCALL(ctx, hipGetLastError());
+    hipLaunchKernelGGL(gemv_n_pass2, dim3((m + 255) / 256, nb), dim3(256), 0, ctx->stream, m, nsplit, F->part, alpha,
+                       beta, y, sv);
+    HIPCALL(ctx, hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(gemv_t_kernel, dim3((n + 3) / 4, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv, lower,
+                       alpha, beta, y, sv);
+    HIPCALL(ctx, hipGetLastError());
+  }
   return 0;
 }
 
 int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
   shellgpu_ctx *ctx = F->ctx;
   if (!F->factored) return ferr(ctx, 1, "factor not computed");
-  int rc = ensure_blas(ctx);
-  if (rc) return rc;
-  rocblas_handle h = (rocblas_handle)ctx->blas;
   const int nb = F->nbatch, n = F->n;
-  const long ps = F->per_design;
-  const double one = 1.0, mone = -1.0, zero = 0.0;
+  int rc = 0;
   hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256, nb), dim3(256), 0, ctx->stream, n, F->d_pos2dof, b, F->vb,
                      1);
   HIPCALL(ctx, hipGetLastError());
+  // forward: y_l = C_l^{-1} (b_l - V_l y_{l-1})
   for (int l = 0; l < F->nlevels; l++) {
     const int ml = F->m[l];
-    double *bl = F->vb + F->voff[l], *zl = F->vz + F->voff[l];
-    if (l > 0) {
-      BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_none, ml, F->m[l - 1], &mone,
-                                                  F->buf + F->L_off[l], ml, ps, F->vz + F->voff[l - 1], 1, n, &one,
-                                                  bl, 1, n, nb));
-    }
-    BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_none, ml, ml, &one, F->buf + F->S_off[l], ml,
-                                                ps, bl, 1, n, &zero, zl, 1, n, nb));
+    double *bl = F->vb + F->voff[l];
+    if (l > 0 && (rc = gemv(F, false, ml, F->m[l - 1], F->buf + F->L_off[l], ml, F->vz + F->voff[l - 1], -1.0, 1.0,
+                             bl, 0)))
+      return rc;
+    if ((rc = gemv(F, false, ml, ml, F->buf + F->S_off[l], ml, bl, 1.0, 0.0, F->vz + F->voff[l], 1))) return rc;
   }
-  for (int l = F->nlevels - 2; l >= 0; l--) {
+  // backward: x_l = C_l^{-T} (y_l - V_{l+1}^T x_{l+1})
+  for (int l = F->nlevels - 1; l >= 0; l--) {
     const int ml = F->m[l];
-    BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_none, ml, F->m[l + 1], &mone,
-                                                F->buf + F->W_off[l], ml, ps, F->vz + F->voff[l + 1], 1, n, &one,
-                                                F->vz + F->voff[l], 1, n, nb));
+    double *yl = F->vz + F->voff[l];
+    if (l < F->nlevels - 1 &&
+        (rc = gemv(F, true, F->m[l + 1], ml, F->buf + F->L_off[l + 1], F->m[l + 1], F->vx + F->voff[l + 1], -1.0, 1.0,
+                   yl, 0)))
+      return rc;
+    if ((rc = gemv(F, true, ml, ml, F->buf + F->S_off[l], ml, yl, 1.0, 0.0, F->vx + F->voff[l], 1))) return rc;
   }
-  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256, nb), dim3(256), 0, ctx->stream, n, F->d_pos2dof, F->vz, x,
+  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256, nb), dim3(256), 0, ctx->stream, n, F->d_pos2dof, F->vx, x,
                      0);
   HIPCALL(ctx, hipGetLastError());
   return 0;
@@ -659,22 +769,136 @@ out:
 // ---------------------------------------------------------------------------
 // linear buckling
 // ---------------------------------------------------------------------------
+namespace {
+
+// A[b] = K[b] + s[b] G[b] over the batch
+__global__ void shifted_kernel(long n, const double *K, const double *G, const double *s, double *A) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long o = (long)blockIdx.y * n + i;
+  A[o] = K[o] + s[blockIdx.y] * G[o];
+}
+
+struct LanczosOut {
+  std::vector<std::vector<double>> alpha, beta;
+  int steps = 0;
+};
+
+// Lanczos in the K inner product, designs in lock-step, full CGS2
+// re-orthogonalisation.  mode 0: w = A^{-1} K q (shift-invert, A = K +
+// sigma G factored in F); mode 1: w = K^{-1} (-G q) (buckling mode, F
+// factors K).  Ritz value theta relates to the buckling load by
+// lambda = -theta sigma / (1 - theta) (mode 0) or lambda = 1 / theta (mode 1).
+// Stops when the nev wanted Ritz values have relative residual <= tol.
+int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
+            const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
+            double *work, LanczosOut &lo) {
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  const double one = 1.0, mone = -1.0, zero = 0.0;
+  VecOps vo;
+  int rc = vo.init(ctx, n, nb);
+  if (rc) return rc;
+  std::vector<double> tmp(nb);
+  lo.alpha.assign(nb, {});
+  lo.beta.assign(nb, {});
+  {
+    std::vector<double> q0((size_t)n * nb);
+    unsigned long long s = 0x5eedULL;
+    for (auto &v : q0) {
+      s += 0x9E3779B97F4A7C15ULL;
+      unsigned long long z = s;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      z ^= z >> 31;
+      v = 2.0 * ((double)(z >> 11) / 9007199254740992.0) - 1.0;
+    }
+    HIPCALL(ctx, hipMemcpyAsync(Q, q0.data(), sizeof(double) * q0.size(), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = shellgpu_vec_zero_bcs(ctx, Q))) return rc;
+    if ((rc = shellgpu_mat_mult(K, Q, KQ))) return rc;
+    if ((rc = vo.dot(Q, KQ, tmp))) return rc;
+    for (auto &t : tmp) t = 1.0 / std::sqrt(t);
+    if ((rc = vo.scale(Q, tmp))) return rc;
+    if ((rc = vo.scale(KQ, tmp))) return rc;
+  }
+  int k;
+  for (k = 0; k < m; k++) {
+    double *qn = Q + (size_t)(k + 1) * nb * n, *Kqn = KQ + (size_t)(k + 1) * nb * n;
+    if (mode == 0) {
+      if ((rc = shellgpu_factor_apply(F, KQ + (size_t)k * nb * n, qn))) return rc;
+    } else {
+      if ((rc = shellgpu_mat_mult(G, Q + (size_t)k * nb * n, work))) return rc;
+      if ((rc = shellgpu_vec_zero_bcs(ctx, work))) return rc;
+      if ((rc = shellgpu_factor_apply(F, work, qn))) return rc;
+      std::vector<double> m1(nb, -1.0);
+      if ((rc = vo.scale(qn, m1))) return rc;
+    }
+    if ((rc = shellgpu_vec_zero_bcs(ctx, qn))) return rc;
+    std::vector<double> hsum((size_t)nb * (k + 1), 0.0), hh((size_t)nb * (k + 1));
+    for (int pass = 0; pass < 2; pass++) {
+      BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_transpose, n, k + 1, &one, KQ, nb * n, n, qn,
+                                                  1, n, &zero, hv, 1, k + 1, nb));
+      BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_none, n, k + 1, &mone, Q, nb * n, n, hv, 1,
+                                                  k + 1, &one, qn, 1, n, nb));
+      HIPCALL(ctx, hipMemcpyAsync(hh.data(), hv, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+      for (size_t i = 0; i < hh.size(); i++) hsum[i] += hh[i];
+    }
+    for (int d = 0; d < nb; d++) lo.alpha[d].push_back(hsum[(size_t)d * (k + 1) + k]);
+    if ((rc = shellgpu_mat_mult(K, qn, Kqn))) return rc;
+    if ((rc = vo.dot(qn, Kqn, tmp))) return rc;
+    for (int d = 0; d < nb; d++) {
+      const double bt = std::sqrt(std::max(tmp[d], 0.0));
+      lo.beta[d].push_back(bt);
+      tmp[d] = bt > 0.0 ? 1.0 / bt : 0.0;
+    }
+    if ((rc = vo.scale(qn, tmp))) return rc;
+    if ((rc = vo.scale(Kqn, tmp))) return rc;
+    if (k + 1 >= nev && ((k + 1) % 5 == 0 || k + 1 == m)) {
+      bool all = true;
+      for (int d = 0; d < nb && all; d++) {
+        std::vector<double> w, z;
+        const int ns = k + 1;
+        std::vector<double> e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
+        if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+        std::vector<int> idx(ns);
+        for (int i = 0; i < ns; i++) idx[i] = i;
+        auto key = [&](int i) {
+          return mode == 0 ? std::fabs(-w[i] * sigma[d] / (1.0 - w[i])) : (w[i] > 0 ? 1.0 / w[i] : 1e300);
+        };
+        std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+        for (int j = 0; j < nev; j++) {
+          const int i = idx[j];
+          const double err = std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]);
+          if (!(err <= tol * std::fabs(w[i]))) all = false;
+        }
+      }
+      if (all) {
+        k++;
+        break;
+      }
+    }
+  }
+  lo.steps = std::min(k, m);
+  return 0;
+}
+
+}  // namespace
+
 extern "C" int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *force, int nev,
                                         int max_lanczos, double tol, double *lambda, double *phi, double *path,
-                                        int *nsteps, double *resid) {
+                                        int *nsteps, double *resid, double *sigma_used) {
   if (!ctx || nev <= 0 || max_lanczos < nev) return ferr(ctx, 1, "invalid buckling arguments");
   const int nb = ctx->nbatch, n = 6 * ctx->nnodes, m = max_lanczos;
   int rc = 0;
   shellgpu_mat *K = nullptr, *G = nullptr, *A = nullptr;
   shellgpu_factor *F = nullptr;
-  double *u = nullptr, *res = nullptr, *f0 = nullptr, *Q = nullptr, *KQ = nullptr, *hv = nullptr;
+  double *u = nullptr, *res = nullptr, *f0 = nullptr, *Q = nullptr, *KQ = nullptr, *hv = nullptr, *dsig = nullptr;
   VecOps vo;
-  std::vector<std::vector<double>> alpha(nb), beta(nb);
-  std::vector<double> tmp(nb), lam_all;
-  std::vector<int> conv(nb, 0);
-  int k = 0;
+  std::vector<double> sig(nb, sigma);
+  LanczosOut lo;
   rocblas_handle h;
-  const double one = 1.0, mone = -1.0, zero = 0.0;
+  const double one = 1.0, zero = 0.0;
 
   if ((rc = vo.init(ctx, n, nb))) return rc;
   if ((rc = ensure_blas(ctx))) return rc;
@@ -689,6 +913,7 @@ extern "C" int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const d
   HIPCALL(ctx, hipMalloc((void **)&Q, sizeof(double) * (size_t)n * nb * (m + 1)));
   HIPCALL(ctx, hipMalloc((void **)&KQ, sizeof(double) * (size_t)n * nb * (m + 1)));
   HIPCALL(ctx, hipMalloc((void **)&hv, sizeof(double) * (size_t)nb * (m + 1)));
+  HIPCALL(ctx, hipMalloc((void **)&dsig, sizeof(double) * nb));
 
   // TACSLinearBuckling::solve with u0 = NULL (src/TACSBuckling.cpp:236-271)
   if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, K))) goto out;
@@ -709,104 +934,66 @@ extern "C" int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const d
   }
   if ((rc = shellgpu_vec_set_bcs(ctx, u))) goto out;
   if (path) HIPCALL(ctx, hipMemcpyAsync(path, u, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream));
-
-  // G at the path; A = K + sigma G with BCs; factor A
   if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_GEOMETRIC, u, G))) goto out;
-  if ((rc = shellgpu_mat_copy(A, K))) goto out;
-  if ((rc = shellgpu_mat_axpy(A, sigma, G))) goto out;
-  if ((rc = shellgpu_mat_apply_bcs(A, 1.0))) goto out;
-  if ((rc = shellgpu_factor_compute(F, A))) goto out;
 
-  // Lanczos, designs in lock-step.  Q and KQ: vector j of design d at
-  // (j * nb + d) * n.
-  {
-    std::vector<double> q0((size_t)n * nb);
-    unsigned long long s = 0x5eedULL;
-    for (auto &v : q0) {
-      s += 0x9E3779B97F4A7C15ULL;
-      unsigned long long z = s;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-      z ^= z >> 31;
-      v = 2.0 * ((double)(z >> 11) / 9007199254740992.0) - 1.0;
-    }
-    HIPCALL(ctx, hipMemcpyAsync(Q, q0.data(), sizeof(double) * q0.size(), hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = shellgpu_vec_zero_bcs(ctx, Q))) goto out;
-    if ((rc = shellgpu_mat_mult(K, Q, KQ))) goto out;
-    if ((rc = vo.dot(Q, KQ, tmp))) goto out;
-    for (auto &t : tmp) t = 1.0 / std::sqrt(t);
-    if ((rc = vo.scale(Q, tmp))) goto out;
-    if ((rc = vo.scale(KQ, tmp))) goto out;
-  }
-  for (k = 0; k < m; k++) {
-    double *qn = Q + (size_t)(k + 1) * nb * n, *Kqn = KQ + (size_t)(k + 1) * nb * n;
-    if ((rc = shellgpu_factor_apply(F, KQ + (size_t)k * nb * n, qn))) goto out;
-    if ((rc = shellgpu_vec_zero_bcs(ctx, qn))) goto out;
-    // classical Gram-Schmidt, twice, in the K inner product:
-    // h = KQ_{0..k}^T qn ; qn -= Q_{0..k} h   (per design: strided by nb*n)
-    std::vector<double> hsum((size_t)nb * (k + 1), 0.0), hh((size_t)nb * (k + 1));
-    for (int pass = 0; pass < 2; pass++) {
-      BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_transpose, n, k + 1, &one, KQ, nb * n, n,
-                                                  qn, 1, n, &zero, hv, 1, k + 1, nb));
-      BLASCALL(ctx, rocblas_dgemv_strided_batched(h, rocblas_operation_none, n, k + 1, &mone, Q, nb * n, n, hv, 1,
-                                                  k + 1, &one, qn, 1, n, nb));
-      HIPCALL(ctx, hipMemcpyAsync(hh.data(), hv, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, ctx->stream));
-      HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
-      for (size_t i = 0; i < hh.size(); i++) hsum[i] += hh[i];
-    }
-    for (int d = 0; d < nb; d++) alpha[d].push_back(hsum[(size_t)d * (k + 1) + k]);
-    if ((rc = shellgpu_mat_mult(K, qn, Kqn))) goto out;
-    if ((rc = vo.dot(qn, Kqn, tmp))) goto out;
+  // shift: sigma > 0 as given; sigma <= 0 places it at 0.95 of a first-load
+  // estimate from a short buckling-mode Lanczos on K^{-1}(-G)
+  if (sigma <= 0.0) {
+    if ((rc = lanczos(ctx, F, K, G, 1, sig, 1, std::min(m, 40), 1e-4, Q, KQ, hv, res, lo))) goto out;
     for (int d = 0; d < nb; d++) {
-      const double bt = std::sqrt(std::max(tmp[d], 0.0));
-      beta[d].push_back(bt);
-      tmp[d] = bt > 0.0 ? 1.0 / bt : 0.0;
-    }
-    if ((rc = vo.scale(qn, tmp))) goto out;
-    if ((rc = vo.scale(Kqn, tmp))) goto out;
-    // convergence test on the Ritz values of interest
-    if (k + 1 >= nev && ((k + 1) % 5 == 0 || k + 1 == m)) {
-      bool all = true;
-      for (int d = 0; d < nb; d++) {
-        std::vector<double> w, z;
-        const int ns = k + 1;
-        std::vector<double> e(beta[d].begin(), beta[d].begin() + (ns - 1));
-        if (tridiag_ql(ns, alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
-        std::vector<int> idx(ns);
-        for (int i = 0; i < ns; i++) idx[i] = i;
-        auto lam_of = [&](int i) { return -w[i] * sigma / (1.0 - w[i]); };
-        std::sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(lam_of(a)) < std::fabs(lam_of(b)); });
-        int c = 0;
-        for (int j = 0; j < nev; j++) {
-          const int i = idx[j];
-          const double err = std::fabs(beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]);
-          if (err <= tol * std::fabs(w[i])) c++;
-        }
-        conv[d] = c;
-        if (c < nev) all = false;
+      const int ns = (int)lo.alpha[d].size();
+      std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
+      if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) {
+        rc = ferr(ctx, 6, "tridiagonal eigensolver failed");
+        goto out;
       }
-      if (all) {
-        k++;
-        break;
+      const double tmax = *std::max_element(w.begin(), w.end());
+      if (!(tmax > 0.0)) {
+        rc = ferr(ctx, 7, "no positive buckling load under this load case");
+        goto out;
       }
+      sig[d] = 0.95 / tmax;
     }
   }
-  if (k > m) k = m;
-  if (nsteps) *nsteps = k;
-  // Ritz pairs per design
+  // A = K + sigma G with BCs; factor, lowering the shift of any design whose
+  // shifted matrix is indefinite (shift above its first load)
+  for (int attempt = 0;; attempt++) {
+    HIPCALL(ctx, hipMemcpyAsync(dsig, sig.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+    {
+      const long nv = 36L * ctx->nnzb;
+      hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
+                         shellgpu_mat_values(K), shellgpu_mat_values(G), dsig, shellgpu_mat_values(A));
+      HIPCALL(ctx, hipGetLastError());
+    }
+    if ((rc = shellgpu_mat_apply_bcs(A, 1.0))) goto out;
+    rc = shellgpu_factor_compute(F, A);
+    if (rc == 0) break;
+    if (rc != 5 || attempt >= 8) goto out;
+    std::vector<int> info(nb);
+    HIPCALL(ctx, hipMemcpy(info.data(), F->info, sizeof(int) * nb, hipMemcpyDeviceToHost));
+    for (int d = 0; d < nb; d++)
+      if (info[d] != 0) sig[d] *= (sigma <= 0.0 ? 0.8 : 0.5);
+  }
+  if (sigma_used)
+    for (int d = 0; d < nb; d++) sigma_used[d] = sig[d];
+
+  if ((rc = lanczos(ctx, F, K, G, 0, sig, nev, m, tol, Q, KQ, hv, res, lo))) goto out;
+  if (nsteps) *nsteps = lo.steps;
   for (int d = 0; d < nb; d++) {
-    const int ns = (int)alpha[d].size();
-    std::vector<double> w, z;
-    std::vector<double> e(beta[d].begin(), beta[d].begin() + (ns - 1));
-    if (tridiag_ql(ns, alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+    const int ns = (int)lo.alpha[d].size();
+    std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
+    if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) {
+      rc = ferr(ctx, 6, "tridiagonal eigensolver failed");
+      goto out;
+    }
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
-    auto lam_of = [&](int i) { return -w[i] * sigma / (1.0 - w[i]); };
+    auto lam_of = [&](int i) { return -w[i] * sig[d] / (1.0 - w[i]); };
     std::sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(lam_of(a)) < std::fabs(lam_of(b)); });
     for (int j = 0; j < nev && j < ns; j++) {
       const int i = idx[j];
       lambda[(size_t)d * nev + j] = lam_of(i);
-      if (resid) resid[(size_t)d * nev + j] = std::fabs(beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]);
+      if (resid) resid[(size_t)d * nev + j] = std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]);
       if (phi) {
         std::vector<double> y(z.begin() + (size_t)i * ns, z.begin() + (size_t)(i + 1) * ns);
         HIPCALL(ctx, hipMemcpyAsync(hv, y.data(), sizeof(double) * ns, hipMemcpyHostToDevice, ctx->stream));
@@ -821,7 +1008,7 @@ out:
   if (G) shellgpu_mat_destroy(G);
   if (A) shellgpu_mat_destroy(A);
   if (F) shellgpu_factor_destroy(F);
-  void *bufs[] = {u, res, f0, Q, KQ, hv};
+  void *bufs[] = {u, res, f0, Q, KQ, hv, dsig};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   return rc;

@@ -81,7 +81,7 @@ def load_library(path=LIB_PATH):
         "shellgpu_factor_apply": (C.c_int, [_VP, _VP, _VP]),
         "shellgpu_gmres": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, _IP, _DP]),
         "shellgpu_linear_buckling": (C.c_int, [_VP, C.c_double, _VP, C.c_int, C.c_int, C.c_double, _DP, _VP, _VP,
-                                               _IP, _DP]),
+                                               _IP, _DP, _DP]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
         "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
         "shellgpu_timer_start": (C.c_int, [_VP]),
@@ -340,13 +340,14 @@ class Context:
         f = force if isinstance(force, DeviceArray) else self.vector(np.broadcast_to(force, (nb, n)))
         lam = np.zeros(nb * nev)
         res = np.zeros(nb * nev)
+        sig = np.zeros(nb)
         steps = C.c_int()
         phi = DeviceArray(self, nb * nev * n) if want_vectors else None
         path = DeviceArray(self, nb * n) if want_path else None
         self._check(self.lib.shellgpu_linear_buckling(self.h, float(sigma), f.ptr, nev, max_lanczos, tol, _dp(lam),
                                                       phi.ptr if phi else None, path.ptr if path else None,
-                                                      C.byref(steps), _dp(res)))
-        out = {"lambda": lam.reshape(nb, nev), "resid": res.reshape(nb, nev), "steps": steps.value}
+                                                      C.byref(steps), _dp(res), _dp(sig)))
+        out = {"lambda": lam.reshape(nb, nev), "resid": res.reshape(nb, nev), "steps": steps.value, "sigma": sig}
         if phi is not None:
             out["phi"] = phi.download().reshape(nb, nev, n)
         if path is not None:

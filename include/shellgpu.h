@@ -148,14 +148,17 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
  * 197-278) for every design: K, static path K u = f, G(u), K + sigma G,
  * shift-invert Lanczos (SEP::solve, src/bpmat/GSEP.cpp:485-599) with
  * max_lanczos steps at most, stopping when the nev eigenvalues of smallest
- * magnitude have relative Ritz residual <= tol.  Outputs: lambda[nbatch*nev]
- * (host), phi (device, nbatch*nev*6*nnodes, K-normalised; may be NULL), path
- * (device, nbatch*6*nnodes; may be NULL), Lanczos steps, relative Ritz
- * residuals resid[nbatch*nev] (host, may be NULL).  force is device,
- * nbatch*6*nnodes. */
+ * magnitude have relative Ritz residual <= tol.  sigma > 0 is the shift as
+ * in the reference (halved while K + sigma G is indefinite); sigma <= 0
+ * selects 0.95 x a first-load estimate per design.  Outputs:
+ * lambda[nbatch*nev] (host), phi (device, nbatch*nev*6*nnodes,
+ * K-normalised; may be NULL), path (device, nbatch*6*nnodes; may be NULL),
+ * Lanczos steps, relative Ritz residuals resid[nbatch*nev] (host, may be
+ * NULL), shift used per design sigma_used[nbatch] (host, may be NULL).
+ * force is device, nbatch*6*nnodes. */
 int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *force, int nev,
                              int max_lanczos, double tol, double *lambda, double *phi,
-                             double *path, int *nsteps, double *resid);
+                             double *path, int *nsteps, double *resid, double *sigma_used);
 
 /* ---- timing (no reference counterpart: measurement plumbing) ---------- */
 /* With timing enabled, HIP events on the context's stream bracket every
