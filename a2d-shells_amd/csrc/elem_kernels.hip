@@ -51,7 +51,10 @@ struct ElemSmem {
   double Jty[216], Jty1[216];  // [tying point][24]
   double sig[36];              // [qp][9]
   double dety[9];
-  double B[216], Y[216], B1[216], Y1[216];  // [strain][24] for one qp
+  // [strain][24] for one qp.  The adjoint mode, which forms no C B
+  // products, keeps in Y: path u [0,24), adjoint psi [24,48), five strain
+  // vectors [48,93) and two weighted stress vectors [93,111)
+  double B[216], Y[216], B1[216], Y1[216];
 };
 
 // rows of the 9-strain vector fed by the tying strains
@@ -209,8 +212,9 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   const long design = blockIdx.y;
   constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
   constexpr bool kState = (MODE != kModeKLin);
-  constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl);
-  constexpr bool kRes = (MODE == kModeRLin || MODE == kModeRNl);
+  constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl);
+  constexpr bool kRes =
+      (MODE == kModeRLin || MODE == kModeRNl || MODE == kModeAdjResLin || MODE == kModeAdjResNl);
   constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG);
 
   const double thick = a.thick[design * a.nelems + e];
@@ -225,6 +229,10 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     sm.s[lane] = (kState && a.state)
                      ? a.state[design * a.state_stride + 6 * cn[lane / 6] + lane % 6]
                      : 0.0;
+    if (MODE == kModeAdjDV)
+      sm.Y[lane] = a.u2 ? a.u2[design * a.u2_stride + 6 * cn[lane / 6] + lane % 6] : 0.0;
+    if (MODE == kModeAdjResLin || MODE == kModeAdjResNl)
+      sm.Y[24 + lane] = a.psi[design * a.psi_stride + 6 * cn[lane / 6] + lane % 6];
   }
   __syncthreads();
 
@@ -475,7 +483,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     sm.ety[t] = kNonlin ? lin + quad : lin;
     // derivative coefficients with a = xfac X + ufac U, nd = xfac n0 + ufac d0
     for (int pass = 0; pass < 2; pass++) {
-      if (pass == 1 && MODE != kModeG) break;
+      if (pass == 1 && MODE != kModeG && MODE != kModeAdjDV) break;
       const double xf = (pass == 0) ? 1.0 : 0.0;
       const double uf = (pass == 0) ? (kNonlin ? 1.0 : 0.0) : 1.0;
       double av[6], nd[3], dU[6] = {0, 0, 0, 0, 0, 0}, dd[3] = {0, 0, 0};
@@ -526,7 +534,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // ---- P3: tying-strain Jacobian rows; strains and stresses at the
   // quadrature points ----
   {
-    const int pass = (lane < 24) ? 0 : ((MODE == kModeG && lane >= 32 && lane < 56) ? 1 : -1);
+    const int pass =
+        (lane < 24) ? 0 : (((MODE == kModeG || MODE == kModeAdjDV) && lane >= 32 && lane < 56) ? 1 : -1);
     if (pass >= 0) {
       const int k = (pass == 0) ? lane : lane - 32;
       const int n = k / 6, r = k % 6;
@@ -578,7 +587,12 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       e[0] -= a.mat.cte * temp;
       e[1] -= a.mat.cte * temp;
       double s9[9];
-      apply_c(sec, sm.wdet[iq], e, s9);
+      if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) s9[i] = e[i];  // mechanical strain
+      } else {
+        apply_c(sec, sm.wdet[iq], e, s9);
+      }
 #pragma unroll
       for (int i = 0; i < 9; i++) sm.sig[9 * iq + i] = s9[i];
     }
@@ -594,6 +608,118 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 #pragma unroll
       for (int r = 0; r < 5; r++) acc += sm.sig[9 * iq + tying_row(r)] * sm.W[45 * iq + 9 * r + t];
     sm.dety[t] = acc;
+  }
+
+  // ---- thickness derivative of phi^T K phi and phi^T G(u) phi, and the
+  // path-adjoint right-hand side d(phi^T G(u) phi)/du ----
+  // With e(q) = B0 q + Q(q, q)/2 (Q symmetric bilinear, Bq(a) b = Q(a, b)):
+  //   phi^T K phi    = sum w e1(phi)^T C e1(phi),            e1 = B0 .
+  //   phi^T G(u) phi = sum w [2 Q(u, phi)^T C e1(phi) + (e1(u) - e_th)^T C Q(phi, phi)]
+  if (MODE == kModeAdjDV) {
+    const Section dsec = make_section_dt(a.mat, thick);
+    const double *su = &sm.Y[0];
+    double *E = &sm.Y[48], *cv = &sm.Y[93];
+    const double ca = a.coef[2 * design], cb = a.coef[2 * design + 1];
+    double gdv = 0.0, gu = 0.0;
+    for (int iq = 0; iq < 4; iq++) {
+      if (lane < 24) {
+        double b[9];
+        build_bcol<kBLin>(sm, iq, lane, sm.Jty, b);
+#pragma unroll
+        for (int i = 0; i < 9; i++) sm.B[24 * i + lane] = b[i];
+      } else if (lane >= 32 && lane < 56) {
+        double b[9];
+        build_bcol<kBQuad>(sm, iq, lane - 32, sm.Jty1, b);
+#pragma unroll
+        for (int i = 0; i < 9; i++) sm.B1[24 * i + lane - 32] = b[i];
+      }
+      __syncthreads();
+      // E = [e1(phi), e1(u), Q(phi, u), Q(phi, phi)]
+      if (lane < 36) {
+        const int v = lane / 9, i = lane % 9;
+        const double *M = (v < 2) ? &sm.B[24 * i] : &sm.B1[24 * i];
+        const double *x = (v == 0 || v == 3) ? sm.s : su;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 24; k++) acc += M[k] * x[k];
+        E[lane] = acc;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const double w = sm.wdet[iq];
+        const double *e1p = &E[0], *e1u = &E[9], *e2up = &E[18], *e2pp = &E[27];
+        double eu[9], t1[9], t2[9], c1[9], c2[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) eu[i] = e1u[i];
+        eu[0] -= a.mat.cte * temp;
+        eu[1] -= a.mat.cte * temp;
+        apply_c(dsec, w, e1p, t1);  // w C' e1(phi)
+        apply_c(dsec, w, eu, t2);   // w C' (e1(u) - e_th)
+        double da = 0.0, db = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+          da += e1p[i] * t1[i];
+          db += 2.0 * e2up[i] * t1[i] + e2pp[i] * t2[i];
+        }
+        gdv += ca * da + cb * db;
+        apply_c(sec, w, e1p, c1);
+        apply_c(sec, w, e2pp, c2);
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+          cv[i] = 2.0 * cb * c1[i];
+          cv[9 + i] = cb * c2[i];
+        }
+      }
+      __syncthreads();
+      // d/du: Bq(phi)^T (2 w C e1(phi)) + B0^T (w C Q(phi, phi))
+      if (lane < 24) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) gu += sm.B1[24 * i + lane] * cv[i] + sm.B[24 * i + lane] * cv[9 + i];
+      }
+      __syncthreads();
+    }
+    if (lane == 0 && a.dv) a.dv[design * a.nelems + e] += gdv;
+    if (lane < 24 && a.res) a.res[design * a.res_stride + 6 * cn[lane / 6] + lane % 6] += gu;
+    return;
+  }
+
+  // ---- psi^T dR_e/dt (TACSShellElement::addAdjResProduct, SE.h:774-868,
+  // with TACSIsoShellConstitutive::addStressDVSens) ----
+  if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
+    const Section dsec = make_section_dt(a.mat, thick);
+    const double *sp = &sm.Y[24];
+    double *E = &sm.Y[48];
+    double g = 0.0;
+    for (int iq = 0; iq < 4; iq++) {
+      if (lane < 24) {
+        double b[9];
+        if (kNonlin)
+          build_bcol<kBFull>(sm, iq, lane, sm.Jty, b);
+        else
+          build_bcol<kBLin>(sm, iq, lane, sm.Jty, b);
+#pragma unroll
+        for (int i = 0; i < 9; i++) sm.B[24 * i + lane] = b[i];
+      }
+      __syncthreads();
+      if (lane < 9) {  // dR/dq^T psi strain: B(u) psi
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 24; k++) acc += sm.B[24 * lane + k] * sp[k];
+        E[lane] = acc;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        double t2[9];
+        apply_c(dsec, sm.wdet[iq], &sm.sig[9 * iq], t2);  // sig holds e(u) - e_th
+        double d = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) d += E[i] * t2[i];
+        g += d;
+      }
+      __syncthreads();
+    }
+    if (lane == 0) a.dv[design * a.nelems + e] += a.coef[design] * g;
+    return;
   }
 
   // ---- P5: quadrature loop: B columns, residual, B^T C B tile ----
@@ -743,6 +869,12 @@ hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int n
       return launch_mode<kModeRLin>(args, nblocks, nbatch, stream);
     case kModeRNl:
       return launch_mode<kModeRNl>(args, nblocks, nbatch, stream);
+    case kModeAdjDV:
+      return launch_mode<kModeAdjDV>(args, nblocks, nbatch, stream);
+    case kModeAdjResLin:
+      return launch_mode<kModeAdjResLin>(args, nblocks, nbatch, stream);
+    case kModeAdjResNl:
+      return launch_mode<kModeAdjResNl>(args, nblocks, nbatch, stream);
     default:
       return hipErrorInvalidValue;
   }

@@ -146,23 +146,6 @@ int check_batch(shellgpu_ctx *ctx, const shellgpu_mat *A) {
   return 0;
 }
 
-ElemArgs base_args(shellgpu_ctx *ctx, const double *u) {
-  ElemArgs a{};
-  a.conn = ctx->d_conn;
-  a.blk = ctx->d_blk;
-  a.Xpts = ctx->d_X;
-  a.state = u;
-  a.state_stride = 6L * ctx->nnodes;
-  a.thick = ctx->d_thick;
-  a.temp = ctx->d_temp;
-  a.nelems = ctx->nelems;
-  a.mat = ctx->mat;
-  a.alpha = 1.0;
-  a.vals_stride = 36L * ctx->nnzb;
-  a.res_stride = 6L * ctx->nnodes;
-  return a;
-}
-
 int record_begin(shellgpu_ctx *ctx, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev) {
   if (!ctx->timing) return 0;
   hipEvent_t a, b;
@@ -187,6 +170,27 @@ void drop_events(std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev) {
   ev.clear();
 }
 
+}  // namespace
+
+namespace shellgpu {
+
+ElemArgs base_args(shellgpu_ctx *ctx, const double *u) {
+  ElemArgs a{};
+  a.conn = ctx->d_conn;
+  a.blk = ctx->d_blk;
+  a.Xpts = ctx->d_X;
+  a.state = u;
+  a.state_stride = 6L * ctx->nnodes;
+  a.thick = ctx->d_thick;
+  a.temp = ctx->d_temp;
+  a.nelems = ctx->nelems;
+  a.mat = ctx->mat;
+  a.alpha = 1.0;
+  a.vals_stride = 36L * ctx->nnzb;
+  a.res_stride = 6L * ctx->nnodes;
+  return a;
+}
+
 int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
   int rc = 0;
   for (int c = 0; c < ctx->ncolors; c++) {
@@ -199,7 +203,7 @@ int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
   return 0;
 }
 
-}  // namespace
+}  // namespace shellgpu
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -228,7 +232,7 @@ void shellgpu_destroy(shellgpu_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   void *bufs[] = {ctx->d_conn, ctx->d_rowp,     ctx->d_cols,    ctx->d_blk,
                   ctx->d_color_elems, ctx->d_X, ctx->d_bc_nodes, ctx->d_bc_mask,
-                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp};
+                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp, ctx->d_coef};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   drop_events(ctx->elem_ev);
@@ -377,6 +381,9 @@ int shellgpu_set_designs(shellgpu_ctx *ctx, int nbatch, const double *thick, con
     (void)hipFree(ctx->d_temp);
     ctx->d_temp = nullptr;
   }
+  if (ctx->d_coef) (void)hipFree(ctx->d_coef);
+  ctx->d_coef = nullptr;
+  HIPCALL(ctx, hipMalloc((void **)&ctx->d_coef, sizeof(double) * 2 * nbatch));
   return 0;
 }
 
@@ -540,6 +547,64 @@ int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u, do
   a.vals = A->vals;
   if ((rc = run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeKNl : kModeKLin, a))) return rc;
   return shellgpu_mat_apply_bcs(A, 1.0);
+}
+
+// per-design coefficients (c0, c1) into ctx->d_coef
+static int set_coef(shellgpu_ctx *ctx, double c0, double c1) {
+  std::vector<double> h(2 * ctx->nbatch);
+  for (int d = 0; d < ctx->nbatch; d++) {
+    h[2 * d] = c0;
+    h[2 * d + 1] = c1;
+  }
+  HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  // h is released on return: the copy must have consumed it
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int shellgpu_add_adj_res_product(shellgpu_ctx *ctx, int model, double scale, const double *psi, const double *u,
+                                 double *dfdt) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!psi || !dfdt) return set_err(ctx, 1, "psi and dfdt are required");
+  std::vector<double> h(ctx->nbatch, scale);
+  HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  ElemArgs a = base_args(ctx, u);
+  a.psi = psi;
+  a.psi_stride = 6L * ctx->nnodes;
+  a.coef = ctx->d_coef;
+  a.dv = dfdt;
+  return run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeAdjResNl : kModeAdjResLin, a);
+}
+
+int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale, const double *phi,
+                                           const double *u, double *dfdt) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!phi || !dfdt) return set_err(ctx, 1, "phi and dfdt are required");
+  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC)
+    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS or SHELLGPU_GEOMETRIC");
+  const bool geo = mattype == SHELLGPU_GEOMETRIC;
+  if ((rc = set_coef(ctx, geo ? 0.0 : scale, geo ? scale : 0.0))) return rc;
+  ElemArgs a = base_args(ctx, phi);
+  a.u2 = u;
+  a.u2_stride = 6L * ctx->nnodes;
+  a.coef = ctx->d_coef;
+  a.dv = dfdt;
+  return run_colored(ctx, kModeAdjDV, a);
+}
+
+int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *phi, double *dfdu) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!phi || !dfdu) return set_err(ctx, 1, "phi and dfdu are required");
+  HIPCALL(ctx, hipMemsetAsync(dfdu, 0, sizeof(double) * 6L * ctx->nnodes * ctx->nbatch, ctx->stream));
+  if ((rc = set_coef(ctx, 0.0, 1.0))) return rc;
+  ElemArgs a = base_args(ctx, phi);
+  a.coef = ctx->d_coef;
+  a.res = dfdu;
+  return run_colored(ctx, kModeAdjDV, a);
 }
 
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x) {

@@ -27,6 +27,7 @@ struct shellgpu_ctx {
   double *d_bc_vals = nullptr;
 
   double *d_thick = nullptr, *d_temp = nullptr;
+  double *d_coef = nullptr;  // 2 * nbatch per-design kernel coefficients
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
@@ -43,3 +44,11 @@ struct shellgpu_mat {
   double *vals = nullptr;
 };
 
+namespace shellgpu {
+// element-kernel arguments for the context's mesh, material and designs
+// with state u (NULL = zero)
+ElemArgs base_args(shellgpu_ctx *ctx, const double *u);
+// launch one element-kernel mode colour by colour (no two elements of a
+// launch share a node, so scatters need no atomics)
+int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a);
+}  // namespace shellgpu

@@ -39,6 +39,9 @@ enum ElemMode {
   kModeG = 2,     // geometric stiffness, exact directional derivative
   kModeRLin = 3,  // linear-model residual
   kModeRNl = 4,   // nonlinear-model residual
+  kModeAdjDV = 5,     // thickness terms of a buckling eigenpair (state = phi)
+  kModeAdjResLin = 6, // psi^T dR/dt, linear model (state = u)
+  kModeAdjResNl = 7,  // psi^T dR/dt, nonlinear model (state = u)
 };
 
 struct ElemArgs {
@@ -59,7 +62,34 @@ struct ElemArgs {
   long vals_stride;
   double *res;
   long res_stride;
+  // thickness-derivative modes (' = d/dt_e, all outputs accumulate):
+  //  kModeAdjDV (state = phi, u2 = path u):
+  //    dv[e] += coef[2d] phi^T K' phi + coef[2d+1] phi^T G'(u) phi
+  //    res   += coef[2d+1] d(phi^T G(u) phi)/du
+  //  kModeAdjRes* (state = u): dv[e] += coef[d] psi^T dR_e/dt
+  const double *u2;
+  const double *psi;
+  long u2_stride, psi_stride;
+  const double *coef;
+  double *dv;
 };
+
+// d/dt of the section stiffness (A' = Q, D' = t^2/4 Q, As' = 5/6 G,
+// drill' = kdrill 5/6 G), the derivative of make_section
+__host__ __device__ inline Section make_section_dt(const Material &m, double t) {
+  Section s;
+  const double G = 0.5 * m.E / (1.0 + m.nu);
+  const double Dm = m.E / (1.0 - m.nu * m.nu);
+  const double Q[6] = {Dm, m.nu * Dm, 0.0, Dm, 0.0, G};
+  for (int i = 0; i < 6; i++) {
+    s.A[i] = Q[i];
+    s.D[i] = 0.25 * t * t * Q[i];
+  }
+  s.As[0] = s.As[2] = (5.0 / 6.0) * G;
+  s.As[1] = 0.0;
+  s.drill = 0.5 * m.kdrill * (s.As[0] + s.As[2]);
+  return s;
+}
 
 hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int nbatch,
                               hipStream_t stream);

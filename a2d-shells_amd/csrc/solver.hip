@@ -885,131 +885,345 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
 
 }  // namespace
 
-extern "C" int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *force, int nev,
-                                        int max_lanczos, double tol, double *lambda, double *phi, double *path,
-                                        int *nsteps, double *resid, double *sigma_used) {
-  if (!ctx || nev <= 0 || max_lanczos < nev) return ferr(ctx, 1, "invalid buckling arguments");
-  const int nb = ctx->nbatch, n = 6 * ctx->nnodes, m = max_lanczos;
-  int rc = 0;
+struct shellgpu_buckling {
+  shellgpu_ctx *ctx = nullptr;
+  int model = 0, nev = 0, m = 0;
+  double sigma = 0.0, tol = 0.0;
   shellgpu_mat *K = nullptr, *G = nullptr, *A = nullptr;
   shellgpu_factor *F = nullptr;
+  // device: path u, work vectors, Lanczos basis Q and K Q, coefficients
   double *u = nullptr, *res = nullptr, *f0 = nullptr, *Q = nullptr, *KQ = nullptr, *hv = nullptr, *dsig = nullptr;
-  VecOps vo;
-  std::vector<double> sig(nb, sigma);
-  LanczosOut lo;
-  rocblas_handle h;
+  double *ph = nullptr, *gv = nullptr, *psi = nullptr, *dcoef = nullptr;
+  std::vector<double> sig;
+  bool solved = false;
+  int steps = 0;
+  // per design: eigenvalues, Ritz residuals and Ritz coefficient vectors
+  // (nev x ns, ns = Lanczos basis size of that design)
+  std::vector<std::vector<double>> lam, err, ritz;
+  std::vector<int> ns;
+};
+
+namespace {
+
+int buckling_free(shellgpu_buckling *B) {
+  if (B->K) shellgpu_mat_destroy(B->K);
+  if (B->G) shellgpu_mat_destroy(B->G);
+  if (B->A) shellgpu_mat_destroy(B->A);
+  if (B->F) shellgpu_factor_destroy(B->F);
+  double *bufs[] = {B->u, B->res, B->f0, B->Q, B->KQ, B->hv, B->dsig, B->ph, B->gv, B->psi, B->dcoef};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  return 0;
+}
+
+// phi_n of every design into out (device, nbatch x n)
+int buckling_vector(shellgpu_buckling *B, int idx, double *out) {
+  shellgpu_ctx *ctx = B->ctx;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
+  rocblas_handle h = (rocblas_handle)ctx->blas;
   const double one = 1.0, zero = 0.0;
-
-  if ((rc = vo.init(ctx, n, nb))) return rc;
-  if ((rc = ensure_blas(ctx))) return rc;
-  h = (rocblas_handle)ctx->blas;
-  if ((rc = shellgpu_mat_create(ctx, nb, &K))) goto out;
-  if ((rc = shellgpu_mat_create(ctx, nb, &G))) goto out;
-  if ((rc = shellgpu_mat_create(ctx, nb, &A))) goto out;
-  if ((rc = shellgpu_factor_create(ctx, &F))) goto out;
-  HIPCALL(ctx, hipMalloc((void **)&u, sizeof(double) * (size_t)n * nb));
-  HIPCALL(ctx, hipMalloc((void **)&res, sizeof(double) * (size_t)n * nb));
-  HIPCALL(ctx, hipMalloc((void **)&f0, sizeof(double) * (size_t)n * nb));
-  HIPCALL(ctx, hipMalloc((void **)&Q, sizeof(double) * (size_t)n * nb * (m + 1)));
-  HIPCALL(ctx, hipMalloc((void **)&KQ, sizeof(double) * (size_t)n * nb * (m + 1)));
-  HIPCALL(ctx, hipMalloc((void **)&hv, sizeof(double) * (size_t)nb * (m + 1)));
-  HIPCALL(ctx, hipMalloc((void **)&dsig, sizeof(double) * nb));
-
-  // TACSLinearBuckling::solve with u0 = NULL (src/TACSBuckling.cpp:236-271)
-  if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, K))) goto out;
-  if ((rc = shellgpu_factor_compute(F, K))) goto out;
-  if ((rc = shellgpu_assemble_res(ctx, SHELLGPU_LINEAR, nullptr, res))) goto out;
-  if (force) {
-    HIPCALL(ctx, hipMemcpyAsync(f0, force, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((rc = shellgpu_vec_zero_bcs(ctx, f0))) goto out;
-    std::vector<double> m1(nb, -1.0);
-    if ((rc = vo.axpy(m1, f0, res))) goto out;
+  for (int d = 0; d < nb; d++) {
+    const int ns = B->ns[d];
+    HIPCALL(ctx, hipMemcpyAsync(B->hv, B->ritz[d].data() + (size_t)idx * ns, sizeof(double) * ns,
+                                hipMemcpyHostToDevice, ctx->stream));
+    BLASCALL(ctx, rocblas_dgemv(h, rocblas_operation_none, n, ns, &one, B->Q + (size_t)d * n, nb * n, B->hv, 1, &zero,
+                                out + (size_t)d * n, 1));
+    // hv is reused by the next design
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shellgpu_buckling_create(shellgpu_ctx *ctx, int model, double sigma, int nev, int max_lanczos, double tol,
+                             shellgpu_buckling **out) {
+  if (!ctx || !out) return 1;
+  *out = nullptr;
+  if (nev <= 0 || max_lanczos < nev || !(tol > 0.0)) return ferr(ctx, 1, "invalid buckling arguments");
+  if (!ctx->d_thick) return ferr(ctx, 3, "designs not set (shellgpu_set_designs)");
+  shellgpu_buckling *B = new shellgpu_buckling();
+  B->ctx = ctx;
+  B->model = model == SHELLGPU_NONLINEAR ? SHELLGPU_NONLINEAR : SHELLGPU_LINEAR;
+  B->sigma = sigma;
+  B->nev = nev;
+  B->m = max_lanczos;
+  B->tol = tol;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes, m = max_lanczos;
+  int rc = 0;
+  if ((rc = ensure_blas(ctx))) goto fail;
+  if ((rc = shellgpu_mat_create(ctx, nb, &B->K))) goto fail;
+  if ((rc = shellgpu_mat_create(ctx, nb, &B->G))) goto fail;
+  if ((rc = shellgpu_mat_create(ctx, nb, &B->A))) goto fail;
+  if ((rc = shellgpu_factor_create(ctx, &B->F))) goto fail;
   {
+    const size_t v = sizeof(double) * (size_t)n * nb;
+    double **vecs[] = {&B->u, &B->res, &B->f0, &B->ph, &B->gv, &B->psi};
+    for (double **p : vecs) {
+      if (hipMalloc((void **)p, v) != hipSuccess) {
+        rc = ferr(ctx, 2, "out of device memory (buckling vectors)");
+        goto fail;
+      }
+    }
+    if (hipMalloc((void **)&B->Q, v * (m + 1)) != hipSuccess ||
+        hipMalloc((void **)&B->KQ, v * (m + 1)) != hipSuccess ||
+        hipMalloc((void **)&B->hv, sizeof(double) * (size_t)nb * (m + 1)) != hipSuccess ||
+        hipMalloc((void **)&B->dsig, sizeof(double) * nb) != hipSuccess ||
+        hipMalloc((void **)&B->dcoef, sizeof(double) * 2 * nb) != hipSuccess) {
+      rc = ferr(ctx, 2, "out of device memory (Lanczos basis)");
+      goto fail;
+    }
+  }
+  *out = B;
+  return 0;
+fail:
+  buckling_free(B);
+  delete B;
+  return rc;
+}
+
+void shellgpu_buckling_destroy(shellgpu_buckling *B) {
+  if (!B) return;
+  buckling_free(B);
+  delete B;
+}
+
+int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const double *u0) {
+  if (!B) return 1;
+  shellgpu_ctx *ctx = B->ctx;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes, m = B->m, nev = B->nev;
+  int rc = 0;
+  VecOps vo;
+  LanczosOut lo;
+  B->solved = false;
+  B->sig.assign(nb, B->sigma);
+  if ((rc = vo.init(ctx, n, nb))) return rc;
+
+  // TACSLinearBuckling::solve (src/TACSBuckling.cpp:197-278): K is
+  // assembled at zero state, the path is the static solution K u = f (with
+  // the BC-driven residual) unless u0 is given
+  if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, B->K))) return rc;
+  if (u0) {
+    HIPCALL(ctx, hipMemcpyAsync(B->u, u0, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    if ((rc = shellgpu_factor_compute(B->F, B->K))) return rc;
+    if ((rc = shellgpu_assemble_res(ctx, SHELLGPU_LINEAR, nullptr, B->res))) return rc;
+    if (force) {
+      HIPCALL(ctx,
+              hipMemcpyAsync(B->f0, force, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream));
+      if ((rc = shellgpu_vec_zero_bcs(ctx, B->f0))) return rc;
+      std::vector<double> m1(nb, -1.0);
+      if ((rc = vo.axpy(m1, B->f0, B->res))) return rc;
+    }
     int its = 0;
     double rn = 0.0;
-    if ((rc = shellgpu_gmres(ctx, K, F, res, u, 10, 15, 1e-12, 1e-30, &its, &rn))) goto out;
+    if ((rc = shellgpu_gmres(ctx, B->K, B->F, B->res, B->u, 10, 15, 1e-12, 1e-30, &its, &rn))) return rc;
     std::vector<double> m1(nb, -1.0);
-    if ((rc = vo.scale(u, m1))) goto out;
+    if ((rc = vo.scale(B->u, m1))) return rc;
   }
-  if ((rc = shellgpu_vec_set_bcs(ctx, u))) goto out;
-  if (path) HIPCALL(ctx, hipMemcpyAsync(path, u, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream));
-  if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_GEOMETRIC, u, G))) goto out;
+  if ((rc = shellgpu_vec_set_bcs(ctx, B->u))) return rc;
+  if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_GEOMETRIC, B->u, B->G))) return rc;
 
   // shift: sigma > 0 as given; sigma <= 0 places it at 0.95 of a first-load
   // estimate from a short buckling-mode Lanczos on K^{-1}(-G)
-  if (sigma <= 0.0) {
-    if ((rc = lanczos(ctx, F, K, G, 1, sig, 1, std::min(m, 40), 1e-4, Q, KQ, hv, res, lo))) goto out;
+  if (B->sigma <= 0.0) {
+    if (u0 && (rc = shellgpu_factor_compute(B->F, B->K))) return rc;
+    if ((rc = lanczos(ctx, B->F, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-4, B->Q, B->KQ, B->hv, B->res, lo)))
+      return rc;
     for (int d = 0; d < nb; d++) {
       const int ns = (int)lo.alpha[d].size();
       std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
-      if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) {
-        rc = ferr(ctx, 6, "tridiagonal eigensolver failed");
-        goto out;
-      }
+      if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
       const double tmax = *std::max_element(w.begin(), w.end());
-      if (!(tmax > 0.0)) {
-        rc = ferr(ctx, 7, "no positive buckling load under this load case");
-        goto out;
-      }
-      sig[d] = 0.95 / tmax;
+      if (!(tmax > 0.0)) return ferr(ctx, 7, "no positive buckling load under this load case");
+      B->sig[d] = 0.95 / tmax;
     }
   }
   // A = K + sigma G with BCs; factor, lowering the shift of any design whose
   // shifted matrix is indefinite (shift above its first load)
   for (int attempt = 0;; attempt++) {
-    HIPCALL(ctx, hipMemcpyAsync(dsig, sig.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+    HIPCALL(ctx, hipMemcpyAsync(B->dsig, B->sig.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
     {
       const long nv = 36L * ctx->nnzb;
       hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
-                         shellgpu_mat_values(K), shellgpu_mat_values(G), dsig, shellgpu_mat_values(A));
+                         shellgpu_mat_values(B->K), shellgpu_mat_values(B->G), B->dsig, shellgpu_mat_values(B->A));
       HIPCALL(ctx, hipGetLastError());
     }
-    if ((rc = shellgpu_mat_apply_bcs(A, 1.0))) goto out;
-    rc = shellgpu_factor_compute(F, A);
+    if ((rc = shellgpu_mat_apply_bcs(B->A, 1.0))) return rc;
+    rc = shellgpu_factor_compute(B->F, B->A);
     if (rc == 0) break;
-    if (rc != 5 || attempt >= 8) goto out;
+    if (rc != 5 || attempt >= 8) return rc;
     std::vector<int> info(nb);
-    HIPCALL(ctx, hipMemcpy(info.data(), F->info, sizeof(int) * nb, hipMemcpyDeviceToHost));
+    HIPCALL(ctx, hipMemcpy(info.data(), B->F->info, sizeof(int) * nb, hipMemcpyDeviceToHost));
     for (int d = 0; d < nb; d++)
-      if (info[d] != 0) sig[d] *= (sigma <= 0.0 ? 0.8 : 0.5);
+      if (info[d] != 0) B->sig[d] *= (B->sigma <= 0.0 ? 0.8 : 0.5);
   }
-  if (sigma_used)
-    for (int d = 0; d < nb; d++) sigma_used[d] = sig[d];
 
-  if ((rc = lanczos(ctx, F, K, G, 0, sig, nev, m, tol, Q, KQ, hv, res, lo))) goto out;
-  if (nsteps) *nsteps = lo.steps;
+  if ((rc = lanczos(ctx, B->F, B->K, B->G, 0, B->sig, nev, m, B->tol, B->Q, B->KQ, B->hv, B->res, lo))) return rc;
+  B->steps = lo.steps;
+  B->lam.assign(nb, {});
+  B->err.assign(nb, {});
+  B->ritz.assign(nb, {});
+  B->ns.assign(nb, 0);
   for (int d = 0; d < nb; d++) {
     const int ns = (int)lo.alpha[d].size();
+    if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
     std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
-    if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) {
-      rc = ferr(ctx, 6, "tridiagonal eigensolver failed");
-      goto out;
-    }
+    if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
-    auto lam_of = [&](int i) { return -w[i] * sig[d] / (1.0 - w[i]); };
+    const double sd = B->sig[d];
+    auto lam_of = [&](int i) { return -w[i] * sd / (1.0 - w[i]); };
+    // sortEigenvalues, SMALLEST_MAGNITUDE (src/bpmat/GSEP.cpp:693-748)
     std::sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(lam_of(a)) < std::fabs(lam_of(b)); });
-    for (int j = 0; j < nev && j < ns; j++) {
+    B->ns[d] = ns;
+    for (int j = 0; j < nev; j++) {
       const int i = idx[j];
-      lambda[(size_t)d * nev + j] = lam_of(i);
-      if (resid) resid[(size_t)d * nev + j] = std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]);
-      if (phi) {
-        std::vector<double> y(z.begin() + (size_t)i * ns, z.begin() + (size_t)(i + 1) * ns);
-        HIPCALL(ctx, hipMemcpyAsync(hv, y.data(), sizeof(double) * ns, hipMemcpyHostToDevice, ctx->stream));
-        BLASCALL(ctx, rocblas_dgemv(h, rocblas_operation_none, n, ns, &one, Q + (size_t)d * n, nb * n, hv, 1, &zero,
-                                    phi + ((size_t)d * nev + j) * n, 1));
-        HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+      B->lam[d].push_back(lam_of(i));
+      B->err[d].push_back(std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]));
+      B->ritz[d].insert(B->ritz[d].end(), z.begin() + (size_t)i * ns, z.begin() + (size_t)(i + 1) * ns);
+    }
+  }
+  B->solved = true;
+  return 0;
+}
+
+int shellgpu_buckling_extract(shellgpu_buckling *B, int idx, double *lambda, double *phi, double *error) {
+  if (!B) return 1;
+  shellgpu_ctx *ctx = B->ctx;
+  if (!B->solved) return ferr(ctx, 9, "buckling problem not solved");
+  if (idx < 0 || idx >= B->nev) return ferr(ctx, 1, "eigenvalue index out of range");
+  const int nb = ctx->nbatch;
+  for (int d = 0; d < nb; d++) {
+    if (lambda) lambda[d] = B->lam[d][idx];
+    if (error) error[d] = B->err[d][idx];
+  }
+  if (phi) return buckling_vector(B, idx, phi);
+  return 0;
+}
+
+int shellgpu_buckling_info(shellgpu_buckling *B, int *nsteps, double *sigma_used, double *path) {
+  if (!B) return 1;
+  shellgpu_ctx *ctx = B->ctx;
+  if (!B->solved) return ferr(ctx, 9, "buckling problem not solved");
+  if (nsteps) *nsteps = B->steps;
+  if (sigma_used)
+    for (int d = 0; d < ctx->nbatch; d++) sigma_used[d] = B->sig[d];
+  if (path)
+    HIPCALL(ctx, hipMemcpyAsync(path, B->u, sizeof(double) * 6L * ctx->nnodes * ctx->nbatch, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  return 0;
+}
+
+int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, double *dfdt, double *dfdX) {
+  if (!B || !coef) return 1;
+  shellgpu_ctx *ctx = B->ctx;
+  if (!B->solved) return ferr(ctx, 9, "buckling problem not solved");
+  if (dfdX) return ferr(ctx, 10, "coordinate sensitivities are not available");
+  if (!dfdt) return 0;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes, nev = B->nev;
+  int rc = 0;
+  VecOps vo;
+  if ((rc = vo.init(ctx, n, nb))) return rc;
+  HIPCALL(ctx, hipMemsetAsync(B->gv, 0, sizeof(double) * (size_t)n * nb, ctx->stream));
+  std::vector<double> hc(2 * nb);
+  for (int j = 0; j < nev; j++) {
+    bool any = false;
+    for (int d = 0; d < nb; d++) any = any || coef[(size_t)d * nev + j] != 0.0;
+    if (!any) continue;
+    // phi_j, and the reference's normalisation phi^T G phi
+    if ((rc = buckling_vector(B, j, B->ph))) return rc;
+    if ((rc = shellgpu_mat_mult(B->G, B->ph, B->res))) return rc;
+    std::vector<double> pgp;
+    if ((rc = vo.dot(B->ph, B->res, pgp))) return rc;
+    for (int d = 0; d < nb; d++) {
+      // d lambda = -(phi^T K' phi + lambda phi^T G' phi - lambda psi^T R') / (phi^T G phi)
+      const double f = -coef[(size_t)d * nev + j] / pgp[d];
+      hc[2 * d] = f;
+      hc[2 * d + 1] = f * B->lam[d][j];
+    }
+    HIPCALL(ctx, hipMemcpyAsync(B->dcoef, hc.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, ctx->stream));
+    ElemArgs a = base_args(ctx, B->ph);
+    a.u2 = B->u;
+    a.u2_stride = n;
+    a.coef = B->dcoef;
+    a.dv = dfdt;
+    a.res = B->gv;
+    if ((rc = run_colored(ctx, kModeAdjDV, a))) return rc;
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  // path adjoint: dR/du psi = sum_j f_j lambda_j d(phi_j^T G phi_j)/du, with
+  // the tangent of the residual that defines the path (K for the linear
+  // model; the nonlinear tangent at u otherwise)
+  if ((rc = shellgpu_vec_zero_bcs(ctx, B->gv))) return rc;
+  if (B->model == SHELLGPU_NONLINEAR) {
+    if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, B->u, nullptr, B->A))) return rc;
+  } else {
+    if ((rc = shellgpu_mat_copy(B->A, B->K))) return rc;
+  }
+  if ((rc = shellgpu_factor_compute(B->F, B->A))) return rc;
+  {
+    int its = 0;
+    double rn = 0.0;
+    if ((rc = shellgpu_gmres(ctx, B->A, B->F, B->gv, B->psi, 10, 3, 1e-12, 1e-300, &its, &rn))) return rc;
+  }
+  // the shifted operator is no longer factored: a later solve refactors
+  std::vector<double> mone(nb, -1.0);
+  HIPCALL(ctx, hipMemcpyAsync(B->dcoef, mone.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+  ElemArgs a = base_args(ctx, B->u);
+  a.psi = B->psi;
+  a.psi_stride = n;
+  a.coef = B->dcoef;
+  a.dv = dfdt;
+  if ((rc = run_colored(ctx, B->model == SHELLGPU_NONLINEAR ? kModeAdjResNl : kModeAdjResLin, a))) return rc;
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *force, int nev, int max_lanczos,
+                             double tol, double *lambda, double *phi, double *path, int *nsteps, double *resid,
+                             double *sigma_used) {
+  shellgpu_buckling *B = nullptr;
+  int rc = shellgpu_buckling_create(ctx, SHELLGPU_LINEAR, sigma, nev, max_lanczos, tol, &B);
+  if (rc) return rc;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
+  std::vector<double> lam(nb), err(nb);
+  if ((rc = shellgpu_buckling_solve(B, force, nullptr))) goto out;
+  if ((rc = shellgpu_buckling_info(B, nsteps, sigma_used, path))) goto out;
+  for (int j = 0; j < nev; j++) {
+    if ((rc = shellgpu_buckling_extract(B, j, lam.data(), phi ? B->ph : nullptr, err.data()))) goto out;
+    for (int d = 0; d < nb; d++) {
+      lambda[(size_t)d * nev + j] = lam[d];
+      if (resid) resid[(size_t)d * nev + j] = err[d];
+      if (phi && hipMemcpyAsync(phi + ((size_t)d * nev + j) * n, B->ph + (size_t)d * n, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+        rc = ferr(ctx, 2, "eigenvector copy failed");
+        goto out;
       }
     }
   }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ferr(ctx, 2, "stream synchronisation failed");
 out:
-  if (K) shellgpu_mat_destroy(K);
-  if (G) shellgpu_mat_destroy(G);
-  if (A) shellgpu_mat_destroy(A);
-  if (F) shellgpu_factor_destroy(F);
-  void *bufs[] = {u, res, f0, Q, KQ, hv, dsig};
-  for (void *b : bufs)
-    if (b) (void)hipFree(b);
+  shellgpu_buckling_destroy(B);
   return rc;
 }
+
+int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double *dks) {
+  if (n <= 0 || !f || !ks || !(rho > 0.0)) return 1;
+  double fmax = f[0];
+  for (int i = 1; i < n; i++) fmax = std::max(fmax, f[i]);
+  double sum = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double v = std::exp(rho * (f[i] - fmax));
+    sum += v;
+    if (dks) dks[i] = v;
+  }
+  if (dks)
+    for (int i = 0; i < n; i++) dks[i] /= sum;
+  *ks = fmax + std::log(sum) / rho;
+  return 0;
+}
+
+}  // extern "C"

@@ -71,6 +71,9 @@ def load_library(path=LIB_PATH):
         "shellgpu_assemble_mattype": (C.c_int, [_VP, C.c_int, _VP, _VP]),
         "shellgpu_assemble_res": (C.c_int, [_VP, C.c_int, _VP, _VP]),
         "shellgpu_assemble_jacobian": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+        "shellgpu_add_adj_res_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
+        "shellgpu_add_mat_dv_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
+        "shellgpu_eval_mat_sv_sens_inner_product": (C.c_int, [_VP, _VP, _VP]),
         "shellgpu_vec_zero_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_vec_set_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_set_ordering_seed": (C.c_int, [_VP, C.c_int, _IP]),
@@ -82,6 +85,14 @@ def load_library(path=LIB_PATH):
         "shellgpu_gmres": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, _IP, _DP]),
         "shellgpu_linear_buckling": (C.c_int, [_VP, C.c_double, _VP, C.c_int, C.c_int, C.c_double, _DP, _VP, _VP,
                                                _IP, _DP, _DP]),
+        "shellgpu_buckling_create": (C.c_int, [_VP, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                               C.POINTER(_VP)]),
+        "shellgpu_buckling_destroy": (None, [_VP]),
+        "shellgpu_buckling_solve": (C.c_int, [_VP, _VP, _VP]),
+        "shellgpu_buckling_extract": (C.c_int, [_VP, C.c_int, _DP, _VP, _DP]),
+        "shellgpu_buckling_info": (C.c_int, [_VP, _IP, _DP, _VP]),
+        "shellgpu_buckling_add_eigen_sens": (C.c_int, [_VP, _DP, _VP, _VP]),
+        "shellgpu_ks_aggregate": (C.c_int, [C.c_int, C.c_double, _DP, _DP, _DP]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
         "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
         "shellgpu_timer_start": (C.c_int, [_VP]),
@@ -229,6 +240,93 @@ class Factor:
             pass
 
 
+class Buckling:
+    """Linear buckling analysis of every design of a context
+    (TACSLinearBuckling): solve, extract modes, eigenvalue sensitivities."""
+
+    def __init__(self, ctx, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12, model=None):
+        self.ctx = ctx
+        self.nev = nev
+        h = _VP()
+        ctx._check(ctx.lib.shellgpu_buckling_create(ctx.h, LINEAR if model is None else model, float(sigma), nev,
+                                                    max_lanczos, tol, C.byref(h)))
+        self.h = h.value
+
+    def solve(self, force=None, u0=None):
+        """Static path (or u0), G, shift, Lanczos; force/u0 are host arrays
+        broadcast to [nbatch, 6n] or DeviceArrays."""
+        ctx = self.ctx
+        nb, n = ctx.nbatch, 6 * ctx.nnodes
+
+        def dev(v):
+            if v is None or isinstance(v, DeviceArray):
+                return v
+            return ctx.vector(np.broadcast_to(v, (nb, n)))
+        f, u = dev(force), dev(u0)
+        ctx._check(ctx.lib.shellgpu_buckling_solve(self.h, f.ptr if f else None, u.ptr if u else None))
+
+    def eigenvalues(self):
+        """[nbatch, nev] eigenvalues and relative Ritz residuals."""
+        nb = self.ctx.nbatch
+        lam, err = np.zeros((self.nev, nb)), np.zeros((self.nev, nb))
+        for j in range(self.nev):
+            self.ctx._check(self.ctx.lib.shellgpu_buckling_extract(self.h, j, _dp(lam[j]), None, _dp(err[j])))
+        return lam.T.copy(), err.T.copy()
+
+    def eigenvector(self, j):
+        """K-normalised eigenvector j of every design, [nbatch, 6n]."""
+        nb, n = self.ctx.nbatch, 6 * self.ctx.nnodes
+        phi = DeviceArray(self.ctx, nb * n)
+        lam = np.zeros(nb)
+        self.ctx._check(self.ctx.lib.shellgpu_buckling_extract(self.h, j, _dp(lam), phi.ptr, None))
+        return phi.download().reshape(nb, n)
+
+    def info(self):
+        nb, n = self.ctx.nbatch, 6 * self.ctx.nnodes
+        steps, sig = C.c_int(), np.zeros(nb)
+        path = DeviceArray(self.ctx, nb * n)
+        self.ctx._check(self.ctx.lib.shellgpu_buckling_info(self.h, C.byref(steps), _dp(sig), path.ptr))
+        return {"steps": steps.value, "sigma": sig, "path": path.download().reshape(nb, n)}
+
+    def eigen_sens(self, coef, thickness=True, coords=False):
+        """sum_j coef[d, j] d(lambda_j)/dx per design: (dfdt [nbatch,
+        nelems] or None, dfdX [nbatch, 3*nnodes] or None)."""
+        ctx = self.ctx
+        nb = ctx.nbatch
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(coef, dtype=np.float64), (nb, self.nev)))
+        dt = DeviceArray(ctx, nb * ctx.nelems) if thickness else None
+        dX = DeviceArray(ctx, nb * 3 * ctx.nnodes) if coords else None
+        if dt is not None:
+            dt.zero()
+        if dX is not None:
+            dX.zero()
+        ctx._check(ctx.lib.shellgpu_buckling_add_eigen_sens(self.h, _dp(c), dt.ptr if dt else None,
+                                                           dX.ptr if dX else None))
+        return (dt.download().reshape(nb, ctx.nelems) if dt else None,
+                dX.download().reshape(nb, 3 * ctx.nnodes) if dX else None)
+
+    def destroy(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.shellgpu_buckling_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def ks_aggregate(f, rho):
+    """KS aggregate (max-like) of f and its gradient d ks / d f."""
+    lib = load_library()
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    ks, dks = C.c_double(), np.zeros_like(f)
+    if lib.shellgpu_ks_aggregate(len(f), float(rho), _dp(f), C.byref(ks), _dp(dks)) != 0:
+        raise ShellGPUError("ks_aggregate: invalid arguments")
+    return ks.value, dks
+
+
 class Context:
     """One GPU context: mesh, material, BCs, designs and a HIP stream."""
 
@@ -353,6 +451,24 @@ class Context:
         if path is not None:
             out["path"] = path.download().reshape(nb, n)
         return out
+
+    def design_vector(self):
+        """Zeroed device array of per-design element values [nbatch*nelems]."""
+        return DeviceArray(self, self.nbatch * self.nelems).zero()
+
+    def add_adj_res_product(self, dfdt, psi, u=None, scale=1.0, model=LINEAR):
+        self._check(self.lib.shellgpu_add_adj_res_product(self.h, model, float(scale), psi.ptr,
+                                                          u.ptr if u else None, dfdt.ptr))
+
+    def add_mat_dv_sens_inner_product(self, dfdt, mattype, phi, u=None, scale=1.0):
+        self._check(self.lib.shellgpu_add_mat_dv_sens_inner_product(self.h, mattype, float(scale), phi.ptr,
+                                                                    u.ptr if u else None, dfdt.ptr))
+
+    def eval_mat_sv_sens_inner_product(self, phi, dfdu):
+        self._check(self.lib.shellgpu_eval_mat_sv_sens_inner_product(self.h, phi.ptr, dfdu.ptr))
+
+    def buckling(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12, model=None):
+        return Buckling(self, sigma, nev, max_lanczos, tol, model)
 
     def vec_zero_bcs(self, x):
         self._check(self.lib.shellgpu_vec_zero_bcs(self.h, x.ptr))

@@ -23,6 +23,7 @@ extern "C" {
 typedef struct shellgpu_ctx shellgpu_ctx;
 typedef struct shellgpu_mat shellgpu_mat;
 typedef struct shellgpu_factor shellgpu_factor;
+typedef struct shellgpu_buckling shellgpu_buckling;
 
 /* matrix types (ElementMatrixType, src/elements/TACSElementTypes.h) */
 enum {
@@ -111,6 +112,25 @@ int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double 
 int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u,
                                double *R, shellgpu_mat *A);
 
+/* ---- design sensitivities (thickness t_e per element; device outputs
+ * accumulate, nbatch*nelems) ---------------------------------------------- */
+/* TACSAssembler::addAdjointResProducts -> TACSShellElement::addAdjResProduct
+ * (src/elements/shell/TACSShellElement.h:774-868): dfdt[d*nelems + e] +=
+ * scale psi^T dR_e/dt_e at state u (NULL = zero) for the given model */
+int shellgpu_add_adj_res_product(shellgpu_ctx *ctx, int model, double scale, const double *psi,
+                                 const double *u, double *dfdt);
+/* TACSAssembler::addMatDVSensInnerProduct(scale, matType, phi, phi, dfdx)
+ * (element default src/elements/TACSElement.cpp:355-432, central
+ * differences there, exact here): dfdt += scale phi^T (dA/dt_e) phi with
+ * A = K (SHELLGPU_STIFFNESS, u ignored) or G(u) (SHELLGPU_GEOMETRIC) */
+int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale,
+                                           const double *phi, const double *u, double *dfdt);
+/* TACSAssembler::evalMatSVSensInnerProduct(GEOMETRIC, phi, phi, dfdu)
+ * (element default src/elements/TACSElement.cpp:509-560): dfdu =
+ * d(phi^T G(u) phi)/du, which is independent of u (G is linear in the
+ * path); no BCs applied */
+int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *phi, double *dfdu);
+
 /* ---- vector BCs (TACSBVec, src/bpmat/TACSBVec.cpp:546-625) ------------- */
 /* x[bc] = 0 for every design of the batch */
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x);
@@ -159,6 +179,52 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
 int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *force, int nev,
                              int max_lanczos, double tol, double *lambda, double *phi,
                              double *path, int *nsteps, double *resid, double *sigma_used);
+
+/* The analysis object behind shellgpu_linear_buckling, for callers that
+ * extract modes one at a time and need eigenvalue sensitivities.
+ * TACSLinearBuckling constructor (src/TACSBuckling.cpp:51-137): model is
+ * the element model whose residual defines a path passed as u0 (the
+ * adjoint of a nonlinear path uses its tangent); sigma, nev, max_lanczos
+ * and tol as in shellgpu_linear_buckling. */
+int shellgpu_buckling_create(shellgpu_ctx *ctx, int model, double sigma, int nev,
+                             int max_lanczos, double tol, shellgpu_buckling **B);
+void shellgpu_buckling_destroy(shellgpu_buckling *B);
+/* TACSLinearBuckling::solve(rhs, u0) (src/TACSBuckling.cpp:197-278): K at
+ * zero state; path = static solution of K u = f (u0 == NULL) or u0 (device,
+ * nbatch*6*nnodes; BC values are imposed on it); G(path); K + sigma G;
+ * Lanczos.  force (device) may be NULL. */
+int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const double *u0);
+/* TACSLinearBuckling::extractEigenvalue / extractEigenvector
+ * (src/TACSBuckling.cpp:283-297): eigenvalue n (0-based, sorted by
+ * magnitude) of every design: lambda[nbatch] and error[nbatch] on the host
+ * (either may be NULL), K-normalised phi on the device (nbatch*6*nnodes,
+ * may be NULL). */
+int shellgpu_buckling_extract(shellgpu_buckling *B, int n, double *lambda, double *phi,
+                              double *error);
+/* Lanczos steps taken, shift per design (host, nbatch), path (device,
+ * nbatch*6*nnodes); any may be NULL. */
+int shellgpu_buckling_info(shellgpu_buckling *B, int *nsteps, double *sigma_used,
+                           double *path);
+/* TACSLinearBuckling::addEigenDVSens / evalEigenDVSens and the Xpt
+ * counterparts (archive/TACSBuckling.cpp:382-592; the built reference has
+ * them commented out, src/TACSBuckling.cpp:383-594):
+ *   dfdt[d*nelems + e] += sum_n coef[d*nev + n] d(lambda_n)/d(t_e)
+ * with
+ *   d lambda = -(phi^T dK phi + lambda phi^T dG phi - lambda psi^T dR)
+ *              / (phi^T G phi),   (dR/du) psi = d(phi^T G phi)/d(path),
+ * all products exact (the reference differentiates matrix inner products
+ * by central differences, src/elements/TACSElement.cpp:355-507).  One
+ * adjoint solve serves all n.  coef is host, nbatch*nev; dfdt is device,
+ * nbatch*nelems; dfdX (device, nbatch*3*nnodes) receives the coordinate
+ * sensitivities the same way; either output may be NULL.  The shifted
+ * operator is refactored by a later solve. */
+int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, double *dfdt,
+                                     double *dfdX);
+
+/* KS aggregate of f[0..n) (ksAggregation / ksAggregationSens,
+ * src/utils/TacsUtilities.cpp:971-1011): ks = m + log(sum exp(rho (f_i -
+ * m))) / rho, m = max f; dks[i] = d ks / d f_i (may be NULL).  Host only. */
+int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double *dks);
 
 /* ---- timing (no reference counterpart: measurement plumbing) ---------- */
 /* With timing enabled, HIP events on the context's stream bracket every
