@@ -88,18 +88,20 @@ __device__ __forceinline__ int field_slot(int f) {
   return f == kG11 ? 0 : (f == kG22 ? 3 : (f == kG12 ? 1 : (f == kG23 ? 4 : 2)));
 }
 
-// ---- 3x3 row-major algebra ----
-__device__ __forceinline__ void cross3(const double *a, const double *b,
-                                       double *c) {
+// ---- 3x3 row-major algebra, generic in the scalar type S (double, or the
+// forward-mode Dual of dual.h for coordinate derivatives) ----
+template <typename S>
+__device__ __forceinline__ void cross3(const S *a, const S *b, S *c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
-__device__ __forceinline__ double dot3(const double *a, const double *b) {
+template <typename S>
+__device__ __forceinline__ S dot3(const S *a, const S *b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
-__device__ __forceinline__ void mm3(const double *A, const double *B,
-                                    double *C) {
+template <typename S>
+__device__ __forceinline__ void mm3(const S *A, const S *B, S *C) {
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -108,8 +110,8 @@ __device__ __forceinline__ void mm3(const double *A, const double *B,
           A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
 }
 // C = A^T B
-__device__ __forceinline__ void mtm3(const double *A, const double *B,
-                                     double *C) {
+template <typename S>
+__device__ __forceinline__ void mtm3(const S *A, const S *B, S *C) {
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -118,11 +120,12 @@ __device__ __forceinline__ void mtm3(const double *A, const double *B,
 }
 // inverse by cofactors; determinant expanded on the third row
 // (TACSElementAlgebra.h:1980-2009)
-__device__ __forceinline__ double inv3(const double *A, double *Ai) {
-  const double det =
+template <typename S>
+__device__ __forceinline__ S inv3(const S *A, S *Ai) {
+  const S det =
       (A[8] * (A[0] * A[4] - A[3] * A[1]) - A[7] * (A[0] * A[5] - A[3] * A[2]) +
        A[6] * (A[1] * A[5] - A[2] * A[4]));
-  const double r = 1.0 / det;
+  const S r = 1.0 / det;
   Ai[0] = (A[4] * A[8] - A[5] * A[7]) * r;
   Ai[1] = -(A[1] * A[8] - A[2] * A[7]) * r;
   Ai[2] = (A[1] * A[5] - A[2] * A[4]) * r;
@@ -140,24 +143,24 @@ __device__ __forceinline__ double inv3(const double *A, double *Ai) {
 // left un-orthogonalised: the reference subtracts d*n[0] from t1[0] three
 // times and never touches t1[1], t1[2] (lines 42-44).  T = [t1 t2 n] as
 // columns.
-__device__ __forceinline__ void natural_transform(const double Xxi[6],
-                                                  const double n0[3],
-                                                  double T[9]) {
-  double n[3] = {n0[0], n0[1], n0[2]};
-  double inv = 1.0 / sqrt(dot3(n, n));
-  n[0] *= inv;
-  n[1] *= inv;
-  n[2] *= inv;
-  double t1[3] = {Xxi[0], Xxi[2], Xxi[4]};
-  const double d = dot3(n, t1);
+template <typename S>
+__device__ __forceinline__ void natural_transform(const S Xxi[6], const S n0[3],
+                                                  S T[9]) {
+  S n[3] = {n0[0], n0[1], n0[2]};
+  S inv = 1.0 / sqrt(dot3(n, n));
+  n[0] = n[0] * inv;
+  n[1] = n[1] * inv;
+  n[2] = n[2] * inv;
+  S t1[3] = {Xxi[0], Xxi[2], Xxi[4]};
+  const S d = dot3(n, t1);
   t1[0] = t1[0] - d * n[0];
   t1[0] = t1[0] - d * n[0];
   t1[0] = t1[0] - d * n[0];
   inv = 1.0 / sqrt(dot3(t1, t1));
-  t1[0] *= inv;
-  t1[1] *= inv;
-  t1[2] *= inv;
-  double t2[3];
+  t1[0] = t1[0] * inv;
+  t1[1] = t1[1] * inv;
+  t1[2] = t1[2] * inv;
+  S t2[3];
   cross3(n, t1, t2);
   T[0] = t1[0];
   T[3] = t1[1];
@@ -171,17 +174,30 @@ __device__ __forceinline__ void natural_transform(const double Xxi[6],
 }
 
 // 3x3 frame [xi-derivatives (interleaved) | c] (TACSShellUtilities.h:8-35)
-__device__ __forceinline__ void frame3(const double xi[6], const double *c,
-                                       double Xd[9]) {
+template <typename S>
+__device__ __forceinline__ void frame3(const S xi[6], const S c[3], S Xd[9]) {
   Xd[0] = xi[0];
   Xd[1] = xi[1];
-  Xd[2] = c ? c[0] : 0.0;
+  Xd[2] = c[0];
   Xd[3] = xi[2];
   Xd[4] = xi[3];
-  Xd[5] = c ? c[1] : 0.0;
+  Xd[5] = c[1];
   Xd[6] = xi[4];
   Xd[7] = xi[5];
-  Xd[8] = c ? c[2] : 0.0;
+  Xd[8] = c[2];
+}
+// the same frame with a zero third column
+template <typename S>
+__device__ __forceinline__ void frame3z(const S xi[6], S Xd[9]) {
+  Xd[0] = xi[0];
+  Xd[1] = xi[1];
+  Xd[2] = 0.0;
+  Xd[3] = xi[2];
+  Xd[4] = xi[3];
+  Xd[5] = 0.0;
+  Xd[6] = xi[4];
+  Xd[7] = xi[5];
+  Xd[8] = 0.0;
 }
 
 }  // namespace shellgpu

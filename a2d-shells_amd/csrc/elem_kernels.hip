@@ -376,7 +376,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     double T[9], Xd[9], Xdz[9], Xdi[9], XT[9], nXdz[9], XzT[9];
     natural_transform(Xxi, n0, T);
     frame3(Xxi, n0, Xd);
-    frame3(nxi, nullptr, Xdz);
+    frame3z(nxi, Xdz);
     const double det = inv3(Xd, Xdi);
     mm3(Xdi, Xdz, nXdz);
 #pragma unroll
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       }
       double u0d[9], u1d[9], tmp[9], tmp2[9], o0[9], o1[9];
       frame3(u0xi, d0, u0d);
-      frame3(d0xi, nullptr, u1d);
+      frame3z(d0xi, u1d);
       mm3(u1d, XT, tmp);
       mm3(u0d, XzT, tmp2);
 #pragma unroll

@@ -203,6 +203,31 @@ int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
   return 0;
 }
 
+int run_xpt(shellgpu_ctx *ctx, const double *phi, const double *u, const double *psi, const double *coef,
+            int nonlin, double *dfdX) {
+  XptArgs a{};
+  a.conn = ctx->d_conn;
+  a.Xpts = ctx->d_X;
+  a.thick = ctx->d_thick;
+  a.temp = ctx->d_temp;
+  a.nelems = ctx->nelems;
+  a.mat = ctx->mat;
+  a.phi = phi;
+  a.u = u;
+  a.psi = psi;
+  a.stride = 6L * ctx->nnodes;
+  a.coef = coef;
+  a.nonlin = nonlin;
+  a.dfdX = dfdX;
+  a.dX_stride = 3L * ctx->nnodes;
+  for (int c = 0; c < ctx->ncolors; c++) {
+    a.elems = ctx->d_color_elems + ctx->h_color_ptr[c];
+    const int nb = ctx->h_color_ptr[c + 1] - ctx->h_color_ptr[c];
+    HIPCALL(ctx, launch_xpt_kernel(a, nb, ctx->nbatch, ctx->stream));
+  }
+  return 0;
+}
+
 }  // namespace shellgpu
 
 // ---------------------------------------------------------------------------
@@ -383,7 +408,7 @@ int shellgpu_set_designs(shellgpu_ctx *ctx, int nbatch, const double *thick, con
   }
   if (ctx->d_coef) (void)hipFree(ctx->d_coef);
   ctx->d_coef = nullptr;
-  HIPCALL(ctx, hipMalloc((void **)&ctx->d_coef, sizeof(double) * 2 * nbatch));
+  HIPCALL(ctx, hipMalloc((void **)&ctx->d_coef, sizeof(double) * 4 * nbatch));
   return 0;
 }
 
@@ -605,6 +630,40 @@ int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *phi
   a.coef = ctx->d_coef;
   a.res = dfdu;
   return run_colored(ctx, kModeAdjDV, a);
+}
+
+// per-design coefficient quadruples (c0, c1, c2, 0) into ctx->d_coef
+static int set_coef4(shellgpu_ctx *ctx, double c0, double c1, double c2) {
+  std::vector<double> h(4 * ctx->nbatch, 0.0);
+  for (int d = 0; d < ctx->nbatch; d++) {
+    h[4 * d] = c0;
+    h[4 * d + 1] = c1;
+    h[4 * d + 2] = c2;
+  }
+  HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale, const double *phi,
+                                            const double *u, double *dfdX) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!phi || !dfdX) return set_err(ctx, 1, "phi and dfdX are required");
+  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC)
+    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS or SHELLGPU_GEOMETRIC");
+  const bool geo = mattype == SHELLGPU_GEOMETRIC;
+  if ((rc = set_coef4(ctx, geo ? 0.0 : scale, geo ? scale : 0.0, 0.0))) return rc;
+  return run_xpt(ctx, phi, geo ? u : nullptr, nullptr, ctx->d_coef, 0, dfdX);
+}
+
+int shellgpu_add_adj_res_xpt_product(shellgpu_ctx *ctx, int model, double scale, const double *psi,
+                                     const double *u, double *dfdX) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!psi || !dfdX) return set_err(ctx, 1, "psi and dfdX are required");
+  if ((rc = set_coef4(ctx, 0.0, 0.0, scale))) return rc;
+  return run_xpt(ctx, nullptr, u, psi, ctx->d_coef, model == SHELLGPU_NONLINEAR, dfdX);
 }
 
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x) {

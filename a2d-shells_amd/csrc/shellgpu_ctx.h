@@ -27,7 +27,7 @@ struct shellgpu_ctx {
   double *d_bc_vals = nullptr;
 
   double *d_thick = nullptr, *d_temp = nullptr;
-  double *d_coef = nullptr;  // 2 * nbatch per-design kernel coefficients
+  double *d_coef = nullptr;  // 4 * nbatch per-design kernel coefficients
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
@@ -51,4 +51,8 @@ ElemArgs base_args(shellgpu_ctx *ctx, const double *u);
 // launch one element-kernel mode colour by colour (no two elements of a
 // launch share a node, so scatters need no atomics)
 int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a);
+// coordinate-derivative kernel over all colours; coef (device, 4 per
+// design), vectors with stride 6*nnodes, dfdX with stride 3*nnodes
+int run_xpt(shellgpu_ctx *ctx, const double *phi, const double *u, const double *psi, const double *coef,
+            int nonlin, double *dfdX);
 }  // namespace shellgpu

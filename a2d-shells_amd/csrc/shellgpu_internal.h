@@ -91,6 +91,26 @@ __host__ __device__ inline Section make_section_dt(const Material &m, double t) 
   return s;
 }
 
+// coordinate-derivative kernel: dfdX[3 node + x] += d/dX of
+// coef[4d] phi^T K_e phi + coef[4d+1] phi^T G_e(u) phi + coef[4d+2] psi^T R_e(u)
+// (residual of the nonlinear model when nonlin != 0); phi, u, psi may be null
+struct XptArgs {
+  const int *elems;
+  const int *conn;
+  const double *Xpts;
+  const double *thick;
+  const double *temp;
+  int nelems;
+  Material mat;
+  const double *phi, *u, *psi;
+  long stride;
+  const double *coef;
+  int nonlin;
+  double *dfdX;
+  long dX_stride;
+};
+hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream_t stream);
+
 hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int nbatch,
                               hipStream_t stream);
 

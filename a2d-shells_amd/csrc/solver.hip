@@ -893,7 +893,7 @@ struct shellgpu_buckling {
   shellgpu_factor *F = nullptr;
   // device: path u, work vectors, Lanczos basis Q and K Q, coefficients
   double *u = nullptr, *res = nullptr, *f0 = nullptr, *Q = nullptr, *KQ = nullptr, *hv = nullptr, *dsig = nullptr;
-  double *ph = nullptr, *gv = nullptr, *psi = nullptr, *dcoef = nullptr;
+  double *ph = nullptr, *gv = nullptr, *psi = nullptr, *dcoef = nullptr;  // dcoef: 6 per design
   std::vector<double> sig;
   bool solved = false;
   int steps = 0;
@@ -971,7 +971,7 @@ int shellgpu_buckling_create(shellgpu_ctx *ctx, int model, double sigma, int nev
         hipMalloc((void **)&B->KQ, v * (m + 1)) != hipSuccess ||
         hipMalloc((void **)&B->hv, sizeof(double) * (size_t)nb * (m + 1)) != hipSuccess ||
         hipMalloc((void **)&B->dsig, sizeof(double) * nb) != hipSuccess ||
-        hipMalloc((void **)&B->dcoef, sizeof(double) * 2 * nb) != hipSuccess) {
+        hipMalloc((void **)&B->dcoef, sizeof(double) * 6 * nb) != hipSuccess) {
       rc = ferr(ctx, 2, "out of device memory (Lanczos basis)");
       goto fail;
     }
@@ -1121,14 +1121,16 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
   if (!B || !coef) return 1;
   shellgpu_ctx *ctx = B->ctx;
   if (!B->solved) return ferr(ctx, 9, "buckling problem not solved");
-  if (dfdX) return ferr(ctx, 10, "coordinate sensitivities are not available");
-  if (!dfdt) return 0;
+  if (!dfdt && !dfdX) return 0;
   const int nb = ctx->nbatch, n = 6 * ctx->nnodes, nev = B->nev;
   int rc = 0;
   VecOps vo;
   if ((rc = vo.init(ctx, n, nb))) return rc;
+  // coefficient layout: [0, 2 nb) pairs for the thickness kernels, [2 nb,
+  // 6 nb) quadruples for the coordinate kernel
+  double *c2 = B->dcoef, *c4 = B->dcoef + 2 * nb;
   HIPCALL(ctx, hipMemsetAsync(B->gv, 0, sizeof(double) * (size_t)n * nb, ctx->stream));
-  std::vector<double> hc(2 * nb);
+  std::vector<double> hc(6 * nb, 0.0);
   for (int j = 0; j < nev; j++) {
     bool any = false;
     for (int d = 0; d < nb; d++) any = any || coef[(size_t)d * nev + j] != 0.0;
@@ -1143,15 +1145,19 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
       const double f = -coef[(size_t)d * nev + j] / pgp[d];
       hc[2 * d] = f;
       hc[2 * d + 1] = f * B->lam[d][j];
+      hc[2 * nb + 4 * d] = f;
+      hc[2 * nb + 4 * d + 1] = f * B->lam[d][j];
+      hc[2 * nb + 4 * d + 2] = 0.0;
     }
-    HIPCALL(ctx, hipMemcpyAsync(B->dcoef, hc.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, ctx->stream));
+    HIPCALL(ctx, hipMemcpyAsync(B->dcoef, hc.data(), sizeof(double) * 6 * nb, hipMemcpyHostToDevice, ctx->stream));
     ElemArgs a = base_args(ctx, B->ph);
     a.u2 = B->u;
     a.u2_stride = n;
-    a.coef = B->dcoef;
+    a.coef = c2;
     a.dv = dfdt;
     a.res = B->gv;
     if ((rc = run_colored(ctx, kModeAdjDV, a))) return rc;
+    if (dfdX && (rc = run_xpt(ctx, B->ph, B->u, nullptr, c4, 0, dfdX))) return rc;
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   }
   // path adjoint: dR/du psi = sum_j f_j lambda_j d(phi_j^T G phi_j)/du, with
@@ -1163,21 +1169,30 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
   } else {
     if ((rc = shellgpu_mat_copy(B->A, B->K))) return rc;
   }
+  // the shifted operator is no longer factored: a later solve refactors
   if ((rc = shellgpu_factor_compute(B->F, B->A))) return rc;
   {
     int its = 0;
     double rn = 0.0;
     if ((rc = shellgpu_gmres(ctx, B->A, B->F, B->gv, B->psi, 10, 3, 1e-12, 1e-300, &its, &rn))) return rc;
   }
-  // the shifted operator is no longer factored: a later solve refactors
-  std::vector<double> mone(nb, -1.0);
-  HIPCALL(ctx, hipMemcpyAsync(B->dcoef, mone.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
-  ElemArgs a = base_args(ctx, B->u);
-  a.psi = B->psi;
-  a.psi_stride = n;
-  a.coef = B->dcoef;
-  a.dv = dfdt;
-  if ((rc = run_colored(ctx, B->model == SHELLGPU_NONLINEAR ? kModeAdjResNl : kModeAdjResLin, a))) return rc;
+  for (int d = 0; d < nb; d++) {
+    hc[d] = -1.0;
+    hc[2 * nb + 4 * d] = 0.0;
+    hc[2 * nb + 4 * d + 1] = 0.0;
+    hc[2 * nb + 4 * d + 2] = -1.0;
+  }
+  HIPCALL(ctx, hipMemcpyAsync(B->dcoef, hc.data(), sizeof(double) * 6 * nb, hipMemcpyHostToDevice, ctx->stream));
+  const int nonlin = B->model == SHELLGPU_NONLINEAR;
+  if (dfdt) {
+    ElemArgs a = base_args(ctx, B->u);
+    a.psi = B->psi;
+    a.psi_stride = n;
+    a.coef = c2;
+    a.dv = dfdt;
+    if ((rc = run_colored(ctx, nonlin ? kModeAdjResNl : kModeAdjResLin, a))) return rc;
+  }
+  if (dfdX && (rc = run_xpt(ctx, nullptr, B->u, B->psi, c4, nonlin, dfdX))) return rc;
   HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
 }

@@ -74,6 +74,8 @@ def load_library(path=LIB_PATH):
         "shellgpu_add_adj_res_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
         "shellgpu_add_mat_dv_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
         "shellgpu_eval_mat_sv_sens_inner_product": (C.c_int, [_VP, _VP, _VP]),
+        "shellgpu_add_mat_xpt_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
+        "shellgpu_add_adj_res_xpt_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
         "shellgpu_vec_zero_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_vec_set_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_set_ordering_seed": (C.c_int, [_VP, C.c_int, _IP]),
@@ -463,6 +465,18 @@ class Context:
     def add_mat_dv_sens_inner_product(self, dfdt, mattype, phi, u=None, scale=1.0):
         self._check(self.lib.shellgpu_add_mat_dv_sens_inner_product(self.h, mattype, float(scale), phi.ptr,
                                                                     u.ptr if u else None, dfdt.ptr))
+
+    def coord_vector(self):
+        """Zeroed device array of per-design nodal coordinates [nbatch*3*nnodes]."""
+        return DeviceArray(self, self.nbatch * 3 * self.nnodes).zero()
+
+    def add_mat_xpt_sens_inner_product(self, dfdX, mattype, phi, u=None, scale=1.0):
+        self._check(self.lib.shellgpu_add_mat_xpt_sens_inner_product(self.h, mattype, float(scale), phi.ptr,
+                                                                     u.ptr if u else None, dfdX.ptr))
+
+    def add_adj_res_xpt_product(self, dfdX, psi, u=None, scale=1.0, model=LINEAR):
+        self._check(self.lib.shellgpu_add_adj_res_xpt_product(self.h, model, float(scale), psi.ptr,
+                                                              u.ptr if u else None, dfdX.ptr))
 
     def eval_mat_sv_sens_inner_product(self, phi, dfdu):
         self._check(self.lib.shellgpu_eval_mat_sv_sens_inner_product(self.h, phi.ptr, dfdu.ptr))

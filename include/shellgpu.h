@@ -131,6 +131,20 @@ int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, doubl
  * path); no BCs applied */
 int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *phi, double *dfdu);
 
+/* ---- coordinate sensitivities (device outputs accumulate,
+ * nbatch*3*nnodes, x/y/z per node) --------------------------------------- */
+/* TACSAssembler::addMatXptSensInnerProduct(scale, matType, phi, phi, dfdX)
+ * (element default src/elements/TACSElement.cpp:434-507, central
+ * differences there, exact forward-mode derivatives here): dfdX += scale
+ * d(phi^T A phi)/dX with A = K (SHELLGPU_STIFFNESS, u ignored) or G(u) */
+int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale,
+                                            const double *phi, const double *u, double *dfdX);
+/* TACSAssembler::addAdjointResXptSensProducts -> addAdjResXptProduct
+ * (element default src/elements/TACSElement.cpp:270-332): dfdX += scale
+ * d(psi^T R(u))/dX at state u (NULL = zero) for the given model */
+int shellgpu_add_adj_res_xpt_product(shellgpu_ctx *ctx, int model, double scale, const double *psi,
+                                     const double *u, double *dfdX);
+
 /* ---- vector BCs (TACSBVec, src/bpmat/TACSBVec.cpp:546-625) ------------- */
 /* x[bc] = 0 for every design of the batch */
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x);

@@ -1,4 +1,4 @@
-"""Thickness sensitivities on the GPU against the oracle.
+"""Thickness and coordinate sensitivities on the GPU against the oracle.
 
 Element level: phi^T (dK/dt) phi and phi^T (dG/dt) phi against a five-point
 central difference of the oracle's element matrices in t (exact for the
@@ -7,10 +7,14 @@ gradient d(phi^T G phi)/du against differences of oracle G matrices (G is
 affine in the path), and psi^T dR/dt against the oracle's restatement of
 addAdjResProduct, both element models.
 
-Analysis level: d(lambda_j)/dt of the linear buckling loads and of a KS
-aggregate along seeded random directions against central differences of
-the oracle's buckling loads (relative step 1e-5), and batched designs
-against one-design runs."""
+Coordinates: d/dX of phi^T K phi, phi^T G(u) phi and psi^T R(u) (both
+models) summed over the mesh against five-point differences of the
+oracle's element matrices and residuals in every nodal coordinate.
+
+Analysis level: d(lambda_j)/dt and d(lambda_j)/dX of the linear buckling
+loads and of a KS aggregate along seeded random directions against central
+differences of the oracle's buckling loads, and batched designs against
+one-design runs."""
 import numpy as np
 import pytest
 
@@ -101,6 +105,59 @@ def test_path_gradient_matches_oracle():
         assert abs(g @ w - ref) <= 1e-10 * max(abs(ref), np.abs(g).max() * np.abs(w).sum() * 1e-6), k
 
 
+@pytest.mark.parametrize("thermal", [False, True])
+def test_coordinate_products_match_oracle(thermal):
+    conn, X = random_mesh(3, 3, seed=51)
+    nn, ne = X.shape[0], conn.shape[0]
+    rng = np.random.default_rng(52)
+    thick = rng.uniform(0.002, 0.02, (1, ne))
+    temp = rng.uniform(-30.0, 60.0, (1, ne)) if thermal else None
+    mat = O.material(cte=1.2e-5 if thermal else 0.0)
+    phi = rng.standard_normal(6 * nn)
+    u = 1e-3 * rng.standard_normal(6 * nn)
+    psi = rng.standard_normal(6 * nn)
+    ctx = _mesh_ctx(conn, X, thick, temp)
+    dphi, du, dpsi = ctx.vector(phi), ctx.vector(u), ctx.vector(psi)
+    out = {}
+    for name in ("K", "G", "R0", "R1"):
+        g = ctx.coord_vector()
+        if name == "K":
+            ctx.add_mat_xpt_sens_inner_product(g, sg.STIFFNESS, dphi)
+        elif name == "G":
+            ctx.add_mat_xpt_sens_inner_product(g, sg.GEOMETRIC, dphi, du)
+        else:
+            ctx.add_adj_res_xpt_product(g, dpsi, du, model=int(name[1]))
+        out[name] = g.download()
+
+    def total(name, Xp):
+        s = 0.0
+        for e in range(ne):
+            Xe = Xp[conn[e]].ravel()
+            idx = (6 * conn[e][:, None] + np.arange(6)).ravel()
+            T = temp[0, e] if thermal else 0.0
+            t = thick[0, e]
+            if name == "K":
+                s += phi[idx] @ O.elem_stiffness(0, mat, t, T, Xe, np.zeros(24)) @ phi[idx]
+            elif name == "G":
+                s += phi[idx] @ O.elem_geometric_step(mat, t, T, Xe, u[idx], 1.0) @ phi[idx]
+            else:
+                s += psi[idx] @ O.elem_residual(int(name[1]), mat, t, T, Xe, u[idx])
+        return s
+
+    h = 1e-3 * 0.3
+    for name, g in out.items():
+        ref = np.zeros(3 * nn)
+        for k in range(3 * nn):
+            def f(x, k=k):
+                Xp = X.copy().ravel()
+                Xp[k] = x
+                return total(name, Xp.reshape(-1, 3))
+            ref[k] = _five_point(f, X.ravel()[k], h)
+        err = np.abs(g - ref).max() / np.abs(ref).max()
+        print(name, err)
+        assert err <= 1e-8, name
+
+
 def _cyl_ctx(cyl, thick):
     ctx = sg.Context(0)
     ctx.set_mesh(cyl.conn, cyl.Xpts)
@@ -141,6 +198,38 @@ def test_buckling_gradient_matches_oracle_fd():
         fdks = (ksp - ksm) / (2 * h)
         print("ks", gks @ dirn, fdks)
         assert abs(gks @ dirn - fdks) <= 1e-6 * max(abs(fdks), 1.0)
+    B.destroy()
+
+
+def test_buckling_coordinate_gradient_matches_oracle_fd():
+    cyl = Cylinder(8, 16)
+    thick = seeded_thickness(cyl.nelems, 61)
+    ctx = _cyl_ctx(cyl, thick)
+    nev = 3
+    B = ctx.buckling(sigma=0.0, nev=nev, max_lanczos=70, tol=1e-12)
+    f = cyl.edge_load(1.0)
+    B.solve(force=f)
+    lam = B.eigenvalues()[0][0]
+    sig = B.info()["sigma"][0]
+    coef = np.array([1.0, -0.5, 0.25])
+    _, gX = B.eigen_sens(coef, thickness=False, coords=True)
+    rng = np.random.default_rng(3001)
+
+    def central(dirn, h):
+        Xp = (cyl.Xpts.ravel() + h * dirn).reshape(-1, 3)
+        Xm = (cyl.Xpts.ravel() - h * dirn).reshape(-1, 3)
+        lp = O.linear_buckling(Xp, cyl.conn, thick[0], cyl.bcs(), f, sig, nev=nev, max_lanczos=70)[0]
+        lm = O.linear_buckling(Xm, cyl.conn, thick[0], cyl.bcs(), f, sig, nev=nev, max_lanczos=70)[0]
+        return coef @ (lp - lm) / (2 * h)
+
+    for k in range(2):
+        # a random nodal perturbation makes the loads strongly curved along
+        # it: Richardson-extrapolated central differences (error O(h^4))
+        dirn = rng.standard_normal(3 * cyl.nnodes)
+        h = 1e-7
+        fd = (4.0 * central(dirn, h) - central(dirn, 2 * h)) / 3.0
+        print(k, gX[0] @ dirn, fd)
+        assert abs(gX[0] @ dirn - fd) <= 1e-6 * abs(fd), k
     B.destroy()
 
 
