@@ -1241,4 +1241,88 @@ int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double
   return 0;
 }
 
+int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, int max_iters, double rtol, double *u,
+                          int *iters, double *relres) {
+  if (!ctx || !force || !u || nsteps <= 0 || max_iters <= 0 || !(rtol > 0.0))
+    return ferr(ctx, 1, "invalid Newton arguments");
+  if (!ctx->d_thick) return ferr(ctx, 3, "designs not set (shellgpu_set_designs)");
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
+  int rc = 0, total = 0;
+  shellgpu_mat *A = nullptr;
+  shellgpu_factor *F = nullptr;
+  double *R = nullptr, *f0 = nullptr, *du = nullptr;
+  VecOps vo;
+  std::vector<double> fn, rn(nb, 0.0);
+  bool factored = false;
+  if ((rc = vo.init(ctx, n, nb))) return rc;
+  if ((rc = ensure_blas(ctx))) return rc;
+  if ((rc = shellgpu_mat_create(ctx, nb, &A))) goto out;
+  if ((rc = shellgpu_factor_create(ctx, &F))) goto out;
+  if (hipMalloc((void **)&R, sizeof(double) * (size_t)n * nb) != hipSuccess ||
+      hipMalloc((void **)&f0, sizeof(double) * (size_t)n * nb) != hipSuccess ||
+      hipMalloc((void **)&du, sizeof(double) * (size_t)n * nb) != hipSuccess) {
+    rc = ferr(ctx, 2, "out of device memory (Newton vectors)");
+    goto out;
+  }
+  if (hipMemcpyAsync(f0, force, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream) !=
+      hipSuccess) {
+    rc = ferr(ctx, 2, "force copy failed");
+    goto out;
+  }
+  if ((rc = shellgpu_vec_zero_bcs(ctx, f0))) goto out;
+  if ((rc = vo.dot(f0, f0, fn))) goto out;
+  for (double &v : fn) v = v > 0.0 ? std::sqrt(v) : 1.0;
+  for (int k = 1; k <= nsteps; k++) {
+    const double s = (double)k / nsteps;
+    for (int it = 0;; it++) {
+      // R(u) - s f with BC rows u - ubar; tangent of the nonlinear model
+      if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, u, R, A))) goto out;
+      {
+        std::vector<double> ms(nb, -s);
+        if ((rc = vo.axpy(ms, f0, R))) goto out;
+      }
+      std::vector<double> r2;
+      if ((rc = vo.dot(R, R, r2))) goto out;
+      bool conv = true;
+      for (int d = 0; d < nb; d++) {
+        rn[d] = std::sqrt(r2[d]) / fn[d];
+        conv = conv && rn[d] <= rtol;
+      }
+      if (conv) break;
+      if (it >= max_iters) {
+        rc = ferr(ctx, 11, "Newton did not converge within max_iters");
+        goto out;
+      }
+      // Newton step K_t du = R by GMRES, preconditioned by the last factored
+      // tangent; refactor at the current state when that stalls
+      int its = 0;
+      double res = 0.0;
+      if (!factored) {
+        if ((rc = shellgpu_factor_compute(F, A))) goto out;
+        factored = true;
+      }
+      if ((rc = shellgpu_gmres(ctx, A, F, R, du, 20, 2, 1e-12, 1e-300, &its, &res))) goto out;
+      if (!(res <= 1e-10)) {
+        if ((rc = shellgpu_factor_compute(F, A))) goto out;
+        if ((rc = shellgpu_gmres(ctx, A, F, R, du, 20, 5, 1e-12, 1e-300, &its, &res))) goto out;
+      }
+      {
+        std::vector<double> m1(nb, -1.0);
+        if ((rc = vo.axpy(m1, du, u))) goto out;
+      }
+      total++;
+    }
+  }
+  if (iters) *iters = total;
+  if (relres)
+    for (int d = 0; d < nb; d++) relres[d] = rn[d];
+out:
+  if (A) shellgpu_mat_destroy(A);
+  if (F) shellgpu_factor_destroy(F);
+  double *bufs[] = {R, f0, du};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  return rc;
+}
+
 }  // extern "C"

@@ -95,6 +95,7 @@ def load_library(path=LIB_PATH):
         "shellgpu_buckling_info": (C.c_int, [_VP, _IP, _DP, _VP]),
         "shellgpu_buckling_add_eigen_sens": (C.c_int, [_VP, _DP, _VP, _VP]),
         "shellgpu_ks_aggregate": (C.c_int, [C.c_int, C.c_double, _DP, _DP, _DP]),
+        "shellgpu_newton_solve": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_double, _VP, _IP, _DP]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
         "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
         "shellgpu_timer_start": (C.c_int, [_VP]),
@@ -480,6 +481,17 @@ class Context:
 
     def eval_mat_sv_sens_inner_product(self, phi, dfdu):
         self._check(self.lib.shellgpu_eval_mat_sv_sens_inner_product(self.h, phi.ptr, dfdu.ptr))
+
+    def newton_solve(self, force, nsteps=5, max_iters=20, rtol=1e-10, u0=None):
+        """Load-stepped Newton on the nonlinear model: returns (u device
+        array [nbatch*6n], Newton iterations, relative residuals [nbatch])."""
+        nb, n = self.nbatch, 6 * self.nnodes
+        f = force if isinstance(force, DeviceArray) else self.vector(np.broadcast_to(force, (nb, n)))
+        u = self.vector(None if u0 is None else np.broadcast_to(u0, (nb, n)))
+        its, rel = C.c_int(), np.zeros(nb)
+        self._check(self.lib.shellgpu_newton_solve(self.h, f.ptr, nsteps, max_iters, rtol, u.ptr, C.byref(its),
+                                                   _dp(rel)))
+        return u, its.value, rel
 
     def buckling(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12, model=None):
         return Buckling(self, sigma, nev, max_lanczos, tol, model)

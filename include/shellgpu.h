@@ -177,6 +177,21 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
                    double *x, int msub, int nrestart, double rtol, double atol, int *iters,
                    double *resnorm);
 
+/* ---- nonlinear static path ------------------------------------------ */
+/* Load-stepped Newton on the nonlinear model (TACSQuad4NonlinearShell).
+ * The built reference has no nonlinear driver (the unbuilt
+ * archive/TACSContinuation.{h,cpp} is an arc-length method); this solves
+ * R_nl(u) = s f for s = k/nsteps, k = 1..nsteps, each load step by full
+ * Newton from the previous state: R and the tangent by
+ * shellgpu_assemble_jacobian(NONLINEAR), K_t du = R by GMRES preconditioned
+ * by a factored tangent (refactored when GMRES stalls), u -= du.  A load
+ * step ends when ||R - s f|| <= rtol ||f|| for every design (BC rows hold
+ * u - ubar; f is zeroed on BC dofs).  u (device, nbatch*6*nnodes) is the
+ * initial guess and the result; iters = Newton iterations over all steps;
+ * relres (host, nbatch, may be NULL) = final relative residuals. */
+int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, int max_iters,
+                          double rtol, double *u, int *iters, double *relres);
+
 /* ---- linear buckling (TACSLinearBuckling, src/TACSBuckling.h:43-107) ---- */
 /* TACSLinearBuckling::solve(rhs = force, u0 = NULL) (src/TACSBuckling.cpp:
  * 197-278) for every design: K, static path K u = f, G(u), K + sigma G,

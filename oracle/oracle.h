@@ -143,8 +143,7 @@ void or_rcm(int nnodes, const int *rowp, const int *cols, int *perm);
 int or_tridiag_eig(int n, const double *d, const double *e, double *eigs,
                    double *z);
 
-/* Linear buckling (TACSLinearBuckling::solve with u0 = NULL,
- * src/TACSBuckling.cpp:197-278) with shift-invert Lanczos, full MGS in the K
+/* Linear buckling (TACSLinearBuckling::solve, src/TACSBuckling.cpp:197-278) with shift-invert Lanczos, full MGS in the K
  * inner product (SEP::solve, GSEP.cpp:485-599), direct solves. */
 typedef struct {
   int nnodes, nelems;
@@ -160,6 +159,8 @@ typedef struct {
   int max_lanczos; /* Lanczos steps (the reference stops at num_eigvals) */
   int nev;
   unsigned long long seed; /* start vector */
+  const double *u0; /* path (TACSLinearBuckling::solve with u0 != NULL); NULL:
+                       static solve of K u = f */
 } or_buckling_problem;
 
 /* lambda[nev] (sorted by magnitude ascending, SEP sortEigenvalues

@@ -165,7 +165,7 @@ class BucklingProblem(C.Structure):
     _fields_ = [("nnodes", C.c_int), ("nelems", C.c_int), ("conn", _IP), ("Xpts", _DP), ("thick", _DP),
                 ("temp", _DP), ("mat", C.POINTER(Material)), ("nbc", C.c_int), ("bc_nodes", _IP),
                 ("bc_mask", _IP), ("force", _DP), ("sigma", C.c_double), ("max_lanczos", C.c_int),
-                ("nev", C.c_int), ("seed", C.c_ulonglong)]
+                ("nev", C.c_int), ("seed", C.c_ulonglong), ("u0", _DP)]
 
 
 def tridiag_eig(d, e):
@@ -180,9 +180,11 @@ def tridiag_eig(d, e):
     return eig, z.reshape(n, n)
 
 
-def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60, mat=None, temp=None, seed=12345):
+def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60, mat=None, temp=None, seed=12345,
+                    u0=None):
     """Oracle linear buckling: returns (lambda[nev], phi[nev, 6n], path[6n],
-    resid[nev], nsteps)."""
+    resid[nev], nsteps).  u0 (optional) is the path; otherwise the static
+    solution of K u = f."""
     mat = mat or material()
     conn = np.ascontiguousarray(conn, np.int32)
     Xpts = _f64(Xpts)
@@ -192,8 +194,10 @@ def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60,
     force = _f64(force)
     tp = _f64(temp) if temp is not None else None
     nn = Xpts.size // 3
+    u0 = _f64(u0) if u0 is not None else None
     p = BucklingProblem(nn, conn.size // 4, _i(conn), _d(Xpts), _d(thick), _d(tp) if tp is not None else None,
-                        C.pointer(mat), bn.size, _i(bn), _i(bm), _d(force), sigma, max_lanczos, nev, seed)
+                        C.pointer(mat), bn.size, _i(bn), _i(bm), _d(force), sigma, max_lanczos, nev, seed,
+                        _d(u0) if u0 is not None else None)
     lam = np.zeros(nev)
     phi = np.zeros((nev, 6 * nn))
     path = np.zeros(6 * nn)

@@ -289,12 +289,18 @@ int or_linear_buckling(const or_buckling_problem *p, double *lambda, double *phi
   or_bsr_apply_bcs(nn, rowp, cols, K, p->nbc, p->bc_nodes, p->bc_mask, 1.0);
   or_rcm(nn, rowp, cols, perm);
   or_band *fk = or_band_factor(nn, rowp, cols, K, perm);
-  /* res = R(0) - f with BC rows u - ubar = 0; path = -K^{-1} res */
-  or_assemble_res(OR_LINEAR, p->mat, nn, p->nelems, p->conn, p->Xpts, p->thick, p->temp, NULL, rhs);
-  for (int i = 0; i < n; i++) rhs[i] -= p->force ? p->force[i] : 0.0;
-  or_vec_zero_bcs(rhs, p->nbc, p->bc_nodes, p->bc_mask);
-  or_band_solve(fk, rhs, u);
-  for (int i = 0; i < n; i++) u[i] = -u[i];
+  if (p->u0) {
+    /* given path with the (zero) BC values set */
+    memcpy(u, p->u0, sizeof(double) * (size_t)n);
+    or_vec_zero_bcs(u, p->nbc, p->bc_nodes, p->bc_mask);
+  } else {
+    /* res = R(0) - f with BC rows u - ubar = 0; path = -K^{-1} res */
+    or_assemble_res(OR_LINEAR, p->mat, nn, p->nelems, p->conn, p->Xpts, p->thick, p->temp, NULL, rhs);
+    for (int i = 0; i < n; i++) rhs[i] -= p->force ? p->force[i] : 0.0;
+    or_vec_zero_bcs(rhs, p->nbc, p->bc_nodes, p->bc_mask);
+    or_band_solve(fk, rhs, u);
+    for (int i = 0; i < n; i++) u[i] = -u[i];
+  }
   or_band_free(fk);
   if (path) memcpy(path, u, sizeof(double) * (size_t)n);
 
