@@ -1,20 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X shell-assembly / buckling path.
 
-Workload: the 200 x 400 axially compressed cylinder (R = 1, L = 2, 80,000
-4-node shell elements, 80,400 nodes, 482,400 dofs) with seeded per-element
-thickness U[0.0015, 0.0025] (seed 1234) and a seeded radial imperfection of
-amplitude 0.1 * 0.002 (wavenumbers from seed 1235); E = 70 GPa, nu = 0.3.
+Workload (the line's `value`): assembly of the stiffness matrix K of the
+200 x 400 axially compressed cylinder (R = 1, L = 2, 80,000 4-node shell
+elements, 80,400 nodes, 482,400 dofs) with seeded per-element thickness
+U[0.0015, 0.0025] (seed 1234) and a seeded radial imperfection of amplitude
+0.1 * 0.002 (wavenumbers from seed 1235); E = 70 GPa, nu = 0.3.  One step =
+every element Jacobian of the linear-model shell, scattered into the
+6x6-block BSR matrix, BCs applied.  `value` = element Jacobians per second
+over the timed steps (inputs resident in HBM).
 
-One step = assembly of the stiffness matrix K (every element Jacobian of the
-linear-model shell, scattered into the 6x6-block BSR matrix, BCs applied).
-`value` = element Jacobians per second over the timed steps (inputs resident in
-HBM).  The line also reports the BSR SpMV bandwidth on the assembled matrix,
-the roofline of the dominant kernel (the element-Jacobian kernel, timed with
-HIP events on the library's stream) and a CPU baseline (the oracle's
-restatement of the reference element loop on a bounded sample).
+The same line reports, as separate fields:
+  * the roofline of the element-Jacobian kernel (HIP events on the
+    library's stream) and the BSR SpMV bandwidth on the assembled K;
+  * `buckling_pipeline`: the 200 x 400 analysis of one design -- 5-step
+    Newton on the nonlinear model, linear buckling about the final state
+    (first 5 loads), KS (rho = 100) gradient of the loads with respect to
+    all 80,000 thicknesses and 241,200 coordinates -- seconds per design;
+  * `batched`: the 100 x 200 cylinder with 64 seeded thickness fields
+    (seeds 2000..2063) solved in one batch -- linear static path, buckling
+    (first 5 loads) and the KS thickness gradient -- seconds per design
+    against the same analysis of one design alone;
+  * `accuracy`: the first 5 loads of the 20 x 40 cylinder against the
+    oracle (relative error), and the oracle's time for the same analysis;
+  * `cpu_baseline`: the oracle's element loop on a bounded sample.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+                       [--skip-pipeline] [--skip-batched] [--skip-accuracy]
 """
 import argparse
 import json
@@ -46,6 +58,7 @@ FLOPS_PER_ELEMENT = {
 }
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBPS = 8000.0    # MI355X HBM3E
+KS_RHO = 100.0
 
 
 def spmv_bytes(nnzb, nnodes):
@@ -68,10 +81,122 @@ def profiled_traffic(name):
     return None
 
 
-def build_problem(na=200, nc=400):
-    cyl = Cylinder(na, nc, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
+def make_context(cyl, thick):
+    ctx = sg.Context(0)
+    ctx.set_mesh(cyl.conn, cyl.Xpts)
+    ctx.set_material(70e9, 0.3, 2700.0, 0.0, 10.0)
+    ctx.set_bcs(*cyl.bcs())
+    ctx.set_designs(thick)
+    ctx.set_ordering_seed(cyl.ring_nodes(0))
+    return ctx
+
+
+def ks_of_loads(lam):
+    """KS aggregate (rho = 100) of -lambda / lambda_1 for each design (a
+    smooth minimum of the loads, normalised by the first): value and
+    d KS / d lambda."""
+    ks, coef = [], []
+    for row in lam:
+        f = -row / row[0]
+        v, d = sg.ks_aggregate(f, KS_RHO)
+        ks.append(v)
+        coef.append(-d / row[0])
+    return np.array(ks), np.array(coef)
+
+
+def timed(ctx, fn):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    ctx.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def buckling_pipeline(newton=True, coords=True):
+    """200 x 400: Newton (5 load steps, nonlinear model) + buckling about the
+    final state + KS gradient (thickness and coordinates); one design."""
+    cyl = Cylinder(200, 400, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
     thick = seeded_thickness(cyl.nelems, 1234)
-    return cyl, thick
+    ctx = make_context(cyl, thick)
+    f = cyl.edge_load(1.0)
+    t = {}
+    u = None
+    if newton:
+        (u, its, rel), t["newton_s"] = timed(ctx, lambda: ctx.newton_solve(f, nsteps=5, max_iters=20, rtol=1e-10))
+    B = ctx.buckling(sigma=0.0, nev=5, max_lanczos=150, tol=1e-10, model=sg.NONLINEAR if newton else sg.LINEAR)
+    _, t["buckling_s"] = timed(ctx, lambda: B.solve(force=None if newton else f, u0=u))
+    lam, err = B.eigenvalues()
+    steps = B.info()["steps"]
+    ks, coef = ks_of_loads(lam)
+    (gt, gX), t["gradient_s"] = timed(ctx, lambda: B.eigen_sens(coef, thickness=True, coords=coords))
+    B.destroy()
+    ctx.close()
+    total = sum(t.values())
+    out = {"workload": "cylinder 200x400, 1 design: 5-step Newton (nonlinear model), buckling about the final "
+                       "state (5 loads), KS(rho=100) gradient w.r.t. 80,000 thicknesses"
+                       + (" and 241,200 coordinates" if coords else ""),
+           "s_per_design": total, **t, "lanczos_steps": steps, "lambda": lam[0].tolist(),
+           "max_ritz_residual": float(err.max()), "ks": float(ks[0]),
+           "grad_t_norm": float(np.linalg.norm(gt)), "grad_X_norm": float(np.linalg.norm(gX)) if coords else None}
+    if newton:
+        out["newton_iterations"] = its
+        out["newton_relres"] = float(rel[0])
+    return out
+
+
+def batched_designs(nb=64):
+    """100 x 200, nb thickness fields (seeds 2000..): linear path, buckling
+    (5 loads), KS thickness gradient -- batched, against one design alone."""
+    cyl = Cylinder(100, 200, R=1.0, L=2.0)
+    thick = np.concatenate([seeded_thickness(cyl.nelems, 2000 + d) for d in range(nb)])
+    f = cyl.edge_load(1.0)
+
+    def run(th):
+        ctx = make_context(cyl, th)
+
+        def analysis():
+            B = ctx.buckling(sigma=0.0, nev=5, max_lanczos=100, tol=1e-10)
+            B.solve(force=f)
+            lam = B.eigenvalues()[0]
+            _, coef = ks_of_loads(lam)
+            gt = B.eigen_sens(coef, thickness=True)[0]
+            B.destroy()
+            return lam, gt
+        (lam, gt), sec = timed(ctx, analysis)
+        ctx.close()
+        return lam, gt, sec
+
+    lam1, gt1, s1 = run(thick[:1])
+    lam1b, gt1b, s1b = run(thick[:1])  # warm
+    lamb, gtb, sb = run(thick)
+    single = min(s1, s1b)
+    return {"workload": f"cylinder 100x200, {nb} thickness designs batched: static path, buckling (5 loads), "
+                        "KS(rho=100) thickness gradient",
+            "designs": nb, "batched_s": sb, "batched_s_per_design": sb / nb, "single_s_per_design": single,
+            "speedup_per_design": single / (sb / nb),
+            "batched_vs_single_lambda_max_rel_diff": float(np.abs(lamb[0] - lam1[0]).max() / abs(lam1[0]).max()),
+            "batched_vs_single_grad_max_rel_diff": float(np.abs(gtb[0] - gt1[0]).max() / np.abs(gt1[0]).max())}
+
+
+def accuracy_vs_oracle():
+    """First 5 loads of the 20 x 40 cylinder (t = 0.002) against the oracle;
+    both run the same shift-invert Lanczos problem."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    cyl = Cylinder(20, 40)
+    thick = np.full((1, cyl.nelems), 0.002)
+    sigma = 0.9 * cyl.classical_load()
+    f = cyl.edge_load(1.0)
+    ctx = make_context(cyl, thick)
+    out, gpu_s = timed(ctx, lambda: ctx.linear_buckling(sigma, f, nev=8, max_lanczos=120, tol=1e-12))
+    ctx.close()
+    t0 = time.perf_counter()
+    lam = O.linear_buckling(cyl.Xpts, cyl.conn, thick[0], cyl.bcs(), f, sigma, nev=8, max_lanczos=120)[0]
+    cpu_s = time.perf_counter() - t0
+    g = out["lambda"][0]
+    return {"workload": "cylinder 20x40, t = 0.002: static path + buckling, first 5 loads",
+            "lambda_max_rel_err": float(np.abs(g[:5] - lam[:5]).max() / np.abs(lam[:5]).max()),
+            "gpu_s": gpu_s, "oracle_cpu_s": cpu_s, "oracle_cores": 1}
 
 
 def cpu_baseline(cyl, thick, seconds):
@@ -107,6 +232,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--skip-pipeline", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-accuracy", action="store_true")
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", choices=["assembly", "calibrate"], default="assembly",
                     help="calibrate: only run a streaming kernel of known traffic (Y += a X over "
                          "3 x 36 x nnzb doubles of the workload's matrices) for counter calibration")
@@ -116,12 +245,9 @@ def main():
                                    "design note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    cyl, thick = build_problem()
-    ctx = sg.Context(0)
-    ctx.set_mesh(cyl.conn, cyl.Xpts)
-    ctx.set_material(70e9, 0.3, 2700.0, 0.0, 10.0)
-    ctx.set_bcs(*cyl.bcs())
-    ctx.set_designs(thick)
+    cyl = Cylinder(200, 400, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
+    thick = seeded_thickness(cyl.nelems, 1234)
+    ctx = make_context(cyl, thick)
     K = ctx.matrix()
 
     if args.mode == "calibrate":
@@ -163,6 +289,11 @@ def main():
     _, _, spmv_ms, spmv_n = ctx.get_timing()
     ctx.set_timing(False)
     spmv_gbps = spmv_bytes(ctx.nnzb, ctx.nnodes) * spmv_n / (spmv_ms * 1e-3) / 1e9
+    nnzb, nnodes, ncolors = ctx.nnzb, ctx.nnodes, ctx.ncolors
+    K.destroy()
+    x.free()
+    y.free()
+    ctx.close()
 
     out = {
         "metric": METRIC,
@@ -178,7 +309,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded cylinder mesh, thickness and imperfection)",
         "config": {"workload": "cylinder 200x400 (80000 quads, 482400 dof): K assembly", "nelems": cyl.nelems,
-                   "nnodes": cyl.nnodes, "nnzb": ctx.nnzb, "colors": ctx.ncolors, "parallelism": "single GPU"},
+                   "nnodes": nnodes, "nnzb": nnzb, "colors": ncolors, "parallelism": "single GPU"},
         "bsr_spmv_GBps": spmv_gbps,
         "roofline": {
             "bound": "mfma",
@@ -192,10 +323,14 @@ def main():
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
         },
     }
+    if not args.skip_pipeline:
+        out["buckling_pipeline"] = buckling_pipeline()
+    if not args.skip_batched:
+        out["batched"] = batched_designs(args.batch)
+    if not args.skip_accuracy:
+        out["accuracy"] = accuracy_vs_oracle()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cyl, thick, args.cpu_seconds)
-    K.destroy()
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 
