@@ -901,15 +901,22 @@ struct shellgpu_buckling {
   // (nev x ns, ns = Lanczos basis size of that design)
   std::vector<std::vector<double>> lam, err, ritz;
   std::vector<int> ns;
+  // K, G, A and F belong to the object unless the caller supplied them
+  bool own_ops = true;
+  // static-path GMRES settings
+  int ksm_m = 10, ksm_restarts = 15;
+  double ksm_rtol = 1e-12, ksm_atol = 1e-30;
 };
 
 namespace {
 
 int buckling_free(shellgpu_buckling *B) {
-  if (B->K) shellgpu_mat_destroy(B->K);
-  if (B->G) shellgpu_mat_destroy(B->G);
-  if (B->A) shellgpu_mat_destroy(B->A);
-  if (B->F) shellgpu_factor_destroy(B->F);
+  if (B->own_ops) {
+    if (B->K) shellgpu_mat_destroy(B->K);
+    if (B->G) shellgpu_mat_destroy(B->G);
+    if (B->A) shellgpu_mat_destroy(B->A);
+    if (B->F) shellgpu_factor_destroy(B->F);
+  }
   double *bufs[] = {B->u, B->res, B->f0, B->Q, B->KQ, B->hv, B->dsig, B->ph, B->gv, B->psi, B->dcoef};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
@@ -990,6 +997,41 @@ void shellgpu_buckling_destroy(shellgpu_buckling *B) {
   delete B;
 }
 
+int shellgpu_buckling_set_operators(shellgpu_buckling *B, shellgpu_mat *K, shellgpu_mat *G, shellgpu_mat *A,
+                                    shellgpu_factor *F) {
+  if (!B) return 1;
+  shellgpu_ctx *ctx = B->ctx;
+  if (!K || !G || !A || !F) return ferr(ctx, 1, "K, G, A and the factor are required");
+  if (K == G || K == A || G == A) return ferr(ctx, 1, "K, G and A must be distinct matrices");
+  const int nb = ctx->nbatch;
+  if (K->nbatch != nb || G->nbatch != nb || A->nbatch != nb)
+    return ferr(ctx, 1, "matrix batch size does not match the context's designs");
+  if (B->own_ops) {
+    shellgpu_mat_destroy(B->K);
+    shellgpu_mat_destroy(B->G);
+    shellgpu_mat_destroy(B->A);
+    shellgpu_factor_destroy(B->F);
+  }
+  B->K = K;
+  B->G = G;
+  B->A = A;
+  B->F = F;
+  B->own_ops = false;
+  B->solved = false;
+  return 0;
+}
+
+int shellgpu_buckling_set_ksm(shellgpu_buckling *B, int msub, int nrestart, double rtol, double atol) {
+  if (!B) return 1;
+  if (msub <= 0 || nrestart < 0 || !(rtol >= 0.0) || !(atol >= 0.0))
+    return ferr(B->ctx, 1, "invalid GMRES settings");
+  B->ksm_m = msub;
+  B->ksm_restarts = nrestart;
+  B->ksm_rtol = rtol;
+  B->ksm_atol = atol;
+  return 0;
+}
+
 int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const double *u0) {
   if (!B) return 1;
   shellgpu_ctx *ctx = B->ctx;
@@ -1019,7 +1061,9 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
     }
     int its = 0;
     double rn = 0.0;
-    if ((rc = shellgpu_gmres(ctx, B->K, B->F, B->res, B->u, 10, 15, 1e-12, 1e-30, &its, &rn))) return rc;
+    if ((rc = shellgpu_gmres(ctx, B->K, B->F, B->res, B->u, B->ksm_m, B->ksm_restarts, B->ksm_rtol, B->ksm_atol,
+                             &its, &rn)))
+      return rc;
     std::vector<double> m1(nb, -1.0);
     if ((rc = vo.scale(B->u, m1))) return rc;
   }
@@ -1323,6 +1367,33 @@ out:
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   return rc;
+}
+
+int shellgpu_vec_axpby(shellgpu_ctx *ctx, size_t n, double alpha, double beta, const double *x, double *y) {
+  if (!ctx || !x || !y || n > 0x7fffffff) return ferr(ctx, 1, "invalid vector arguments");
+  int rc = ensure_blas(ctx);
+  if (rc) return rc;
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  if (beta != 1.0) BLASCALL(ctx, rocblas_dscal(h, (rocblas_int)n, &beta, y, 1));
+  BLASCALL(ctx, rocblas_daxpy(h, (rocblas_int)n, &alpha, x, 1, y, 1));
+  return 0;
+}
+
+int shellgpu_vec_scale(shellgpu_ctx *ctx, size_t n, double alpha, double *x) {
+  if (!ctx || !x || n > 0x7fffffff) return ferr(ctx, 1, "invalid vector arguments");
+  int rc = ensure_blas(ctx);
+  if (rc) return rc;
+  BLASCALL(ctx, rocblas_dscal((rocblas_handle)ctx->blas, (rocblas_int)n, &alpha, x, 1));
+  return 0;
+}
+
+int shellgpu_vec_dot(shellgpu_ctx *ctx, size_t n, const double *x, const double *y, double *out) {
+  if (!ctx || !x || !y || !out || n > 0x7fffffff) return ferr(ctx, 1, "invalid vector arguments");
+  int rc = ensure_blas(ctx);
+  if (rc) return rc;
+  // host pointer mode: the call returns when the result is in *out
+  BLASCALL(ctx, rocblas_ddot((rocblas_handle)ctx->blas, (rocblas_int)n, x, 1, y, 1, out));
+  return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,518 @@
+// TACSShellGPU.cpp -- the TACS-shaped host classes over the shellgpu C ABI
+// (elements, vectors, matrices, preconditioner, GMRES, assembler, buckling).
+// The BDF reader is in TACSMeshLoader.cpp.
+#include "TACSShellGPU.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace shellgpu {
+
+namespace {
+void report(shellgpu_ctx *ctx, int rc, const char *what) {
+  if (rc) fprintf(stderr, "shellgpu: %s failed (%d): %s\n", what, rc, shellgpu_last_error(ctx));
+}
+}  // namespace
+
+// ---- constitutive and elements --------------------------------------------
+
+TACSIsoShellConstitutive::TACSIsoShellConstitutive(TACSMaterialProperties *props, TacsScalar t, int tNum,
+                                                   TacsScalar, TacsScalar)
+    : props(props), t(t), tNum(tNum) {
+  props->incref();
+}
+TACSIsoShellConstitutive::~TACSIsoShellConstitutive() { props->decref(); }
+
+TACSElement::TACSElement(TACSShellConstitutive *con, int nonlinear) : con(con), nonlinear(nonlinear), temperature(0.0) {
+  con->incref();
+}
+TACSElement::~TACSElement() { con->decref(); }
+
+TACSQuad4Shell::TACSQuad4Shell(TACSShellTransform *transform, TACSShellConstitutive *con)
+    : TACSElement(con, 0), transform(transform) {
+  transform->incref();
+}
+TACSQuad4Shell::~TACSQuad4Shell() { transform->decref(); }
+
+TACSQuad4NonlinearShell::TACSQuad4NonlinearShell(TACSShellTransform *transform, TACSShellConstitutive *con)
+    : TACSElement(con, 1), transform(transform) {
+  transform->incref();
+}
+TACSQuad4NonlinearShell::~TACSQuad4NonlinearShell() { transform->decref(); }
+
+TACSElement *TacsCreateShellByName(const char *name, TACSShellTransform *transform, TACSShellConstitutive *con) {
+  if (!name) return NULL;
+  if (strcmp(name, "CQUAD4") == 0 || strcmp(name, "CQUADR") == 0) return new TACSQuad4Shell(transform, con);
+  return NULL;
+}
+
+// ---- vectors ----------------------------------------------------------------
+
+TACSBVec::TACSBVec(TACSAssembler *assembler, size_t size)
+    : assembler(assembler), ctx(assembler->getContext()), size(size), data(NULL) {
+  void *p = NULL;
+  int rc = shellgpu_malloc(ctx, &p, sizeof(double) * size);
+  report(ctx, rc, "TACSBVec allocation");
+  data = (double *)p;
+  zeroEntries();
+}
+TACSBVec::~TACSBVec() {
+  if (data) shellgpu_free(ctx, data);
+}
+int TACSBVec::getValues(std::vector<TacsScalar> &vals) {
+  vals.resize(size);
+  int rc = shellgpu_memcpy_d2h(ctx, vals.data(), data, sizeof(double) * size);
+  report(ctx, rc, "TACSBVec::getValues");
+  return rc;
+}
+int TACSBVec::setValues(const std::vector<TacsScalar> &vals) {
+  if (vals.size() != size) {
+    fprintf(stderr, "TACSBVec::setValues: size mismatch\n");
+    return 1;
+  }
+  int rc = shellgpu_memcpy_h2d(ctx, data, vals.data(), sizeof(double) * size);
+  report(ctx, rc, "TACSBVec::setValues");
+  return rc;
+}
+TacsScalar TACSBVec::dot(TACSBVec *x) {
+  double out = 0.0;
+  report(ctx, shellgpu_vec_dot(ctx, size, data, x->data, &out), "TACSBVec::dot");
+  return out;
+}
+TacsScalar TACSBVec::norm() { return std::sqrt(dot(this)); }
+void TACSBVec::axpy(TacsScalar alpha, TACSBVec *x) {
+  report(ctx, shellgpu_vec_axpby(ctx, size, alpha, 1.0, x->data, data), "TACSBVec::axpy");
+}
+void TACSBVec::axpby(TacsScalar alpha, TacsScalar beta, TACSBVec *x) {
+  report(ctx, shellgpu_vec_axpby(ctx, size, alpha, beta, x->data, data), "TACSBVec::axpby");
+}
+void TACSBVec::scale(TacsScalar alpha) { report(ctx, shellgpu_vec_scale(ctx, size, alpha, data), "TACSBVec::scale"); }
+void TACSBVec::copyValues(TACSBVec *x) {
+  report(ctx, shellgpu_memcpy_d2d(ctx, data, x->data, sizeof(double) * size), "TACSBVec::copyValues");
+}
+void TACSBVec::zeroEntries() { report(ctx, shellgpu_memset(ctx, data, 0, sizeof(double) * size), "TACSBVec::zero"); }
+void TACSBVec::setRand(double lower, double upper) {
+  // splitmix64 stream with a fixed seed: the same values on every call
+  std::vector<TacsScalar> v(size);
+  unsigned long long z = 0x2545F4914F6CDD1DULL;
+  for (size_t i = 0; i < size; i++) {
+    z += 0x9E3779B97F4A7C15ULL;
+    unsigned long long w = z;
+    w = (w ^ (w >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    w = (w ^ (w >> 27)) * 0x94D049BB133111EBULL;
+    w ^= w >> 31;
+    v[i] = lower + (upper - lower) * ((double)(w >> 11) / 9007199254740992.0);
+  }
+  setValues(v);
+}
+void TACSBVec::applyBCs() { report(ctx, shellgpu_vec_zero_bcs(ctx, data), "TACSBVec::applyBCs"); }
+void TACSBVec::setBCs() { report(ctx, shellgpu_vec_set_bcs(ctx, data), "TACSBVec::setBCs"); }
+
+// ---- matrices, preconditioner, GMRES ----------------------------------------
+
+TACSMat::TACSMat(TACSAssembler *assembler) : assembler(assembler), mat(NULL) {
+  report(assembler->getContext(), shellgpu_mat_create(assembler->getContext(), 1, &mat), "TACSMat allocation");
+}
+TACSMat::~TACSMat() {
+  if (mat) shellgpu_mat_destroy(mat);
+}
+void TACSMat::mult(TACSBVec *x, TACSBVec *y) {
+  report(assembler->getContext(), shellgpu_mat_mult(mat, x->getDeviceArray(), y->getDeviceArray()), "TACSMat::mult");
+}
+void TACSMat::zeroEntries() { report(assembler->getContext(), shellgpu_mat_zero(mat), "TACSMat::zeroEntries"); }
+void TACSMat::copyValues(TACSMat *m) {
+  report(assembler->getContext(), shellgpu_mat_copy(mat, m->mat), "TACSMat::copyValues");
+}
+void TACSMat::axpy(TacsScalar alpha, TACSMat *m) {
+  report(assembler->getContext(), shellgpu_mat_axpy(mat, alpha, m->mat), "TACSMat::axpy");
+}
+void TACSMat::applyBCs() { report(assembler->getContext(), shellgpu_mat_apply_bcs(mat, 1.0), "TACSMat::applyBCs"); }
+
+TACSSchurPc::TACSSchurPc(TACSMat *mat, int, double, int) : mat(mat), fac(NULL) {
+  mat->incref();
+  shellgpu_ctx *ctx = mat->getAssembler()->getContext();
+  report(ctx, shellgpu_factor_create(ctx, &fac), "TACSSchurPc allocation");
+}
+TACSSchurPc::~TACSSchurPc() {
+  if (fac) shellgpu_factor_destroy(fac);
+  mat->decref();
+}
+void TACSSchurPc::factor() {
+  report(mat->getAssembler()->getContext(), shellgpu_factor_compute(fac, mat->getHandle()), "TACSSchurPc::factor");
+}
+void TACSSchurPc::applyFactor(TACSBVec *x, TACSBVec *y) {
+  report(mat->getAssembler()->getContext(), shellgpu_factor_apply(fac, x->getDeviceArray(), y->getDeviceArray()),
+         "TACSSchurPc::applyFactor");
+}
+
+GMRES::GMRES(TACSMat *mat, TACSPc *pc, int m, int nrestart, int)
+    : mat(mat), pc(pc), m(m), nrestart(nrestart), rtol(1e-8), atol(1e-30), last_iters(0) {
+  mat->incref();
+  if (pc) pc->incref();
+}
+GMRES::GMRES(TACSMat *mat, int m, int nrestart) : GMRES(mat, NULL, m, nrestart, 0) {}
+GMRES::~GMRES() {
+  mat->decref();
+  if (pc) pc->decref();
+}
+void GMRES::setTolerances(double r, double a) {
+  rtol = r;
+  atol = a;
+}
+void GMRES::getOperators(TACSMat **m_, TACSPc **pc_) {
+  if (m_) *m_ = mat;
+  if (pc_) *pc_ = pc;
+}
+int GMRES::solve(TACSBVec *b, TACSBVec *x, int zero_guess) {
+  shellgpu_ctx *ctx = mat->getAssembler()->getContext();
+  int its = 0;
+  double rn = 0.0;
+  int rc = 0;
+  if (zero_guess) {
+    rc = shellgpu_gmres(ctx, mat->getHandle(), pc ? pc->getHandle() : NULL, b->getDeviceArray(),
+                        x->getDeviceArray(), m, nrestart, rtol, atol, &its, &rn);
+  } else {
+    // correction form: A dx = b - A x, x += dx
+    TACSBVec r(mat->getAssembler(), b->getSize()), dx(mat->getAssembler(), b->getSize());
+    mat->mult(x, &r);
+    r.axpby(1.0, -1.0, b);
+    rc = shellgpu_gmres(ctx, mat->getHandle(), pc ? pc->getHandle() : NULL, r.getDeviceArray(),
+                        dx.getDeviceArray(), m, nrestart, rtol, atol, &its, &rn);
+    if (!rc) x->axpy(1.0, &dx);
+  }
+  report(ctx, rc, "GMRES::solve");
+  last_iters = its;
+  return (!rc && rn <= rtol) ? 1 : 0;
+}
+
+// ---- assembler ----------------------------------------------------------------
+
+TACSAssembler::TACSAssembler(int num_nodes, int num_elements, const int *conn, const TacsScalar *Xpts, int device)
+    : ctx(NULL), num_nodes(num_nodes), num_elements(num_elements), model(0), elements(num_elements, NULL),
+      thick(num_elements, 1.0), vars(NULL), designs_dirty(true) {
+  int rc = shellgpu_create(&ctx, device);
+  if (rc) {
+    fprintf(stderr, "TACSAssembler: no GPU context (%d)\n", rc);
+    return;
+  }
+  report(ctx, shellgpu_set_mesh(ctx, num_nodes, num_elements, conn, Xpts), "TACSAssembler::setMesh");
+  vars = createVec();
+  vars->incref();
+}
+TACSAssembler::~TACSAssembler() {
+  if (vars) vars->decref();
+  for (TACSElement *e : elements)
+    if (e) e->decref();
+  if (ctx) shellgpu_destroy(ctx);
+}
+
+int TACSAssembler::check(int rc, const char *what) {
+  report(ctx, rc, what);
+  return rc;
+}
+
+int TACSAssembler::setElements(TACSElement **elems) {
+  int nl = -1;
+  TACSMaterialProperties *mat = NULL;
+  for (int e = 0; e < num_elements; e++) {
+    if (!elems[e]) {
+      fprintf(stderr, "TACSAssembler::setElements: element %d is NULL\n", e);
+      return 1;
+    }
+    if (nl < 0) nl = elems[e]->isNonlinear();
+    if (nl != elems[e]->isNonlinear()) {
+      fprintf(stderr, "TACSAssembler::setElements: linear and nonlinear shells cannot be mixed\n");
+      return 1;
+    }
+    TACSMaterialProperties *m = elems[e]->getConstitutive()->getMaterialProperties();
+    if (!mat) mat = m;
+    if (m->E != mat->E || m->nu != mat->nu || m->rho != mat->rho || m->alpha != mat->alpha) {
+      fprintf(stderr, "TACSAssembler::setElements: all elements must share one isotropic material\n");
+      return 1;
+    }
+  }
+  for (int e = 0; e < num_elements; e++) {
+    elems[e]->incref();
+    if (elements[e]) elements[e]->decref();
+    elements[e] = elems[e];
+    thick[e] = elems[e]->getConstitutive()->getThickness();
+  }
+  model = nl > 0 ? 1 : 0;
+  if (mat && check(shellgpu_set_material(ctx, mat->E, mat->nu, mat->rho, mat->alpha, 10.0), "setMaterial")) return 1;
+  designs_dirty = true;
+  return syncDesigns();
+}
+
+int TACSAssembler::addBCs(int nbcs, const int *nodes, const int *masks, const TacsScalar *vals) {
+  return check(shellgpu_set_bcs(ctx, nbcs, nodes, masks, vals), "TACSAssembler::addBCs");
+}
+
+void TACSAssembler::setOrderingSeed(int n, const int *nodes) {
+  check(shellgpu_set_ordering_seed(ctx, n, nodes), "TACSAssembler::setOrderingSeed");
+}
+
+int TACSAssembler::syncDesigns() {
+  // temperatures live on the element objects (TACSQuad4Shell::setTemperature)
+  std::vector<TacsScalar> temp(num_elements, 0.0);
+  bool any = false;
+  for (int e = 0; e < num_elements; e++) {
+    if (elements[e]) temp[e] = elements[e]->getTemperature();
+    any = any || temp[e] != 0.0;
+  }
+  if (!designs_dirty && thick == up_thick && temp == up_temp) return 0;
+  designs_dirty = false;
+  up_thick = thick;
+  up_temp = temp;
+  return check(shellgpu_set_designs(ctx, 1, thick.data(), any ? temp.data() : NULL), "TACSAssembler::setDesigns");
+}
+
+TACSBVec *TACSAssembler::createVec() { return new TACSBVec(this, 6 * (size_t)num_nodes); }
+TACSBVec *TACSAssembler::createNodeVec() { return new TACSBVec(this, 3 * (size_t)num_nodes); }
+TACSBVec *TACSAssembler::createDesignVec() { return new TACSBVec(this, (size_t)num_elements); }
+TACSSchurMat *TACSAssembler::createSchurMat() { return new TACSSchurMat(this); }
+
+void TACSAssembler::getDesignVars(TACSBVec *x) { x->setValues(thick); }
+void TACSAssembler::setDesignVars(TACSBVec *x) {
+  x->getValues(thick);
+  syncDesigns();
+}
+void TACSAssembler::zeroVariables() { vars->zeroEntries(); }
+void TACSAssembler::setVariables(TACSBVec *u) { vars->copyValues(u); }
+void TACSAssembler::getVariables(TACSBVec *u) { u->copyValues(vars); }
+void TACSAssembler::applyBCs(TACSBVec *vec) { vec->applyBCs(); }
+void TACSAssembler::applyBCs(TACSMat *mat) { mat->applyBCs(); }
+void TACSAssembler::setBCs(TACSBVec *vec) { vec->setBCs(); }
+
+void TACSAssembler::assembleRes(TACSBVec *residual) {
+  syncDesigns();
+  check(shellgpu_assemble_res(ctx, model, vars->getDeviceArray(), residual->getDeviceArray()),
+        "TACSAssembler::assembleRes");
+}
+
+void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, TACSBVec *residual,
+                                     TACSMat *A) {
+  if (beta != 0.0 || gamma != 0.0) {
+    fprintf(stderr, "TACSAssembler::assembleJacobian: only static Jacobians (beta = gamma = 0) are available\n");
+    return;
+  }
+  syncDesigns();
+  if (check(shellgpu_assemble_jacobian(ctx, model, vars->getDeviceArray(),
+                                       residual ? residual->getDeviceArray() : NULL, A->getHandle()),
+            "TACSAssembler::assembleJacobian"))
+    return;
+  if (alpha != 1.0) {
+    int nnodes, nelems, nnzb, ncolors, nbatch;
+    shellgpu_get_sizes(ctx, &nnodes, &nelems, &nnzb, &ncolors, &nbatch);
+    check(shellgpu_vec_scale(ctx, 36 * (size_t)nnzb, alpha, shellgpu_mat_values(A->getHandle())), "scale");
+    A->applyBCs();
+  }
+}
+
+void TACSAssembler::assembleMatType(ElementMatrixType matType, TACSMat *A) {
+  syncDesigns();
+  int kind;
+  if (matType == TACS_GEOMETRIC_STIFFNESS_MATRIX) {
+    kind = SHELLGPU_GEOMETRIC;
+  } else {
+    // getMatType(STIFFNESS) is the element's own Jacobian at the variables
+    kind = model ? SHELLGPU_TANGENT_NONLINEAR : SHELLGPU_STIFFNESS;
+  }
+  check(shellgpu_assemble_mattype(ctx, kind, vars->getDeviceArray(), A->getHandle()), "TACSAssembler::assembleMatType");
+}
+
+void TACSAssembler::addAdjointResProducts(TacsScalar scale, int n, TACSBVec **adjoint, TACSBVec **dfdx) {
+  syncDesigns();
+  for (int i = 0; i < n; i++)
+    check(shellgpu_add_adj_res_product(ctx, model, scale, adjoint[i]->getDeviceArray(), vars->getDeviceArray(),
+                                       dfdx[i]->getDeviceArray()),
+          "TACSAssembler::addAdjointResProducts");
+}
+
+void TACSAssembler::addAdjointResXptSensProducts(TacsScalar scale, int n, TACSBVec **adjoint, TACSBVec **dfdX) {
+  syncDesigns();
+  for (int i = 0; i < n; i++)
+    check(shellgpu_add_adj_res_xpt_product(ctx, model, scale, adjoint[i]->getDeviceArray(), vars->getDeviceArray(),
+                                           dfdX[i]->getDeviceArray()),
+          "TACSAssembler::addAdjointResXptSensProducts");
+}
+
+void TACSAssembler::addMatDVSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
+                                             TACSBVec *phi, TACSBVec *dfdx) {
+  if (psi != phi) {
+    fprintf(stderr, "TACSAssembler::addMatDVSensInnerProduct: psi and phi must be the same vector\n");
+    return;
+  }
+  syncDesigns();
+  const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC : SHELLGPU_STIFFNESS;
+  check(shellgpu_add_mat_dv_sens_inner_product(ctx, kind, scale, phi->getDeviceArray(), vars->getDeviceArray(),
+                                               dfdx->getDeviceArray()),
+        "TACSAssembler::addMatDVSensInnerProduct");
+}
+
+void TACSAssembler::addMatXptSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
+                                              TACSBVec *phi, TACSBVec *dfdX) {
+  if (psi != phi) {
+    fprintf(stderr, "TACSAssembler::addMatXptSensInnerProduct: psi and phi must be the same vector\n");
+    return;
+  }
+  syncDesigns();
+  const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC : SHELLGPU_STIFFNESS;
+  check(shellgpu_add_mat_xpt_sens_inner_product(ctx, kind, scale, phi->getDeviceArray(), vars->getDeviceArray(),
+                                                dfdX->getDeviceArray()),
+        "TACSAssembler::addMatXptSensInnerProduct");
+}
+
+void TACSAssembler::evalMatSVSensInnerProduct(ElementMatrixType matType, TACSBVec *psi, TACSBVec *phi,
+                                              TACSBVec *res) {
+  if (matType != TACS_GEOMETRIC_STIFFNESS_MATRIX || psi != phi) {
+    fprintf(stderr, "TACSAssembler::evalMatSVSensInnerProduct: geometric stiffness with psi == phi only\n");
+    return;
+  }
+  syncDesigns();
+  check(shellgpu_eval_mat_sv_sens_inner_product(ctx, phi->getDeviceArray(), res->getDeviceArray()),
+        "TACSAssembler::evalMatSVSensInnerProduct");
+}
+
+// ---- linear buckling ----------------------------------------------------------
+
+TACSLinearBuckling::TACSLinearBuckling(TACSAssembler *assembler, TacsScalar sigma, TACSMat *gmat, TACSMat *kmat,
+                                       TACSMat *aux_mat, TACSKsm *solver, int max_lanczos_vecs, int num_eigvals,
+                                       double eig_tol)
+    : assembler(assembler), gmat(gmat), kmat(kmat), aux_mat(aux_mat), solver(solver), sigma(sigma),
+      max_lanczos_vecs(max_lanczos_vecs), num_eigvals(num_eigvals), eig_tol(eig_tol), buck(NULL) {
+  assembler->incref();
+  gmat->incref();
+  kmat->incref();
+  aux_mat->incref();
+  solver->incref();
+  shellgpu_ctx *ctx = assembler->getContext();
+  TACSMat *smat = NULL;
+  TACSPc *pc = NULL;
+  solver->getOperators(&smat, &pc);
+  if (gmat == kmat || gmat == aux_mat || kmat == aux_mat) {
+    fprintf(stderr, "TACSLinearBuckling: K, G and the auxiliary matrix must be distinct\n");
+    return;
+  }
+  if (smat != aux_mat || !pc || pc->getMat() != aux_mat) {
+    fprintf(stderr, "TACSLinearBuckling: the solver and its preconditioner must be bound to aux_mat\n");
+    return;
+  }
+  int rc = shellgpu_buckling_create(ctx, assembler->getModel(), sigma, num_eigvals, max_lanczos_vecs, eig_tol, &buck);
+  if (!rc)
+    rc = shellgpu_buckling_set_operators(buck, kmat->getHandle(), gmat->getHandle(), aux_mat->getHandle(),
+                                         pc->getHandle());
+  GMRES *g = dynamic_cast<GMRES *>(solver);
+  if (!rc && g)
+    rc = shellgpu_buckling_set_ksm(buck, g->getSubspaceSize(), g->getNumRestarts(), g->getRelTol(), g->getAbsTol());
+  report(ctx, rc, "TACSLinearBuckling");
+}
+
+TACSLinearBuckling::~TACSLinearBuckling() {
+  if (buck) shellgpu_buckling_destroy(buck);
+  solver->decref();
+  aux_mat->decref();
+  kmat->decref();
+  gmat->decref();
+  assembler->decref();
+}
+
+void TACSLinearBuckling::solve(TACSBVec *rhs, TACSBVec *u0, KSMPrint *ksm_print) {
+  if (!buck) return;
+  shellgpu_ctx *ctx = assembler->getContext();
+  assembler->syncDesigns();
+  int rc = shellgpu_buckling_solve(buck, rhs ? rhs->getDeviceArray() : NULL, u0 ? u0->getDeviceArray() : NULL);
+  report(ctx, rc, "TACSLinearBuckling::solve");
+  if (rc) return;
+  // the reference leaves the assembler at the path (setVariables(path))
+  shellgpu_buckling_info(buck, NULL, NULL, assembler->getVariables()->getDeviceArray());
+  if (ksm_print) {
+    int steps = 0;
+    shellgpu_buckling_info(buck, &steps, NULL, NULL);
+    char line[128];
+    snprintf(line, sizeof(line), "Lanczos steps %d\n", steps);
+    ksm_print->print(line);
+  }
+}
+
+TacsScalar TACSLinearBuckling::extractEigenvalue(int n, TacsScalar *error) {
+  double lam = 0.0, err = 0.0;
+  if (buck) report(assembler->getContext(), shellgpu_buckling_extract(buck, n, &lam, NULL, &err), "extractEigenvalue");
+  if (error) *error = err;
+  return lam;
+}
+
+TacsScalar TACSLinearBuckling::extractEigenvector(int n, TACSBVec *ans, TacsScalar *error) {
+  double lam = 0.0, err = 0.0;
+  if (buck)
+    report(assembler->getContext(), shellgpu_buckling_extract(buck, n, &lam, ans->getDeviceArray(), &err),
+           "extractEigenvector");
+  if (error) *error = err;
+  return lam;
+}
+
+void TACSLinearBuckling::evalEigenDVSens(int n, TACSBVec *dfdx) {
+  dfdx->zeroEntries();
+  std::vector<double> coef(num_eigvals, 0.0);
+  if (n < 0 || n >= num_eigvals || !buck) return;
+  coef[n] = 1.0;
+  report(assembler->getContext(),
+         shellgpu_buckling_add_eigen_sens(buck, coef.data(), dfdx->getDeviceArray(), NULL), "evalEigenDVSens");
+}
+
+void TACSLinearBuckling::evalEigenXptSens(int n, TACSBVec *dfdX) {
+  dfdX->zeroEntries();
+  std::vector<double> coef(num_eigvals, 0.0);
+  if (n < 0 || n >= num_eigvals || !buck) return;
+  coef[n] = 1.0;
+  report(assembler->getContext(),
+         shellgpu_buckling_add_eigen_sens(buck, coef.data(), NULL, dfdX->getDeviceArray()), "evalEigenXptSens");
+}
+
+// partial derivatives only, as the reference's addEigenDVSens /
+// addEigenXptSens: -coef (phi^T dK phi + lambda phi^T dG phi) / (phi^T G phi)
+void TACSLinearBuckling::addEigenDVSens(TacsScalar coef, int n, TACSBVec *dfdx) {
+  TACSBVec *phi = assembler->createVec(), *t = assembler->createVec();
+  phi->incref();
+  t->incref();
+  TacsScalar err;
+  TacsScalar eig = extractEigenvector(n, phi, &err);
+  gmat->mult(phi, t);
+  TacsScalar scale = t->dot(phi);
+  assembler->addMatDVSensInnerProduct(-coef / scale, TACS_STIFFNESS_MATRIX, phi, phi, dfdx);
+  assembler->addMatDVSensInnerProduct(-eig * coef / scale, TACS_GEOMETRIC_STIFFNESS_MATRIX, phi, phi, dfdx);
+  t->decref();
+  phi->decref();
+}
+
+void TACSLinearBuckling::addEigenXptSens(TacsScalar coef, int n, TACSBVec *dfdX) {
+  TACSBVec *phi = assembler->createVec(), *t = assembler->createVec();
+  phi->incref();
+  t->incref();
+  TacsScalar err;
+  TacsScalar eig = extractEigenvector(n, phi, &err);
+  gmat->mult(phi, t);
+  TacsScalar scale = t->dot(phi);
+  assembler->addMatXptSensInnerProduct(-coef / scale, TACS_STIFFNESS_MATRIX, phi, phi, dfdX);
+  assembler->addMatXptSensInnerProduct(-eig * coef / scale, TACS_GEOMETRIC_STIFFNESS_MATRIX, phi, phi, dfdX);
+  t->decref();
+  phi->decref();
+}
+
+void TACSLinearBuckling::checkEigenvector(int n) {
+  TACSBVec *phi = assembler->createVec(), *t1 = assembler->createVec(), *t2 = assembler->createVec();
+  phi->incref();
+  t1->incref();
+  t2->incref();
+  TacsScalar err;
+  TacsScalar eig = extractEigenvector(n, phi, &err);
+  kmat->mult(phi, t1);
+  gmat->mult(phi, t2);
+  printf("|K*e| = %15.5e  \n|G*e| = %15.5e \n", t1->norm(), t2->norm());
+  t1->axpy(eig, t2);
+  printf("||K*e + lambda*G*e|| = %15.5e \n", t1->norm());
+  t2->decref();
+  t1->decref();
+  phi->decref();
+}
+
+}  // namespace shellgpu

@@ -1,0 +1,398 @@
+// TACSShellGPU.h -- C++ host classes shaped like the reference's TACS API
+// (TACSAssembler, TACSBVec, TACSMat/TACSSchurMat, TACSPc/TACSSchurPc,
+// GMRES, TACSLinearBuckling, TACSMeshLoader and the shell element / material
+// objects), implemented over the shellgpu C ABI.  Driver code written
+// against the reference (examples/cylinder-buckling/mechBuckling.cpp) ports
+// to these classes call for call; the differences are listed per class.
+//
+// Common conventions of the reference kept here: intrusive reference
+// counts (TACSObject incref/decref, src/utils/TACSObject.h:112-135; new
+// objects start at 0, holders incref their inputs); setup calls return an
+// int (0 = success); run-time failures print to stderr.  All data live on
+// one GPU; a TACSAssembler owns one shellgpu context (one HIP stream).
+#pragma once
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "shellgpu.h"
+
+namespace shellgpu {
+
+typedef double TacsScalar;
+
+// ElementMatrixType (src/elements/TACSElementTypes.h)
+enum ElementMatrixType { TACS_STIFFNESS_MATRIX = 0, TACS_GEOMETRIC_STIFFNESS_MATRIX = 1 };
+
+class TACSObject {
+ public:
+  TACSObject() : ref_count(0) {}
+  virtual ~TACSObject() {}
+  void incref() { ref_count++; }
+  void decref() {
+    if (--ref_count <= 0) delete this;
+  }
+  int refcount() const { return ref_count; }
+
+ private:
+  int ref_count;
+};
+
+// TACSMaterialProperties (src/constitutive/TACSMaterialProperties.h),
+// isotropic constructor
+class TACSMaterialProperties : public TACSObject {
+ public:
+  TACSMaterialProperties(TacsScalar rho, TacsScalar specific_heat, TacsScalar E, TacsScalar nu,
+                         TacsScalar ys, TacsScalar alpha, TacsScalar kappa)
+      : rho(rho), specific_heat(specific_heat), E(E), nu(nu), ys(ys), alpha(alpha), kappa(kappa) {}
+  TacsScalar rho, specific_heat, E, nu, ys, alpha, kappa;
+};
+
+// Shell transforms (src/elements/shell/TACSShellElementTransform.h).  Only
+// the natural transform is implemented on the GPU.
+class TACSShellTransform : public TACSObject {};
+class TACSShellNaturalTransform : public TACSShellTransform {};
+
+// TACSIsoShellConstitutive(props, t, tNum, tlb, tub)
+// (src/constitutive/TACSIsoShellConstitutive.cpp:28-81); k_drill = 10,
+// tOffset = 0
+class TACSShellConstitutive : public TACSObject {
+ public:
+  virtual ~TACSShellConstitutive() {}
+  virtual TACSMaterialProperties *getMaterialProperties() = 0;
+  virtual TacsScalar getThickness() = 0;
+};
+class TACSIsoShellConstitutive : public TACSShellConstitutive {
+ public:
+  TACSIsoShellConstitutive(TACSMaterialProperties *props, TacsScalar t = 1.0, int tNum = -1,
+                           TacsScalar tlb = 0.0, TacsScalar tub = 1.0);
+  ~TACSIsoShellConstitutive();
+  TACSMaterialProperties *getMaterialProperties() { return props; }
+  TacsScalar getThickness() { return t; }
+  int getThicknessNum() { return tNum; }
+
+ private:
+  TACSMaterialProperties *props;
+  TacsScalar t;
+  int tNum;
+};
+
+// Shell elements (src/elements/shell/TACSShellElementDefs.h:12-29): the
+// 4-node MITC director shell with the linearised rotation, linear or
+// nonlinear strain model
+class TACSElement : public TACSObject {
+ public:
+  TACSElement(TACSShellConstitutive *con, int nonlinear);
+  ~TACSElement();
+  void setTemperature(TacsScalar T) { temperature = T; }
+  TacsScalar getTemperature() { return temperature; }
+  TACSShellConstitutive *getConstitutive() { return con; }
+  int isNonlinear() const { return nonlinear; }
+  int getVarsPerNode() { return 6; }
+  int getNumNodes() { return 4; }
+
+ private:
+  TACSShellConstitutive *con;
+  int nonlinear;
+  TacsScalar temperature;
+};
+class TACSQuad4Shell : public TACSElement {
+ public:
+  TACSQuad4Shell(TACSShellTransform *transform, TACSShellConstitutive *con);
+  ~TACSQuad4Shell();
+
+ private:
+  TACSShellTransform *transform;
+};
+class TACSQuad4NonlinearShell : public TACSElement {
+ public:
+  TACSQuad4NonlinearShell(TACSShellTransform *transform, TACSShellConstitutive *con);
+  ~TACSQuad4NonlinearShell();
+
+ private:
+  TACSShellTransform *transform;
+};
+// TacsCreateShellByName (src/elements/shell/TACSShellElementDefs.cpp):
+// "CQUAD4" / "CQUADR" -> TACSQuad4Shell; NULL for element types this build
+// does not implement
+TACSElement *TacsCreateShellByName(const char *name, TACSShellTransform *transform,
+                                   TACSShellConstitutive *con);
+
+class TACSAssembler;
+
+// TACSBVec (src/bpmat/TACSBVec.h): a device vector of vars_per_node = 6
+// values per node (or 3 per node for coordinate vectors, 1 per element for
+// design vectors)
+class TACSBVec : public TACSObject {
+ public:
+  TACSBVec(TACSAssembler *assembler, size_t size);
+  ~TACSBVec();
+  size_t getSize() const { return size; }
+  double *getDeviceArray() { return data; }
+  // host copies of the values
+  int getValues(std::vector<TacsScalar> &vals);
+  int setValues(const std::vector<TacsScalar> &vals);
+  TacsScalar norm();
+  TacsScalar dot(TACSBVec *x);
+  void axpy(TacsScalar alpha, TACSBVec *x);
+  void axpby(TacsScalar alpha, TacsScalar beta, TACSBVec *x);
+  void scale(TacsScalar alpha);
+  void copyValues(TACSBVec *x);
+  void zeroEntries();
+  // uniform values in [lower, upper] from a fixed host generator
+  // (TACSBVec::setRand, src/bpmat/TACSBVec.cpp:462-491)
+  void setRand(double lower = -1.0, double upper = 1.0);
+  // zero the BC entries / set them to the BC values (TACSBVec.cpp:546-625)
+  void applyBCs();
+  void setBCs();
+
+ private:
+  TACSAssembler *assembler;
+  shellgpu_ctx *ctx;
+  size_t size;
+  double *data;
+};
+
+// TACSMat (src/bpmat/KSM.h:146-208) over the shared 6x6-block BSR pattern;
+// TACSSchurMat is the name the examples use for the assembler's matrix
+class TACSMat : public TACSObject {
+ public:
+  explicit TACSMat(TACSAssembler *assembler);
+  ~TACSMat();
+  void mult(TACSBVec *x, TACSBVec *y);
+  void zeroEntries();
+  void copyValues(TACSMat *mat);
+  void axpy(TacsScalar alpha, TACSMat *mat);
+  void applyBCs();
+  shellgpu_mat *getHandle() { return mat; }
+  TACSAssembler *getAssembler() { return assembler; }
+
+ private:
+  TACSAssembler *assembler;
+  shellgpu_mat *mat;
+};
+class TACSSchurMat : public TACSMat {
+ public:
+  explicit TACSSchurMat(TACSAssembler *assembler) : TACSMat(assembler) {}
+};
+
+// TACSPc (src/bpmat/KSM.h:209-230).  TACSSchurPc(mat, levFill, fill,
+// reorder) (src/bpmat/TACSSchurMat.h:59-214) factors the matrix it is
+// bound to; here that factorisation is the level-set block Cholesky of the
+// shellgpu library whatever levFill says (a complete factorisation, which
+// is what levFill = 1e6 gives in the reference).
+class TACSPc : public TACSObject {
+ public:
+  virtual ~TACSPc() {}
+  virtual void factor() = 0;
+  virtual void applyFactor(TACSBVec *x, TACSBVec *y) = 0;
+  virtual TACSMat *getMat() = 0;
+  virtual shellgpu_factor *getHandle() = 0;
+};
+class TACSSchurPc : public TACSPc {
+ public:
+  TACSSchurPc(TACSMat *mat, int levFill = 1000000, double fill = 10.0, int reorder_schur = 1);
+  ~TACSSchurPc();
+  void factor();
+  void applyFactor(TACSBVec *x, TACSBVec *y);
+  TACSMat *getMat() { return mat; }
+  shellgpu_factor *getHandle() { return fac; }
+
+ private:
+  TACSMat *mat;
+  shellgpu_factor *fac;
+};
+
+// KSMPrint (src/bpmat/KSM.h:232-262)
+class KSMPrint : public TACSObject {
+ public:
+  virtual ~KSMPrint() {}
+  virtual void print(const char *cstr) = 0;
+};
+class KSMPrintStdout : public KSMPrint {
+ public:
+  KSMPrintStdout(const char *descript, int rank, int freq) : descript(descript), rank(rank), freq(freq) {}
+  void print(const char *cstr) {
+    if (rank == 0) printf("%s: %s", descript.c_str(), cstr);
+  }
+
+ private:
+  std::string descript;
+  int rank, freq;
+};
+
+// TACSKsm / GMRES (src/bpmat/KSM.h:264-278, KSM.cpp:547-955):
+// right-preconditioned restarted GMRES; solve returns 1 when converged
+class TACSKsm : public TACSObject {
+ public:
+  virtual ~TACSKsm() {}
+  virtual int solve(TACSBVec *b, TACSBVec *x, int zero_guess = 1) = 0;
+  virtual void setTolerances(double rtol, double atol) = 0;
+  virtual void getOperators(TACSMat **mat, TACSPc **pc) = 0;
+  virtual void setMonitor(KSMPrint *print) {}
+};
+class GMRES : public TACSKsm {
+ public:
+  GMRES(TACSMat *mat, TACSPc *pc, int m, int nrestart, int isFlexible);
+  GMRES(TACSMat *mat, int m, int nrestart);
+  ~GMRES();
+  int solve(TACSBVec *b, TACSBVec *x, int zero_guess = 1);
+  void setTolerances(double rtol, double atol);
+  void getOperators(TACSMat **mat, TACSPc **pc);
+  int getSubspaceSize() const { return m; }
+  int getNumRestarts() const { return nrestart; }
+  double getRelTol() const { return rtol; }
+  double getAbsTol() const { return atol; }
+  int getIterCount() const { return last_iters; }
+
+ private:
+  TACSMat *mat;
+  TACSPc *pc;
+  int m, nrestart;
+  double rtol, atol;
+  int last_iters;
+};
+
+// TACSAssembler (src/TACSAssembler.h:59-431), one rank, the GPU context
+// behind it.  Per-element thickness is the design vector (one value per
+// element, TACSIsoShellConstitutive with tNum = element index); elements
+// are set per component by TACSMeshLoader (or setElements).
+class TACSAssembler : public TACSObject {
+ public:
+  // nodes Xpts[3*num_nodes] and 4-node connectivity conn[4*num_elements]
+  // in the element node order of the shell basis (tensor order)
+  TACSAssembler(int num_nodes, int num_elements, const int *conn, const TacsScalar *Xpts,
+                int device = 0);
+  ~TACSAssembler();
+
+  int getNumNodes() const { return num_nodes; }
+  int getNumElements() const { return num_elements; }
+  int getVarsPerNode() const { return 6; }
+  shellgpu_ctx *getContext() { return ctx; }
+
+  // one element object per element (the same object may be shared)
+  int setElements(TACSElement **elements);
+  TACSElement *getElement(int elem) { return elements[elem]; }
+  // boundary conditions: node, bitmask of dofs, values[6] per node
+  int addBCs(int nbcs, const int *nodes, const int *masks, const TacsScalar *vals);
+
+  TACSBVec *createVec();
+  TACSBVec *createNodeVec();
+  TACSBVec *createDesignVec();
+  TACSSchurMat *createSchurMat();
+  TACSMat *createMat() { return createSchurMat(); }
+
+  void getDesignVars(TACSBVec *x);
+  void setDesignVars(TACSBVec *x);
+  void zeroVariables();
+  void setVariables(TACSBVec *u);
+  void getVariables(TACSBVec *u);
+  TACSBVec *getVariables() { return vars; }
+  void applyBCs(TACSBVec *vec);
+  void applyBCs(TACSMat *mat);
+  void setBCs(TACSBVec *vec);
+
+  // element loops at the current variables (TACSAssembler.cpp:4000-4250)
+  void assembleRes(TACSBVec *residual);
+  void assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, TACSBVec *residual,
+                        TACSMat *A);
+  void assembleMatType(ElementMatrixType matType, TACSMat *A);
+
+  // sensitivities (the reference's archived assembler API)
+  void addAdjointResProducts(TacsScalar scale, int num_adjoints, TACSBVec **adjoint,
+                             TACSBVec **dfdx);
+  void addAdjointResXptSensProducts(TacsScalar scale, int num_adjoints, TACSBVec **adjoint,
+                                    TACSBVec **dfdX);
+  // psi must be phi: the symmetric products are what the buckling
+  // sensitivities use
+  void addMatDVSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
+                                TACSBVec *phi, TACSBVec *dfdx);
+  void addMatXptSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
+                                 TACSBVec *phi, TACSBVec *dfdX);
+  void evalMatSVSensInnerProduct(ElementMatrixType matType, TACSBVec *psi, TACSBVec *phi,
+                                 TACSBVec *res);
+
+  // the element model of all elements (0 linear, 1 nonlinear)
+  int getModel() const { return model; }
+  // upload design-dependent data (thickness, temperatures, material) if
+  // the element objects or design variables changed
+  int syncDesigns();
+  void setOrderingSeed(int n, const int *nodes);
+
+ private:
+  int check(int rc, const char *what);
+  shellgpu_ctx *ctx;
+  int num_nodes, num_elements, model;
+  std::vector<TACSElement *> elements;
+  std::vector<TacsScalar> thick;
+  std::vector<TacsScalar> up_thick, up_temp;  // values last uploaded
+  TACSBVec *vars;
+  bool designs_dirty;
+  friend class TACSBVec;
+};
+
+// TACSLinearBuckling (src/TACSBuckling.h:43-107, TACSBuckling.cpp:51-350
+// and the archived sensitivities archive/TACSBuckling.cpp:382-592)
+class TACSLinearBuckling : public TACSObject {
+ public:
+  TACSLinearBuckling(TACSAssembler *assembler, TacsScalar sigma, TACSMat *gmat, TACSMat *kmat,
+                     TACSMat *aux_mat, TACSKsm *solver, int max_lanczos_vecs, int num_eigvals,
+                     double eig_tol);
+  ~TACSLinearBuckling();
+  void solve(TACSBVec *rhs = NULL, TACSBVec *u0 = NULL, KSMPrint *ksm_print = NULL);
+  TacsScalar extractEigenvalue(int n, TacsScalar *error);
+  TacsScalar extractEigenvector(int n, TACSBVec *ans, TacsScalar *error);
+  void evalEigenDVSens(int n, TACSBVec *dfdx);
+  void evalEigenXptSens(int n, TACSBVec *dfdX);
+  void addEigenDVSens(TacsScalar coef, int n, TACSBVec *dfdx);
+  void addEigenXptSens(TacsScalar coef, int n, TACSBVec *dfdX);
+  void checkEigenvector(int n);
+  int getNumEigs() const { return num_eigvals; }
+  TacsScalar getSigma() const { return sigma; }
+
+ private:
+  TACSAssembler *assembler;
+  TACSMat *gmat, *kmat, *aux_mat;
+  TACSKsm *solver;
+  TacsScalar sigma;
+  int max_lanczos_vecs, num_eigvals;
+  double eig_tol;
+  shellgpu_buckling *buck;
+};
+
+// TACSMeshLoader (src/io/TACSMeshLoader.{h,cpp}): Nastran bulk-data reader
+// for GRID / GRID* / CQUAD4 (and CQUADR) / SPC / SPC* in short, long and
+// free-field formats, nodes and elements sorted by their file ids, CQUAD4
+// corners permuted to the tensor order [0, 1, 3, 2] (TACSMeshLoader.cpp:
+// 569-1098); createTACS applies the first-appearance node renumbering of
+// TACSCreator (src/TACSCreator.cpp:1141-1202).
+class TACSMeshLoader : public TACSObject {
+ public:
+  TACSMeshLoader();
+  ~TACSMeshLoader();
+  int scanBDFFile(const char *file_name);
+  int getNumComponents() const { return num_components; }
+  const char *getElementDescript(int comp);
+  void setElement(int comp, TACSElement *elem);
+  int getNumNodes() const { return (int)Xpts.size() / 3; }
+  int getNumElements() const { return (int)elem_comp.size(); }
+  // file-ordered data (after the id sort), before renumbering
+  const std::vector<TacsScalar> &getNodes() const { return Xpts; }
+  const std::vector<int> &getConnectivity() const { return conn; }
+  const std::vector<int> &getComponents() const { return elem_comp; }
+  void getBCs(std::vector<int> &nodes, std::vector<int> &masks, std::vector<TacsScalar> &vals) const;
+  TACSAssembler *createTACS(int vars_per_node, int device = 0);
+
+ private:
+  int num_components;
+  std::vector<std::string> comp_elems, comp_descript;
+  std::vector<TACSElement *> comp_element;
+  std::vector<TacsScalar> Xpts;
+  std::vector<int> conn, elem_comp;
+  // one entry per SPC card: node, dof index, value
+  std::vector<int> bc_node, bc_dof;
+  std::vector<TacsScalar> bc_val;
+};
+
+}  // namespace shellgpu

@@ -145,6 +145,14 @@ int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, doub
 int shellgpu_add_adj_res_xpt_product(shellgpu_ctx *ctx, int model, double scale, const double *psi,
                                      const double *u, double *dfdX);
 
+/* ---- vector algebra on length-n device vectors (TACSBVec::axpby, scale,
+ * dot, src/bpmat/TACSBVec.cpp:237-281 and the BLAS calls of TACSBVec);
+ * results of shellgpu_vec_dot are returned on the host ---------------- */
+int shellgpu_vec_axpby(shellgpu_ctx *ctx, size_t n, double alpha, double beta, const double *x,
+                       double *y); /* y = alpha x + beta y */
+int shellgpu_vec_scale(shellgpu_ctx *ctx, size_t n, double alpha, double *x);
+int shellgpu_vec_dot(shellgpu_ctx *ctx, size_t n, const double *x, const double *y, double *out);
+
 /* ---- vector BCs (TACSBVec, src/bpmat/TACSBVec.cpp:546-625) ------------- */
 /* x[bc] = 0 for every design of the batch */
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x);
@@ -218,6 +226,19 @@ int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *forc
 int shellgpu_buckling_create(shellgpu_ctx *ctx, int model, double sigma, int nev,
                              int max_lanczos, double tol, shellgpu_buckling **B);
 void shellgpu_buckling_destroy(shellgpu_buckling *B);
+/* The matrices kmat, gmat, aux_mat and the preconditioner bound to aux_mat
+ * that the TACSLinearBuckling constructor receives (src/TACSBuckling.cpp:
+ * 51-137; K, G, A distinct, as the reference requires at :75-101): the
+ * analysis assembles K and G into them and factors A = K + sigma G with F.
+ * They stay owned by the caller.  Without this call the object uses its
+ * own. */
+int shellgpu_buckling_set_operators(shellgpu_buckling *B, shellgpu_mat *K, shellgpu_mat *G,
+                                    shellgpu_mat *A, shellgpu_factor *F);
+/* GMRES settings of the static path solve (GMRES constructor and
+ * setTolerances, src/bpmat/KSM.cpp:547-560, 727-730; default 10, 15, 1e-12,
+ * 1e-30) */
+int shellgpu_buckling_set_ksm(shellgpu_buckling *B, int msub, int nrestart, double rtol,
+                              double atol);
 /* TACSLinearBuckling::solve(rhs, u0) (src/TACSBuckling.cpp:197-278): K at
  * zero state; path = static solution of K u = f (u0 == NULL) or u0 (device,
  * nbatch*6*nnodes; BC values are imposed on it); G(path); K + sigma G;

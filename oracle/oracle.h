@@ -161,6 +161,7 @@ typedef struct {
   unsigned long long seed; /* start vector */
   const double *u0; /* path (TACSLinearBuckling::solve with u0 != NULL); NULL:
                        static solve of K u = f */
+  const double *bc_vals; /* 6 per BC entry (prescribed values); NULL = 0 */
 } or_buckling_problem;
 
 /* lambda[nev] (sorted by magnitude ascending, SEP sortEigenvalues

@@ -165,7 +165,7 @@ class BucklingProblem(C.Structure):
     _fields_ = [("nnodes", C.c_int), ("nelems", C.c_int), ("conn", _IP), ("Xpts", _DP), ("thick", _DP),
                 ("temp", _DP), ("mat", C.POINTER(Material)), ("nbc", C.c_int), ("bc_nodes", _IP),
                 ("bc_mask", _IP), ("force", _DP), ("sigma", C.c_double), ("max_lanczos", C.c_int),
-                ("nev", C.c_int), ("seed", C.c_ulonglong), ("u0", _DP)]
+                ("nev", C.c_int), ("seed", C.c_ulonglong), ("u0", _DP), ("bc_vals", _DP)]
 
 
 def tridiag_eig(d, e):
@@ -184,7 +184,8 @@ def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60,
                     u0=None):
     """Oracle linear buckling: returns (lambda[nev], phi[nev, 6n], path[6n],
     resid[nev], nsteps).  u0 (optional) is the path; otherwise the static
-    solution of K u = f."""
+    solution of K u = f.  bcs = (nodes, masks) or (nodes, masks, values[6
+    per node]) for prescribed displacements."""
     mat = mat or material()
     conn = np.ascontiguousarray(conn, np.int32)
     Xpts = _f64(Xpts)
@@ -195,9 +196,10 @@ def linear_buckling(Xpts, conn, thick, bcs, force, sigma, nev=5, max_lanczos=60,
     tp = _f64(temp) if temp is not None else None
     nn = Xpts.size // 3
     u0 = _f64(u0) if u0 is not None else None
+    bv = _f64(bcs[2]) if len(bcs) > 2 and bcs[2] is not None else None
     p = BucklingProblem(nn, conn.size // 4, _i(conn), _d(Xpts), _d(thick), _d(tp) if tp is not None else None,
                         C.pointer(mat), bn.size, _i(bn), _i(bm), _d(force), sigma, max_lanczos, nev, seed,
-                        _d(u0) if u0 is not None else None)
+                        _d(u0) if u0 is not None else None, _d(bv) if bv is not None else None)
     lam = np.zeros(nev)
     phi = np.zeros((nev, 6 * nn))
     path = np.zeros(6 * nn)

@@ -290,14 +290,20 @@ int or_linear_buckling(const or_buckling_problem *p, double *lambda, double *phi
   or_rcm(nn, rowp, cols, perm);
   or_band *fk = or_band_factor(nn, rowp, cols, K, perm);
   if (p->u0) {
-    /* given path with the (zero) BC values set */
+    /* given path with the BC values set (TACSBVec::setBCs) */
     memcpy(u, p->u0, sizeof(double) * (size_t)n);
     or_vec_zero_bcs(u, p->nbc, p->bc_nodes, p->bc_mask);
+    for (int b = 0; b < p->nbc && p->bc_vals; b++)
+      for (int k = 0; k < 6; k++)
+        if (p->bc_mask[b] >> k & 1) u[6 * p->bc_nodes[b] + k] = p->bc_vals[6 * b + k];
   } else {
-    /* res = R(0) - f with BC rows u - ubar = 0; path = -K^{-1} res */
+    /* res = R(0) - f with BC rows u - ubar = -ubar; path = -K^{-1} res */
     or_assemble_res(OR_LINEAR, p->mat, nn, p->nelems, p->conn, p->Xpts, p->thick, p->temp, NULL, rhs);
     for (int i = 0; i < n; i++) rhs[i] -= p->force ? p->force[i] : 0.0;
     or_vec_zero_bcs(rhs, p->nbc, p->bc_nodes, p->bc_mask);
+    for (int b = 0; b < p->nbc && p->bc_vals; b++)
+      for (int k = 0; k < 6; k++)
+        if (p->bc_mask[b] >> k & 1) rhs[6 * p->bc_nodes[b] + k] = -p->bc_vals[6 * b + k];
     or_band_solve(fk, rhs, u);
     for (int i = 0; i < n; i++) u[i] = -u[i];
   }

@@ -1,0 +1,55 @@
+"""The C++ host layer end to end on the GPU: the port of the reference's
+mechBuckling driver (lib/mech_buckling) reads a BDF of a cylinder with the
+reference fixture's geometry and boundary conditions (R = 0.2, L = 0.4,
+translations of both end rings constrained, u1 = -1e-5 prescribed on one
+ring), written with shuffled node and element ids, and its buckling loads
+are compared with the oracle's on the same mesh."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import LIB_DIR
+from shellgpu.bdf import write_bdf
+from shellgpu.problems import Cylinder
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(LIB_DIR, "mech_buckling")
+
+
+def test_mech_buckling_driver_matches_oracle(tmp_path):
+    cyl = Cylinder(10, 20, R=0.2, L=0.4)
+    t, sigma, nev, nlanczos = 0.002, 10.0, 8, 120
+    ring0, ring1 = cyl.ring_nodes(0), cyl.ring_nodes(cyl.na)
+    spcs = [(int(n), "123", 0.0) for n in ring0]
+    spcs += [(int(n), "1", -1e-5) for n in ring1] + [(int(n), "23", 0.0) for n in ring1]
+    rng = np.random.default_rng(9)
+    node_ids = rng.permutation(np.arange(1, cyl.nnodes + 1) * 2)
+    # element ids increasing in element order: first-appearance numbering
+    # then reproduces the generator's node order
+    elem_ids = np.arange(1, cyl.nelems + 1) * 3
+    path = str(tmp_path / "cyl.bdf")
+    write_bdf(path, cyl.Xpts, cyl.conn, spcs, node_ids=node_ids, elem_ids=elem_ids)
+    # the driver runs as a child process: its own HIP context
+    out = subprocess.run([DRIVER, path, str(t), str(sigma), str(nev), str(nlanczos)], capture_output=True,
+                         text=True, timeout=120)
+    print(out.stdout[-2000:], out.stderr[-2000:])
+    assert out.returncode == 0
+    lam = np.array([float(m.group(2)) for m in re.finditer(r"lambda\[(\d+)\] = (\S+)", out.stdout)])
+    assert lam.size == nev
+
+    # the BDF's coordinates are written with 9 significant digits
+    Xf = np.array([[float(f"{v:.9e}") for v in row] for row in cyl.Xpts])
+    bnodes = np.concatenate([ring0, ring1]).astype(np.int32)
+    masks = np.full(bnodes.size, 7, np.int32)
+    vals = np.zeros((bnodes.size, 6))
+    vals[ring0.size:, 0] = -1e-5
+    ref = O.linear_buckling(Xf, cyl.conn, np.full(cyl.nelems, t), (bnodes, masks, vals.ravel()),
+                            np.zeros(6 * cyl.nnodes), sigma, nev=nev, max_lanczos=nlanczos,
+                            mat=O.material(E=72e9, nu=0.33, rho=2718.0, cte=10e-6))[0]
+    print("driver", lam[:5], "oracle", ref[:5])
+    assert np.abs(lam[:5] - ref[:5]).max() <= 1e-8 * np.abs(ref[:5]).max()
