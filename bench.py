@@ -92,15 +92,16 @@ def make_context(cyl, thick):
 
 
 def ks_of_loads(lam):
-    """KS aggregate (rho = 100) of -lambda / lambda_1 for each design (a
-    smooth minimum of the loads, normalised by the first): value and
-    d KS / d lambda."""
+    """KS aggregate (rho = 100) of -lambda_i / lambda_1 for each design (a
+    smooth minimum of the loads, normalised by the first, the normalisation
+    differentiated too): value and d KS / d lambda."""
     ks, coef = [], []
     for row in lam:
-        f = -row / row[0]
-        v, d = sg.ks_aggregate(f, KS_RHO)
+        v, c = sg.ks_aggregate(-row / row[0], KS_RHO)
+        d = -c / row[0]
+        d[0] += float(c @ row) / row[0] ** 2
         ks.append(v)
-        coef.append(-d / row[0])
+        coef.append(d)
     return np.array(ks), np.array(coef)
 
 

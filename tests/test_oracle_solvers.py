@@ -40,3 +40,51 @@ def test_buckling_matches_dense_generalized_eigensolve():
     w = w[np.argsort(np.abs(w))][:5]
     assert np.abs(lam - w).max() <= 1e-9 * np.abs(w).max()
     assert resid.max() < 1e-8
+
+
+def _dense_buckling(cyl, thick, u):
+    rowp, cols = O.node_csr(cyl.nnodes, cyl.conn)
+    m = O.material()
+    K = O.bsr_to_dense(rowp, cols, O.assemble_mat(O.KIND_K, 0, m, cyl.Xpts, cyl.conn, thick, rowp, cols))
+    G = O.bsr_to_dense(rowp, cols, O.assemble_mat(O.KIND_G, 1, m, cyl.Xpts, cyl.conn, thick, rowp, cols, u=u))
+    return K, G
+
+
+def test_buckling_with_prescribed_displacements_and_given_path():
+    cyl = Cylinder(4, 8)
+    thick = np.random.default_rng(6).uniform(0.0015, 0.0025, cyl.nelems)
+    ring0, ring1 = cyl.ring_nodes(0), cyl.ring_nodes(cyl.na)
+    bn = np.concatenate([ring0, ring1]).astype(np.int32)
+    bm = np.full(bn.size, 0x3F, np.int32)
+    vals = np.zeros((bn.size, 6))
+    vals[ring0.size:, 0] = -1e-5  # axial end shortening, displacement control
+    fixed = np.array(sorted({6 * b + r for b in bn for r in range(6)}))
+    free = np.array(sorted(set(range(6 * cyl.nnodes)) - set(fixed)))
+    ubar = np.zeros(6 * cyl.nnodes)
+    for b, v in zip(bn, vals):
+        ubar[6 * b:6 * b + 6] = v
+    K, _ = _dense_buckling(cyl, thick, None)
+    u = ubar.copy()
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], -K[np.ix_(free, fixed)] @ ubar[fixed])
+    sigma = 100.0
+    lam, _, path, _, _ = O.linear_buckling(cyl.Xpts, cyl.conn, thick, (bn, bm, vals.ravel()),
+                                           np.zeros(6 * cyl.nnodes), sigma, nev=4, max_lanczos=80)
+    assert np.abs(path - u).max() <= 1e-9 * np.abs(u).max()
+    _, G = _dense_buckling(cyl, thick, u)
+    w = sl.eig(K[np.ix_(free, free)], -G[np.ix_(free, free)], right=False)
+    w = np.real(w[np.isfinite(w)])
+    w = w[np.argsort(np.abs(w))][:4]
+    assert np.abs(lam - w).max() <= 1e-9 * np.abs(w).max()
+
+    # a given path (TACSLinearBuckling::solve with u0): BC values imposed on it
+    u0 = 3.0 * u + 1e-7 * np.random.default_rng(7).standard_normal(u.size)
+    lam0, _, path0, _, _ = O.linear_buckling(cyl.Xpts, cyl.conn, thick, (bn, bm, vals.ravel()),
+                                             np.zeros(6 * cyl.nnodes), sigma, nev=4, max_lanczos=80, u0=u0)
+    ref0 = u0.copy()
+    ref0[fixed] = ubar[fixed]
+    assert np.array_equal(path0, ref0)
+    _, G0 = _dense_buckling(cyl, thick, ref0)
+    w0 = sl.eig(K[np.ix_(free, free)], -G0[np.ix_(free, free)], right=False)
+    w0 = np.real(w0[np.isfinite(w0)])
+    w0 = w0[np.argsort(np.abs(w0))][:4]
+    assert np.abs(lam0 - w0).max() <= 1e-9 * np.abs(w0).max()
