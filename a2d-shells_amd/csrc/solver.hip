@@ -114,58 +114,118 @@ __global__ void scatter_dense_kernel(int nnzb, const double *__restrict__ vals, 
   dst[base[p] + (long)c * ld[p] + r] = v;
 }
 
-// y = beta y + alpha A x, A column-major m x n (lda); lower != 0 restricts to
-// the lower triangle (col <= row).  Pass 1: 64 rows x 4 column groups per
-// block, columns split over gridDim.y chunks -> partial sums.
+// y = beta y + alpha A x, A column-major m x n (lda, even; A and x 16-byte
+// aligned); LOWER restricts to the lower triangle (col <= row).
+// Pass 1: a block of 4 wavefronts covers 128 rows (two per lane, one
+// 16-byte load per column) of one column chunk (gridDim.y chunks), the
+// wavefronts taking every fourth column, 4 columns in flight per lane;
+// partial sums go to part[design][chunk][row].
+template <int LOWER>
 __global__ __launch_bounds__(256) void gemv_n_pass1(int m, int n, const double *__restrict__ A, int lda, long sA,
-                                                    const double *__restrict__ x, long sx, int lower, int chunk,
+                                                    const double *__restrict__ x, long sx, int chunk,
                                                     double *__restrict__ part) {
-  __shared__ double red[256];
+  __shared__ double2 red[256];
   const int b = blockIdx.z;
   A += b * sA;
   x += b * sx;
-  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int g = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 128 + 2 * lane;
   const int c0 = blockIdx.y * chunk;
   int c1 = min(n, c0 + chunk);
-  double acc = 0.0;
+  double a0 = 0.0, a1 = 0.0;
   if (row < m) {
-    if (lower) c1 = min(c1, row + 1);
-    for (int c = c0 + g; c < c1; c += 4) acc += A[row + (long)c * lda] * x[c];
+    if (LOWER) c1 = min(c1, row + 2);
+    int c = c0 + wave;
+    for (; c + 12 < c1; c += 16) {
+      const double2 v0 = *reinterpret_cast<const double2 *>(A + row + (long)c * lda);
+      const double2 v1 = *reinterpret_cast<const double2 *>(A + row + (long)(c + 4) * lda);
+      const double2 v2 = *reinterpret_cast<const double2 *>(A + row + (long)(c + 8) * lda);
+      const double2 v3 = *reinterpret_cast<const double2 *>(A + row + (long)(c + 12) * lda);
+      const double x0 = x[c], x1 = x[c + 4], x2 = x[c + 8], x3 = x[c + 12];
+      if (LOWER) {
+        a0 += (c <= row ? v0.x : 0.0) * x0 + (c + 4 <= row ? v1.x : 0.0) * x1 + (c + 8 <= row ? v2.x : 0.0) * x2 +
+              (c + 12 <= row ? v3.x : 0.0) * x3;
+      } else {
+        a0 += v0.x * x0 + v1.x * x1 + v2.x * x2 + v3.x * x3;
+      }
+      a1 += v0.y * x0 + v1.y * x1 + v2.y * x2 + v3.y * x3;
+    }
+    for (; c < c1; c += 4) {
+      const double2 v = *reinterpret_cast<const double2 *>(A + row + (long)c * lda);
+      const double xc = x[c];
+      a0 += ((!LOWER || c <= row) ? v.x : 0.0) * xc;
+      a1 += v.y * xc;
+    }
   }
-  red[threadIdx.x] = acc;
+  red[threadIdx.x] = make_double2(a0, a1);
   __syncthreads();
-  if (threadIdx.x < 64 && row < m) {
-    const double s = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
-    part[((long)b * gridDim.y + blockIdx.y) * m + row] = s;
+  if (wave == 0 && row < m) {
+    const double2 r0 = red[lane], r1 = red[lane + 64], r2 = red[lane + 128], r3 = red[lane + 192];
+    double2 out;
+    out.x = (r0.x + r1.x) + (r2.x + r3.x);
+    out.y = (r0.y + r1.y) + (r2.y + r3.y);
+    *reinterpret_cast<double2 *>(part + ((long)b * gridDim.y + blockIdx.y) * m + row) = out;
   }
 }
 
+// Pass 2: y = beta y + alpha sum over chunks, 8 partials in flight per thread.
 __global__ void gemv_n_pass2(int m, int nsplit, const double *__restrict__ part, double alpha, double beta,
                              double *__restrict__ y, long sy) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
   const int b = blockIdx.y;
+  const double *p = part + (long)b * nsplit * m + row;
   double s = 0.0;
-  for (int k = 0; k < nsplit; k++) s += part[((long)b * nsplit + k) * m + row];
+  int k = 0;
+  for (; k + 8 <= nsplit; k += 8) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) t[u] = p[(long)(k + u) * m];
+    s += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+  }
+  for (; k < nsplit; k++) s += p[(long)k * m];
   double *yy = y + b * sy;
   yy[row] = (beta == 0.0 ? 0.0 : beta * yy[row]) + alpha * s;
 }
 
-// y = beta y + alpha A^T x, A column-major m x n (lda): one wavefront per
-// output entry j (a column of A read coalesced); lower != 0 restricts to
-// rows i >= j.
+// y = beta y + alpha A^T x, A column-major m x n (lda, even; 16-byte
+// aligned): one wavefront per output entry j, reading column j with one
+// 16-byte load per lane and 4 loads in flight; LOWER restricts to rows
+// i >= j.
+template <int LOWER>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double *__restrict__ A, int lda, long sA,
-                                                     const double *__restrict__ x, long sx, int lower, double alpha,
-                                                     double beta, double *__restrict__ y, long sy) {
+                                                     const double *__restrict__ x, long sx, double alpha, double beta,
+                                                     double *__restrict__ y, long sy) {
   const int b = blockIdx.y;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (j >= n) return;
   const double *col = A + b * sA + (long)j * lda;
   const double *xx = x + b * sx;
-  double acc = 0.0;
-  for (int i = (lower ? j : 0) + lane; i < m; i += 64) acc += col[i] * xx[i];
+  const int i0 = LOWER ? (j & ~1) : 0;  // even start keeps the 16-byte alignment
+  double a0 = 0.0, a1 = 0.0;
+  int i = i0 + 2 * lane;
+  for (; i + 384 < m; i += 512) {
+    const double2 v0 = *reinterpret_cast<const double2 *>(col + i);
+    const double2 v1 = *reinterpret_cast<const double2 *>(col + i + 128);
+    const double2 v2 = *reinterpret_cast<const double2 *>(col + i + 256);
+    const double2 v3 = *reinterpret_cast<const double2 *>(col + i + 384);
+    const double2 x0 = *reinterpret_cast<const double2 *>(xx + i);
+    const double2 x1 = *reinterpret_cast<const double2 *>(xx + i + 128);
+    const double2 x2 = *reinterpret_cast<const double2 *>(xx + i + 256);
+    const double2 x3 = *reinterpret_cast<const double2 *>(xx + i + 384);
+    // in a LOWER column only the first pair can hold a row above the
+    // diagonal (row j - 1 when j is odd)
+    a0 += ((LOWER && i < j) ? 0.0 : v0.x * x0.x) + v1.x * x1.x + v2.x * x2.x + v3.x * x3.x;
+    a1 += v0.y * x0.y + v1.y * x1.y + v2.y * x2.y + v3.y * x3.y;
+  }
+  for (; i < m; i += 128) {
+    const double2 v = *reinterpret_cast<const double2 *>(col + i);
+    const double2 xv = *reinterpret_cast<const double2 *>(xx + i);
+    a0 += (LOWER && i < j) ? 0.0 : v.x * xv.x;
+    a1 += v.y * xv.y;
+  }
+  double acc = a0 + a1;
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) {
     double *yy = y + b * sy;
@@ -280,7 +340,7 @@ int setup_plan(shellgpu_factor *F) {
   F->m.assign(nl, 0);
   F->voff.assign(nl + 1, 0);
   for (int l = 0; l < nl; l++) {
-    F->m[l] = 6 * cnt[l];
+    F->m[l] = 6 * cnt[l];  // even: the level GEMVs load 16-byte pairs
     F->voff[l + 1] = F->voff[l] + F->m[l];
     F->maxm = std::max(F->maxm, F->m[l]);
   }
@@ -531,15 +591,22 @@ static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, i
   if (!trans) {
     const int nsplit = std::max(1, std::min(32, n / 64));
     const int chunk = (n + nsplit - 1) / nsplit;
-    hipLaunchKernelGGL(gemv_n_pass1, dim3((m + 63) / 64, nsplit, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x,
-                       sv, lower, chunk, F->part);
+    const dim3 grid((m + 127) / 128, nsplit, nb);
+    if (lower)
+      hipLaunchKernelGGL(gemv_n_pass1<1>, grid, dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv, chunk, F->part);
+    else
+      hipLaunchKernelGGL(gemv_n_pass1<0>, grid, dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv, chunk, F->part);
     HIPCALL(ctx, hipGetLastError());
     hipLaunchKernelGGL(gemv_n_pass2, dim3((m + 255) / 256, nb), dim3(256), 0, ctx->stream, m, nsplit, F->part, alpha,
                        beta, y, sv);
     HIPCALL(ctx, hipGetLastError());
   } else {
-    hipLaunchKernelGGL(gemv_t_kernel, dim3((n + 3) / 4, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv, lower,
-                       alpha, beta, y, sv);
+    if (lower)
+      hipLaunchKernelGGL(gemv_t_kernel<1>, dim3((n + 3) / 4, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv,
+                         alpha, beta, y, sv);
+    else
+      hipLaunchKernelGGL(gemv_t_kernel<0>, dim3((n + 3) / 4, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv,
+                         alpha, beta, y, sv);
     HIPCALL(ctx, hipGetLastError());
   }
   return 0;
