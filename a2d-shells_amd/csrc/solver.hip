@@ -44,6 +44,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,6 +53,12 @@
 #include "shellgpu_ctx.h"
 
 using namespace shellgpu;
+
+// one round of block cyclic reduction: levels s, 3s, 5s, ... eliminated
+struct FactorRound {
+  int s, nE, nR;  // stride, levels eliminated, of which with a right neighbour
+  long off;       // first pointer of this round in the device pointer arrays
+};
 
 struct shellgpu_factor {
   shellgpu_ctx *ctx = nullptr;
@@ -74,6 +81,18 @@ struct shellgpu_factor {
   double *vb = nullptr, *vz = nullptr, *vx = nullptr;  // level-ordered work vectors
   double *part = nullptr;    // GEMV partial sums
   bool factored = false;
+  long nv = 0;               // level-ordered vector length per design
+  // block cyclic reduction over uniform (padded) levels of M dofs: per
+  // design, blocks D_i (-> C_i^{-1}), B_i (coupling to the left active
+  // neighbour -> Y_i = C_i^{-1} B_i), Z_i (-> W_i = B_{i+s} C_i^{-T})
+  bool cr = false;
+  int M = 0;
+  int nkmax = 1;             // largest number of levels eliminated in one round
+  int *d_mlev = nullptr;     // dofs per level (padding below)
+  double **d_ptr = nullptr;  // device pointer arrays of all rounds
+  int *cr_info = nullptr;    // potrf / trtri status per (round entry)
+  std::vector<FactorRound> rounds;
+  long root_off = 0;         // pointer group of the root level D_0
 };
 
 namespace {
@@ -114,20 +133,27 @@ __global__ void scatter_dense_kernel(int nnzb, const double *__restrict__ vals, 
   dst[base[p] + (long)c * ld[p] + r] = v;
 }
 
+// Batch of GEMVs: entry q = k * nd + d (k: level entry, d: design) uses
+// A + d sAd + k sAk, x + d sxd + k sxk, y + d syd + k syk.
+struct GemvBatch {
+  int nd;
+  long sAd, sAk, sxd, sxk, syd, syk;
+};
+
 // y = beta y + alpha A x, A column-major m x n (lda, even; A and x 16-byte
 // aligned); LOWER restricts to the lower triangle (col <= row).
 // Pass 1: a block of 4 wavefronts covers 128 rows (two per lane, one
 // 16-byte load per column) of one column chunk (gridDim.y chunks), the
 // wavefronts taking every fourth column, 4 columns in flight per lane;
-// partial sums go to part[design][chunk][row].
+// partial sums go to part[entry][chunk][row].
 template <int LOWER>
-__global__ __launch_bounds__(256) void gemv_n_pass1(int m, int n, const double *__restrict__ A, int lda, long sA,
-                                                    const double *__restrict__ x, long sx, int chunk,
+__global__ __launch_bounds__(256) void gemv_n_pass1(int m, int n, const double *__restrict__ A, int lda,
+                                                    const double *__restrict__ x, GemvBatch g, int chunk,
                                                     double *__restrict__ part) {
   __shared__ double2 red[256];
-  const int b = blockIdx.z;
-  A += b * sA;
-  x += b * sx;
+  const int q = blockIdx.z, d = q % g.nd, k = q / g.nd;
+  A += d * g.sAd + k * g.sAk;
+  x += d * g.sxd + k * g.sxk;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = blockIdx.x * 128 + 2 * lane;
   const int c0 = blockIdx.y * chunk;
@@ -164,27 +190,27 @@ __global__ __launch_bounds__(256) void gemv_n_pass1(int m, int n, const double *
     double2 out;
     out.x = (r0.x + r1.x) + (r2.x + r3.x);
     out.y = (r0.y + r1.y) + (r2.y + r3.y);
-    *reinterpret_cast<double2 *>(part + ((long)b * gridDim.y + blockIdx.y) * m + row) = out;
+    *reinterpret_cast<double2 *>(part + ((long)q * gridDim.y + blockIdx.y) * m + row) = out;
   }
 }
 
 // Pass 2: y = beta y + alpha sum over chunks, 8 partials in flight per thread.
 __global__ void gemv_n_pass2(int m, int nsplit, const double *__restrict__ part, double alpha, double beta,
-                             double *__restrict__ y, long sy) {
+                             double *__restrict__ y, GemvBatch g) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
-  const int b = blockIdx.y;
-  const double *p = part + (long)b * nsplit * m + row;
+  const int q = blockIdx.y, d = q % g.nd, k = q / g.nd;
+  const double *p = part + (long)q * nsplit * m + row;
   double s = 0.0;
-  int k = 0;
-  for (; k + 8 <= nsplit; k += 8) {
+  int c = 0;
+  for (; c + 8 <= nsplit; c += 8) {
     double t[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) t[u] = p[(long)(k + u) * m];
+    for (int u = 0; u < 8; u++) t[u] = p[(long)(c + u) * m];
     s += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
   }
-  for (; k < nsplit; k++) s += p[(long)k * m];
-  double *yy = y + b * sy;
+  for (; c < nsplit; c++) s += p[(long)c * m];
+  double *yy = y + d * g.syd + k * g.syk;
   yy[row] = (beta == 0.0 ? 0.0 : beta * yy[row]) + alpha * s;
 }
 
@@ -193,15 +219,15 @@ __global__ void gemv_n_pass2(int m, int nsplit, const double *__restrict__ part,
 // 16-byte load per lane and 4 loads in flight; LOWER restricts to rows
 // i >= j.
 template <int LOWER>
-__global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double *__restrict__ A, int lda, long sA,
-                                                     const double *__restrict__ x, long sx, double alpha, double beta,
-                                                     double *__restrict__ y, long sy) {
-  const int b = blockIdx.y;
+__global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double *__restrict__ A, int lda,
+                                                     const double *__restrict__ x, GemvBatch g, double alpha,
+                                                     double beta, double *__restrict__ y) {
+  const int q = blockIdx.y, d = q % g.nd, k = q / g.nd;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (j >= n) return;
-  const double *col = A + b * sA + (long)j * lda;
-  const double *xx = x + b * sx;
+  const double *col = A + d * g.sAd + k * g.sAk + (long)j * lda;
+  const double *xx = x + d * g.sxd + k * g.sxk;
   const int i0 = LOWER ? (j & ~1) : 0;  // even start keeps the 16-byte alignment
   double a0 = 0.0, a1 = 0.0;
   int i = i0 + 2 * lane;
@@ -228,20 +254,32 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double 
   double acc = a0 + a1;
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) {
-    double *yy = y + b * sy;
+    double *yy = y + d * g.syd + k * g.syk;
     yy[j] = (beta == 0.0 ? 0.0 : beta * yy[j]) + alpha * acc;
   }
 }
 
-__global__ void permute_kernel(int n, const int *__restrict__ pos2dof, const double *__restrict__ src,
+// identity on the padded diagonal entries of the cyclic-reduction levels
+__global__ void pad_identity_kernel(int M, long M2, const int *__restrict__ mlev, double *__restrict__ buf,
+                                    long per_design) {
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  if (kk >= M || kk < mlev[l]) return;
+  buf[(long)blockIdx.z * per_design + (long)l * M2 + (long)kk * M + kk] = 1.0;
+}
+
+// level order <-> dof order; pos2dof[pos] < 0 marks a padding position
+// (zero in level order)
+__global__ void permute_kernel(long nv, long n, const int *__restrict__ pos2dof, const double *__restrict__ src,
                                double *__restrict__ dst, int to_level) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long off = (long)blockIdx.y * n;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int p = pos2dof[i];
+  const long lo = (long)blockIdx.y * nv, doff = (long)blockIdx.y * n;
   if (to_level) {
-    dst[off + i] = src[off + pos2dof[i]];
-  } else {
-    dst[off + pos2dof[i]] = src[off + i];
+    dst[lo + i] = p >= 0 ? src[doff + p] : 0.0;
+  } else if (p >= 0) {
+    dst[doff + p] = src[lo + i];
   }
 }
 
@@ -344,20 +382,56 @@ int setup_plan(shellgpu_factor *F) {
     F->voff[l + 1] = F->voff[l] + F->m[l];
     F->maxm = std::max(F->maxm, F->m[l]);
   }
-  F->S_off.assign(nl, 0);
-  F->L_off.assign(nl, -1);
-  long off = 0;
-  F->tmp_elems = 1;
-  for (int l = 0; l < nl; l++) {
-    F->S_off[l] = off;
-    off += (long)F->m[l] * F->m[l];
-    if (l > 0) {
-      F->L_off[l] = off;
-      off += (long)F->m[l] * F->m[l - 1];
-      F->tmp_elems = std::max(F->tmp_elems, (long)F->m[l] * F->m[l - 1]);
-    }
+
+  // Block cyclic reduction when the levels are (nearly) uniform and the
+  // padded factor (three M x M blocks per level and design) fits in 3/4 of
+  // the free device memory; otherwise the sequential level Cholesky.
+  // SHELLGPU_FACTOR=thomas selects the sequential factorisation.
+  {
+    const long M = F->maxm;
+    size_t freeb = 0, totalb = 0;
+    HIPCALL(ctx, hipMemGetInfo(&freeb, &totalb));
+    const double need = 3.0 * nl * (double)M * (double)M * 8.0 * ctx->nbatch;
+    const char *env = getenv("SHELLGPU_FACTOR");
+    F->cr = nl >= 3 && (double)nl * M <= 1.15 * F->n && need <= 0.75 * (double)freeb;
+    if (env && strcmp(env, "thomas") == 0) F->cr = false;
   }
-  F->per_design = off;
+  if (F->cr) {
+    const long M = F->maxm, M2 = M * M;
+    F->M = (int)M;
+    F->nv = (long)nl * M;
+    F->per_design = 3L * nl * M2;
+    F->rounds.clear();
+    F->nkmax = 1;
+    for (int st = 1; st < nl; st *= 2) {
+      FactorRound r;
+      r.s = st;
+      r.nE = r.nR = 0;
+      for (int i = st; i < nl; i += 2 * st) {
+        r.nE++;
+        if (i + st < nl) r.nR++;
+      }
+      r.off = 0;
+      F->rounds.push_back(r);
+      F->nkmax = std::max(F->nkmax, r.nE);
+    }
+  } else {
+    F->nv = F->n;
+    F->S_off.assign(nl, 0);
+    F->L_off.assign(nl, -1);
+    long off = 0;
+    F->tmp_elems = 1;
+    for (int l = 0; l < nl; l++) {
+      F->S_off[l] = off;
+      off += (long)F->m[l] * F->m[l];
+      if (l > 0) {
+        F->L_off[l] = off;
+        off += (long)F->m[l] * F->m[l - 1];
+        F->tmp_elems = std::max(F->tmp_elems, (long)F->m[l] * F->m[l - 1]);
+      }
+    }
+    F->per_design = off;
+  }
 
   // BC dofs (symmetric elimination)
   std::vector<int> isbc(F->n, 0);
@@ -365,15 +439,17 @@ int setup_plan(shellgpu_factor *F) {
     for (int r = 0; r < 6; r++)
       if (ctx->h_bc_mask[b] & (1 << r)) isbc[6 * ctx->h_bc_nodes[b] + r] = 1;
 
-  // dof permutation and block scatter map
-  std::vector<int> pos2dof(F->n), bcpos(F->n);
-  for (int i = 0; i < nn; i++)
-    for (int d = 0; d < 6; d++) {
-      pos2dof[6L * node_pos[i] + d] = 6 * i + d;
-      bcpos[6L * node_pos[i] + d] = isbc[6 * i + d];
-    }
+  // dof permutation (level order, padded to M per level under cyclic
+  // reduction) and block scatter map
+  std::vector<int> pos2dof(F->nv, -1);
+  for (int i = 0; i < nn; i++) {
+    const int li = level[i];
+    const long p0 = F->cr ? (long)li * F->M + 6L * (node_pos[i] - noff[li]) : 6L * node_pos[i];
+    for (int d = 0; d < 6; d++) pos2dof[p0 + d] = 6 * i + d;
+  }
   std::vector<long> base(ctx->nnzb);
   std::vector<int> ld(ctx->nnzb), colbc(ctx->nnzb);
+  const long M2 = (long)F->M * F->M;
   for (int i = 0; i < nn; i++) {
     const int li = level[i];
     const long rl = 6L * (node_pos[i] - noff[li]);
@@ -384,25 +460,29 @@ int setup_plan(shellgpu_factor *F) {
       for (int d = 0; d < 6; d++) mask |= (isbc[6 * i + d] << d) | (isbc[6 * j + d] << (6 + d));
       if (i == j) mask |= 1 << 12;
       colbc[p] = mask;
-      ld[p] = F->m[li];
+      ld[p] = F->cr ? F->M : F->m[li];
       if (lj == li) {
-        base[p] = F->S_off[li] + cl * F->m[li] + rl;
+        base[p] = F->cr ? (long)li * M2 + cl * F->M + rl : F->S_off[li] + cl * F->m[li] + rl;
       } else if (lj == li - 1) {
-        base[p] = F->L_off[li] + cl * F->m[li] + rl;
+        base[p] = F->cr ? (long)(nl + li) * M2 + cl * F->M + rl : F->L_off[li] + cl * F->m[li] + rl;
       } else if (lj == li + 1) {
-        base[p] = -1;  // upper block: the transpose of the next level's L
+        base[p] = -1;  // upper block: the transpose of the next level's coupling
       } else {
         return ferr(ctx, 1, "level ordering is not block tridiagonal");
       }
     }
   }
+  if (F->cr) {
+    HIPCALL(ctx, hipMalloc((void **)&F->d_mlev, sizeof(int) * nl));
+    HIPCALL(ctx, hipMemcpy(F->d_mlev, F->m.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+  }
   HIPCALL(ctx, hipMalloc((void **)&F->d_bcflag, sizeof(int) * ctx->nnzb));
   HIPCALL(ctx, hipMemcpyAsync(F->d_bcflag, colbc.data(), sizeof(int) * ctx->nnzb, hipMemcpyHostToDevice,
                               ctx->stream));
-  HIPCALL(ctx, hipMalloc((void **)&F->d_pos2dof, sizeof(int) * F->n));
+  HIPCALL(ctx, hipMalloc((void **)&F->d_pos2dof, sizeof(int) * F->nv));
   HIPCALL(ctx, hipMalloc((void **)&F->d_blk_base, sizeof(long) * ctx->nnzb));
   HIPCALL(ctx, hipMalloc((void **)&F->d_blk_ld, sizeof(int) * ctx->nnzb));
-  HIPCALL(ctx, hipMemcpyAsync(F->d_pos2dof, pos2dof.data(), sizeof(int) * F->n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCALL(ctx, hipMemcpyAsync(F->d_pos2dof, pos2dof.data(), sizeof(int) * F->nv, hipMemcpyHostToDevice, ctx->stream));
   HIPCALL(ctx, hipMemcpyAsync(F->d_blk_base, base.data(), sizeof(long) * ctx->nnzb, hipMemcpyHostToDevice,
                               ctx->stream));
   HIPCALL(ctx, hipMemcpyAsync(F->d_blk_ld, ld.data(), sizeof(int) * ctx->nnzb, hipMemcpyHostToDevice, ctx->stream));
@@ -475,6 +555,47 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
   return 0;
 }
 
+// Pointer arrays of every cyclic-reduction round, entry q = k nb + d
+// (level entry k, design d): D_i, B_i, Z_i, D_{i-s}, D_{i+s}, B_{i+s} for
+// i = s (2k + 1); the last three exist for the first nR nb entries only.
+// A final group of nb entries points at D_0 (the root).
+int build_cr_pointers(shellgpu_factor *F) {
+  shellgpu_ctx *ctx = F->ctx;
+  const int nb = F->nbatch, nl = F->nlevels;
+  const long M2 = (long)F->M * F->M, PD = F->per_design;
+  std::vector<double *> h;
+  for (FactorRound &r : F->rounds) {
+    r.off = (long)h.size();
+    const long cnt = (long)nb * r.nE;
+    h.resize(h.size() + 6 * cnt, nullptr);
+    double **P = h.data() + r.off;
+    for (int k = 0; k < r.nE; k++) {
+      const int i = r.s * (2 * k + 1);
+      for (int d = 0; d < nb; d++) {
+        const long q = (long)k * nb + d;
+        double *base = F->buf + d * PD;
+        P[0 * cnt + q] = base + (long)i * M2;
+        P[1 * cnt + q] = base + (long)(nl + i) * M2;
+        P[2 * cnt + q] = base + (long)(2 * nl + i) * M2;
+        P[3 * cnt + q] = base + (long)(i - r.s) * M2;
+        if (i + r.s < nl) {
+          P[4 * cnt + q] = base + (long)(i + r.s) * M2;
+          P[5 * cnt + q] = base + (long)(nl + i + r.s) * M2;
+        } else {
+          P[4 * cnt + q] = P[3 * cnt + q];  // unused (entries beyond nR nb)
+          P[5 * cnt + q] = P[1 * cnt + q];
+        }
+      }
+    }
+  }
+  const long root = (long)h.size();
+  for (int d = 0; d < nb; d++) h.push_back(F->buf + d * PD);
+  HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
+  HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
+  F->root_off = root;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -497,18 +618,26 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
     delete F;
     return rc;
   }
-  const size_t bytes = sizeof(double) * (size_t)F->per_design * F->nbatch;
+  const int nb = F->nbatch;
+  const size_t bytes = sizeof(double) * (size_t)F->per_design * nb;
   hipError_t e = hipMalloc((void **)&F->buf, bytes);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->tmp, sizeof(double) * std::max(1L, F->tmp_elems) * F->nbatch);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->info, sizeof(int) * F->nbatch);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->vb, sizeof(double) * (size_t)F->n * F->nbatch);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->vz, sizeof(double) * (size_t)F->n * F->nbatch);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->vx, sizeof(double) * (size_t)F->n * F->nbatch);
-  if (e == hipSuccess) e = hipMalloc((void **)&F->part, sizeof(double) * (size_t)F->maxm * 32 * F->nbatch);
+  if (e == hipSuccess && !F->cr)
+    e = hipMalloc((void **)&F->tmp, sizeof(double) * std::max(1L, F->tmp_elems) * nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->info, sizeof(int) * nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->vb, sizeof(double) * (size_t)F->nv * nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->vz, sizeof(double) * (size_t)F->nv * nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->vx, sizeof(double) * (size_t)F->nv * nb);
+  if (e == hipSuccess)
+    e = hipMalloc((void **)&F->part, sizeof(double) * (size_t)F->maxm * 32 * nb * F->nkmax);
+  if (e == hipSuccess && F->cr) e = hipMalloc((void **)&F->cr_info, sizeof(int) * 2 * nb * (F->nlevels + 1));
   if (e != hipSuccess) {
     shellgpu_factor_destroy(F);
     return ferr(ctx, 2, std::string("factor allocation (") + std::to_string(bytes >> 20) +
                             " MiB): " + hipGetErrorString(e));
+  }
+  if (F->cr && (rc = build_cr_pointers(F))) {
+    shellgpu_factor_destroy(F);
+    return rc;
   }
   *out = F;
   return 0;
@@ -518,7 +647,7 @@ void shellgpu_factor_destroy(shellgpu_factor *F) {
   if (!F) return;
   (void)hipStreamSynchronize(F->ctx->stream);
   void *bufs[] = {F->buf, F->tmp, F->info, F->d_bcflag, F->d_pos2dof, F->d_blk_base, F->d_blk_ld,
-                  F->vb,  F->vz,  F->vx,   F->part};
+                  F->vb,  F->vz,  F->vx,   F->part, F->d_mlev, F->d_ptr,  F->cr_info};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete F;
@@ -531,6 +660,9 @@ int shellgpu_factor_info(shellgpu_factor *F, int *nlevels, int *max_level_dofs, 
   if (gib) *gib = sizeof(double) * (double)F->per_design * F->nbatch / (1024.0 * 1024.0 * 1024.0);
   return 0;
 }
+
+static int factor_cr(shellgpu_factor *F);
+static int apply_cr(shellgpu_factor *F);
 
 int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   shellgpu_ctx *ctx = F->ctx;
@@ -549,6 +681,7 @@ int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
                        F->buf, ps);
     HIPCALL(ctx, hipGetLastError());
   }
+  if (F->cr) return factor_cr(F);
   HIPCALL(ctx, hipMemsetAsync(F->info, 0, sizeof(int) * nb, ctx->stream));
   const double one = 1.0, mone = -1.0;
   for (int l = 0; l < F->nlevels; l++) {
@@ -582,32 +715,154 @@ int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   return 0;
 }
 
-// y = beta y + alpha op(A) x over the batch (device, strides per design)
-static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, int lda, const double *x, double alpha,
-                double beta, double *y, int lower) {
+// y = beta y + alpha op(A) x over nq batch entries (GemvBatch strides)
+static int gemv_batch(shellgpu_factor *F, bool trans, int m, int n, const double *A, int lda, const double *x,
+                      double alpha, double beta, double *y, int lower, const GemvBatch &g, int nq) {
   shellgpu_ctx *ctx = F->ctx;
-  const int nb = F->nbatch;
-  const long ps = F->per_design, sv = F->n;
+  if (nq <= 0) return 0;
   if (!trans) {
     const int nsplit = std::max(1, std::min(32, n / 64));
     const int chunk = (n + nsplit - 1) / nsplit;
-    const dim3 grid((m + 127) / 128, nsplit, nb);
+    const dim3 grid((m + 127) / 128, nsplit, nq);
     if (lower)
-      hipLaunchKernelGGL(gemv_n_pass1<1>, grid, dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv, chunk, F->part);
+      hipLaunchKernelGGL(gemv_n_pass1<1>, grid, dim3(256), 0, ctx->stream, m, n, A, lda, x, g, chunk, F->part);
     else
-      hipLaunchKernelGGL(gemv_n_pass1<0>, grid, dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv, chunk, F->part);
+      hipLaunchKernelGGL(gemv_n_pass1<0>, grid, dim3(256), 0, ctx->stream, m, n, A, lda, x, g, chunk, F->part);
     HIPCALL(ctx, hipGetLastError());
-    hipLaunchKernelGGL(gemv_n_pass2, dim3((m + 255) / 256, nb), dim3(256), 0, ctx->stream, m, nsplit, F->part, alpha,
-                       beta, y, sv);
+    hipLaunchKernelGGL(gemv_n_pass2, dim3((m + 255) / 256, nq), dim3(256), 0, ctx->stream, m, nsplit, F->part, alpha,
+                       beta, y, g);
     HIPCALL(ctx, hipGetLastError());
   } else {
     if (lower)
-      hipLaunchKernelGGL(gemv_t_kernel<1>, dim3((n + 3) / 4, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv,
-                         alpha, beta, y, sv);
+      hipLaunchKernelGGL(gemv_t_kernel<1>, dim3((n + 3) / 4, nq), dim3(256), 0, ctx->stream, m, n, A, lda, x, g,
+                         alpha, beta, y);
     else
-      hipLaunchKernelGGL(gemv_t_kernel<0>, dim3((n + 3) / 4, nb), dim3(256), 0, ctx->stream, m, n, A, lda, ps, x, sv,
-                         alpha, beta, y, sv);
+      hipLaunchKernelGGL(gemv_t_kernel<0>, dim3((n + 3) / 4, nq), dim3(256), 0, ctx->stream, m, n, A, lda, x, g,
+                         alpha, beta, y);
     HIPCALL(ctx, hipGetLastError());
+  }
+  return 0;
+}
+
+// the same for the sequential factorisation: one entry per design
+static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, int lda, const double *x, double alpha,
+                double beta, double *y, int lower) {
+  const GemvBatch g{F->nbatch, F->per_design, 0, F->nv, 0, F->nv, 0};
+  return gemv_batch(F, trans, m, n, A, lda, x, alpha, beta, y, lower, g, F->nbatch);
+}
+
+// Block cyclic reduction of the level-block-tridiagonal matrix in F->buf
+// (after the scatter).  Round with stride s eliminates levels i = s, 3s,
+// ... with active neighbours l = i - s and r = i + s:
+//   C_i C_i^T = D_i,  X_i = C_i^{-1},  Y_i = X_i B_i,  W_i = B_r X_i^T,
+//   D_l -= Y_i^T Y_i,  D_r -= W_i W_i^T,  B_r <- -W_i Y_i,
+// all entries of a round (levels x designs) in one batched call each;
+// level 0 is factored last.
+static int factor_cr(shellgpu_factor *F) {
+  shellgpu_ctx *ctx = F->ctx;
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  const int nb = F->nbatch, nl = F->nlevels, M = F->M;
+  const double one = 1.0, mone = -1.0, zero = 0.0;
+  if (*std::min_element(F->m.begin(), F->m.end()) < M) {
+    hipLaunchKernelGGL(pad_identity_kernel, dim3((M + 255) / 256, nl, nb), dim3(256), 0, ctx->stream, M, (long)M * M,
+                       F->d_mlev, F->buf, F->per_design);
+    HIPCALL(ctx, hipGetLastError());
+  }
+  const long half = (long)nb * (nl + 1);
+  HIPCALL(ctx, hipMemsetAsync(F->cr_info, 0, sizeof(int) * 2 * half, ctx->stream));
+  long ioff = 0;
+  for (const FactorRound &r : F->rounds) {
+    const int cnt = nb * r.nE, cntR = nb * r.nR;
+    double **P = F->d_ptr + r.off;
+    double **pD = P, **pB = P + cnt, **pZ = P + 2 * cnt, **pDl = P + 3 * cnt, **pDr = P + 4 * cnt,
+           **pBr = P + 5 * cnt;
+    BLASCALL(ctx, rocsolver_dpotrf_batched(h, rocblas_fill_lower, M, pD, M, F->cr_info + ioff, cnt));
+    BLASCALL(ctx, rocsolver_dtrtri_batched(h, rocblas_fill_lower, rocblas_diagonal_non_unit, M, pD, M,
+                                           F->cr_info + half + ioff, cnt));
+    // Y_i = X_i B_i (in place)
+    BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                        rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
+                                        (const double *const *)pB, M, pB, M, cnt));
+    if (cntR > 0)  // W_i = B_r X_i^T
+      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                          rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
+                                          (const double *const *)pBr, M, pZ, M, cntR));
+    // D_l -= Y_i^T Y_i (full GEMM: rocBLAS's batched SYRK runs its
+    // diagonal tiles in a slow non-MFMA kernel at these sizes)
+    BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_transpose, rocblas_operation_none, M, M, M, &mone,
+                                        (const double *const *)pB, M, (const double *const *)pB, M, &one, pDl, M,
+                                        cnt));
+    if (cntR > 0) {
+      // D_r -= W_i W_i^T
+      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_none, rocblas_operation_transpose, M, M, M, &mone,
+                                          (const double *const *)pZ, M, (const double *const *)pZ, M, &one, pDr, M,
+                                          cntR));
+      // B_r = -W_i Y_i (coupling of r to l)
+      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_none, rocblas_operation_none, M, M, M, &mone,
+                                          (const double *const *)pZ, M, (const double *const *)pB, M, &zero, pBr, M,
+                                          cntR));
+    }
+    ioff += cnt;
+  }
+  double **pRoot = F->d_ptr + F->root_off;
+  BLASCALL(ctx, rocsolver_dpotrf_batched(h, rocblas_fill_lower, M, pRoot, M, F->cr_info + ioff, nb));
+  BLASCALL(ctx, rocsolver_dtrtri_batched(h, rocblas_fill_lower, rocblas_diagonal_non_unit, M, pRoot, M,
+                                         F->cr_info + half + ioff, nb));
+  ioff += nb;
+  std::vector<int> info(ioff);
+  HIPCALL(ctx, hipMemcpyAsync(info.data(), F->cr_info, sizeof(int) * ioff, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int> bad(nb, 0);
+  for (long q = 0; q < ioff; q++)
+    if (info[q] != 0) bad[q % nb] = 1;
+  HIPCALL(ctx, hipMemsetAsync(F->info, 0, sizeof(int) * nb, ctx->stream));
+  HIPCALL(ctx, hipMemcpyAsync(F->info, bad.data(), sizeof(int) * nb, hipMemcpyHostToDevice, ctx->stream));
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < nb; b++)
+    if (bad[b])
+      return ferr(ctx, 5, "matrix not positive definite in the level Cholesky (design " + std::to_string(b) +
+                              "): for K + sigma G the shift is above the first buckling load");
+  F->factored = true;
+  return 0;
+}
+
+// x = A^{-1} b by the cyclic-reduction factors: reduction sweep, root,
+// back-substitution sweep (level-ordered vectors vb, vz, vx)
+static int apply_cr(shellgpu_factor *F) {
+  const int nb = F->nbatch, nl = F->nlevels, M = F->M;
+  const long M2 = (long)M * M, PD = F->per_design, NV = F->nv;
+  double *D0 = F->buf, *B0 = F->buf + (long)nl * M2, *Z0 = F->buf + 2L * nl * M2;
+  int rc = 0;
+  for (const FactorRound &r : F->rounds) {
+    const long s = r.s;
+    const GemvBatch g{nb, PD, 2 * s * M2, NV, 2 * s * M, NV, 2 * s * M};
+    const int cnt = nb * r.nE, cntR = nb * r.nR;
+    // c_i = X_i b_i
+    if ((rc = gemv_batch(F, false, M, M, D0 + s * M2, M, F->vb + s * M, 1.0, 0.0, F->vz + s * M, 1, g, cnt)))
+      return rc;
+    // b_l -= Y_i^T c_i
+    if ((rc = gemv_batch(F, true, M, M, B0 + s * M2, M, F->vz + s * M, -1.0, 1.0, F->vb, 0, g, cnt))) return rc;
+    // b_r -= W_i c_i
+    if ((rc = gemv_batch(F, false, M, M, Z0 + s * M2, M, F->vz + s * M, -1.0, 1.0, F->vb + 2 * s * M, 0, g, cntR)))
+      return rc;
+  }
+  {
+    const GemvBatch g{nb, PD, 0, NV, 0, NV, 0};
+    if ((rc = gemv_batch(F, false, M, M, D0, M, F->vb, 1.0, 0.0, F->vz, 1, g, nb))) return rc;
+    if ((rc = gemv_batch(F, true, M, M, D0, M, F->vz, 1.0, 0.0, F->vx, 1, g, nb))) return rc;
+  }
+  for (auto it = F->rounds.rbegin(); it != F->rounds.rend(); ++it) {
+    const long s = it->s;
+    const GemvBatch g{nb, PD, 2 * s * M2, NV, 2 * s * M, NV, 2 * s * M};
+    const int cnt = nb * it->nE, cntR = nb * it->nR;
+    // c_i -= Y_i x_l
+    if ((rc = gemv_batch(F, false, M, M, B0 + s * M2, M, F->vx, -1.0, 1.0, F->vz + s * M, 0, g, cnt))) return rc;
+    // c_i -= W_i^T x_r
+    if ((rc = gemv_batch(F, true, M, M, Z0 + s * M2, M, F->vx + 2 * s * M, -1.0, 1.0, F->vz + s * M, 0, g, cntR)))
+      return rc;
+    // x_i = X_i^T c_i
+    if ((rc = gemv_batch(F, true, M, M, D0 + s * M2, M, F->vz + s * M, 1.0, 0.0, F->vx + s * M, 1, g, cnt)))
+      return rc;
   }
   return 0;
 }
@@ -615,32 +870,36 @@ static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, i
 int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
   shellgpu_ctx *ctx = F->ctx;
   if (!F->factored) return ferr(ctx, 1, "factor not computed");
-  const int nb = F->nbatch, n = F->n;
+  const int nb = F->nbatch;
+  const long nv = F->nv, n = F->n;
   int rc = 0;
-  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256, nb), dim3(256), 0, ctx->stream, n, F->d_pos2dof, b, F->vb,
-                     1);
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv, n,
+                     F->d_pos2dof, b, F->vb, 1);
   HIPCALL(ctx, hipGetLastError());
-  // forward: y_l = C_l^{-1} (b_l - V_l y_{l-1})
-  for (int l = 0; l < F->nlevels; l++) {
-    const int ml = F->m[l];
-    double *bl = F->vb + F->voff[l];
-    if (l > 0 && (rc = gemv(F, false, ml, F->m[l - 1], F->buf + F->L_off[l], ml, F->vz + F->voff[l - 1], -1.0, 1.0,
-                             bl, 0)))
-      return rc;
-    if ((rc = gemv(F, false, ml, ml, F->buf + F->S_off[l], ml, bl, 1.0, 0.0, F->vz + F->voff[l], 1))) return rc;
+  if (F->cr) {
+    if ((rc = apply_cr(F))) return rc;
+  } else {
+    // forward: y_l = C_l^{-1} (b_l - V_l y_{l-1})
+    for (int l = 0; l < F->nlevels; l++) {
+      const int ml = F->m[l];
+      double *bl = F->vb + F->voff[l];
+      if (l > 0 && (rc = gemv(F, false, ml, F->m[l - 1], F->buf + F->L_off[l], ml, F->vz + F->voff[l - 1], -1.0,
+                               1.0, bl, 0)))
+        return rc;
+      if ((rc = gemv(F, false, ml, ml, F->buf + F->S_off[l], ml, bl, 1.0, 0.0, F->vz + F->voff[l], 1))) return rc;
+    }
+    // backward: x_l = C_l^{-T} (y_l - V_{l+1}^T x_{l+1})
+    for (int l = F->nlevels - 1; l >= 0; l--) {
+      const int ml = F->m[l];
+      double *yl = F->vz + F->voff[l];
+      if (l < F->nlevels - 1 && (rc = gemv(F, true, F->m[l + 1], ml, F->buf + F->L_off[l + 1], F->m[l + 1],
+                                           F->vx + F->voff[l + 1], -1.0, 1.0, yl, 0)))
+        return rc;
+      if ((rc = gemv(F, true, ml, ml, F->buf + F->S_off[l], ml, yl, 1.0, 0.0, F->vx + F->voff[l], 1))) return rc;
+    }
   }
-  // backward: x_l = C_l^{-T} (y_l - V_{l+1}^T x_{l+1})
-  for (int l = F->nlevels - 1; l >= 0; l--) {
-    const int ml = F->m[l];
-    double *yl = F->vz + F->voff[l];
-    if (l < F->nlevels - 1 &&
-        (rc = gemv(F, true, F->m[l + 1], ml, F->buf + F->L_off[l + 1], F->m[l + 1], F->vx + F->voff[l + 1], -1.0, 1.0,
-                   yl, 0)))
-      return rc;
-    if ((rc = gemv(F, true, ml, ml, F->buf + F->S_off[l], ml, yl, 1.0, 0.0, F->vx + F->voff[l], 1))) return rc;
-  }
-  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256, nb), dim3(256), 0, ctx->stream, n, F->d_pos2dof, F->vx, x,
-                     0);
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv, n,
+                     F->d_pos2dof, F->vx, x, 0);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }

@@ -22,7 +22,11 @@ def make_ctx(cyl, thick):
     return ctx
 
 
-def test_direct_solve_and_gmres():
+@pytest.mark.parametrize("factor", ["auto", "thomas"])
+def test_direct_solve_and_gmres(factor, monkeypatch):
+    # auto: block cyclic reduction over the (uniform) ring levels;
+    # thomas: the sequential level Cholesky
+    monkeypatch.setenv("SHELLGPU_FACTOR", factor)
     cyl = Cylinder(10, 20)
     thick = np.random.default_rng(1).uniform(0.0015, 0.0025, cyl.nelems)
     ctx = make_ctx(cyl, thick)
@@ -54,8 +58,10 @@ def _subspace_angle(V, W, rowp, cols, Kvals):
     return float(np.arccos(min(1.0, s.min())))
 
 
+@pytest.mark.parametrize("factor", ["auto", "thomas"])
 @pytest.mark.parametrize("na,nc", [(20, 40)])
-def test_buckling_matches_oracle(na, nc):
+def test_buckling_matches_oracle(na, nc, factor, monkeypatch):
+    monkeypatch.setenv("SHELLGPU_FACTOR", factor)
     cyl = Cylinder(na, nc)
     thick = np.full(cyl.nelems, 0.002)
     sigma = 0.9 * cyl.classical_load()
