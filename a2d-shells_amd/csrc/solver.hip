@@ -60,6 +60,14 @@ struct FactorRound {
   long off;       // first pointer of this round in the device pointer arrays
 };
 
+// one step of the recursive Cholesky-inverse of a diagonal block, in
+// execution order: a leaf (n2 == 0: factor and invert rows/cols [r0, r0 +
+// n1) in LDS), the update after the leading half (first == 1) or the
+// off-diagonal inverse after the trailing half (first == 0)
+struct CholNode {
+  int r0, n1, n2, first;
+};
+
 struct shellgpu_factor {
   shellgpu_ctx *ctx = nullptr;
   int nbatch = 0;
@@ -93,6 +101,11 @@ struct shellgpu_factor {
   int *cr_info = nullptr;    // potrf / trtri status per (round entry)
   std::vector<FactorRound> rounds;
   long root_off = 0;         // pointer group of the root level D_0
+  // recursive Cholesky-inverse of the M x M diagonal blocks: the node
+  // sequence (shared by all rounds) and, per round (and the root, last),
+  // the offset of each node's pointer arrays in d_ptr
+  std::vector<CholNode> chol;
+  std::vector<std::vector<long>> chol_off;
 };
 
 namespace {
@@ -259,6 +272,65 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double 
   }
 }
 
+// Cholesky factor and its inverse of one n x n SPD block (n <= 96) per
+// workgroup, in LDS: A (column-major, lda) -> X = chol(A)^{-1}, lower, with
+// zeros above the diagonal of the block; info[entry] = 1 for a
+// non-positive pivot.
+constexpr int kCholBase = 96;
+__global__ __launch_bounds__(256) void cholinv_base_kernel(double *const *ptrs, int n, int lda, int *info) {
+  constexpr int ld = kCholBase + 1;  // odd row stride: column walks hit distinct banks
+  __shared__ double L[kCholBase * ld];
+  __shared__ double X[kCholBase * ld];
+  __shared__ int bad;
+  double *A = ptrs[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t == 0) bad = 0;
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e % n, j = e / n;
+    L[i * ld + j] = A[i + (long)j * lda];
+  }
+  __syncthreads();
+  for (int j = 0; j < n; j++) {
+    if (t == 0) {
+      double d = L[j * ld + j];
+      if (!(d > 0.0)) {
+        bad = 1;
+        d = 1.0;
+      }
+      L[j * ld + j] = sqrt(d);
+    }
+    __syncthreads();
+    const double djj = L[j * ld + j];
+    for (int i = j + 1 + t; i < n; i += 256) L[i * ld + j] /= djj;
+    __syncthreads();
+    const int m = n - j - 1;
+    for (int e = t; e < m * m; e += 256) {
+      const int ii = e / m, kk = e % m;
+      if (kk <= ii) {
+        const int i = j + 1 + ii, k = j + 1 + kk;
+        L[i * ld + k] -= L[i * ld + j] * L[k * ld + j];
+      }
+    }
+    __syncthreads();
+  }
+  // X = L^{-1}: thread c forms column c by forward substitution
+  if (t < n) {
+    const int c = t;
+    X[c * ld + c] = 1.0 / L[c * ld + c];
+    for (int i = c + 1; i < n; i++) {
+      double acc = 0.0;
+      for (int k = c; k < i; k++) acc += L[i * ld + k] * X[k * ld + c];
+      X[i * ld + c] = -acc / L[i * ld + i];
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e % n, j = e / n;
+    A[i + (long)j * lda] = (i >= j) ? X[i * ld + j] : 0.0;
+  }
+  if (t == 0 && bad) info[blockIdx.x] = 1;
+}
+
 // identity on the padded diagonal entries of the cyclic-reduction levels
 __global__ void pad_identity_kernel(int M, long M2, const int *__restrict__ mlev, double *__restrict__ buf,
                                     long per_design) {
@@ -313,6 +385,21 @@ __global__ void dot_batched_kernel(int n, const double *x, long xs, const double
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicAdd(&out[b], red[0]);
+}
+
+// recursive halving of an n-block at r0 down to leaves of <= kCholBase
+void chol_plan(int r0, int n, std::vector<CholNode> &seq) {
+  if (n <= kCholBase) {
+    seq.push_back({r0, n, 0, 0});
+    return;
+  }
+  int n1 = n / 2;
+  n1 -= n1 & 1;  // even split keeps 16-byte alignment of the sub-blocks
+  const int n2 = n - n1;
+  chol_plan(r0, n1, seq);
+  seq.push_back({r0, n1, n2, 1});
+  chol_plan(r0 + n1, n2, seq);
+  seq.push_back({r0, n1, n2, 0});
 }
 
 int setup_plan(shellgpu_factor *F) {
@@ -415,6 +502,8 @@ int setup_plan(shellgpu_factor *F) {
       F->rounds.push_back(r);
       F->nkmax = std::max(F->nkmax, r.nE);
     }
+    F->chol.clear();
+    chol_plan(0, (int)M, F->chol);
   } else {
     F->nv = F->n;
     F->S_off.assign(nl, 0);
@@ -590,6 +679,27 @@ int build_cr_pointers(shellgpu_factor *F) {
   }
   const long root = (long)h.size();
   for (int d = 0; d < nb; d++) h.push_back(F->buf + d * PD);
+  // Cholesky-inverse nodes: per group of diagonal blocks (each round's
+  // D_i, then the root) and node, arrays of A11, A21, A22 (leaves: A11)
+  const long Ml = F->M;
+  F->chol_off.clear();
+  for (size_t g = 0; g <= F->rounds.size(); g++) {
+    const bool is_root = g == F->rounds.size();
+    const long cnt = is_root ? nb : (long)nb * F->rounds[g].nE;
+    const long src = is_root ? root : F->rounds[g].off;  // D pointers of the group
+    std::vector<double *> D(h.begin() + src, h.begin() + src + cnt);
+    std::vector<long> offs;
+    for (const CholNode &c : F->chol) {
+      offs.push_back((long)h.size());
+      const long a11 = (long)c.r0 * (Ml + 1), a21 = a11 + c.n1, a22 = a11 + (long)c.n1 * (Ml + 1);
+      for (long q = 0; q < cnt; q++) h.push_back(D[q] + a11);
+      if (c.n2 > 0) {
+        for (long q = 0; q < cnt; q++) h.push_back(D[q] + a21);
+        for (long q = 0; q < cnt; q++) h.push_back(D[q] + a22);
+      }
+    }
+    F->chol_off.push_back(offs);
+  }
   HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
   HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
   F->root_off = root;
@@ -751,6 +861,42 @@ static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, i
   return gemv_batch(F, trans, m, n, A, lda, x, alpha, beta, y, lower, g, F->nbatch);
 }
 
+// X = chol(A)^{-1} (lower) in place for the cnt diagonal blocks of group g:
+// leaves in LDS; internal nodes by batched BLAS on the halves,
+//   C21 = A21 X11^T,  A22 -= C21 C21^T,  (trailing half),
+//   X21 = -X22 C21 X11.
+static int cholinv(shellgpu_factor *F, size_t g, int cnt, int *info) {
+  shellgpu_ctx *ctx = F->ctx;
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  const int M = F->M;
+  const double one = 1.0, mone = -1.0;
+  const std::vector<long> &offs = F->chol_off[g];
+  for (size_t k = 0; k < F->chol.size(); k++) {
+    const CholNode &c = F->chol[k];
+    double **P = F->d_ptr + offs[k];
+    double **A11 = P, **A21 = P + cnt, **A22 = P + 2 * (long)cnt;
+    if (c.n2 == 0) {
+      hipLaunchKernelGGL(cholinv_base_kernel, dim3(cnt), dim3(256), 0, ctx->stream, A11, c.n1, M, info);
+      HIPCALL(ctx, hipGetLastError());
+    } else if (c.first) {
+      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                          rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A11, M,
+                                          (const double *const *)A21, M, A21, M, cnt));
+      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_none, rocblas_operation_transpose, c.n2, c.n2, c.n1,
+                                          &mone, (const double *const *)A21, M, (const double *const *)A21, M, &one,
+                                          A22, M, cnt));
+    } else {
+      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A11, M,
+                                          (const double *const *)A21, M, A21, M, cnt));
+      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, c.n2, c.n1, &mone, (const double *const *)A22, M,
+                                          (const double *const *)A21, M, A21, M, cnt));
+    }
+  }
+  return 0;
+}
+
 // Block cyclic reduction of the level-block-tridiagonal matrix in F->buf
 // (after the scatter).  Round with stride s eliminates levels i = s, 3s,
 // ... with active neighbours l = i - s and r = i + s:
@@ -771,14 +917,13 @@ static int factor_cr(shellgpu_factor *F) {
   const long half = (long)nb * (nl + 1);
   HIPCALL(ctx, hipMemsetAsync(F->cr_info, 0, sizeof(int) * 2 * half, ctx->stream));
   long ioff = 0;
+  int rc = 0;
   for (const FactorRound &r : F->rounds) {
     const int cnt = nb * r.nE, cntR = nb * r.nR;
     double **P = F->d_ptr + r.off;
     double **pD = P, **pB = P + cnt, **pZ = P + 2 * cnt, **pDl = P + 3 * cnt, **pDr = P + 4 * cnt,
            **pBr = P + 5 * cnt;
-    BLASCALL(ctx, rocsolver_dpotrf_batched(h, rocblas_fill_lower, M, pD, M, F->cr_info + ioff, cnt));
-    BLASCALL(ctx, rocsolver_dtrtri_batched(h, rocblas_fill_lower, rocblas_diagonal_non_unit, M, pD, M,
-                                           F->cr_info + half + ioff, cnt));
+    if ((rc = cholinv(F, &r - F->rounds.data(), cnt, F->cr_info + ioff))) return rc;
     // Y_i = X_i B_i (in place)
     BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                                         rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
@@ -804,10 +949,7 @@ static int factor_cr(shellgpu_factor *F) {
     }
     ioff += cnt;
   }
-  double **pRoot = F->d_ptr + F->root_off;
-  BLASCALL(ctx, rocsolver_dpotrf_batched(h, rocblas_fill_lower, M, pRoot, M, F->cr_info + ioff, nb));
-  BLASCALL(ctx, rocsolver_dtrtri_batched(h, rocblas_fill_lower, rocblas_diagonal_non_unit, M, pRoot, M,
-                                         F->cr_info + half + ioff, nb));
+  if ((rc = cholinv(F, F->rounds.size(), nb, F->cr_info + ioff))) return rc;
   ioff += nb;
   std::vector<int> info(ioff);
   HIPCALL(ctx, hipMemcpyAsync(info.data(), F->cr_info, sizeof(int) * ioff, hipMemcpyDeviceToHost, ctx->stream));
