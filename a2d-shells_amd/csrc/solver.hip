@@ -1371,6 +1371,11 @@ struct shellgpu_buckling {
   std::vector<int> ns;
   // K, G, A and F belong to the object unless the caller supplied them
   bool own_ops = true;
+  // a second factor holding K (zero state) for the static path, the shift
+  // estimate and the path adjoint, when device memory allows; otherwise
+  // FK == F and K is refactored where needed
+  shellgpu_factor *FK = nullptr;
+  bool fk_valid = false;
   // static-path GMRES settings
   int ksm_m = 10, ksm_restarts = 15;
   double ksm_rtol = 1e-12, ksm_atol = 1e-30;
@@ -1379,6 +1384,8 @@ struct shellgpu_buckling {
 namespace {
 
 int buckling_free(shellgpu_buckling *B) {
+  if (B->FK && B->FK != B->F) shellgpu_factor_destroy(B->FK);
+  B->FK = nullptr;
   if (B->own_ops) {
     if (B->K) shellgpu_mat_destroy(B->K);
     if (B->G) shellgpu_mat_destroy(B->G);
@@ -1515,10 +1522,19 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // assembled at zero state, the path is the static solution K u = f (with
   // the BC-driven residual) unless u0 is given
   if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, B->K))) return rc;
+  if (!B->FK && shellgpu_factor_create(ctx, &B->FK) != 0) {
+    B->FK = B->F;  // no room for a second factor
+    ctx->err.clear();
+    (void)hipGetLastError();  // drop the failed allocation's error state
+  }
+  B->fk_valid = false;
+  if (!u0 || B->sigma <= 0.0) {
+    if ((rc = shellgpu_factor_compute(B->FK, B->K))) return rc;
+    B->fk_valid = true;
+  }
   if (u0) {
     HIPCALL(ctx, hipMemcpyAsync(B->u, u0, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice, ctx->stream));
   } else {
-    if ((rc = shellgpu_factor_compute(B->F, B->K))) return rc;
     if ((rc = shellgpu_assemble_res(ctx, SHELLGPU_LINEAR, nullptr, B->res))) return rc;
     if (force) {
       HIPCALL(ctx,
@@ -1529,7 +1545,7 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
     }
     int its = 0;
     double rn = 0.0;
-    if ((rc = shellgpu_gmres(ctx, B->K, B->F, B->res, B->u, B->ksm_m, B->ksm_restarts, B->ksm_rtol, B->ksm_atol,
+    if ((rc = shellgpu_gmres(ctx, B->K, B->FK, B->res, B->u, B->ksm_m, B->ksm_restarts, B->ksm_rtol, B->ksm_atol,
                              &its, &rn)))
       return rc;
     std::vector<double> m1(nb, -1.0);
@@ -1541,8 +1557,7 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // shift: sigma > 0 as given; sigma <= 0 places it at 0.95 of a first-load
   // estimate from a short buckling-mode Lanczos on K^{-1}(-G)
   if (B->sigma <= 0.0) {
-    if (u0 && (rc = shellgpu_factor_compute(B->F, B->K))) return rc;
-    if ((rc = lanczos(ctx, B->F, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-4, B->Q, B->KQ, B->hv, B->res, lo)))
+    if ((rc = lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-4, B->Q, B->KQ, B->hv, B->res, lo)))
       return rc;
     for (int d = 0; d < nb; d++) {
       const int ns = (int)lo.alpha[d].size();
@@ -1564,6 +1579,7 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
       HIPCALL(ctx, hipGetLastError());
     }
     if ((rc = shellgpu_mat_apply_bcs(B->A, 1.0))) return rc;
+    if (B->FK == B->F) B->fk_valid = false;
     rc = shellgpu_factor_compute(B->F, B->A);
     if (rc == 0) break;
     if (rc != 5 || attempt >= 8) return rc;
@@ -1676,17 +1692,30 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
   // the tangent of the residual that defines the path (K for the linear
   // model; the nonlinear tangent at u otherwise)
   if ((rc = shellgpu_vec_zero_bcs(ctx, B->gv))) return rc;
+  shellgpu_mat *T = B->K;
   if (B->model == SHELLGPU_NONLINEAR) {
     if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, B->u, nullptr, B->A))) return rc;
-  } else {
-    if ((rc = shellgpu_mat_copy(B->A, B->K))) return rc;
+    T = B->A;
   }
-  // the shifted operator is no longer factored: a later solve refactors
-  if ((rc = shellgpu_factor_compute(B->F, B->A))) return rc;
   {
     int its = 0;
-    double rn = 0.0;
-    if ((rc = shellgpu_gmres(ctx, B->A, B->F, B->gv, B->psi, 10, 3, 1e-12, 1e-300, &its, &rn))) return rc;
+    double rn = 1.0;
+    if (B->FK != B->F) {
+      // GMRES on the path tangent preconditioned by the factor of K (exact
+      // for the linear path; a few iterations for a nonlinear path)
+      if (!B->fk_valid) {
+        if ((rc = shellgpu_factor_compute(B->FK, B->K))) return rc;
+        B->fk_valid = true;
+      }
+      if ((rc = shellgpu_gmres(ctx, T, B->FK, B->gv, B->psi, 20, 4, 1e-12, 1e-300, &its, &rn))) return rc;
+    }
+    if (!(rn <= 1e-10)) {
+      // factor the tangent itself (the shifted operator's factor is lost
+      // when the factors are shared: a later solve refactors)
+      if (B->FK == B->F) B->fk_valid = false;
+      if ((rc = shellgpu_factor_compute(B->F, T))) return rc;
+      if ((rc = shellgpu_gmres(ctx, T, B->F, B->gv, B->psi, 10, 3, 1e-12, 1e-300, &its, &rn))) return rc;
+    }
   }
   for (int d = 0; d < nb; d++) {
     hc[d] = -1.0;
