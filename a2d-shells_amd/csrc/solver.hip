@@ -105,7 +105,14 @@ struct shellgpu_factor {
   // sequence (shared by all rounds) and, per round (and the root, last),
   // the offset of each node's pointer arrays in d_ptr
   std::vector<CholNode> chol;
-  std::vector<std::vector<long>> chol_off;
+  // pointer-array groups of the Cholesky-inverse: cyclic reduction, one per
+  // round and the root (last); sequential factorisation, one per level
+  struct CholGroup {
+    int n, lda, cnt;
+    std::vector<CholNode> nodes;
+    std::vector<long> offs;  // per node: first pointer (A11, A21, A22 arrays of cnt)
+  };
+  std::vector<CholGroup> cgroups;
 };
 
 namespace {
@@ -471,7 +478,7 @@ int setup_plan(shellgpu_factor *F) {
   }
 
   // Block cyclic reduction when the levels are (nearly) uniform and the
-  // padded factor (three M x M blocks per level and design) fits in 3/4 of
+  // padded factor (three M x M blocks per level and design) fits in 85 % of
   // the free device memory; otherwise the sequential level Cholesky.
   // SHELLGPU_FACTOR=thomas selects the sequential factorisation.
   {
@@ -480,7 +487,7 @@ int setup_plan(shellgpu_factor *F) {
     HIPCALL(ctx, hipMemGetInfo(&freeb, &totalb));
     const double need = 3.0 * nl * (double)M * (double)M * 8.0 * ctx->nbatch;
     const char *env = getenv("SHELLGPU_FACTOR");
-    F->cr = nl >= 3 && (double)nl * M <= 1.15 * F->n && need <= 0.75 * (double)freeb;
+    F->cr = nl >= 3 && (double)nl * M <= 1.15 * F->n && need <= 0.85 * (double)freeb;
     if (env && strcmp(env, "thomas") == 0) F->cr = false;
   }
   if (F->cr) {
@@ -644,6 +651,42 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
   return 0;
 }
 
+// Cholesky-inverse node sequence of an n-block and, per node, the pointer
+// arrays (A11, A21, A22; leaves A11 only) of the blocks D[q] (lda)
+void add_chol_group(shellgpu_factor *F, std::vector<double *> &h, const std::vector<double *> &D, int n, int lda) {
+  shellgpu_factor::CholGroup grp;
+  grp.n = n;
+  grp.lda = lda;
+  grp.cnt = (int)D.size();
+  chol_plan(0, n, grp.nodes);
+  for (const CholNode &c : grp.nodes) {
+    grp.offs.push_back((long)h.size());
+    const long a11 = (long)c.r0 * (lda + 1), a21 = a11 + c.n1, a22 = a11 + (long)c.n1 * (lda + 1);
+    for (double *p : D) h.push_back(p + a11);
+    if (c.n2 > 0) {
+      for (double *p : D) h.push_back(p + a21);
+      for (double *p : D) h.push_back(p + a22);
+    }
+  }
+  F->cgroups.push_back(grp);
+}
+
+// the sequential factorisation's Cholesky-inverse groups: one per level
+int build_thomas_pointers(shellgpu_factor *F) {
+  shellgpu_ctx *ctx = F->ctx;
+  const int nb = F->nbatch;
+  std::vector<double *> h;
+  F->cgroups.clear();
+  for (int l = 0; l < F->nlevels; l++) {
+    std::vector<double *> D(nb);
+    for (int d = 0; d < nb; d++) D[d] = F->buf + d * F->per_design + F->S_off[l];
+    add_chol_group(F, h, D, F->m[l], F->m[l]);
+  }
+  HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
+  HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
 // Pointer arrays of every cyclic-reduction round, entry q = k nb + d
 // (level entry k, design d): D_i, B_i, Z_i, D_{i-s}, D_{i+s}, B_{i+s} for
 // i = s (2k + 1); the last three exist for the first nR nb entries only.
@@ -679,26 +722,13 @@ int build_cr_pointers(shellgpu_factor *F) {
   }
   const long root = (long)h.size();
   for (int d = 0; d < nb; d++) h.push_back(F->buf + d * PD);
-  // Cholesky-inverse nodes: per group of diagonal blocks (each round's
-  // D_i, then the root) and node, arrays of A11, A21, A22 (leaves: A11)
-  const long Ml = F->M;
-  F->chol_off.clear();
+  F->cgroups.clear();
   for (size_t g = 0; g <= F->rounds.size(); g++) {
     const bool is_root = g == F->rounds.size();
     const long cnt = is_root ? nb : (long)nb * F->rounds[g].nE;
     const long src = is_root ? root : F->rounds[g].off;  // D pointers of the group
     std::vector<double *> D(h.begin() + src, h.begin() + src + cnt);
-    std::vector<long> offs;
-    for (const CholNode &c : F->chol) {
-      offs.push_back((long)h.size());
-      const long a11 = (long)c.r0 * (Ml + 1), a21 = a11 + c.n1, a22 = a11 + (long)c.n1 * (Ml + 1);
-      for (long q = 0; q < cnt; q++) h.push_back(D[q] + a11);
-      if (c.n2 > 0) {
-        for (long q = 0; q < cnt; q++) h.push_back(D[q] + a21);
-        for (long q = 0; q < cnt; q++) h.push_back(D[q] + a22);
-      }
-    }
-    F->chol_off.push_back(offs);
+    add_chol_group(F, h, D, F->M, F->M);
   }
   HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
   HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
@@ -739,13 +769,13 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
   if (e == hipSuccess) e = hipMalloc((void **)&F->vx, sizeof(double) * (size_t)F->nv * nb);
   if (e == hipSuccess)
     e = hipMalloc((void **)&F->part, sizeof(double) * (size_t)F->maxm * 32 * nb * F->nkmax);
-  if (e == hipSuccess && F->cr) e = hipMalloc((void **)&F->cr_info, sizeof(int) * 2 * nb * (F->nlevels + 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&F->cr_info, sizeof(int) * 2 * nb * (F->nlevels + 1));
   if (e != hipSuccess) {
     shellgpu_factor_destroy(F);
     return ferr(ctx, 2, std::string("factor allocation (") + std::to_string(bytes >> 20) +
                             " MiB): " + hipGetErrorString(e));
   }
-  if (F->cr && (rc = build_cr_pointers(F))) {
+  if ((rc = F->cr ? build_cr_pointers(F) : build_thomas_pointers(F))) {
     shellgpu_factor_destroy(F);
     return rc;
   }
@@ -773,6 +803,7 @@ int shellgpu_factor_info(shellgpu_factor *F, int *nlevels, int *max_level_dofs, 
 
 static int factor_cr(shellgpu_factor *F);
 static int apply_cr(shellgpu_factor *F);
+static int cholinv(shellgpu_factor *F, size_t g, int *info);
 
 int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   shellgpu_ctx *ctx = F->ctx;
@@ -792,7 +823,7 @@ int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
     HIPCALL(ctx, hipGetLastError());
   }
   if (F->cr) return factor_cr(F);
-  HIPCALL(ctx, hipMemsetAsync(F->info, 0, sizeof(int) * nb, ctx->stream));
+  HIPCALL(ctx, hipMemsetAsync(F->cr_info, 0, sizeof(int) * (long)F->nlevels * nb, ctx->stream));
   const double one = 1.0, mone = -1.0;
   for (int l = 0; l < F->nlevels; l++) {
     const int ml = F->m[l];
@@ -806,17 +837,20 @@ int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
                                                   mp, ps, V, ml, ps, F->tmp, ml, (long)ml * mp, nb));
       HIPCALL(ctx, hipMemcpy2DAsync(V, sizeof(double) * ps, F->tmp, sizeof(double) * (size_t)ml * mp,
                                     sizeof(double) * (size_t)ml * mp, nb, hipMemcpyDeviceToDevice, ctx->stream));
-      // S_l = D_l - V_l V_l^T (lower triangle)
-      BLASCALL(ctx, rocblas_dsyrk_strided_batched(h, rocblas_fill_lower, rocblas_operation_none, ml, mp, &mone, V, ml,
-                                                  ps, &one, S, ml, ps, nb));
+      // S_l = D_l - V_l V_l^T (full GEMM: rocBLAS's batched SYRK runs its
+      // diagonal tiles in a slow non-MFMA kernel at these sizes)
+      BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, ml, ml, mp,
+                                                  &mone, V, ml, ps, V, ml, ps, &one, S, ml, ps, nb));
     }
-    BLASCALL(ctx, rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, ml, S, ml, ps, F->info, nb));
-    BLASCALL(ctx, rocsolver_dtrtri_strided_batched(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ml, S, ml, ps,
-                                                   F->info, nb));
+    // C_l^{-1} in place (recursive Cholesky-inverse, per design)
+    if ((rc = cholinv(F, l, F->cr_info + (long)l * nb))) return rc;
   }
-  std::vector<int> info(nb);
-  HIPCALL(ctx, hipMemcpyAsync(info.data(), F->info, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<int> all((long)F->nlevels * nb), info(nb, 0);
+  HIPCALL(ctx, hipMemcpyAsync(all.data(), F->cr_info, sizeof(int) * all.size(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  for (size_t q = 0; q < all.size(); q++)
+    if (all[q] != 0) info[q % nb] = 1;
+  HIPCALL(ctx, hipMemcpy(F->info, info.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   for (int b = 0; b < nb; b++)
     if (info[b] != 0)
       return ferr(ctx, 5, "matrix not positive definite in the level Cholesky (design " + std::to_string(b) +
@@ -865,15 +899,15 @@ static int gemv(shellgpu_factor *F, bool trans, int m, int n, const double *A, i
 // leaves in LDS; internal nodes by batched BLAS on the halves,
 //   C21 = A21 X11^T,  A22 -= C21 C21^T,  (trailing half),
 //   X21 = -X22 C21 X11.
-static int cholinv(shellgpu_factor *F, size_t g, int cnt, int *info) {
+static int cholinv(shellgpu_factor *F, size_t g, int *info) {
   shellgpu_ctx *ctx = F->ctx;
   rocblas_handle h = (rocblas_handle)ctx->blas;
-  const int M = F->M;
+  const shellgpu_factor::CholGroup &grp = F->cgroups[g];
+  const int M = grp.lda, cnt = grp.cnt;
   const double one = 1.0, mone = -1.0;
-  const std::vector<long> &offs = F->chol_off[g];
-  for (size_t k = 0; k < F->chol.size(); k++) {
-    const CholNode &c = F->chol[k];
-    double **P = F->d_ptr + offs[k];
+  for (size_t k = 0; k < grp.nodes.size(); k++) {
+    const CholNode &c = grp.nodes[k];
+    double **P = F->d_ptr + grp.offs[k];
     double **A11 = P, **A21 = P + cnt, **A22 = P + 2 * (long)cnt;
     if (c.n2 == 0) {
       hipLaunchKernelGGL(cholinv_base_kernel, dim3(cnt), dim3(256), 0, ctx->stream, A11, c.n1, M, info);
@@ -923,7 +957,7 @@ static int factor_cr(shellgpu_factor *F) {
     double **P = F->d_ptr + r.off;
     double **pD = P, **pB = P + cnt, **pZ = P + 2 * cnt, **pDl = P + 3 * cnt, **pDr = P + 4 * cnt,
            **pBr = P + 5 * cnt;
-    if ((rc = cholinv(F, &r - F->rounds.data(), cnt, F->cr_info + ioff))) return rc;
+    if ((rc = cholinv(F, &r - F->rounds.data(), F->cr_info + ioff))) return rc;
     // Y_i = X_i B_i (in place)
     BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                                         rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
@@ -949,7 +983,7 @@ static int factor_cr(shellgpu_factor *F) {
     }
     ioff += cnt;
   }
-  if ((rc = cholinv(F, F->rounds.size(), nb, F->cr_info + ioff))) return rc;
+  if ((rc = cholinv(F, F->rounds.size(), F->cr_info + ioff))) return rc;
   ioff += nb;
   std::vector<int> info(ioff);
   HIPCALL(ctx, hipMemcpyAsync(info.data(), F->cr_info, sizeof(int) * ioff, hipMemcpyDeviceToHost, ctx->stream));
