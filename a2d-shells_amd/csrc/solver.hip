@@ -477,18 +477,24 @@ int setup_plan(shellgpu_factor *F) {
     F->maxm = std::max(F->maxm, F->m[l]);
   }
 
-  // Block cyclic reduction when the levels are (nearly) uniform and the
+  // Block cyclic reduction when the levels are (nearly) uniform, the
   // padded factor (three M x M blocks per level and design) fits in 85 % of
-  // the free device memory; otherwise the sequential level Cholesky.
-  // SHELLGPU_FACTOR=thomas selects the sequential factorisation.
+  // the free device memory and the batch is small; otherwise the sequential
+  // level Cholesky.  SHELLGPU_FACTOR=thomas / =cr selects either.
   {
     const long M = F->maxm;
     size_t freeb = 0, totalb = 0;
     HIPCALL(ctx, hipMemGetInfo(&freeb, &totalb));
     const double need = 3.0 * nl * (double)M * (double)M * 8.0 * ctx->nbatch;
     const char *env = getenv("SHELLGPU_FACTOR");
-    F->cr = nl >= 3 && (double)nl * M <= 1.15 * F->n && need <= 0.85 * (double)freeb;
+    const bool can = nl >= 3 && (double)nl * M <= 1.15 * F->n && need <= 0.85 * (double)freeb;
+    // cyclic reduction pays off while one level's batched work is too small
+    // to fill the GPU (few designs); with many designs the sequential
+    // factorisation is already parallel and its factor is smaller (two
+    // blocks per level instead of three: fewer bytes per solve)
+    F->cr = can && (long)ctx->nbatch * M <= 8192;
     if (env && strcmp(env, "thomas") == 0) F->cr = false;
+    if (env && strcmp(env, "cr") == 0) F->cr = can;
   }
   if (F->cr) {
     const long M = F->maxm, M2 = M * M;

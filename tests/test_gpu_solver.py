@@ -22,9 +22,9 @@ def make_ctx(cyl, thick):
     return ctx
 
 
-@pytest.mark.parametrize("factor", ["auto", "thomas"])
+@pytest.mark.parametrize("factor", ["cr", "thomas"])
 def test_direct_solve_and_gmres(factor, monkeypatch):
-    # auto: block cyclic reduction over the (uniform) ring levels;
+    # cr: block cyclic reduction over the (uniform) ring levels;
     # thomas: the sequential level Cholesky
     monkeypatch.setenv("SHELLGPU_FACTOR", factor)
     cyl = Cylinder(10, 20)
@@ -58,7 +58,7 @@ def _subspace_angle(V, W, rowp, cols, Kvals):
     return float(np.arccos(min(1.0, s.min())))
 
 
-@pytest.mark.parametrize("factor", ["auto", "thomas"])
+@pytest.mark.parametrize("factor", ["cr", "thomas"])
 @pytest.mark.parametrize("na,nc", [(20, 40)])
 def test_buckling_matches_oracle(na, nc, factor, monkeypatch):
     monkeypatch.setenv("SHELLGPU_FACTOR", factor)
@@ -92,7 +92,9 @@ def test_buckling_matches_oracle(na, nc, factor, monkeypatch):
         i = j
 
 
-def test_batched_designs_match_single_designs():
+@pytest.mark.parametrize("factor", ["cr", "thomas"])
+def test_batched_designs_match_single_designs(factor, monkeypatch):
+    monkeypatch.setenv("SHELLGPU_FACTOR", factor)
     cyl = Cylinder(8, 16, imperfection=(2e-4, 2, 3))
     rng = np.random.default_rng(9)
     thick = rng.uniform(0.0015, 0.0025, (3, cyl.nelems))
