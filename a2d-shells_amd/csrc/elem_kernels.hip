@@ -1,5 +1,5 @@
 // elem_kernels.hip -- per-element residual / stiffness / geometric-stiffness
-// kernels for the 4-node director shell, one wavefront per element.
+// kernels for the 4-node director shell, two elements per wavefront.
 //
 // Formulation.  The reference (TACSShellElement.h:303-672) builds R_e and
 // K_e by chaining first/second derivatives of the strain energy through
@@ -24,37 +24,53 @@
 //
 // with B1 = dB/d(eps) along u and sig(u) = w detXd C (B0 u - e_thermal).
 //
-// Work split inside the wavefront (64 lanes): geometry per node / quadrature
-// point / tying point on lanes 0-16, strain-Jacobian columns on lanes 0-23
-// (and 32-55 for B1), and the 24 x 24 product as one 3 x 3 register tile per
-// lane.  Elements of one colour share no node, so each wavefront adds its 16
-// blocks into the BSR values without atomics.
+// Work split: each half-wavefront (32 lanes) owns one element and its LDS.
+// Geometry per node / quadrature point / tying point runs on lanes 0-16,
+// strain-Jacobian columns on lanes 0-23 (B1 after B), and the 24 x 24 product
+// as one 3 x 6 register tile per lane.  Packing two elements per wavefront
+// fills the lanes that the per-element geometry phases leave idle.  Elements
+// of one colour share no node, so each half adds its 16 blocks into the BSR
+// values without atomics.
 #include "elem_common.h"
 #include "shellgpu_internal.h"
 
 namespace shellgpu {
 
+// LDS of one element.  kTwo: the mode also forms B1 = dB/dq along the state
+// (geometric stiffness and the thickness adjoint), which doubles the tying
+// Jacobian and B storage.  The union holds arrays of disjoint phases: the
+// node frames and tying coefficients (P1-P3) and the B / C B columns (P5 on).
+template <bool kTwo>
 struct ElemSmem {
+  static constexpr int kNB = kTwo ? 2 : 1;
   double X[12];
   double s[24];
+  double v2[24];  // second state of the adjoint modes: path u or adjoint psi
   double fn[12];
-  double Tn[36], XTn[36];
   double T[36], XT[36], XzT[36], wdet[4];
-  double tXxi[54], tn0[27];
-  double tcU[54], tcd[27];    // tying derivative coefficients (B or B0)
-  double tcU1[54], tcd1[27];  // tying derivative coefficients of B1
   double ety[9];
   double W[180];  // [qp][5][9] tying value -> strain rows (0,1,2,6,7)
   double u0x[36], u1x[36];
   double drot[96];  // [node][24] d(drill strain at node)/dq
   double etn[4];
-  double Jty[216], Jty1[216];  // [tying point][24]
-  double sig[36];              // [qp][9]
+  double Jty[kNB][216];  // [B or B1][tying point][24]
+  double sig[36];        // [qp][9]
   double dety[9];
-  // [strain][24] for one qp.  The adjoint mode, which forms no C B
-  // products, keeps in Y: path u [0,24), adjoint psi [24,48), five strain
-  // vectors [48,93) and two weighted stress vectors [93,111)
-  double B[216], Y[216], B1[216], Y1[216];
+  Section sec;  // section stiffness of the element's thickness
+  struct Frames {
+    double Tn[36], XTn[36], tXxi[54];
+    double tcU[kNB][54], tcd[kNB][27];  // tying derivative coefficients
+  };
+  // [strain][24] for one qp.  The adjoint modes, which form no C B
+  // products, keep in Y[0]: five strain vectors [0,45) and two weighted
+  // stress vectors [45,63)
+  struct Cols {
+    double B[kNB][216], Y[kNB][216];
+  };
+  union {
+    Frames f;
+    Cols c;
+  };
 };
 
 // rows of the 9-strain vector fed by the tying strains
@@ -66,7 +82,8 @@ __device__ __forceinline__ int tying_row(int r) {
 // d(u0x)/dq_k = alpha beta^T, d(u1x)/dq_k = alpha gamma^T
 // (from TacsShellComputeDispGrad, TACSShellUtilities.h:361-422, with the
 // linearized-rotation director d = q x n, TACSDirector.h:215-242)
-__device__ __forceinline__ void dof_abg(const ElemSmem &sm, int iq, int k,
+template <class SM>
+__device__ __forceinline__ void dof_abg(const SM &sm, int iq, int k,
                                         const double N[4], const double Na[4],
                                         const double Nb[4], double al[3],
                                         double be[3], double ga[3]) {
@@ -103,8 +120,8 @@ __device__ __forceinline__ void dof_abg(const ElemSmem &sm, int iq, int k,
 enum BVariant { kBLin = 0, kBFull = 1, kBQuad = 2 };
 
 // one column of B_qp (9 strains) for dof k
-template <int VAR>
-__device__ __forceinline__ void build_bcol(const ElemSmem &sm, int iq, int k,
+template <int VAR, class SM>
+__device__ __forceinline__ void build_bcol(const SM &sm, int iq, int k,
                                            const double *J, double out[9]) {
   double u, v, N[4], Na[4], Nb[4];
   quad_point(iq, u, v);
@@ -163,7 +180,8 @@ __device__ __forceinline__ void apply_c(const Section &c, double w,
 // second derivative of the tying strains summed with weights dety
 // (the dety * d2(ety)/dq2 terms of addComputeTyingStrainHessian,
 // TACSShellElementModel.h:782-1031)
-__device__ __forceinline__ double tying_geo(const ElemSmem &sm, int k, int l) {
+template <class SM>
+__device__ __forceinline__ double tying_geo(const SM &sm, int k, int l) {
   const int n = k / 6, r = k % 6, m = l / 6, rp = l % 6;
   if (r >= 3 && rp >= 3) return 0.0;
   if (r < 3 && rp < 3) {
@@ -204,11 +222,19 @@ __device__ __forceinline__ double tying_geo(const ElemSmem &sm, int k, int l) {
   return h;
 }
 
+// Two elements per wavefront: lanes 0-31 work on element 2 blockIdx.x of the
+// launch list, lanes 32-63 on the next one, each half with its own LDS.  An
+// odd list leaves the last upper half computing a copy of the lower half's
+// element with its writes suppressed.
 template <int MODE>
 __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
-  __shared__ ElemSmem sm;
-  const int lane = threadIdx.x;
-  const int e = a.elems[blockIdx.x];
+  constexpr bool kTwo = (MODE == kModeG || MODE == kModeAdjDV);
+  __shared__ ElemSmem<kTwo> smem[2];
+  const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
+  ElemSmem<kTwo> &sm = smem[half];
+  const int slot = 2 * blockIdx.x + half;
+  const bool valid = slot < a.count;
+  const int e = a.elems[valid ? slot : slot - 1];
   const long design = blockIdx.y;
   constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
   constexpr bool kState = (MODE != kModeKLin);
@@ -220,7 +246,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   const double thick = a.thick[design * a.nelems + e];
   double temp = a.temp ? a.temp[design * a.nelems + e] : 0.0;
   double alpha = a.alpha;
-  const Section sec = make_section(a.mat, thick);
+  if (lane == 0) sm.sec = make_section(a.mat, thick);
+  const Section &sec = sm.sec;
   const int *cn = a.conn + 4 * e;
 
   // ---- P0: gather coordinates and state ----
@@ -230,9 +257,9 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
                      ? a.state[design * a.state_stride + 6 * cn[lane / 6] + lane % 6]
                      : 0.0;
     if (MODE == kModeAdjDV)
-      sm.Y[lane] = a.u2 ? a.u2[design * a.u2_stride + 6 * cn[lane / 6] + lane % 6] : 0.0;
+      sm.v2[lane] = a.u2 ? a.u2[design * a.u2_stride + 6 * cn[lane / 6] + lane % 6] : 0.0;
     if (MODE == kModeAdjResLin || MODE == kModeAdjResNl)
-      sm.Y[24 + lane] = a.psi[design * a.psi_stride + 6 * cn[lane / 6] + lane % 6];
+      sm.v2[lane] = a.psi[design * a.psi_stride + 6 * cn[lane / 6] + lane % 6];
   }
   __syncthreads();
 
@@ -287,8 +314,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     mm3(Xdi, Tn, XTn);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      sm.Tn[9 * n + i] = Tn[i];
-      sm.XTn[9 * n + i] = XTn[i];
+      sm.f.Tn[9 * n + i] = Tn[i];
+      sm.f.XTn[9 * n + i] = XTn[i];
     }
   } else if (lane >= 8 && lane < 17) {
     const int t = lane - 8;
@@ -304,7 +331,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
       }
 #pragma unroll
-    for (int i = 0; i < 6; i++) sm.tXxi[6 * t + i] = Xxi[i];
+    for (int i = 0; i < 6; i++) sm.f.tXxi[6 * t + i] = Xxi[i];
   }
   __syncthreads();
 
@@ -314,10 +341,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     // TacsShellAddDrillStrainSens map (TACSShellUtilities.h:787-835) with
     // TACSLinearizedRotation: et = 0.5 (Ct[3] + u0x[3] - Ct[1] - u0x[1])
     const int n = lane;
-    const double *Tn = &sm.Tn[9 * n], *XTn = &sm.XTn[9 * n];
-    double row[24];
-#pragma unroll
-    for (int i = 0; i < 24; i++) row[i] = 0.0;
+    const double *Tn = &sm.f.Tn[9 * n], *XTn = &sm.f.XTn[9 * n];
     const double dA[9] = {0, -0.5, 0, 0.5, 0, 0, 0, 0, 0};
     double tmp[9], dC[9], du0d[9];
     // dC = Tn dA Tn^T ; du0d = Tn dA XTn^T
@@ -337,23 +361,22 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
                          dA[3 * i + 2] * XTn[3 * j + 2];
       }
     mm3(Tn, tmp, du0d);
-    row[6 * n + 3] += -(dC[7] - dC[5]);
-    row[6 * n + 4] += -(dC[2] - dC[6]);
-    row[6 * n + 5] += -(dC[3] - dC[1]);
+    // rotation dofs of node n enter through C, displacements of every node
+    // through u0x
+    const double rq[3] = {-(dC[7] - dC[5]), -(dC[2] - dC[6]), -(dC[3] - dC[1])};
     double u, v, N[4], Na[4], Nb[4];
     node_point(n, u, v);
     shape4(u, v, N, Na, Nb);
     const double g[6] = {du0d[0], du0d[1], du0d[3], du0d[4], du0d[6], du0d[7]};
+    double et = 0.0;
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-      for (int d = 0; d < 3; d++) row[6 * m + d] += Na[m] * g[2 * d] + Nb[m] * g[2 * d + 1];
-    double et = 0.0;
-#pragma unroll
-    for (int i = 0; i < 24; i++) {
-      sm.drot[24 * n + i] = row[i];
-      et += row[i] * sm.s[i];
-    }
+      for (int d = 0; d < 6; d++) {
+        const double val = (d < 3) ? Na[m] * g[2 * d] + Nb[m] * g[2 * d + 1] : (m == n ? rq[d - 3] : 0.0);
+        sm.drot[24 * n + 6 * m + d] = val;
+        et += val * sm.s[6 * m + d];
+      }
     sm.etn[n] = et;
   } else if (lane < 8) {
     // quadrature point kinematics (TacsShellComputeDispGrad,
@@ -460,7 +483,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         U[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
       }
     }
-    const double *X = &sm.tXxi[6 * t];
+    const double *X = &sm.f.tXxi[6 * t];
     const int f = tying_field(t);
     // tying strain value (TACSShellElementModel.h:32-77 and 643-697)
     double lin = 0.0, quad = 0.0;
@@ -521,8 +544,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         dd[1] = 0.5 * av[2];
         dd[2] = 0.5 * av[4];
       }
-      double *cU = pass == 0 ? sm.tcU : sm.tcU1;
-      double *cd = pass == 0 ? sm.tcd : sm.tcd1;
+      double *cU = sm.f.tcU[pass];
+      double *cd = sm.f.tcd[pass];
 #pragma unroll
       for (int i = 0; i < 6; i++) cU[6 * t + i] = dU[i];
 #pragma unroll
@@ -534,14 +557,12 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // ---- P3: tying-strain Jacobian rows; strains and stresses at the
   // quadrature points ----
   {
-    const int pass =
-        (lane < 24) ? 0 : (((MODE == kModeG || MODE == kModeAdjDV) && lane >= 32 && lane < 56) ? 1 : -1);
-    if (pass >= 0) {
-      const int k = (pass == 0) ? lane : lane - 32;
+    for (int pass = 0; pass < ElemSmem<kTwo>::kNB && lane < 24; pass++) {
+      const int k = lane;
       const int n = k / 6, r = k % 6;
-      const double *cU = pass == 0 ? sm.tcU : sm.tcU1;
-      const double *cd = pass == 0 ? sm.tcd : sm.tcd1;
-      double *J = pass == 0 ? sm.Jty : sm.Jty1;
+      const double *cU = sm.f.tcU[pass];
+      const double *cd = sm.f.tcd[pass];
+      double *J = sm.Jty[pass];
 #pragma unroll
       for (int t = 0; t < 9; t++) {
         double u, v, N[4], Na[4], Nb[4];
@@ -558,8 +579,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         J[24 * t + k] = val;
       }
     }
-    if ((kRes || kGeo) && lane >= 56 && lane < 60) {
-      const int iq = lane - 56;
+    if ((kRes || kGeo) && lane >= 24 && lane < 28) {
+      const int iq = lane - 24;
       double u, v, N[4], Na[4], Nb[4];
       quad_point(iq, u, v);
       shape4(u, v, N, Na, Nb);
@@ -617,32 +638,30 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   //   phi^T G(u) phi = sum w [2 Q(u, phi)^T C e1(phi) + (e1(u) - e_th)^T C Q(phi, phi)]
   if (MODE == kModeAdjDV) {
     const Section dsec = make_section_dt(a.mat, thick);
-    const double *su = &sm.Y[0];
-    double *E = &sm.Y[48], *cv = &sm.Y[93];
+    const double *su = sm.v2;
+    double *E = &sm.c.Y[0][0], *cv = &sm.c.Y[0][45];
     const double ca = a.coef[2 * design], cb = a.coef[2 * design + 1];
     double gdv = 0.0, gu = 0.0;
     for (int iq = 0; iq < 4; iq++) {
       if (lane < 24) {
         double b[9];
-        build_bcol<kBLin>(sm, iq, lane, sm.Jty, b);
+        build_bcol<kBLin>(sm, iq, lane, sm.Jty[0], b);
 #pragma unroll
-        for (int i = 0; i < 9; i++) sm.B[24 * i + lane] = b[i];
-      } else if (lane >= 32 && lane < 56) {
-        double b[9];
-        build_bcol<kBQuad>(sm, iq, lane - 32, sm.Jty1, b);
+        for (int i = 0; i < 9; i++) sm.c.B[0][24 * i + lane] = b[i];
+        build_bcol<kBQuad>(sm, iq, lane, sm.Jty[1], b);
 #pragma unroll
-        for (int i = 0; i < 9; i++) sm.B1[24 * i + lane - 32] = b[i];
+        for (int i = 0; i < 9; i++) sm.c.B[1][24 * i + lane] = b[i];
       }
       __syncthreads();
       // E = [e1(phi), e1(u), Q(phi, u), Q(phi, phi)]
-      if (lane < 36) {
-        const int v = lane / 9, i = lane % 9;
-        const double *M = (v < 2) ? &sm.B[24 * i] : &sm.B1[24 * i];
+      for (int it = lane; it < 36; it += 32) {
+        const int v = it / 9, i = it % 9;
+        const double *M = (v < 2) ? &sm.c.B[0][24 * i] : &sm.c.B[1][24 * i];
         const double *x = (v == 0 || v == 3) ? sm.s : su;
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < 24; k++) acc += M[k] * x[k];
-        E[lane] = acc;
+        E[it] = acc;
       }
       __syncthreads();
       if (lane == 0) {
@@ -674,12 +693,12 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       // d/du: Bq(phi)^T (2 w C e1(phi)) + B0^T (w C Q(phi, phi))
       if (lane < 24) {
 #pragma unroll
-        for (int i = 0; i < 9; i++) gu += sm.B1[24 * i + lane] * cv[i] + sm.B[24 * i + lane] * cv[9 + i];
+        for (int i = 0; i < 9; i++) gu += sm.c.B[1][24 * i + lane] * cv[i] + sm.c.B[0][24 * i + lane] * cv[9 + i];
       }
       __syncthreads();
     }
-    if (lane == 0 && a.dv) a.dv[design * a.nelems + e] += gdv;
-    if (lane < 24 && a.res) a.res[design * a.res_stride + 6 * cn[lane / 6] + lane % 6] += gu;
+    if (valid && lane == 0 && a.dv) a.dv[design * a.nelems + e] += gdv;
+    if (valid && lane < 24 && a.res) a.res[design * a.res_stride + 6 * cn[lane / 6] + lane % 6] += gu;
     return;
   }
 
@@ -687,24 +706,24 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // with TACSIsoShellConstitutive::addStressDVSens) ----
   if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
     const Section dsec = make_section_dt(a.mat, thick);
-    const double *sp = &sm.Y[24];
-    double *E = &sm.Y[48];
+    const double *sp = sm.v2;
+    double *E = &sm.c.Y[0][0];
     double g = 0.0;
     for (int iq = 0; iq < 4; iq++) {
       if (lane < 24) {
         double b[9];
         if (kNonlin)
-          build_bcol<kBFull>(sm, iq, lane, sm.Jty, b);
+          build_bcol<kBFull>(sm, iq, lane, sm.Jty[0], b);
         else
-          build_bcol<kBLin>(sm, iq, lane, sm.Jty, b);
+          build_bcol<kBLin>(sm, iq, lane, sm.Jty[0], b);
 #pragma unroll
-        for (int i = 0; i < 9; i++) sm.B[24 * i + lane] = b[i];
+        for (int i = 0; i < 9; i++) sm.c.B[0][24 * i + lane] = b[i];
       }
       __syncthreads();
       if (lane < 9) {  // dR/dq^T psi strain: B(u) psi
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < 24; k++) acc += sm.B[24 * lane + k] * sp[k];
+        for (int k = 0; k < 24; k++) acc += sm.c.B[0][24 * lane + k] * sp[k];
         E[lane] = acc;
       }
       __syncthreads();
@@ -718,24 +737,26 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       }
       __syncthreads();
     }
-    if (lane == 0) a.dv[design * a.nelems + e] += a.coef[design] * g;
+    if (valid && lane == 0) a.dv[design * a.nelems + e] += a.coef[design] * g;
     return;
   }
 
   // ---- P5: quadrature loop: B columns, residual, B^T C B tile ----
-  const int row0 = 3 * (lane >> 3), col0 = 3 * (lane & 7);
-  double acc[9];
+  // lane tile: rows row0..row0+2 (one node's translations or rotations),
+  // columns col0..col0+5 (all dofs of node col0 / 6)
+  const int row0 = 3 * (lane >> 2), col0 = 6 * (lane & 3);
+  double acc[18];
 #pragma unroll
-  for (int i = 0; i < 9; i++) acc[i] = 0.0;
+  for (int i = 0; i < 18; i++) acc[i] = 0.0;
   double racc = 0.0;
 
   for (int iq = 0; iq < 4; iq++) {
     if (lane < 24) {
       double b[9], y[9];
       if (kNonlin) {
-        build_bcol<kBFull>(sm, iq, lane, sm.Jty, b);
+        build_bcol<kBFull>(sm, iq, lane, sm.Jty[0], b);
       } else {
-        build_bcol<kBLin>(sm, iq, lane, sm.Jty, b);
+        build_bcol<kBLin>(sm, iq, lane, sm.Jty[0], b);
       }
       if (kRes) {
 #pragma unroll
@@ -745,55 +766,48 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         apply_c(sec, sm.wdet[iq], b, y);
 #pragma unroll
         for (int i = 0; i < 9; i++) {
-          sm.B[24 * i + lane] = b[i];
-          sm.Y[24 * i + lane] = y[i];
+          sm.c.B[0][24 * i + lane] = b[i];
+          sm.c.Y[0][24 * i + lane] = y[i];
         }
       }
-    } else if (MODE == kModeG && lane >= 32 && lane < 56) {
-      const int k = lane - 32;
-      double b[9], y[9];
-      build_bcol<kBQuad>(sm, iq, k, sm.Jty1, b);
-      apply_c(sec, sm.wdet[iq], b, y);
+      if constexpr (MODE == kModeG) {
+        build_bcol<kBQuad>(sm, iq, lane, sm.Jty[1], b);
+        apply_c(sec, sm.wdet[iq], b, y);
 #pragma unroll
-      for (int i = 0; i < 9; i++) {
-        sm.B1[24 * i + k] = b[i];
-        sm.Y1[24 * i + k] = y[i];
+        for (int i = 0; i < 9; i++) {
+          sm.c.B[kTwo ? 1 : 0][24 * i + lane] = b[i];
+          sm.c.Y[kTwo ? 1 : 0][24 * i + lane] = y[i];
+        }
       }
     }
     if (kMat) {
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < 9; s++) {
-        const double b0 = sm.B[24 * s + row0], b1 = sm.B[24 * s + row0 + 1],
-                     b2 = sm.B[24 * s + row0 + 2];
-        if (MODE == kModeG) {
-          const double y0 = sm.Y1[24 * s + col0], y1 = sm.Y1[24 * s + col0 + 1],
-                       y2 = sm.Y1[24 * s + col0 + 2];
-          const double c0 = sm.B1[24 * s + row0], c1 = sm.B1[24 * s + row0 + 1],
-                       c2 = sm.B1[24 * s + row0 + 2];
-          const double z0 = sm.Y[24 * s + col0], z1 = sm.Y[24 * s + col0 + 1],
-                       z2 = sm.Y[24 * s + col0 + 2];
-          acc[0] += b0 * y0 + c0 * z0;
-          acc[1] += b0 * y1 + c0 * z1;
-          acc[2] += b0 * y2 + c0 * z2;
-          acc[3] += b1 * y0 + c1 * z0;
-          acc[4] += b1 * y1 + c1 * z1;
-          acc[5] += b1 * y2 + c1 * z2;
-          acc[6] += b2 * y0 + c2 * z0;
-          acc[7] += b2 * y1 + c2 * z1;
-          acc[8] += b2 * y2 + c2 * z2;
+        double b[3], y[6];
+#pragma unroll
+        for (int i = 0; i < 3; i++) b[i] = sm.c.B[0][24 * s + row0 + i];
+        if constexpr (MODE == kModeG) {
+          // B0^T C B1 + B1^T C B0
+          double c[3], z[6];
+#pragma unroll
+          for (int i = 0; i < 3; i++) c[i] = sm.c.B[kTwo ? 1 : 0][24 * s + row0 + i];
+#pragma unroll
+          for (int j = 0; j < 6; j++) {
+            y[j] = sm.c.Y[kTwo ? 1 : 0][24 * s + col0 + j];
+            z[j] = sm.c.Y[0][24 * s + col0 + j];
+          }
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) acc[6 * i + j] += b[i] * y[j] + c[i] * z[j];
         } else {
-          const double y0 = sm.Y[24 * s + col0], y1 = sm.Y[24 * s + col0 + 1],
-                       y2 = sm.Y[24 * s + col0 + 2];
-          acc[0] += b0 * y0;
-          acc[1] += b0 * y1;
-          acc[2] += b0 * y2;
-          acc[3] += b1 * y0;
-          acc[4] += b1 * y1;
-          acc[5] += b1 * y2;
-          acc[6] += b2 * y0;
-          acc[7] += b2 * y1;
-          acc[8] += b2 * y2;
+#pragma unroll
+          for (int j = 0; j < 6; j++) y[j] = sm.c.Y[0][24 * s + col0 + j];
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) acc[6 * i + j] += b[i] * y[j];
         }
       }
       __syncthreads();
@@ -805,44 +819,44 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) acc[3 * i + j] += tying_geo(sm, row0 + i, col0 + j);
+      for (int j = 0; j < 6; j++) acc[6 * i + j] += tying_geo(sm, row0 + i, col0 + j);
     for (int iq = 0; iq < 4; iq++) {
       double u, v, N[4], Na[4], Nb[4];
       quad_point(iq, u, v);
       shape4(u, v, N, Na, Nb);
       const double S00 = sm.sig[9 * iq + 3], S11 = sm.sig[9 * iq + 4], S01 = sm.sig[9 * iq + 5];
-      double ar[3][3], br[3][3], gr[3][3], ac[3][3], bc[3][3], gc[3][3];
+      double ar[3][3], br[3][3], gr[3][3];
 #pragma unroll
-      for (int i = 0; i < 3; i++) {
-        dof_abg(sm, iq, row0 + i, N, Na, Nb, ar[i], br[i], gr[i]);
-        dof_abg(sm, iq, col0 + i, N, Na, Nb, ac[i], bc[i], gc[i]);
-      }
+      for (int i = 0; i < 3; i++) dof_abg(sm, iq, row0 + i, N, Na, Nb, ar[i], br[i], gr[i]);
 #pragma unroll
-      for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 6; j++) {
+        double ac[3], bc[3], gc[3];
+        dof_abg(sm, iq, col0 + j, N, Na, Nb, ac, bc, gc);
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-          const double aa = ar[i][0] * ac[j][0] + ar[i][1] * ac[j][1] + ar[i][2] * ac[j][2];
-          const double h = S00 * (br[i][0] * gc[j][0] + bc[j][0] * gr[i][0]) +
-                           S11 * (br[i][1] * gc[j][1] + bc[j][1] * gr[i][1]) +
-                           S01 * (br[i][0] * gc[j][1] + bc[j][0] * gr[i][1] + br[i][1] * gc[j][0] +
-                                  bc[j][1] * gr[i][0]);
-          acc[3 * i + j] += aa * h;
+        for (int i = 0; i < 3; i++) {
+          const double aa = ar[i][0] * ac[0] + ar[i][1] * ac[1] + ar[i][2] * ac[2];
+          const double h = S00 * (br[i][0] * gc[0] + bc[0] * gr[i][0]) +
+                           S11 * (br[i][1] * gc[1] + bc[1] * gr[i][1]) +
+                           S01 * (br[i][0] * gc[1] + bc[0] * gr[i][1] + br[i][1] * gc[0] +
+                                  bc[1] * gr[i][0]);
+          acc[6 * i + j] += aa * h;
         }
+      }
     }
   }
 
   // ---- P7: scatter (elements of one colour share no node) ----
-  if (kMat) {
+  if (kMat && valid) {
     const int ni = row0 / 6, nj = col0 / 6;
-    const int rr = row0 % 6, cc = col0 % 6;
+    const int rr = row0 % 6;
     const long blk = a.blk[16 * e + 4 * ni + nj];
     double *dst = a.vals + design * a.vals_stride + 36 * blk;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) dst[6 * (rr + i) + cc + j] += alpha * acc[3 * i + j];
+      for (int j = 0; j < 6; j++) dst[6 * (rr + i) + j] += alpha * acc[6 * i + j];
   }
-  if (kRes && lane < 24) {
+  if (kRes && valid && lane < 24) {
     double *dst = a.res + design * a.res_stride;
     dst[6 * cn[lane / 6] + lane % 6] += alpha * racc;
   }
@@ -850,31 +864,33 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 
 // explicit instantiation + launch wrappers
 template <int MODE>
-static hipError_t launch_mode(const ElemArgs &args, int nblocks, int nbatch, hipStream_t stream) {
-  if (nblocks <= 0 || nbatch <= 0) return hipSuccess;
-  hipLaunchKernelGGL(elem_kernel<MODE>, dim3(nblocks, nbatch), dim3(64), 0, stream, args);
+static hipError_t launch_mode(const ElemArgs &args, int count, int nbatch, hipStream_t stream) {
+  if (count <= 0 || nbatch <= 0) return hipSuccess;
+  ElemArgs a = args;
+  a.count = count;
+  hipLaunchKernelGGL(elem_kernel<MODE>, dim3((count + 1) / 2, nbatch), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int nbatch,
+hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream) {
   switch (mode) {
     case kModeKLin:
-      return launch_mode<kModeKLin>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeKLin>(args, count, nbatch, stream);
     case kModeKNl:
-      return launch_mode<kModeKNl>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeKNl>(args, count, nbatch, stream);
     case kModeG:
-      return launch_mode<kModeG>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeG>(args, count, nbatch, stream);
     case kModeRLin:
-      return launch_mode<kModeRLin>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeRLin>(args, count, nbatch, stream);
     case kModeRNl:
-      return launch_mode<kModeRNl>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeRNl>(args, count, nbatch, stream);
     case kModeAdjDV:
-      return launch_mode<kModeAdjDV>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeAdjDV>(args, count, nbatch, stream);
     case kModeAdjResLin:
-      return launch_mode<kModeAdjResLin>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeAdjResLin>(args, count, nbatch, stream);
     case kModeAdjResNl:
-      return launch_mode<kModeAdjResNl>(args, nblocks, nbatch, stream);
+      return launch_mode<kModeAdjResNl>(args, count, nbatch, stream);
     default:
       return hipErrorInvalidValue;
   }

@@ -46,6 +46,7 @@ enum ElemMode {
 
 struct ElemArgs {
   const int *elems;  // element ids of this launch (one colour)
+  int count;         // number of them (set by the launcher)
   const int *conn;   // 4 node ids per element
   const int *blk;    // 16 BSR block ids per element, (row node i, col node j)
   const double *Xpts;
@@ -111,7 +112,7 @@ struct XptArgs {
 };
 hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream_t stream);
 
-hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int nblocks, int nbatch,
+hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream);
 
 }  // namespace shellgpu

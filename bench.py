@@ -319,7 +319,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": profiled_traffic("elem_stiffness_cylinder200x400.md"),
-            "kernel": "elem_kernel<0> (stiffness, one wavefront per element)",
+            "kernel": "elem_kernel<0> (stiffness, two elements per wavefront)",
             "avg_launch_ms": elem_ms / max(elem_launches, 1),
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
         },
