@@ -56,7 +56,6 @@ struct ElemSmem {
   double Jty[kNB][216];  // [B or B1][tying point][24]
   double sig[36];        // [qp][9]
   double dety[9];
-  Section sec;  // section stiffness of the element's thickness
   struct Frames {
     double Tn[36], XTn[36], tXxi[54];
     double tcU[kNB][54], tcd[kNB][27];  // tying derivative coefficients
@@ -163,17 +162,29 @@ __device__ __forceinline__ void build_bcol(const SM &sm, int iq, int k,
   out[8] = e8;
 }
 
+// Isotropic section stiffness with the entries that are zero by construction
+// dropped: A = [[a11 a12 0][a12 a11 0][0 0 a66]], D likewise, As = as I,
+// drill.  make_section and make_section_dt produce exactly this pattern, and
+// the dropped terms add exact zeros, so C b is unchanged bit for bit.
+struct IsoSection {
+  double a11, a12, a66, d11, d12, d66, as, drill;
+};
+
+__device__ __forceinline__ IsoSection iso_section(const Section &c) {
+  return IsoSection{c.A[0], c.A[1], c.A[5], c.D[0], c.D[1], c.D[5], c.As[0], c.drill};
+}
+
 // Y = C B for one column (C: A membrane, D bending, As shear, drill)
-__device__ __forceinline__ void apply_c(const Section &c, double w,
+__device__ __forceinline__ void apply_c(const IsoSection &c, double w,
                                         const double b[9], double y[9]) {
-  y[0] = w * (c.A[0] * b[0] + c.A[1] * b[1] + c.A[2] * b[2]);
-  y[1] = w * (c.A[1] * b[0] + c.A[3] * b[1] + c.A[4] * b[2]);
-  y[2] = w * (c.A[2] * b[0] + c.A[4] * b[1] + c.A[5] * b[2]);
-  y[3] = w * (c.D[0] * b[3] + c.D[1] * b[4] + c.D[2] * b[5]);
-  y[4] = w * (c.D[1] * b[3] + c.D[3] * b[4] + c.D[4] * b[5]);
-  y[5] = w * (c.D[2] * b[3] + c.D[4] * b[4] + c.D[5] * b[5]);
-  y[6] = w * (c.As[0] * b[6] + c.As[1] * b[7]);
-  y[7] = w * (c.As[1] * b[6] + c.As[2] * b[7]);
+  y[0] = w * (c.a11 * b[0] + c.a12 * b[1]);
+  y[1] = w * (c.a12 * b[0] + c.a11 * b[1]);
+  y[2] = w * (c.a66 * b[2]);
+  y[3] = w * (c.d11 * b[3] + c.d12 * b[4]);
+  y[4] = w * (c.d12 * b[3] + c.d11 * b[4]);
+  y[5] = w * (c.d66 * b[5]);
+  y[6] = w * (c.as * b[6]);
+  y[7] = w * (c.as * b[7]);
   y[8] = w * (c.drill * b[8]);
 }
 
@@ -246,8 +257,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   const double thick = a.thick[design * a.nelems + e];
   double temp = a.temp ? a.temp[design * a.nelems + e] : 0.0;
   double alpha = a.alpha;
-  if (lane == 0) sm.sec = make_section(a.mat, thick);
-  const Section &sec = sm.sec;
+  const IsoSection sec = iso_section(make_section(a.mat, thick));
   const int *cn = a.conn + 4 * e;
 
   // ---- P0: gather coordinates and state ----
@@ -557,26 +567,32 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // ---- P3: tying-strain Jacobian rows; strains and stresses at the
   // quadrature points ----
   {
-    for (int pass = 0; pass < ElemSmem<kTwo>::kNB && lane < 24; pass++) {
+    if (lane < 24) {
       const int k = lane;
       const int n = k / 6, r = k % 6;
-      const double *cU = sm.f.tcU[pass];
-      const double *cd = sm.f.tcd[pass];
-      double *J = sm.Jty[pass];
+      // shape values of node n at the tying points (coordinates in {-1, 0, 1},
+      // so every value below is exact and equals shape4's)
+      const double ha = (n & 1) ? 0.5 : -0.5, hb = (n & 2) ? 0.5 : -0.5;
+      // rotation rows: component c of fn[n] x dd
+      const int c = r < 3 ? 0 : r - 3, i1 = (c + 1) % 3, i2 = (c + 2) % 3;
+      const double fa = sm.fn[3 * n + i1], fb = sm.fn[3 * n + i2];
+      for (int pass = 0; pass < ElemSmem<kTwo>::kNB; pass++) {
+        const double *cU = sm.f.tcU[pass];
+        const double *cd = sm.f.tcd[pass];
+        double *J = sm.Jty[pass];
 #pragma unroll
-      for (int t = 0; t < 9; t++) {
-        double u, v, N[4], Na[4], Nb[4];
-        tying_point(t, u, v);
-        shape4(u, v, N, Na, Nb);
-        double val;
-        if (r < 3) {
-          val = cU[6 * t + 2 * r] * Na[n] + cU[6 * t + 2 * r + 1] * Nb[n];
-        } else {
-          double fx[3];
-          cross3(&sm.fn[3 * n], &cd[3 * t], fx);  // n x dd ; d(d.dd)/dq_r
-          val = N[n] * fx[r - 3];
+        for (int t = 0; t < 9; t++) {
+          double u, v;
+          tying_point(t, u, v);
+          const double na = 0.5 + ha * u, nb = 0.5 + hb * v;
+          double val;
+          if (r < 3) {  // N,a = ha nb and N,b = na hb
+            val = cU[6 * t + 2 * r] * (ha * nb) + cU[6 * t + 2 * r + 1] * (na * hb);
+          } else {
+            val = (na * nb) * (fa * cd[3 * t + i2] - fb * cd[3 * t + i1]);
+          }
+          J[24 * t + k] = val;
         }
-        J[24 * t + k] = val;
       }
     }
     if ((kRes || kGeo) && lane >= 24 && lane < 28) {
@@ -637,7 +653,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   //   phi^T K phi    = sum w e1(phi)^T C e1(phi),            e1 = B0 .
   //   phi^T G(u) phi = sum w [2 Q(u, phi)^T C e1(phi) + (e1(u) - e_th)^T C Q(phi, phi)]
   if (MODE == kModeAdjDV) {
-    const Section dsec = make_section_dt(a.mat, thick);
+    const IsoSection dsec = iso_section(make_section_dt(a.mat, thick));
     const double *su = sm.v2;
     double *E = &sm.c.Y[0][0], *cv = &sm.c.Y[0][45];
     const double ca = a.coef[2 * design], cb = a.coef[2 * design + 1];
@@ -705,7 +721,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // ---- psi^T dR_e/dt (TACSShellElement::addAdjResProduct, SE.h:774-868,
   // with TACSIsoShellConstitutive::addStressDVSens) ----
   if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
-    const Section dsec = make_section_dt(a.mat, thick);
+    const IsoSection dsec = iso_section(make_section_dt(a.mat, thick));
     const double *sp = sm.v2;
     double *E = &sm.c.Y[0][0];
     double g = 0.0;
