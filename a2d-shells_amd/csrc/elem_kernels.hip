@@ -36,26 +36,44 @@
 
 namespace shellgpu {
 
-// LDS of one element.  kTwo: the mode also forms B1 = dB/dq along the state
-// (geometric stiffness and the thickness adjoint), which doubles the tying
-// Jacobian and B storage.  The union holds arrays of disjoint phases: the
-// node frames and tying coefficients (P1-P3) and the B / C B columns (P5 on).
-template <bool kTwo>
+// What each kernel mode computes.
+template <int MODE>
+struct ModeTraits {
+  static constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
+  static constexpr bool kState = (MODE != kModeKLin);
+  static constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl);
+  static constexpr bool kRes =
+      (MODE == kModeRLin || MODE == kModeRNl || MODE == kModeAdjResLin || MODE == kModeAdjResNl);
+  static constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG);
+  // B1 = dB/dq along the state is formed too (geometric stiffness and the
+  // thickness adjoint), which doubles the tying Jacobian and B storage
+  static constexpr bool kTwo = (MODE == kModeG || MODE == kModeAdjDV);
+  static constexpr bool kAdj =
+      (MODE == kModeAdjDV || MODE == kModeAdjResLin || MODE == kModeAdjResNl);
+  static constexpr bool kStrain = kRes || kGeo;  // stresses at the quadrature points
+};
+
+// LDS of one element, arrays sized to what the mode uses.  The union holds
+// arrays of disjoint phases: the node frames and tying coefficients (P1-P3)
+// and the B / C B columns (P5 on).
+template <int MODE>
 struct ElemSmem {
-  static constexpr int kNB = kTwo ? 2 : 1;
+  using MT = ModeTraits<MODE>;
+  static constexpr int kNB = MT::kTwo ? 2 : 1;
   double X[12];
   double s[24];
-  double v2[24];  // second state of the adjoint modes: path u or adjoint psi
+  double v2[MT::kAdj ? 24 : 1];  // second state of the adjoint modes: path u or psi
   double fn[12];
   double T[36], XT[36], XzT[36], wdet[4];
-  double ety[9];
-  double W[180];  // [qp][5][9] tying value -> strain rows (0,1,2,6,7)
-  double u0x[36], u1x[36];
+  double ety[MT::kState ? 9 : 1];
+  // [qp][5 strain rows (0,1,2,6,7)][5 tying fields]: transformation of the
+  // interpolated tying fields to the strain rows
+  double Wf[100];
+  double u0x[MT::kState ? 36 : 1], u1x[MT::kState ? 36 : 1];
   double drot[96];  // [node][24] d(drill strain at node)/dq
   double etn[4];
-  double Jty[kNB][216];  // [B or B1][tying point][24]
-  double sig[36];        // [qp][9]
-  double dety[9];
+  double sig[MT::kStrain ? 36 : 1];  // [qp][9]
+  double dety[MT::kGeo ? 9 : 1];
   struct Frames {
     double Tn[36], XTn[36], tXxi[54];
     double tcU[kNB][54], tcd[kNB][27];  // tying derivative coefficients
@@ -71,6 +89,17 @@ struct ElemSmem {
     Cols c;
   };
 };
+
+// Tying fields interpolated at a point with the weights Nt of tying_interp:
+// each tying point feeds only its own field (g11: points 0,1; g22: 2,3;
+// g12: 4; g23: 5,6; g13: 7,8)
+__device__ __forceinline__ void tying_fields(const double Nt[9], const double x[9], double g[5]) {
+  g[kG11] = Nt[0] * x[0] + Nt[1] * x[1];
+  g[kG22] = Nt[2] * x[2] + Nt[3] * x[3];
+  g[kG12] = Nt[4] * x[4];
+  g[kG23] = Nt[5] * x[5] + Nt[6] * x[6];
+  g[kG13] = Nt[7] * x[7] + Nt[8] * x[8];
+}
 
 // rows of the 9-strain vector fed by the tying strains
 __device__ __forceinline__ int tying_row(int r) {
@@ -119,18 +148,22 @@ __device__ __forceinline__ void dof_abg(const SM &sm, int iq, int k,
 enum BVariant { kBLin = 0, kBFull = 1, kBQuad = 2 };
 
 // one column of B_qp (9 strains) for dof k
+// (J: this dof's column of the tying-strain Jacobian, one value per tying point)
 template <int VAR, class SM>
 __device__ __forceinline__ void build_bcol(const SM &sm, int iq, int k,
-                                           const double *J, double out[9]) {
+                                           const double J[9], double out[9]) {
   double u, v, N[4], Na[4], Nb[4];
   quad_point(iq, u, v);
   shape4(u, v, N, Na, Nb);
-  const double *Wq = &sm.W[45 * iq];
+  double Nt[9], g[5];
+  tying_interp(u, v, Nt);
+  tying_fields(Nt, J, g);
+  const double *Wq = &sm.Wf[25 * iq];
 #pragma unroll
   for (int r = 0; r < 5; r++) {
     double acc = 0.0;
 #pragma unroll
-    for (int t = 0; t < 9; t++) acc += Wq[9 * r + t] * J[24 * t + k];
+    for (int f = 0; f < 5; f++) acc += Wq[5 * r + f] * g[f];
     out[tying_row(r)] = acc;
   }
   double al[3], be[3], ga[3];
@@ -239,20 +272,18 @@ __device__ __forceinline__ double tying_geo(const SM &sm, int k, int l) {
 // element with its writes suppressed.
 template <int MODE>
 __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
-  constexpr bool kTwo = (MODE == kModeG || MODE == kModeAdjDV);
-  __shared__ ElemSmem<kTwo> smem[2];
+  using MT = ModeTraits<MODE>;
+  constexpr bool kTwo = MT::kTwo;
+  constexpr int kNB = ElemSmem<MODE>::kNB;
+  __shared__ ElemSmem<MODE> smem[2];
   const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
-  ElemSmem<kTwo> &sm = smem[half];
+  ElemSmem<MODE> &sm = smem[half];
   const int slot = 2 * blockIdx.x + half;
   const bool valid = slot < a.count;
   const int e = a.elems[valid ? slot : slot - 1];
   const long design = blockIdx.y;
-  constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
-  constexpr bool kState = (MODE != kModeKLin);
-  constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl);
-  constexpr bool kRes =
-      (MODE == kModeRLin || MODE == kModeRNl || MODE == kModeAdjResLin || MODE == kModeAdjResNl);
-  constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG);
+  constexpr bool kMat = MT::kMat, kState = MT::kState, kNonlin = MT::kNonlin;
+  constexpr bool kRes = MT::kRes, kGeo = MT::kGeo;
 
   const double thick = a.thick[design * a.nelems + e];
   double temp = a.temp ? a.temp[design * a.nelems + e] : 0.0;
@@ -424,10 +455,10 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     }
     sm.wdet[iq] = det;  // Gauss weight 1
 
-    // tying value -> strain weights: e0ty = XT^T gty XT, gty from the tying
-    // interpolation (mat3x3SymmTransformTranspose, TACSElementAlgebra.h:1094)
-    double Nt[9];
-    tying_interp(u, v, Nt);
+    // tying field -> strain weights: e0ty = XT^T gty XT, gty the tying
+    // fields interpolated at the point (mat3x3SymmTransformTranspose,
+    // TACSElementAlgebra.h:1094); the interpolation weights are applied
+    // where the fields are formed (tying_fields)
     const int kk[5] = {0, 3, 1, 4, 2};
     const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
     const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
@@ -435,12 +466,12 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     for (int r = 0; r < 5; r++) {
       const int ka = pa[kk[r]], kb = pb[kk[r]];
 #pragma unroll
-      for (int t = 0; t < 9; t++) {
-        const int sl = field_slot(tying_field(t));
+      for (int f = 0; f < 5; f++) {
+        const int sl = field_slot(f);
         const int c = pa[sl], d = pb[sl];
         const double M = (c == d) ? XT[3 * c + ka] * XT[3 * c + kb]
                                   : XT[3 * c + ka] * XT[3 * d + kb] + XT[3 * d + ka] * XT[3 * c + kb];
-        sm.W[45 * iq + 9 * r + t] = ff[r] * M * Nt[t];
+        sm.Wf[25 * iq + 5 * r + f] = ff[r] * M;
       }
     }
     if (kState) {
@@ -513,7 +544,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       lin = 0.5 * (X[0] * d0[0] + X[2] * d0[1] + X[4] * d0[2] + n0[0] * U[0] + n0[1] * U[2] + n0[2] * U[4]);
       quad = 0.5 * (d0[0] * U[0] + d0[1] * U[2] + d0[2] * U[4]);
     }
-    sm.ety[t] = kNonlin ? lin + quad : lin;
+    if (kState) sm.ety[t] = kNonlin ? lin + quad : lin;
     // derivative coefficients with a = xfac X + ufac U, nd = xfac n0 + ufac d0
     for (int pass = 0; pass < 2; pass++) {
       if (pass == 1 && MODE != kModeG && MODE != kModeAdjDV) break;
@@ -566,6 +597,9 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 
   // ---- P3: tying-strain Jacobian rows; strains and stresses at the
   // quadrature points ----
+  // lane k < 24 keeps column k of the tying-strain Jacobian (one value per
+  // tying point) in registers: only it reads that column when building B
+  double Jr[kNB][9];
   {
     if (lane < 24) {
       const int k = lane;
@@ -576,10 +610,11 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       // rotation rows: component c of fn[n] x dd
       const int c = r < 3 ? 0 : r - 3, i1 = (c + 1) % 3, i2 = (c + 2) % 3;
       const double fa = sm.fn[3 * n + i1], fb = sm.fn[3 * n + i2];
-      for (int pass = 0; pass < ElemSmem<kTwo>::kNB; pass++) {
+#pragma unroll
+      for (int pass = 0; pass < kNB; pass++) {
         const double *cU = sm.f.tcU[pass];
         const double *cd = sm.f.tcd[pass];
-        double *J = sm.Jty[pass];
+        double *J = Jr[pass];
 #pragma unroll
         for (int t = 0; t < 9; t++) {
           double u, v;
@@ -591,7 +626,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
           } else {
             val = (na * nb) * (fa * cd[3 * t + i2] - fb * cd[3 * t + i1]);
           }
-          J[24 * t + k] = val;
+          J[t] = val;
         }
       }
     }
@@ -600,13 +635,17 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       double u, v, N[4], Na[4], Nb[4];
       quad_point(iq, u, v);
       shape4(u, v, N, Na, Nb);
-      double e[9];
-      const double *Wq = &sm.W[45 * iq];
+      double e[9], Nt[9], g[5], ety[9];
+      tying_interp(u, v, Nt);
+#pragma unroll
+      for (int t = 0; t < 9; t++) ety[t] = sm.ety[t];
+      tying_fields(Nt, ety, g);
+      const double *Wq = &sm.Wf[25 * iq];
 #pragma unroll
       for (int r = 0; r < 5; r++) {
         double acc = 0.0;
 #pragma unroll
-        for (int t = 0; t < 9; t++) acc += Wq[9 * r + t] * sm.ety[t];
+        for (int f = 0; f < 5; f++) acc += Wq[5 * r + f] * g[f];
         e[tying_row(r)] = acc;
       }
       const double *U0 = &sm.u0x[9 * iq], *U1 = &sm.u1x[9 * iq];
@@ -638,12 +677,18 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 
   // ---- P4: stress resultants at the tying points ----
   if (kGeo && lane >= 8 && lane < 17) {
-    const int t = lane - 8;
+    const int t = lane - 8, f = tying_field(t);
     double acc = 0.0;
 #pragma unroll
-    for (int iq = 0; iq < 4; iq++)
+    for (int iq = 0; iq < 4; iq++) {
+      double u, v, Nt[9];
+      quad_point(iq, u, v);
+      tying_interp(u, v, Nt);
+      double w = 0.0;
 #pragma unroll
-      for (int r = 0; r < 5; r++) acc += sm.sig[9 * iq + tying_row(r)] * sm.W[45 * iq + 9 * r + t];
+      for (int r = 0; r < 5; r++) w += sm.sig[9 * iq + tying_row(r)] * sm.Wf[25 * iq + 5 * r + f];
+      acc += Nt[t] * w;
+    }
     sm.dety[t] = acc;
   }
 
@@ -661,10 +706,10 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     for (int iq = 0; iq < 4; iq++) {
       if (lane < 24) {
         double b[9];
-        build_bcol<kBLin>(sm, iq, lane, sm.Jty[0], b);
+        build_bcol<kBLin>(sm, iq, lane, Jr[0], b);
 #pragma unroll
         for (int i = 0; i < 9; i++) sm.c.B[0][24 * i + lane] = b[i];
-        build_bcol<kBQuad>(sm, iq, lane, sm.Jty[1], b);
+        build_bcol<kBQuad>(sm, iq, lane, Jr[kNB - 1], b);
 #pragma unroll
         for (int i = 0; i < 9; i++) sm.c.B[1][24 * i + lane] = b[i];
       }
@@ -729,9 +774,9 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       if (lane < 24) {
         double b[9];
         if (kNonlin)
-          build_bcol<kBFull>(sm, iq, lane, sm.Jty[0], b);
+          build_bcol<kBFull>(sm, iq, lane, Jr[0], b);
         else
-          build_bcol<kBLin>(sm, iq, lane, sm.Jty[0], b);
+          build_bcol<kBLin>(sm, iq, lane, Jr[0], b);
 #pragma unroll
         for (int i = 0; i < 9; i++) sm.c.B[0][24 * i + lane] = b[i];
       }
@@ -770,9 +815,9 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     if (lane < 24) {
       double b[9], y[9];
       if (kNonlin) {
-        build_bcol<kBFull>(sm, iq, lane, sm.Jty[0], b);
+        build_bcol<kBFull>(sm, iq, lane, Jr[0], b);
       } else {
-        build_bcol<kBLin>(sm, iq, lane, sm.Jty[0], b);
+        build_bcol<kBLin>(sm, iq, lane, Jr[0], b);
       }
       if (kRes) {
 #pragma unroll
@@ -787,7 +832,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         }
       }
       if constexpr (MODE == kModeG) {
-        build_bcol<kBQuad>(sm, iq, lane, sm.Jty[1], b);
+        build_bcol<kBQuad>(sm, iq, lane, Jr[kNB - 1], b);
         apply_c(sec, sm.wdet[iq], b, y);
 #pragma unroll
         for (int i = 0; i < 9; i++) {
