@@ -923,6 +923,149 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   }
 }
 
+// ---- mass matrix ----
+// getMatType(TACS_MASS_MATRIX) (TACSShellElement.h:675-704, gamma = 1) keeps
+// only the inertial terms of addJacobian (TACSShellElement.h:613-648) and the
+// gamma d2Tdotd part of TACSLinearizedRotation::addDirectorJacobian
+// (TACSDirector.h:368-430).  With the mass moments of TACSIsoShellConstitutive
+// (TACSIsoShellConstitutive.cpp:120-129, tOffset = 0) m0 = rho t, m1 = 0,
+// m2 = rho t^3 / 12, and S_ij = sum_qp detXd N_i N_j:
+//   M[u_i, u_j] = m0 S_ij I
+//   M[q_i, q_j] = m2 S_ij ((t_i . t_j) I - t_j t_i^T)
+// since the director d = q x t moves by -[t]x dq and [t_i]x^T [t_j]x =
+// (t_i . t_j) I - t_j t_i^T (t = node normals).  Translations and rotations do
+// not couple (m1 = 0).  kModeMassDV contracts dM/dt (dm0/dt = rho,
+// dm2/dt = rho t^2 / 4) with the state phi: dv[e] += coef[d] phi^T dM_e/dt phi.
+// Same layout as elem_kernel: two elements per wavefront, a 3 x 6 tile per lane.
+template <int MODE>
+__global__ __launch_bounds__(64) void mass_kernel(ElemArgs a) {
+  struct Smem {
+    double X[12], fn[12], wdet[4], part[32];
+  };
+  __shared__ Smem smem[2];
+  const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
+  Smem &sm = smem[half];
+  const int slot = 2 * blockIdx.x + half;
+  const bool valid = slot < a.count;
+  const int e = a.elems[valid ? slot : slot - 1];
+  const long design = blockIdx.y;
+  const int *cn = a.conn + 4 * e;
+  const double t = a.thick[design * a.nelems + e];
+
+  if (lane < 12) sm.X[lane] = a.Xpts[3 * cn[lane / 3] + lane % 3];
+  __syncthreads();
+  // node normals (TacsShellComputeNodeNormals, TACSShellUtilities.h:301-343)
+  if (lane < 4) {
+    double u, v, N[4], Na[4], Nb[4];
+    node_point(lane, u, v);
+    shape4(u, v, N, Na, Nb);
+    double Xxi[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Xxi[2 * d] += Na[m] * sm.X[3 * m + d];
+        Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
+      }
+    const double a3[3] = {Xxi[0], Xxi[2], Xxi[4]}, b3[3] = {Xxi[1], Xxi[3], Xxi[5]};
+    double f[3];
+    cross3(a3, b3, f);
+    const double nrm = sqrt(dot3(f, f));
+    if (nrm != 0.0) {
+      const double inv = 1.0 / nrm;
+      f[0] *= inv;
+      f[1] *= inv;
+      f[2] *= inv;
+    }
+    sm.fn[3 * lane] = f[0];
+    sm.fn[3 * lane + 1] = f[1];
+    sm.fn[3 * lane + 2] = f[2];
+  }
+  __syncthreads();
+  // detXd at the quadrature points (Gauss weight 1)
+  if (lane < 4) {
+    double u, v, N[4], Na[4], Nb[4];
+    quad_point(lane, u, v);
+    shape4(u, v, N, Na, Nb);
+    double Xxi[6] = {0, 0, 0, 0, 0, 0}, n0[3] = {0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Xxi[2 * d] += Na[m] * sm.X[3 * m + d];
+        Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
+        n0[d] += N[m] * sm.fn[3 * m + d];
+      }
+    double Xd[9], Xdi[9];
+    frame3(Xxi, n0, Xd);
+    sm.wdet[lane] = inv3(Xd, Xdi);
+  }
+  __syncthreads();
+
+  const int row0 = 3 * (lane >> 2), col0 = 6 * (lane & 3);
+  const int ni = row0 / 6, nj = col0 / 6, rr = row0 % 6;
+  double S = 0.0;
+#pragma unroll
+  for (int iq = 0; iq < 4; iq++) {
+    double u, v, N[4], Na[4], Nb[4];
+    quad_point(iq, u, v);
+    shape4(u, v, N, Na, Nb);
+    const double ha = (ni & 1) ? 0.5 : -0.5, hb = (ni & 2) ? 0.5 : -0.5;
+    const double ga = (nj & 1) ? 0.5 : -0.5, gb = (nj & 2) ? 0.5 : -0.5;
+    // N of nodes ni and nj at the point, (0.5 + h u)(0.5 + h v) as in shape4
+    const double Ni = (0.5 + ha * u) * (0.5 + hb * v), Nj = (0.5 + ga * u) * (0.5 + gb * v);
+    S += sm.wdet[iq] * (Ni * Nj);
+  }
+  const double rho = a.mat.rho;
+  const double m0 = (MODE == kModeMass) ? rho * t : rho;
+  const double m2 = (MODE == kModeMass) ? rho * t * t * t * (1.0 / 12.0) : rho * t * t * 0.25;
+  double acc[18];
+  if (rr == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 6; j++) acc[6 * i + j] = (j == i) ? m0 * S : 0.0;
+  } else {
+    const double *ti = &sm.fn[3 * ni], *tj = &sm.fn[3 * nj];
+    const double tt = ti[0] * tj[0] + ti[1] * tj[1] + ti[2] * tj[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 6; j++)
+        acc[6 * i + j] = (j < 3) ? 0.0 : m2 * S * ((j - 3 == i ? tt : 0.0) - tj[i] * ti[j - 3]);
+  }
+
+  if (MODE == kModeMass) {
+    if (valid) {
+      const long blk = a.blk[16 * e + 4 * ni + nj];
+      double *dst = a.vals + design * a.vals_stride + 36 * blk;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) dst[6 * (rr + i) + j] += a.alpha * acc[6 * i + j];
+    }
+  } else {
+    const double *phi = a.state + design * a.state_stride;
+    double pr[3], pc[6];
+#pragma unroll
+    for (int i = 0; i < 3; i++) pr[i] = phi[6 * cn[ni] + rr + i];
+#pragma unroll
+    for (int j = 0; j < 6; j++) pc[j] = phi[6 * cn[nj] + j];
+    double p = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 6; j++) p += pr[i] * acc[6 * i + j] * pc[j];
+    sm.part[lane] = p;
+    __syncthreads();
+    if (lane == 0 && valid) {
+      double sum = 0.0;
+      for (int k = 0; k < 32; k++) sum += sm.part[k];
+      a.dv[design * a.nelems + e] += a.coef[design] * sum;
+    }
+  }
+}
+
 // explicit instantiation + launch wrappers
 template <int MODE>
 static hipError_t launch_mode(const ElemArgs &args, int count, int nbatch, hipStream_t stream) {
@@ -952,6 +1095,18 @@ hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nba
       return launch_mode<kModeAdjResLin>(args, count, nbatch, stream);
     case kModeAdjResNl:
       return launch_mode<kModeAdjResNl>(args, count, nbatch, stream);
+    case kModeMass:
+    case kModeMassDV: {
+      if (count <= 0 || nbatch <= 0) return hipSuccess;
+      ElemArgs m = args;
+      m.count = count;
+      const dim3 grid((count + 1) / 2, nbatch);
+      if (mode == kModeMass)
+        hipLaunchKernelGGL(mass_kernel<kModeMass>, grid, dim3(64), 0, stream, m);
+      else
+        hipLaunchKernelGGL(mass_kernel<kModeMassDV>, grid, dim3(64), 0, stream, m);
+      return hipGetLastError();
+    }
     default:
       return hipErrorInvalidValue;
   }

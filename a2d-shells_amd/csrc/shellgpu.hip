@@ -542,6 +542,8 @@ int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, s
     if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
   } else if (mattype == SHELLGPU_TANGENT_NONLINEAR) {
     if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
+  } else if (mattype == SHELLGPU_MASS) {
+    if ((rc = run_colored(ctx, kModeMass, a))) return rc;
   } else {
     return set_err(ctx, 1, "unknown matrix type");
   }
@@ -608,8 +610,19 @@ int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, doubl
   int rc = check_batch(ctx, nullptr);
   if (rc) return rc;
   if (!phi || !dfdt) return set_err(ctx, 1, "phi and dfdt are required");
+  if (mattype == SHELLGPU_MASS) {
+    // one coefficient per design
+    std::vector<double> h(ctx->nbatch, scale);
+    HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    ElemArgs a = base_args(ctx, phi);
+    a.coef = ctx->d_coef;
+    a.dv = dfdt;
+    return run_colored(ctx, kModeMassDV, a);
+  }
   if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC)
-    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS or SHELLGPU_GEOMETRIC");
+    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS, SHELLGPU_GEOMETRIC or SHELLGPU_MASS");
   const bool geo = mattype == SHELLGPU_GEOMETRIC;
   if ((rc = set_coef(ctx, geo ? 0.0 : scale, geo ? scale : 0.0))) return rc;
   ElemArgs a = base_args(ctx, phi);

@@ -42,6 +42,8 @@ enum ElemMode {
   kModeAdjDV = 5,     // thickness terms of a buckling eigenpair (state = phi)
   kModeAdjResLin = 6, // psi^T dR/dt, linear model (state = u)
   kModeAdjResNl = 7,  // psi^T dR/dt, nonlinear model (state = u)
+  kModeMass = 8,      // mass matrix (getMatType MASS)
+  kModeMassDV = 9,    // dv[e] += coef[d] phi^T dM_e/dt phi (state = phi)
 };
 
 struct ElemArgs {

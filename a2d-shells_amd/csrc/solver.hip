@@ -1295,9 +1295,12 @@ struct LanczosOut {
 // Lanczos in the K inner product, designs in lock-step, full CGS2
 // re-orthogonalisation.  mode 0: w = A^{-1} K q (shift-invert, A = K +
 // sigma G factored in F); mode 1: w = K^{-1} (-G q) (buckling mode, F
-// factors K).  Ritz value theta relates to the buckling load by
-// lambda = -theta sigma / (1 - theta) (mode 0) or lambda = 1 / theta (mode 1).
-// Stops when the nev wanted Ritz values have relative residual <= tol.
+// factors K); mode 2: w = A^{-1} K q with the argument K holding the mass
+// matrix M and A = K - sigma M (generalised shift-invert, M inner product).
+// Ritz value theta relates to the eigenvalue by lambda = -theta sigma /
+// (1 - theta) (mode 0), lambda = 1 / theta (mode 1) or lambda = 1 / theta +
+// sigma (mode 2).  Stops when the nev wanted Ritz values have relative
+// residual <= tol.
 int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
             const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
             double *work, LanczosOut &lo) {
@@ -1332,7 +1335,7 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
   int k;
   for (k = 0; k < m; k++) {
     double *qn = Q + (size_t)(k + 1) * nb * n, *Kqn = KQ + (size_t)(k + 1) * nb * n;
-    if (mode == 0) {
+    if (mode == 0 || mode == 2) {
       if ((rc = shellgpu_factor_apply(F, KQ + (size_t)k * nb * n, qn))) return rc;
     } else {
       if ((rc = shellgpu_mat_mult(G, Q + (size_t)k * nb * n, work))) return rc;
@@ -1372,6 +1375,7 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
         std::vector<int> idx(ns);
         for (int i = 0; i < ns; i++) idx[i] = i;
         auto key = [&](int i) {
+          if (mode == 2) return std::fabs(1.0 / w[i] + sigma[d]);
           return mode == 0 ? std::fabs(-w[i] * sigma[d] / (1.0 - w[i])) : (w[i] > 0 ? 1.0 / w[i] : 1e300);
         };
         std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
@@ -1438,22 +1442,27 @@ int buckling_free(shellgpu_buckling *B) {
   return 0;
 }
 
-// phi_n of every design into out (device, nbatch x n)
-int buckling_vector(shellgpu_buckling *B, int idx, double *out) {
-  shellgpu_ctx *ctx = B->ctx;
+// Ritz vector idx of every design, Q z, into out (device, nbatch x n); ritz[d]
+// holds the wanted Ritz coefficient vectors of design d (ns[d] each)
+int ritz_vector(shellgpu_ctx *ctx, const double *Q, const std::vector<int> &ns,
+                const std::vector<std::vector<double>> &ritz, int idx, double *hv, double *out) {
   const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
   rocblas_handle h = (rocblas_handle)ctx->blas;
   const double one = 1.0, zero = 0.0;
   for (int d = 0; d < nb; d++) {
-    const int ns = B->ns[d];
-    HIPCALL(ctx, hipMemcpyAsync(B->hv, B->ritz[d].data() + (size_t)idx * ns, sizeof(double) * ns,
+    HIPCALL(ctx, hipMemcpyAsync(hv, ritz[d].data() + (size_t)idx * ns[d], sizeof(double) * ns[d],
                                 hipMemcpyHostToDevice, ctx->stream));
-    BLASCALL(ctx, rocblas_dgemv(h, rocblas_operation_none, n, ns, &one, B->Q + (size_t)d * n, nb * n, B->hv, 1, &zero,
+    BLASCALL(ctx, rocblas_dgemv(h, rocblas_operation_none, n, ns[d], &one, Q + (size_t)d * n, nb * n, hv, 1, &zero,
                                 out + (size_t)d * n, 1));
     // hv is reused by the next design
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   }
   return 0;
+}
+
+// phi_n of every design into out (device, nbatch x n)
+int buckling_vector(shellgpu_buckling *B, int idx, double *out) {
+  return ritz_vector(B->ctx, B->Q, B->ns, B->ritz, idx, B->hv, out);
 }
 
 }  // namespace
@@ -1804,6 +1813,223 @@ int shellgpu_linear_buckling(shellgpu_ctx *ctx, double sigma, const double *forc
 out:
   shellgpu_buckling_destroy(B);
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// natural frequencies (TACSFrequencyAnalysis, Lanczos path)
+// ---------------------------------------------------------------------------
+struct shellgpu_frequency {
+  shellgpu_ctx *ctx = nullptr;
+  int nev = 0, m = 0;
+  double sigma = 0.0, tol = 0.0;
+  shellgpu_mat *K = nullptr, *M = nullptr, *A = nullptr;
+  shellgpu_factor *F = nullptr;
+  // device: Lanczos basis Q and M Q, coefficients, work vectors, shifts
+  double *Q = nullptr, *MQ = nullptr, *hv = nullptr, *work = nullptr, *ph = nullptr, *dsig = nullptr;
+  std::vector<double> sig;
+  bool solved = false;
+  int steps = 0;
+  std::vector<std::vector<double>> lam, err, ritz;
+  std::vector<int> ns;
+};
+
+namespace {
+
+void frequency_free(shellgpu_frequency *F) {
+  if (F->K) shellgpu_mat_destroy(F->K);
+  if (F->M) shellgpu_mat_destroy(F->M);
+  if (F->A) shellgpu_mat_destroy(F->A);
+  if (F->F) shellgpu_factor_destroy(F->F);
+  double *bufs[] = {F->Q, F->MQ, F->hv, F->work, F->ph, F->dsig};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int shellgpu_frequency_create(shellgpu_ctx *ctx, double sigma, int nev, int max_lanczos, double tol,
+                              shellgpu_frequency **out) {
+  if (!ctx || !out) return 1;
+  *out = nullptr;
+  if (nev <= 0 || max_lanczos < nev || !(tol > 0.0) || !(sigma >= 0.0))
+    return ferr(ctx, 1, "invalid frequency-analysis arguments (nev > 0, max_lanczos >= nev, tol > 0, sigma >= 0)");
+  if (!ctx->d_thick) return ferr(ctx, 3, "designs not set (shellgpu_set_designs)");
+  shellgpu_frequency *F = new shellgpu_frequency();
+  F->ctx = ctx;
+  F->sigma = sigma;
+  F->nev = nev;
+  F->m = max_lanczos;
+  F->tol = tol;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes, m = max_lanczos;
+  const size_t v = sizeof(double) * (size_t)n * nb;
+  int rc = 0;
+  if ((rc = ensure_blas(ctx))) goto fail;
+  if ((rc = shellgpu_mat_create(ctx, nb, &F->K))) goto fail;
+  if ((rc = shellgpu_mat_create(ctx, nb, &F->M))) goto fail;
+  if ((rc = shellgpu_mat_create(ctx, nb, &F->A))) goto fail;
+  if ((rc = shellgpu_factor_create(ctx, &F->F))) goto fail;
+  if (hipMalloc((void **)&F->Q, v * (m + 1)) != hipSuccess || hipMalloc((void **)&F->MQ, v * (m + 1)) != hipSuccess ||
+      hipMalloc((void **)&F->hv, sizeof(double) * (size_t)nb * (m + 1)) != hipSuccess ||
+      hipMalloc((void **)&F->work, v) != hipSuccess || hipMalloc((void **)&F->ph, v) != hipSuccess ||
+      hipMalloc((void **)&F->dsig, sizeof(double) * nb) != hipSuccess) {
+    rc = ferr(ctx, 2, "out of device memory (frequency analysis)");
+    goto fail;
+  }
+  *out = F;
+  return 0;
+fail:
+  frequency_free(F);
+  delete F;
+  return rc;
+}
+
+void shellgpu_frequency_destroy(shellgpu_frequency *F) {
+  if (!F) return;
+  frequency_free(F);
+  delete F;
+}
+
+int shellgpu_frequency_solve(shellgpu_frequency *F) {
+  if (!F) return 1;
+  shellgpu_ctx *ctx = F->ctx;
+  const int nb = ctx->nbatch, nev = F->nev;
+  int rc = 0;
+  LanczosOut lo;
+  F->solved = false;
+  F->sig.assign(nb, F->sigma);
+  // TACSFrequencyAnalysis::solve (src/TACSBuckling.cpp:810-893): K and M at
+  // zero state, A = K - sigma M with BCs, factor, SEP::solve
+  if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, F->K))) return rc;
+  if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_MASS, nullptr, F->M))) return rc;
+  for (int attempt = 0;; attempt++) {
+    std::vector<double> ms(nb);
+    for (int d = 0; d < nb; d++) ms[d] = -F->sig[d];
+    HIPCALL(ctx, hipMemcpyAsync(F->dsig, ms.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+    {
+      const long nv = 36L * ctx->nnzb;
+      hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
+                         shellgpu_mat_values(F->K), shellgpu_mat_values(F->M), F->dsig, shellgpu_mat_values(F->A));
+      HIPCALL(ctx, hipGetLastError());
+    }
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));  // ms is released next iteration
+    if ((rc = shellgpu_mat_apply_bcs(F->A, 1.0))) return rc;
+    rc = shellgpu_factor_compute(F->F, F->A);
+    if (rc == 0) break;
+    if (rc != 5 || attempt >= 12) return rc;
+    // A indefinite: the shift lies above the first eigenvalue of a design
+    std::vector<int> info(nb);
+    HIPCALL(ctx, hipMemcpy(info.data(), F->F->info, sizeof(int) * nb, hipMemcpyDeviceToHost));
+    for (int d = 0; d < nb; d++) {
+      if (info[d] == 0) continue;
+      if (F->sig[d] == 0.0) return ferr(ctx, 5, "stiffness matrix is not positive definite");
+      F->sig[d] = attempt < 10 ? 0.5 * F->sig[d] : 0.0;
+    }
+    ctx->err.clear();
+  }
+  if ((rc = lanczos(ctx, F->F, F->M, nullptr, 2, F->sig, nev, F->m, F->tol, F->Q, F->MQ, F->hv, F->work, lo)))
+    return rc;
+  F->steps = lo.steps;
+  F->lam.assign(nb, {});
+  F->err.assign(nb, {});
+  F->ritz.assign(nb, {});
+  F->ns.assign(nb, 0);
+  for (int d = 0; d < nb; d++) {
+    const int ns = (int)lo.alpha[d].size();
+    if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
+    std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
+    if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+    std::vector<int> idx(ns);
+    for (int i = 0; i < ns; i++) idx[i] = i;
+    const double sd = F->sig[d];
+    // EPGeneralizedShiftInvert::convertEigenvalue (src/bpmat/GSEP.cpp:184-186)
+    auto lam_of = [&](int i) { return 1.0 / w[i] + sd; };
+    // sortEigenvalues, SMALLEST_MAGNITUDE (src/bpmat/GSEP.cpp:693-748)
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(lam_of(a)) < std::fabs(lam_of(b)); });
+    F->ns[d] = ns;
+    for (int j = 0; j < nev; j++) {
+      const int i = idx[j];
+      F->lam[d].push_back(lam_of(i));
+      F->err[d].push_back(std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]));
+      F->ritz[d].insert(F->ritz[d].end(), z.begin() + (size_t)i * ns, z.begin() + (size_t)(i + 1) * ns);
+    }
+  }
+  F->solved = true;
+  return 0;
+}
+
+int shellgpu_frequency_extract(shellgpu_frequency *F, int idx, double *lambda, double *phi, double *error) {
+  if (!F) return 1;
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->solved) return ferr(ctx, 9, "frequency problem not solved");
+  if (idx < 0 || idx >= F->nev) return ferr(ctx, 1, "eigenvalue index out of range");
+  for (int d = 0; d < ctx->nbatch; d++) {
+    if (lambda) lambda[d] = F->lam[d][idx];
+    if (error) error[d] = F->err[d][idx];
+  }
+  if (phi) return ritz_vector(ctx, F->Q, F->ns, F->ritz, idx, F->hv, phi);
+  return 0;
+}
+
+int shellgpu_frequency_info(shellgpu_frequency *F, int *nsteps, double *sigma_used) {
+  if (!F) return 1;
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->solved) return ferr(ctx, 9, "frequency problem not solved");
+  if (nsteps) *nsteps = F->steps;
+  if (sigma_used)
+    for (int d = 0; d < ctx->nbatch; d++) sigma_used[d] = F->sig[d];
+  return 0;
+}
+
+int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef, double *dfdt) {
+  if (!F || !coef) return 1;
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->solved) return ferr(ctx, 9, "frequency problem not solved");
+  if (!dfdt) return 0;
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes, nev = F->nev;
+  int rc = 0;
+  VecOps vo;
+  if ((rc = vo.init(ctx, n, nb))) return rc;
+  std::vector<double> hc(2 * nb, 0.0);
+  for (int j = 0; j < nev; j++) {
+    bool any = false;
+    for (int d = 0; d < nb; d++) any = any || coef[(size_t)d * nev + j] != 0.0;
+    if (!any) continue;
+    // d lambda = phi^T (K' - lambda M') phi / (phi^T M phi)
+    if ((rc = ritz_vector(ctx, F->Q, F->ns, F->ritz, j, F->hv, F->ph))) return rc;
+    if ((rc = shellgpu_mat_mult(F->M, F->ph, F->work))) return rc;
+    std::vector<double> pmp;
+    if ((rc = vo.dot(F->ph, F->work, pmp))) return rc;
+    // stiffness term: (c, 0) pairs of the thickness-adjoint kernel
+    for (int d = 0; d < nb; d++) {
+      hc[2 * d] = coef[(size_t)d * nev + j] / pmp[d];
+      hc[2 * d + 1] = 0.0;
+    }
+    HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, hc.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, ctx->stream));
+    {
+      ElemArgs a = base_args(ctx, F->ph);
+      a.u2 = nullptr;
+      a.res = nullptr;
+      a.coef = ctx->d_coef;
+      a.dv = dfdt;
+      if ((rc = run_colored(ctx, kModeAdjDV, a))) return rc;
+    }
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    // mass term: one coefficient per design
+    for (int d = 0; d < nb; d++) hc[d] = -coef[(size_t)d * nev + j] * F->lam[d][j] / pmp[d];
+    HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, hc.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+    {
+      ElemArgs a = base_args(ctx, F->ph);
+      a.coef = ctx->d_coef;
+      a.dv = dfdt;
+      if ((rc = run_colored(ctx, kModeMassDV, a))) return rc;
+    }
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return 0;
 }
 
 int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double *dks) {

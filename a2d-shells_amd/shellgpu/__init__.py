@@ -15,7 +15,7 @@ PKG_ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libshellgpu.so")
 HEADER_PATH = os.path.join(os.path.dirname(PKG_ROOT), "include", "shellgpu.h")
 
-STIFFNESS, GEOMETRIC, GEOMETRIC_FD, TANGENT_NONLINEAR = 0, 1, 2, 3
+STIFFNESS, GEOMETRIC, GEOMETRIC_FD, TANGENT_NONLINEAR, MASS = 0, 1, 2, 3, 4
 LINEAR, NONLINEAR = 0, 1
 
 
@@ -94,6 +94,12 @@ def load_library(path=LIB_PATH):
         "shellgpu_buckling_extract": (C.c_int, [_VP, C.c_int, _DP, _VP, _DP]),
         "shellgpu_buckling_info": (C.c_int, [_VP, _IP, _DP, _VP]),
         "shellgpu_buckling_add_eigen_sens": (C.c_int, [_VP, _DP, _VP, _VP]),
+        "shellgpu_frequency_create": (C.c_int, [_VP, C.c_double, C.c_int, C.c_int, C.c_double, C.POINTER(_VP)]),
+        "shellgpu_frequency_destroy": (None, [_VP]),
+        "shellgpu_frequency_solve": (C.c_int, [_VP]),
+        "shellgpu_frequency_extract": (C.c_int, [_VP, C.c_int, _DP, _VP, _DP]),
+        "shellgpu_frequency_info": (C.c_int, [_VP, _IP, _DP]),
+        "shellgpu_frequency_add_eigen_sens": (C.c_int, [_VP, _DP, _VP]),
         "shellgpu_ks_aggregate": (C.c_int, [C.c_int, C.c_double, _DP, _DP, _DP]),
         "shellgpu_newton_solve": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_double, _VP, _IP, _DP]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
@@ -234,6 +240,63 @@ class Factor:
     def destroy(self):
         if self.h and self.ctx.h:
             self.ctx.lib.shellgpu_factor_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Frequency:
+    """Natural-frequency analysis of every design of a context
+    (TACSFrequencyAnalysis, Lanczos path): K x = lambda M x, lambda = omega^2."""
+
+    def __init__(self, ctx, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12):
+        self.ctx = ctx
+        self.nev = nev
+        h = _VP()
+        ctx._check(ctx.lib.shellgpu_frequency_create(ctx.h, float(sigma), nev, max_lanczos, tol, C.byref(h)))
+        self.h = h.value
+
+    def solve(self):
+        self.ctx._check(self.ctx.lib.shellgpu_frequency_solve(self.h))
+
+    def eigenvalues(self):
+        """[nbatch, nev] eigenvalues omega^2 and relative Ritz residuals."""
+        nb = self.ctx.nbatch
+        lam, err = np.zeros((self.nev, nb)), np.zeros((self.nev, nb))
+        for j in range(self.nev):
+            self.ctx._check(self.ctx.lib.shellgpu_frequency_extract(self.h, j, _dp(lam[j]), None, _dp(err[j])))
+        return lam.T.copy(), err.T.copy()
+
+    def eigenvector(self, j):
+        """M-normalised eigenvector j of every design, [nbatch, 6n]."""
+        nb, n = self.ctx.nbatch, 6 * self.ctx.nnodes
+        phi = DeviceArray(self.ctx, nb * n)
+        lam = np.zeros(nb)
+        self.ctx._check(self.ctx.lib.shellgpu_frequency_extract(self.h, j, _dp(lam), phi.ptr, None))
+        return phi.download().reshape(nb, n)
+
+    def info(self):
+        steps, sig = C.c_int(), np.zeros(self.ctx.nbatch)
+        self.ctx._check(self.ctx.lib.shellgpu_frequency_info(self.h, C.byref(steps), _dp(sig)))
+        return {"steps": steps.value, "sigma": sig}
+
+    def eigen_sens(self, coef):
+        """sum_j coef[d, j] d(lambda_j)/dt per design, [nbatch, nelems]."""
+        ctx = self.ctx
+        nb = ctx.nbatch
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(coef, dtype=np.float64), (nb, self.nev)))
+        dt = DeviceArray(ctx, nb * ctx.nelems)
+        dt.zero()
+        ctx._check(ctx.lib.shellgpu_frequency_add_eigen_sens(self.h, _dp(c), dt.ptr))
+        return dt.download().reshape(nb, ctx.nelems)
+
+    def destroy(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.shellgpu_frequency_destroy(self.h)
         self.h = None
 
     def __del__(self):
@@ -495,6 +558,9 @@ class Context:
 
     def buckling(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12, model=None):
         return Buckling(self, sigma, nev, max_lanczos, tol, model)
+
+    def frequency(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12):
+        return Frequency(self, sigma, nev, max_lanczos, tol)
 
     def vec_zero_bcs(self, x):
         self._check(self.lib.shellgpu_vec_zero_bcs(self.h, x.ptr))

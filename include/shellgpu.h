@@ -24,13 +24,15 @@ typedef struct shellgpu_ctx shellgpu_ctx;
 typedef struct shellgpu_mat shellgpu_mat;
 typedef struct shellgpu_factor shellgpu_factor;
 typedef struct shellgpu_buckling shellgpu_buckling;
+typedef struct shellgpu_frequency shellgpu_frequency;
 
 /* matrix types (ElementMatrixType, src/elements/TACSElementTypes.h) */
 enum {
   SHELLGPU_STIFFNESS = 0,        /* TACS_STIFFNESS_MATRIX, linear model     */
   SHELLGPU_GEOMETRIC = 1,        /* TACS_GEOMETRIC_STIFFNESS_MATRIX, exact  */
   SHELLGPU_GEOMETRIC_FD = 2,     /* same, reference central difference      */
-  SHELLGPU_TANGENT_NONLINEAR = 3 /* addJacobian(alpha=1) of the nonlinear model */
+  SHELLGPU_TANGENT_NONLINEAR = 3, /* addJacobian(alpha=1) of the nonlinear model */
+  SHELLGPU_MASS = 4               /* TACS_MASS_MATRIX (getMatType, gamma = 1)  */
 };
 /* element models (src/elements/shell/TACSShellElementDefs.h:12-29) */
 enum { SHELLGPU_LINEAR = 0, SHELLGPU_NONLINEAR = 1 };
@@ -122,7 +124,8 @@ int shellgpu_add_adj_res_product(shellgpu_ctx *ctx, int model, double scale, con
 /* TACSAssembler::addMatDVSensInnerProduct(scale, matType, phi, phi, dfdx)
  * (element default src/elements/TACSElement.cpp:355-432, central
  * differences there, exact here): dfdt += scale phi^T (dA/dt_e) phi with
- * A = K (SHELLGPU_STIFFNESS, u ignored) or G(u) (SHELLGPU_GEOMETRIC) */
+ * A = K (SHELLGPU_STIFFNESS, u ignored), G(u) (SHELLGPU_GEOMETRIC) or M
+ * (SHELLGPU_MASS, u ignored) */
 int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale,
                                            const double *phi, const double *u, double *dfdt);
 /* TACSAssembler::evalMatSVSensInnerProduct(GEOMETRIC, phi, phi, dfdu)
@@ -270,6 +273,34 @@ int shellgpu_buckling_info(shellgpu_buckling *B, int *nsteps, double *sigma_used
  * operator is refactored by a later solve. */
 int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, double *dfdt,
                                      double *dfdX);
+
+/* ---- natural frequencies ----------------------------------------------- */
+/* TACSFrequencyAnalysis with the Lanczos solver (src/TACSBuckling.h:128-200,
+ * constructor src/TACSBuckling.cpp:614-702, solve :810-893, extract
+ * :895-918): K x = lambda M x with lambda = omega^2, K and M assembled at
+ * zero state with BCs (unit diagonal), A = K - sigma M factored, shift-invert
+ * Lanczos on A^{-1} M in the M inner product (EPGeneralizedShiftInvert,
+ * src/bpmat/GSEP.cpp:106-190): lambda = 1/theta + sigma, sorted by |lambda|
+ * (SMALLEST_MAGNITUDE).  One object serves every design of the context.
+ * The direct factorisation needs A positive definite: a shift above the
+ * first eigenvalue of a design is halved until A factors (the reference's LU
+ * accepts it).  Stops when the nev wanted Ritz pairs have relative residual
+ * <= tol. */
+int shellgpu_frequency_create(shellgpu_ctx *ctx, double sigma, int nev, int max_lanczos, double tol,
+                              shellgpu_frequency **F);
+void shellgpu_frequency_destroy(shellgpu_frequency *F);
+int shellgpu_frequency_solve(shellgpu_frequency *F);
+/* TACSFrequencyAnalysis::extractEigenvalue / extractEigenvector
+ * (src/TACSBuckling.cpp:895-918): lambda[d] (host, nbatch), phi (device,
+ * nbatch x 6 nnodes, M-normalised; may be NULL), error[d] = relative Ritz
+ * residual (may be NULL) */
+int shellgpu_frequency_extract(shellgpu_frequency *F, int n, double *lambda, double *phi, double *error);
+int shellgpu_frequency_info(shellgpu_frequency *F, int *nsteps, double *sigma_used);
+/* d(sum_n coef[n] lambda_n)/dt added into dfdt (device, nbatch*nelems) with
+ * d lambda / dt = phi^T (dK/dt - lambda dM/dt) phi / (phi^T M phi)
+ * (TACSFrequencyAnalysis::evalEigenDVSens, src/TACSBuckling.cpp:950-995, where
+ * the element products are central differences; exact here) */
+int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef, double *dfdt);
 
 /* KS aggregate of f[0..n) (ksAggregation / ksAggregationSens,
  * src/utils/TacsUtilities.cpp:971-1011): ks = m + log(sum exp(rho (f_i -

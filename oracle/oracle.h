@@ -57,6 +57,9 @@ void or_elem_jacobian(int model, const or_material *m, double t, double temp,
 void or_elem_stiffness(int model, const or_material *m, double t, double temp,
                        const double X[12], const double q[24],
                        double mat[576]);
+/* getMatType(TACS_MASS_MATRIX): consistent mass of the element, mat
+ * overwritten.  SE.h:613-648, 675-704; SD.h:368-430; ISC:120-129 */
+void or_elem_mass(const or_material *m, double t, const double X[12], double mat[576]);
 void or_elem_geometric_fd(const or_material *m, double t, double temp,
                           const double X[12], const double u[24],
                           double mat[576]);
@@ -95,7 +98,7 @@ void or_first_appearance(int nnodes, int nelems, const int *conn,
  * 1 = G (geometric, FD step 1e-4), 2 = nonlinear Jacobian at u (alpha = 1),
  * 3 = G by the same formula with a unit step (accurate value when T = 0),
  * 4 = the same with step 1e5 (accurate value when a temperature dominates the
- * state norm).
+ * state norm), 5 = the mass matrix (getMatType(TACS_MASS_MATRIX)).
  * thick[nelems] per-element thickness; temp per-element temperature (may be
  * NULL).  u is the 6*nnodes state (may be NULL = 0).  Zeroes vals first. */
 void or_assemble_mat(int kind, int model, const or_material *m, int nnodes,

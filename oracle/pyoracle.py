@@ -60,6 +60,12 @@ def elem_stiffness(model, mat, t, temp, X, q):
     return K.reshape(24, 24)
 
 
+def elem_mass(mat, t, X):
+    M = np.zeros(576)
+    lib().or_elem_mass(C.byref(mat), C.c_double(t), _d(_f64(X)), _d(M))
+    return M.reshape(24, 24)
+
+
 def elem_jacobian(model, mat, t, temp, X, q, alpha=1.0):
     K = np.zeros(576)
     res = np.zeros(24)
@@ -111,7 +117,7 @@ def first_appearance(nnodes, conn):
     return out
 
 
-KIND_K, KIND_G_FD, KIND_KNL, KIND_G, KIND_G_THERMAL = 0, 1, 2, 3, 4
+KIND_K, KIND_G_FD, KIND_KNL, KIND_G, KIND_G_THERMAL, KIND_M = 0, 1, 2, 3, 4, 5
 
 
 def assemble_mat(kind, model, mat, Xpts, conn, thick, rowp, cols, u=None, temp=None):

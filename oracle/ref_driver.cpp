@@ -7,7 +7,7 @@
 // TACSQuad4Shell / TACSQuad4NonlinearShell compute:
 //
 //   record in  (66 doubles): Xpts[12] vars[24] psi[24] t E nu rho cte temperature
-//   record out (1780 doubles):
+//   record out (2356 doubles):
 //     res_lin[24]   TACSShellElement::addResidual, linear model      (SE.h:303-449)
 //     res_nl[24]    same, nonlinear model
 //     K_lin[576]    getMatType(TACS_STIFFNESS_MATRIX), linear model   (SE.h:675-771)
@@ -17,6 +17,7 @@
 //     dRdt_lin      psi^T dR_lin/dt from a 5-point stencil in t
 //     dRdt_nl       psi^T dR_nl/dt from a 5-point stencil in t
 //     energy_lin    strain energy from computeEnergies, linear model  (SE.h:202-297)
+//     M[576]        getMatType(TACS_MASS_MATRIX), linear model        (SE.h:675-704)
 //
 // The residual is a polynomial of degree 3 in the thickness (A ~ t, D ~ t^3,
 // drilling ~ t), so the 5-point central stencil with step 0.1 t is exact up to
@@ -34,7 +35,7 @@
 #include "TACSShellElementTransform.h"
 
 static const int NIN = 66;
-static const int NOUT = 24 + 24 + 3 * 576 + 4;
+static const int NOUT = 24 + 24 + 3 * 576 + 4 + 576;
 
 static double psi_dot_res(TACSElement *elem, const double *X, const double *q,
                           const double *psi, const double *zero) {
@@ -130,6 +131,7 @@ int main(int argc, char **argv) {
     double Te = 0.0, Ue = 0.0;
     lin->computeEnergies(0, 0.0, X, q, zero, &Te, &Ue);
     tail[3] = Ue;
+    lin->getMatType(TACS_MASS_MATRIX, 0, 0.0, X, q, tail + 4);
 
     nl->decref();
     lin->decref();

@@ -1147,6 +1147,53 @@ void or_elem_stiffness(int model, const or_material *m, double t, double temp, c
   elem_core(model, m, t, temp, 1.0, X, q, res, mat);
 }
 
+/* getMatType(TACS_MASS_MATRIX) (SE.h:675-704 with gamma = 1, alpha = beta =
+ * 0): the inertial terms of addJacobian (SE.h:613-648) -- the translational
+ * block gamma detXd m0 I through addInterpFieldsOuterProduct and the
+ * director block detXd m2 I accumulated in d2Tdotd -- and the gamma d2Tdotd
+ * part of addDirectorJacobian for TACSLinearizedRotation (SD.h:368-430).
+ * With alpha = 0 every stiffness Hessian (d2d, d2du, tying and drilling
+ * terms) is scaled to zero.  Mass moments of TACSIsoShellConstitutive
+ * (ISC:120-129) with tOffset = 0: m0 = rho t, m1 = -rho t^2 tOffset = 0 (so
+ * d2Tdotu vanishes), m2 = rho t^3 (tOffset^2 + 1/12).  mat is overwritten.
+ * pinned by: golden vectors of the compiled reference getMatType(MASS);
+ * rigid translation / in-plane rotation energies (closed forms) */
+void or_elem_mass(const or_material *m, double t, const double Xpts[12], double mat[576]) {
+  memset(mat, 0, 576 * sizeof(double));
+  double fn[12], Xdn[36];
+  node_normals(Xpts, fn, Xdn);
+  const double gamma = 1.0, tOffset = 0.0;
+  const double m0 = m->rho * t;
+  const double m2 = m->rho * t * t * t * (tOffset * tOffset + 1.0 / 12.0);
+  double d2Tdotd[144];
+  memset(d2Tdotd, 0, sizeof(d2Tdotd));
+  for (int iq = 0; iq < 4; iq++) {
+    double pt[2], Xxi[6], n0[3], Xd[9], Xdinv[9];
+    quad_point(iq, pt);
+    grad3(pt, 3, Xpts, Xxi);
+    interp3(pt, 3, fn, n0);
+    frame(Xxi, n0, Xd);
+    const double detXd = m_inv(Xd, Xdinv) * 1.0; /* Gauss weight 1 */
+    double d2u0dot[9], d2Td[9];
+    memset(d2u0dot, 0, sizeof(d2u0dot));
+    d2u0dot[0] = d2u0dot[4] = d2u0dot[8] = gamma * detXd * m0;
+    outer_ff(pt, VPN, d2u0dot, mat);
+    memset(d2Td, 0, sizeof(d2Td));
+    d2Td[0] = d2Td[4] = d2Td[8] = detXd * m2;
+    outer_ff(pt, 3, d2Td, d2Tdotd);
+  }
+  /* d = q x t: mat[q_i, q_j] -= [t_i]x (gamma d2Tdotd_ij) [t_j]x */
+  for (int i = 0; i < NN; i++)
+    for (int j = 0; j < NN; j++) {
+      double blk[9], tmp[9];
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) blk[3 * a + b] = gamma * d2Tdotd[12 * (3 * i + a) + 3 * j + b];
+      skew_mat_skew(&fn[3 * i], blk, &fn[3 * j], tmp);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) mat[NV * (VPN * i + 3 + a) + VPN * j + 3 + b] -= tmp[3 * a + b];
+    }
+}
+
 /* getMatType(TACS_GEOMETRIC_STIFFNESS_MATRIX) (SE.h:705-766): central
  * difference of the nonlinear tangent along the state with the temperature
  * perturbed along with it.  The reference's step is dh = 1e-4.  Because the

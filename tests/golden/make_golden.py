@@ -85,7 +85,7 @@ def main():
         K_nl=out[:, 624:1200].reshape(-1, 24, 24),
         G=out[:, 1200:1776].reshape(-1, 24, 24),
         adj_lin=out[:, 1776], dRdt_lin=out[:, 1777], dRdt_nl=out[:, 1778],
-        energy_lin=out[:, 1779])
+        energy_lin=out[:, 1779], M=out[:, 1780:2356].reshape(-1, 24, 24))
     print("wrote", len(recs), "records")
 
 

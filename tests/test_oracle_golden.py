@@ -99,3 +99,45 @@ def test_nonlinear_tangent_is_residual_derivative():
         e[j] = h
         col = (O.elem_residual(1, m, 0.003, 0.0, X, q + e) - O.elem_residual(1, m, 0.003, 0.0, X, q - e)) / (2 * h)
         assert np.abs(col - K[:, j]).max() <= 1e-6 * np.abs(K).max()
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_mass_matches_reference(i):
+    m, t, T, X, q, psi = _rec(i)
+    M = O.elem_mass(m, t, X)
+    assert rel(M, GOLD["M"][i]) < 1e-13
+
+
+def _plate(a=0.7, b=0.4, z=0.3):
+    # flat rectangle in the plane z = const, tensor node order
+    return np.array([[0, 0, z], [a, 0, z], [0, b, z], [a, b, z]], float)
+
+
+def test_mass_rigid_motion_energies():
+    """Closed forms on a flat a x b plate: a rigid translation carries
+    rho t a b, an in-plane rotation vector rho t^3/12 a b per unit square
+    component, and a rotation about the normal (the drilling direction,
+    where the director d = q x n does not move) carries none."""
+    rho, t, a, b = 2700.0, 0.01, 0.7, 0.4
+    m = O.material(rho=rho)
+    M = O.elem_mass(m, t, _plate(a, b))
+    assert np.abs(M - M.T).max() <= 1e-14 * np.abs(M).max()
+    area = a * b
+    for d in range(3):
+        u = np.zeros(24)
+        u[d::6] = 1.0
+        assert abs(u @ M @ u - rho * t * area) <= 1e-12 * rho * t * area
+    mr = rho * t ** 3 / 12.0
+    for d, want in [(3, mr * area), (4, mr * area), (5, 0.0)]:
+        u = np.zeros(24)
+        u[d::6] = 1.0
+        assert abs(u @ M @ u - want) <= 1e-12 * mr * area
+    # translations and rotations do not couple (m1 = 0 for tOffset = 0)
+    tr = [6 * n + d for n in range(4) for d in range(3)]
+    rot = [6 * n + 3 + d for n in range(4) for d in range(3)]
+    assert np.abs(M[np.ix_(tr, rot)]).max() == 0.0
+    # consistent bilinear mass: the translational block of node pairs is
+    # rho t a b / 36 [4 2 2 1] pattern
+    base = np.array([[4, 2, 2, 1], [2, 4, 1, 2], [2, 1, 4, 2], [1, 2, 2, 4]]) * rho * t * area / 36.0
+    Mx = M[0::6, 0::6]
+    assert np.abs(Mx - base).max() <= 1e-12 * base.max()
