@@ -313,6 +313,8 @@ void TACSAssembler::assembleMatType(ElementMatrixType matType, TACSMat *A) {
   int kind;
   if (matType == TACS_GEOMETRIC_STIFFNESS_MATRIX) {
     kind = SHELLGPU_GEOMETRIC;
+  } else if (matType == TACS_MASS_MATRIX) {
+    kind = SHELLGPU_MASS;
   } else {
     // getMatType(STIFFNESS) is the element's own Jacobian at the variables
     kind = model ? SHELLGPU_TANGENT_NONLINEAR : SHELLGPU_STIFFNESS;
@@ -343,7 +345,9 @@ void TACSAssembler::addMatDVSensInnerProduct(TacsScalar scale, ElementMatrixType
     return;
   }
   syncDesigns();
-  const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC : SHELLGPU_STIFFNESS;
+  const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC
+                   : matType == TACS_MASS_MATRIX              ? SHELLGPU_MASS
+                                                              : SHELLGPU_STIFFNESS;
   check(shellgpu_add_mat_dv_sens_inner_product(ctx, kind, scale, phi->getDeviceArray(), vars->getDeviceArray(),
                                                dfdx->getDeviceArray()),
         "TACSAssembler::addMatDVSensInnerProduct");
@@ -356,6 +360,10 @@ void TACSAssembler::addMatXptSensInnerProduct(TacsScalar scale, ElementMatrixTyp
     return;
   }
   syncDesigns();
+  if (matType == TACS_MASS_MATRIX) {
+    fprintf(stderr, "TACSAssembler::addMatXptSensInnerProduct: no coordinate derivative of the mass matrix\n");
+    return;
+  }
   const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC : SHELLGPU_STIFFNESS;
   check(shellgpu_add_mat_xpt_sens_inner_product(ctx, kind, scale, phi->getDeviceArray(), vars->getDeviceArray(),
                                                 dfdX->getDeviceArray()),
@@ -510,6 +518,116 @@ void TACSLinearBuckling::checkEigenvector(int n) {
   printf("|K*e| = %15.5e  \n|G*e| = %15.5e \n", t1->norm(), t2->norm());
   t1->axpy(eig, t2);
   printf("||K*e + lambda*G*e|| = %15.5e \n", t1->norm());
+  t2->decref();
+  t1->decref();
+  phi->decref();
+}
+
+// ---- natural frequencies ---------------------------------------------------
+
+TACSFrequencyAnalysis::TACSFrequencyAnalysis(TACSAssembler *assembler, TacsScalar sigma, TACSMat *mmat,
+                                             TACSMat *kmat, TACSKsm *solver, int max_lanczos, int num_eigvals,
+                                             double eig_tol)
+    : assembler(assembler), mmat(mmat), kmat(kmat), solver(solver), sigma(sigma), max_lanczos(max_lanczos),
+      num_eigvals(num_eigvals), eig_tol(eig_tol), freq(NULL) {
+  assembler->incref();
+  mmat->incref();
+  kmat->incref();
+  solver->incref();
+  TACSMat *smat = NULL;
+  TACSPc *pc = NULL;
+  solver->getOperators(&smat, &pc);
+  if (smat != kmat) fprintf(stderr, "Error, solver must be associated with the stiffness matrix\n");
+  recreate();
+}
+
+void TACSFrequencyAnalysis::recreate() {
+  if (freq) shellgpu_frequency_destroy(freq);
+  freq = NULL;
+  shellgpu_ctx *ctx = assembler->getContext();
+  report(ctx, shellgpu_frequency_create(ctx, sigma, num_eigvals, max_lanczos, eig_tol, &freq),
+         "TACSFrequencyAnalysis");
+}
+
+TACSFrequencyAnalysis::~TACSFrequencyAnalysis() {
+  if (freq) shellgpu_frequency_destroy(freq);
+  solver->decref();
+  kmat->decref();
+  mmat->decref();
+  assembler->decref();
+}
+
+void TACSFrequencyAnalysis::setSigma(TacsScalar s) {
+  sigma = s;
+  recreate();
+}
+
+void TACSFrequencyAnalysis::solve(KSMPrint *ksm_print, int print_level) {
+  (void)print_level;
+  if (!freq) return;
+  shellgpu_ctx *ctx = assembler->getContext();
+  assembler->zeroVariables();
+  assembler->syncDesigns();
+  int rc = shellgpu_frequency_solve(freq);
+  report(ctx, rc, "TACSFrequencyAnalysis::solve");
+  if (rc) return;
+  // the reference leaves M in mmat and the shifted operator in kmat
+  double sig = sigma;
+  shellgpu_frequency_info(freq, NULL, &sig);
+  assembler->assembleMatType(TACS_STIFFNESS_MATRIX, kmat);
+  assembler->assembleMatType(TACS_MASS_MATRIX, mmat);
+  kmat->axpy(-sig, mmat);
+  kmat->applyBCs();
+  if (ksm_print) {
+    int steps = 0;
+    shellgpu_frequency_info(freq, &steps, NULL);
+    char line[128];
+    snprintf(line, sizeof(line), "Lanczos steps %d\n", steps);
+    ksm_print->print(line);
+  }
+}
+
+TacsScalar TACSFrequencyAnalysis::extractEigenvalue(int n, TacsScalar *error) {
+  double lam = 0.0, err = 0.0;
+  if (freq) report(assembler->getContext(), shellgpu_frequency_extract(freq, n, &lam, NULL, &err), "extractEigenvalue");
+  if (error) *error = err;
+  return lam;
+}
+
+TacsScalar TACSFrequencyAnalysis::extractEigenvector(int n, TACSBVec *ans, TacsScalar *error) {
+  double lam = 0.0, err = 0.0;
+  if (freq)
+    report(assembler->getContext(), shellgpu_frequency_extract(freq, n, &lam, ans->getDeviceArray(), &err),
+           "extractEigenvector");
+  if (error) *error = err;
+  return lam;
+}
+
+// (u^T M u) d lambda / dx = u^T (dK/dx - lambda dM/dx) u
+void TACSFrequencyAnalysis::evalEigenDVSens(int n, TACSBVec *dfdx) {
+  dfdx->zeroEntries();
+  if (n < 0 || n >= num_eigvals || !freq) return;
+  std::vector<double> coef(num_eigvals, 0.0);
+  coef[n] = 1.0;
+  report(assembler->getContext(), shellgpu_frequency_add_eigen_sens(freq, coef.data(), dfdx->getDeviceArray()),
+         "evalEigenDVSens");
+}
+
+void TACSFrequencyAnalysis::checkEigenvector(int n) {
+  assembler->zeroVariables();
+  assembler->assembleMatType(TACS_STIFFNESS_MATRIX, kmat);
+  assembler->assembleMatType(TACS_MASS_MATRIX, mmat);
+  TACSBVec *phi = assembler->createVec(), *t1 = assembler->createVec(), *t2 = assembler->createVec();
+  phi->incref();
+  t1->incref();
+  t2->incref();
+  TacsScalar err;
+  TacsScalar eig = extractEigenvector(n, phi, &err);
+  kmat->mult(phi, t1);
+  mmat->mult(phi, t2);
+  printf("|K*e| = %15.5e  \n|M*e| = %15.5e \n", t1->norm(), t2->norm());
+  t1->axpy(-eig, t2);
+  printf("||K*e - lambda*M*e|| = %15.5e \n", t1->norm());
   t2->decref();
   t1->decref();
   phi->decref();

@@ -23,7 +23,14 @@ namespace shellgpu {
 typedef double TacsScalar;
 
 // ElementMatrixType (src/elements/TACSElementTypes.h)
-enum ElementMatrixType { TACS_STIFFNESS_MATRIX = 0, TACS_GEOMETRIC_STIFFNESS_MATRIX = 1 };
+// ElementMatrixType (src/elements/TACSElementTypes.h:101-107), same values
+enum ElementMatrixType {
+  TACS_JACOBIAN_MATRIX,
+  TACS_STIFFNESS_MATRIX,
+  TACS_MASS_MATRIX,
+  TACS_GEOMETRIC_STIFFNESS_MATRIX,
+  TACS_STIFFNESS_PRODUCT_DERIVATIVE
+};
 
 class TACSObject {
  public:
@@ -359,6 +366,39 @@ class TACSLinearBuckling : public TACSObject {
   int max_lanczos_vecs, num_eigvals;
   double eig_tol;
   shellgpu_buckling *buck;
+};
+
+// TACSFrequencyAnalysis, Lanczos constructor (src/TACSBuckling.h:128-200,
+// src/TACSBuckling.cpp:614-702, 793-1062): K u = lambda M u, lambda = omega^2,
+// by shift-invert Lanczos on (K - sigma M)^{-1} M in the M inner product.
+// The solver must be bound to kmat, as in the reference.  solve() leaves
+// mmat = M and kmat = K - sigma M with BCs, as the reference does; the GPU
+// object factors its own copy of the shifted operator.  evalEigenDVSens is
+// exact (the reference's version, commented out there, takes element
+// central differences).
+class TACSFrequencyAnalysis : public TACSObject {
+ public:
+  TACSFrequencyAnalysis(TACSAssembler *assembler, TacsScalar sigma, TACSMat *mmat, TACSMat *kmat, TACSKsm *solver,
+                        int max_lanczos, int num_eigvals, double eig_tol);
+  ~TACSFrequencyAnalysis();
+  TacsScalar getSigma() const { return sigma; }
+  void setSigma(TacsScalar s);
+  void solve(KSMPrint *ksm_print = NULL, int print_level = 0);
+  TacsScalar extractEigenvalue(int n, TacsScalar *error);
+  TacsScalar extractEigenvector(int n, TACSBVec *ans, TacsScalar *error);
+  void evalEigenDVSens(int n, TACSBVec *dfdx);
+  void checkEigenvector(int n);
+  int getNumEigs() const { return num_eigvals; }
+
+ private:
+  void recreate();
+  TACSAssembler *assembler;
+  TACSMat *mmat, *kmat;
+  TACSKsm *solver;
+  TacsScalar sigma;
+  int max_lanczos, num_eigvals;
+  double eig_tol;
+  shellgpu_frequency *freq;
 };
 
 // TACSMeshLoader (src/io/TACSMeshLoader.{h,cpp}): Nastran bulk-data reader

@@ -145,6 +145,26 @@ def buckling_pipeline(newton=True, coords=True):
     return out
 
 
+def frequency_analysis():
+    """200 x 400 (seeded thickness): mass and stiffness assembly, factor of K,
+    Lanczos for the first 5 natural frequencies, their thickness gradient."""
+    cyl = Cylinder(200, 400, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
+    thick = seeded_thickness(cyl.nelems, 1234)
+    ctx = make_context(cyl, thick)
+    Fr = ctx.frequency(sigma=0.0, nev=5, max_lanczos=150, tol=1e-10)
+    _, solve_s = timed(ctx, Fr.solve)
+    lam, err = Fr.eigenvalues()
+    steps = Fr.info()["steps"]
+    gt, grad_s = timed(ctx, lambda: Fr.eigen_sens(np.ones(5)))
+    Fr.destroy()
+    ctx.close()
+    return {"workload": "cylinder 200x400, 1 design: K and M, first 5 natural frequencies (shift-invert "
+                        "Lanczos in the M inner product), thickness gradient of their sum",
+            "solve_s": solve_s, "gradient_s": grad_s, "lanczos_steps": steps,
+            "omega2": lam[0].tolist(), "freq_hz": (np.sqrt(lam[0]) / (2 * np.pi)).tolist(),
+            "max_ritz_residual": float(err.max()), "grad_t_norm": float(np.linalg.norm(gt))}
+
+
 def batched_designs(nb=64):
     """100 x 200, nb thickness fields (seeds 2000..): linear path, buckling
     (5 loads), KS thickness gradient -- batched, against one design alone."""
@@ -236,6 +256,7 @@ def main():
     ap.add_argument("--skip-pipeline", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-accuracy", action="store_true")
+    ap.add_argument("--skip-frequency", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", choices=["assembly", "calibrate"], default="assembly",
                     help="calibrate: only run a streaming kernel of known traffic (Y += a X over "
@@ -326,6 +347,8 @@ def main():
     }
     if not args.skip_pipeline:
         out["buckling_pipeline"] = buckling_pipeline()
+    if not args.skip_frequency:
+        out["frequency"] = frequency_analysis()
     if not args.skip_batched:
         out["batched"] = batched_designs(args.batch)
     if not args.skip_accuracy:
