@@ -477,24 +477,27 @@ int setup_plan(shellgpu_factor *F) {
     F->maxm = std::max(F->maxm, F->m[l]);
   }
 
-  // Block cyclic reduction when the levels are (nearly) uniform, the
-  // padded factor (three M x M blocks per level and design) fits in 85 % of
-  // the free device memory and the batch is small; otherwise the sequential
-  // level Cholesky.  SHELLGPU_FACTOR=thomas / =cr selects either.
+  // Block cyclic reduction when the padded factor (three M x M blocks per
+  // level and design) fits in 85 % of the free device memory, padding the
+  // levels to M adds at most half the unknowns, and the batch is small;
+  // otherwise the sequential level Cholesky.  SHELLGPU_FACTOR=thomas / =cr
+  // selects either (cr only where it fits).
   {
     const long M = F->maxm;
     size_t freeb = 0, totalb = 0;
     HIPCALL(ctx, hipMemGetInfo(&freeb, &totalb));
     const double need = 3.0 * nl * (double)M * (double)M * 8.0 * ctx->nbatch;
     const char *env = getenv("SHELLGPU_FACTOR");
-    const bool can = nl >= 3 && (double)nl * M <= 1.15 * F->n && need <= 0.85 * (double)freeb;
+    const bool fits = nl >= 3 && need <= 0.85 * (double)freeb;
+    const bool uniform = (double)nl * M <= 1.5 * F->n;
     // cyclic reduction pays off while one level's batched work is too small
     // to fill the GPU (few designs); with many designs the sequential
     // factorisation is already parallel and its factor is smaller (two
-    // blocks per level instead of three: fewer bytes per solve)
-    F->cr = can && (long)ctx->nbatch * M <= 8192;
+    // blocks per level instead of three, and no padding: fewer bytes per
+    // solve)
+    F->cr = fits && uniform && (long)ctx->nbatch * M <= 8192;
     if (env && strcmp(env, "thomas") == 0) F->cr = false;
-    if (env && strcmp(env, "cr") == 0) F->cr = can;
+    if (env && strcmp(env, "cr") == 0) F->cr = fits;
   }
   if (F->cr) {
     const long M = F->maxm, M2 = M * M;
