@@ -41,7 +41,11 @@ template <int MODE>
 struct ModeTraits {
   static constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
   static constexpr bool kState = (MODE != kModeKLin);
-  static constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl);
+  static constexpr bool kNonlin =
+      (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl || MODE == kModeOutNl);
+  // strains, stresses and failure values at the element nodes instead of
+  // the quadrature points (getOutputData)
+  static constexpr bool kOut = (MODE == kModeOutLin || MODE == kModeOutNl);
   static constexpr bool kRes =
       (MODE == kModeRLin || MODE == kModeRNl || MODE == kModeAdjResLin || MODE == kModeAdjResNl);
   static constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG);
@@ -282,6 +286,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   const bool valid = slot < a.count;
   const int e = a.elems[valid ? slot : slot - 1];
   const long design = blockIdx.y;
+  // output modes: the element's 4 x 31 output values
+  [[maybe_unused]] double *obase = a.out ? a.out + (design * a.nelems + e) * 124 : nullptr;
   constexpr bool kMat = MT::kMat, kState = MT::kState, kNonlin = MT::kNonlin;
   constexpr bool kRes = MT::kRes, kGeo = MT::kGeo;
 
@@ -424,7 +430,10 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     // TACSShellUtilities.h:361-422)
     const int iq = lane - 4;
     double u, v, N[4], Na[4], Nb[4];
-    quad_point(iq, u, v);
+    if (MT::kOut)
+      node_point(iq, u, v);
+    else
+      quad_point(iq, u, v);
     shape4(u, v, N, Na, Nb);
     double Xxi[6] = {0, 0, 0, 0, 0, 0}, nxi[6] = {0, 0, 0, 0, 0, 0}, n0[3] = {0, 0, 0};
 #pragma unroll
@@ -601,7 +610,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // tying point) in registers: only it reads that column when building B
   double Jr[kNB][9];
   {
-    if (lane < 24) {
+    if (!MT::kOut && lane < 24) {
       const int k = lane;
       const int n = k / 6, r = k % 6;
       // shape values of node n at the tying points (coordinates in {-1, 0, 1},
@@ -630,10 +639,13 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         }
       }
     }
-    if ((kRes || kGeo) && lane >= 24 && lane < 28) {
+    if ((kRes || kGeo || MT::kOut) && lane >= 24 && lane < 28) {
       const int iq = lane - 24;
       double u, v, N[4], Na[4], Nb[4];
-      quad_point(iq, u, v);
+      if (MT::kOut)
+        node_point(iq, u, v);
+      else
+        quad_point(iq, u, v);
       shape4(u, v, N, Na, Nb);
       double e[9], Nt[9], g[5], ety[9];
       tying_interp(u, v, Nt);
@@ -659,21 +671,60 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
                 U1[6] * U0[7];
       }
       e[8] = N[0] * sm.etn[0] + N[1] * sm.etn[1] + N[2] * sm.etn[2] + N[3] * sm.etn[3];
-      // mechanical strain (TACSIsoShellConstitutive.cpp:438-456)
-      e[0] -= a.mat.cte * temp;
-      e[1] -= a.mat.cte * temp;
-      double s9[9];
-      if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
+      if constexpr (MT::kOut) {
+        // getOutputData (TACSShellElement.h:1402-1500): total strain, stress
+        // s = C e (TACSIsoShellConstitutive::evalStress, no thermal strain),
+        // failure0 = evalFailure (TACSIsoShellConstitutive.cpp:254-297: KS,
+        // weight 100, of the top and bottom von Mises indices), dv1 = t
+        double s9[9];
+        apply_c(sec, 1.0, e, s9);
+        const double G = 0.5 * a.mat.E / (1.0 + a.mat.nu);
+        const double Dm = a.mat.E / (1.0 - a.mat.nu * a.mat.nu);
+        const double Q0 = Dm, Q1 = a.mat.nu * Dm, Q5 = G;
+        const double ht = 0.5 * thick, hb = -0.5 * thick;
+        double et[3], eb[3];
 #pragma unroll
-        for (int i = 0; i < 9; i++) s9[i] = e[i];  // mechanical strain
+        for (int i = 0; i < 3; i++) {
+          et[i] = e[i] + ht * e[3 + i];
+          eb[i] = e[i] + hb * e[3 + i];
+        }
+        const double st0 = Q0 * et[0] + Q1 * et[1], st1 = Q1 * et[0] + Q0 * et[1], st2 = Q5 * et[2];
+        const double sb0 = Q0 * eb[0] + Q1 * eb[1], sb1 = Q1 * eb[0] + Q0 * eb[1], sb2 = Q5 * eb[2];
+        const double top = sqrt(st0 * st0 + st1 * st1 - st0 * st1 + 3.0 * st2 * st2) / a.ys;
+        const double bottom = sqrt(sb0 * sb0 + sb1 * sb1 - sb0 * sb1 + 3.0 * sb2 * sb2) / a.ys;
+        const double mx = top > bottom ? top : bottom;
+        const double fail = mx + log(exp(100.0 * (top - mx)) + exp(100.0 * (bottom - mx))) / 100.0;
+        if (valid) {
+          double *o = obase + 31 * iq;
+#pragma unroll
+          for (int i = 0; i < 9; i++) {
+            o[i] = e[i];
+            o[9 + i] = s9[i];
+          }
+#pragma unroll
+          for (int i = 18; i < 31; i++) o[i] = 0.0;
+          o[18] = fail;
+          o[24] = thick;
+        }
       } else {
-        apply_c(sec, sm.wdet[iq], e, s9);
-      }
+        // mechanical strain (TACSIsoShellConstitutive.cpp:438-456)
+        e[0] -= a.mat.cte * temp;
+        e[1] -= a.mat.cte * temp;
+        double s9[9];
+        if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
 #pragma unroll
-      for (int i = 0; i < 9; i++) sm.sig[9 * iq + i] = s9[i];
+          for (int i = 0; i < 9; i++) s9[i] = e[i];  // mechanical strain
+        } else {
+          apply_c(sec, sm.wdet[iq], e, s9);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) sm.sig[9 * iq + i] = s9[i];
+      }
     }
   }
   __syncthreads();
+
+  if (MT::kOut) return;
 
   // ---- P4: stress resultants at the tying points ----
   if (kGeo && lane >= 8 && lane < 17) {
@@ -697,7 +748,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // With e(q) = B0 q + Q(q, q)/2 (Q symmetric bilinear, Bq(a) b = Q(a, b)):
   //   phi^T K phi    = sum w e1(phi)^T C e1(phi),            e1 = B0 .
   //   phi^T G(u) phi = sum w [2 Q(u, phi)^T C e1(phi) + (e1(u) - e_th)^T C Q(phi, phi)]
-  if (MODE == kModeAdjDV) {
+  if constexpr (MODE == kModeAdjDV) {
     const IsoSection dsec = iso_section(make_section_dt(a.mat, thick));
     const double *su = sm.v2;
     double *E = &sm.c.Y[0][0], *cv = &sm.c.Y[0][45];
@@ -765,7 +816,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 
   // ---- psi^T dR_e/dt (TACSShellElement::addAdjResProduct, SE.h:774-868,
   // with TACSIsoShellConstitutive::addStressDVSens) ----
-  if (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
+  if constexpr (MODE == kModeAdjResLin || MODE == kModeAdjResNl) {
     const IsoSection dsec = iso_section(make_section_dt(a.mat, thick));
     const double *sp = sm.v2;
     double *E = &sm.c.Y[0][0];
@@ -1095,6 +1146,10 @@ hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nba
       return launch_mode<kModeAdjResLin>(args, count, nbatch, stream);
     case kModeAdjResNl:
       return launch_mode<kModeAdjResNl>(args, count, nbatch, stream);
+    case kModeOutLin:
+      return launch_mode<kModeOutLin>(args, count, nbatch, stream);
+    case kModeOutNl:
+      return launch_mode<kModeOutNl>(args, count, nbatch, stream);
     case kModeMass:
     case kModeMassDV: {
       if (count <= 0 || nbatch <= 0) return hipSuccess;

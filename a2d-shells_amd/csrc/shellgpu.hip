@@ -565,6 +565,17 @@ int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double 
   return 0;
 }
 
+int shellgpu_element_output(shellgpu_ctx *ctx, int model, const double *u, double ys, double *out) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!out) return set_err(ctx, 1, "out is required");
+  if (!(ys > 0.0)) return set_err(ctx, 1, "the yield stress must be positive");
+  ElemArgs a = base_args(ctx, u);
+  a.out = out;
+  a.ys = ys;
+  return run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeOutNl : kModeOutLin, a);
+}
+
 int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u, double *R, shellgpu_mat *A) {
   int rc = check_batch(ctx, A);
   if (rc) return rc;

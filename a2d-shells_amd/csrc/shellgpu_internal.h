@@ -44,6 +44,8 @@ enum ElemMode {
   kModeAdjResNl = 7,  // psi^T dR/dt, nonlinear model (state = u)
   kModeMass = 8,      // mass matrix (getMatType MASS)
   kModeMassDV = 9,    // dv[e] += coef[d] phi^T dM_e/dt phi (state = phi)
+  kModeOutLin = 10,   // output data at the element nodes, linear model (getOutputData)
+  kModeOutNl = 11,    // the same, nonlinear model
 };
 
 struct ElemArgs {
@@ -75,6 +77,9 @@ struct ElemArgs {
   long u2_stride, psi_stride;
   const double *coef;
   double *dv;
+  // output modes: out[((d nelems + e) 4 + node) 31 + k], ys = yield stress
+  double *out;
+  double ys;
 };
 
 // d/dt of the section stiffness (A' = Q, D' = t^2/4 Q, As' = 5/6 G,

@@ -3,15 +3,18 @@
 // BDF, one isotropic shell per component, static path, shift-invert
 // buckling, eigenvalues and their residual check.
 //
-// Usage: mech_buckling [file.bdf] [thickness] [sigma] [num_eigvals] [max_lanczos]
+// Usage: mech_buckling [file.bdf] [thickness] [sigma] [num_eigvals] [max_lanczos] [f5 prefix]
 // Defaults are the reference driver's: mech-cylinder.bdf, t = 0.010,
-// sigma = 10, 50 eigenvalues, 100 Lanczos vectors, tolerance 1e-12.
-// Differences from the reference driver: the solve is called with u0 =
+// sigma = 10, 50 eigenvalues, 100 Lanczos vectors, tolerance 1e-12.  With
+// an f5 prefix every mode is written, as the reference driver does, to
+// <prefix><mode>.f5 with TACSToFH5 (connectivity, nodes, displacements,
+// strains, stresses, extras).
+// Difference from the reference driver: the solve is called with u0 =
 // NULL, so the static path is solved (the reference passes a zero vector,
-// which skips that solve and builds G from the BC values alone); no .f5
-// output is written.
+// which skips that solve and builds G from the BC values alone).
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "TACSShellGPU.h"
 
@@ -23,6 +26,7 @@ int main(int argc, char *argv[]) {
   const TacsScalar sigma = argc > 3 ? atof(argv[3]) : 10.0;
   const int num_eigvals = argc > 4 ? atoi(argv[4]) : 50;
   const int max_lanczos_vecs = argc > 5 ? atoi(argv[5]) : 100;
+  const char *f5prefix = argc > 6 ? argv[6] : NULL;
 
   printf("Scanning BDF file\n");
   TACSMeshLoader *mesh = new TACSMeshLoader();
@@ -101,6 +105,24 @@ int main(int argc, char *argv[]) {
     printf("lambda[%d] = %.16e error %.3e\n", imode, eig, error);
   }
   buckling->checkEigenvector(0);
+
+  if (f5prefix) {
+    // write each of the buckling modes to a file
+    int write_flag = (TACS_OUTPUT_CONNECTIVITY | TACS_OUTPUT_NODES | TACS_OUTPUT_DISPLACEMENTS |
+                      TACS_OUTPUT_STRAINS | TACS_OUTPUT_STRESSES | TACS_OUTPUT_EXTRAS);
+    TACSToFH5 *f5 = new TACSToFH5(assembler, TACS_BEAM_OR_SHELL_ELEMENT, write_flag);
+    f5->incref();
+    TACSBVec *phi = assembler->createVec();
+    phi->incref();
+    for (int imode = 0; imode < num_eigvals; imode++) {
+      buckling->extractEigenvector(imode, phi, &error);
+      assembler->setVariables(phi);
+      std::string filename = std::string(f5prefix) + std::to_string(imode) + ".f5";
+      f5->writeToFile(filename.c_str());
+    }
+    phi->decref();
+    f5->decref();
+  }
 
   buckling->decref();
   ksm_print->decref();

@@ -319,6 +319,7 @@ TACSAssembler *TACSMeshLoader::createTACS(int vars_per_node, int device) {
     delete tacs;
     return NULL;
   }
+  tacs->setComponents(num_components, elem_comp.data());
   std::vector<int> bn, bm;
   std::vector<TacsScalar> bv;
   getBCs(bn, bm, bv);

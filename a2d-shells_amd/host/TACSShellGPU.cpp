@@ -189,7 +189,8 @@ int GMRES::solve(TACSBVec *b, TACSBVec *x, int zero_guess) {
 
 TACSAssembler::TACSAssembler(int num_nodes, int num_elements, const int *conn, const TacsScalar *Xpts, int device)
     : ctx(NULL), num_nodes(num_nodes), num_elements(num_elements), model(0), elements(num_elements, NULL),
-      thick(num_elements, 1.0), vars(NULL), designs_dirty(true) {
+      thick(num_elements, 1.0), vars(NULL), designs_dirty(true), conn_h(conn, conn + 4 * (size_t)num_elements),
+      comp_h(num_elements, 0), X_h(Xpts, Xpts + 3 * (size_t)num_nodes), num_comp(1) {
   int rc = shellgpu_create(&ctx, device);
   if (rc) {
     fprintf(stderr, "TACSAssembler: no GPU context (%d)\n", rc);
@@ -204,6 +205,42 @@ TACSAssembler::~TACSAssembler() {
   for (TACSElement *e : elements)
     if (e) e->decref();
   if (ctx) shellgpu_destroy(ctx);
+}
+
+void TACSAssembler::setComponents(int ncomp, const int *elem_comp) {
+  num_comp = ncomp;
+  comp_h.assign(elem_comp, elem_comp + num_elements);
+}
+
+int TACSAssembler::getElementOutputData(int write_flag, int *len, int *nvals, std::vector<TacsScalar> &data) {
+  if (syncDesigns()) return 1;
+  const bool want[3] = {(write_flag & TACS_OUTPUT_STRAINS) != 0, (write_flag & TACS_OUTPUT_STRESSES) != 0,
+                        (write_flag & TACS_OUTPUT_EXTRAS) != 0};
+  const int group_off[3] = {0, 9, 18}, group_len[3] = {9, 9, 13};
+  int nv = 0;
+  for (int g = 0; g < 3; g++) nv += want[g] ? group_len[g] : 0;
+  *len = 4 * num_elements;
+  *nvals = nv;
+  data.assign((size_t)(*len) * nv, 0.0);
+  if (nv == 0) return 0;
+  TacsScalar ys = 1e11;
+  if (elements[0] && elements[0]->getConstitutive()->getMaterialProperties())
+    ys = elements[0]->getConstitutive()->getMaterialProperties()->ys;
+  const size_t n31 = (size_t)(*len) * 31;
+  double *dout = NULL;
+  if (check(shellgpu_malloc(ctx, (void **)&dout, sizeof(double) * n31), "getElementOutputData")) return 1;
+  std::vector<double> all(n31);
+  int rc = check(shellgpu_element_output(ctx, model, vars->getDeviceArray(), ys, dout), "getElementOutputData");
+  if (!rc) rc = check(shellgpu_memcpy_d2h(ctx, all.data(), dout, sizeof(double) * n31), "getElementOutputData");
+  shellgpu_free(ctx, dout);
+  if (rc) return rc;
+  for (size_t p = 0; p < (size_t)(*len); p++) {
+    int k = 0;
+    for (int g = 0; g < 3; g++)
+      if (want[g])
+        for (int i = 0; i < group_len[g]; i++) data[p * nv + k++] = all[p * 31 + group_off[g] + i];
+  }
+  return 0;
 }
 
 int TACSAssembler::check(int rc, const char *what) {
@@ -631,6 +668,189 @@ void TACSFrequencyAnalysis::checkEigenvector(int n) {
   t2->decref();
   t1->decref();
   phi->decref();
+}
+
+// ---- .f5 output ---------------------------------------------------------------
+
+TACSFH5File::TACSFH5File() : fp(NULL) {}
+TACSFH5File::~TACSFH5File() { close(); }
+
+int TACSFH5File::createFile(const char *file_name, int num_components, char **component_names) {
+  if (fp) {
+    fprintf(stderr, "[0] FH5: Error, cannot create new file\n");
+    return 1;
+  }
+  fp = fopen(file_name, "wb");
+  if (!fp) return 1;
+  // number of components, then (length with the terminating zero, name)
+  fwrite(&num_components, sizeof(int), 1, fp);
+  for (int k = 0; k < num_components; k++) {
+    const char *name = component_names[k] ? component_names[k] : "";
+    int slen = (int)strlen(name) + 1;
+    fwrite(&slen, sizeof(int), 1, fp);
+    fwrite(name, 1, slen, fp);
+  }
+  return 0;
+}
+
+int TACSFH5File::writeZoneData(char *zone_name, char *var_names, FH5DataType data_name, int dim1, int dim2,
+                               void *data) {
+  if (!fp) return 0;
+  int pre[5] = {(int)data_name, dim1, dim2, (int)strlen(zone_name) + 1, (int)strlen(var_names) + 1};
+  fwrite(pre, sizeof(int), 5, fp);
+  fwrite(zone_name, 1, pre[3], fp);
+  fwrite(var_names, 1, pre[4], fp);
+  const size_t sz = data_name == FH5_INT ? sizeof(int) : (data_name == FH5_FLOAT ? sizeof(float) : sizeof(double));
+  fwrite(data, sz, (size_t)dim1 * dim2, fp);
+  return 1;
+}
+
+void TACSFH5File::close() {
+  if (fp) fclose(fp);
+  fp = NULL;
+}
+
+namespace {
+const char *kShellDisp[6] = {"u", "v", "w", "rotx", "roty", "rotz"};
+const char *kShellLoads[6] = {"fx", "fy", "fz", "mx", "my", "mz"};
+const char *kShellStrains[9] = {"ex0", "ey0", "exy0", "ex1", "ey1", "exy1", "eyz0", "exz0", "erot"};
+const char *kShellStresses[9] = {"sx0", "sy0", "sxy0", "sx1", "sy1", "sxy1", "syz0", "sxz0", "srot"};
+const char *kShellExtras[13] = {"failure0", "failure1", "failure2", "failure3", "failure4", "failure5", "dv1",
+                                "dv2",      "dv3",      "dv4",      "dv5",      "dv6",      "dv7"};
+
+std::string join(const char *const *names, int n) {
+  std::string s;
+  for (int i = 0; i < n; i++) {
+    if (i) s += ",";
+    s += names[i];
+  }
+  return s;
+}
+}  // namespace
+
+TACSToFH5::TACSToFH5(TACSAssembler *assembler, ElementType elem_type, int write_flag)
+    : assembler(assembler), elem_type(elem_type), write_flag(write_flag) {
+  assembler->incref();
+  element_write_flag =
+      write_flag & (~(TACS_OUTPUT_CONNECTIVITY | TACS_OUTPUT_NODES | TACS_OUTPUT_DISPLACEMENTS | TACS_OUTPUT_LOADS));
+  // getElementVarNames (src/io/TACSToFH5.cpp:480-545)
+  nvals = 0;
+  std::string v;
+  if (element_write_flag & TACS_OUTPUT_STRAINS) {
+    v += join(kShellStrains, 9);
+    nvals += 9;
+  }
+  if (element_write_flag & TACS_OUTPUT_STRESSES) {
+    v += (v.empty() ? "" : ",") + join(kShellStresses, 9);
+    nvals += 9;
+  }
+  if (element_write_flag & TACS_OUTPUT_EXTRAS) {
+    v += (v.empty() ? "" : ",") + join(kShellExtras, 13);
+    nvals += 13;
+  }
+  variable_names = v;
+  if (elem_type != TACS_BEAM_OR_SHELL_ELEMENT) {
+    fprintf(stderr, "TACSToFH5: only TACS_BEAM_OR_SHELL_ELEMENT output is implemented\n");
+    nvals = 0;
+  }
+  const int nc = assembler->getNumComponents();
+  for (int k = 0; k < nc; k++) {
+    char comp_name[128];
+    snprintf(comp_name, sizeof(comp_name), "Component %d", k);
+    component_names.push_back(comp_name);
+  }
+}
+
+TACSToFH5::~TACSToFH5() { assembler->decref(); }
+
+void TACSToFH5::setComponentName(int comp_num, const char *comp_name) {
+  if (comp_num >= 0 && comp_num < (int)component_names.size()) component_names[comp_num] = comp_name;
+}
+
+int TACSToFH5::writeConnectivity(TACSFH5File *file) {
+  // TACSToFH5::writeConnectivity (src/io/TACSToFH5.cpp:369-476), one rank
+  const int ne = assembler->getNumElements();
+  std::vector<int> comp(ne), ltypes(ne, TACS_QUAD_ELEMENT), ptr(ne + 1);
+  for (int e = 0; e < ne; e++) comp[e] = assembler->getComponentNum(e);
+  for (int e = 0; e <= ne; e++) ptr[e] = 4 * e;
+  std::vector<int> conn = assembler->getConnectivity();
+  char comp_name[] = "components", layout_name[] = "ltypes", ptr_name[] = "ptr", conn_name[] = "connectivity";
+  file->writeZoneData(comp_name, comp_name, TACSFH5File::FH5_INT, ne, 1, comp.data());
+  file->writeZoneData(layout_name, layout_name, TACSFH5File::FH5_INT, ne, 1, ltypes.data());
+  file->writeZoneData(ptr_name, ptr_name, TACSFH5File::FH5_INT, ne + 1, 1, ptr.data());
+  file->writeZoneData(conn_name, conn_name, TACSFH5File::FH5_INT, 4 * ne, 1, conn.data());
+  return 0;
+}
+
+int TACSToFH5::writeToFile(const char *filename) {
+  // TACSToFH5::writeToFile (src/io/TACSToFH5.cpp:126-362), one rank
+  TACSFH5File *file = new TACSFH5File();
+  file->incref();
+  std::vector<char *> names;
+  for (std::string &n : component_names) names.push_back(&n[0]);
+  if (file->createFile(filename, (int)names.size(), names.data())) {
+    file->decref();
+    fprintf(stderr, "[0] TACSToFH5 error: Could not create file\n");
+    return 1;
+  }
+  if (write_flag & TACS_OUTPUT_CONNECTIVITY) writeConnectivity(file);
+
+  if (write_flag & (TACS_OUTPUT_NODES | TACS_OUTPUT_DISPLACEMENTS | TACS_OUTPUT_LOADS)) {
+    std::string var_names;
+    int dim2 = 0;
+    if (write_flag & TACS_OUTPUT_NODES) {
+      var_names = "X,Y,Z";
+      dim2 += 3;
+    }
+    if (write_flag & TACS_OUTPUT_DISPLACEMENTS) {
+      var_names += (var_names.empty() ? "" : ",") + join(kShellDisp, 6);
+      dim2 += 6;
+    }
+    if (write_flag & TACS_OUTPUT_LOADS) {
+      var_names += (var_names.empty() ? "" : ",") + join(kShellLoads, 6);
+      dim2 += 6;
+    }
+    const int nn = assembler->getNumNodes();
+    std::vector<TacsScalar> u, F;
+    if (write_flag & TACS_OUTPUT_DISPLACEMENTS) assembler->getVariables()->getValues(u);
+    if (write_flag & TACS_OUTPUT_LOADS) {
+      // the residual without external loads infers the applied load
+      TACSBVec *R = assembler->createVec();
+      R->incref();
+      assembler->assembleRes(R);
+      R->getValues(F);
+      R->decref();
+    }
+    const std::vector<TacsScalar> &X = assembler->getNodeLocations();
+    std::vector<float> fd((size_t)nn * dim2);
+    for (int i = 0; i < nn; i++) {
+      int k = 0;
+      if (write_flag & TACS_OUTPUT_NODES)
+        for (int j = 0; j < 3; j++) fd[(size_t)dim2 * i + k++] = (float)X[3 * i + j];
+      if (write_flag & TACS_OUTPUT_DISPLACEMENTS)
+        for (int j = 0; j < 6; j++) fd[(size_t)dim2 * i + k++] = (float)u[6 * i + j];
+      if (write_flag & TACS_OUTPUT_LOADS)
+        for (int j = 0; j < 6; j++) fd[(size_t)dim2 * i + k++] = (float)F[6 * i + j];
+    }
+    char data_name[128];
+    snprintf(data_name, sizeof(data_name), "continuous data t=%.10e", 0.0);
+    file->writeZoneData(data_name, &var_names[0], TACSFH5File::FH5_FLOAT, nn, dim2, fd.data());
+  }
+
+  if (nvals > 0) {
+    int len = 0, nv = 0;
+    std::vector<TacsScalar> data;
+    if (assembler->getElementOutputData(element_write_flag, &len, &nv, data) == 0) {
+      std::vector<float> fd(data.size());
+      for (size_t i = 0; i < data.size(); i++) fd[i] = (float)data[i];
+      char data_name[128];
+      snprintf(data_name, sizeof(data_name), "element data t=%.10e", 0.0);
+      file->writeZoneData(data_name, &variable_names[0], TACSFH5File::FH5_FLOAT, len, nv, fd.data());
+    }
+  }
+  file->close();
+  file->decref();
+  return 0;
 }
 
 }  // namespace shellgpu

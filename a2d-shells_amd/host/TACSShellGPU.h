@@ -23,6 +23,18 @@ namespace shellgpu {
 typedef double TacsScalar;
 
 // ElementMatrixType (src/elements/TACSElementTypes.h)
+// Output flags, element types and layouts (src/elements/TACSElementTypes.h:
+// 37-43, 49-60, 65-77), same values
+const int TACS_OUTPUT_CONNECTIVITY = 1;
+const int TACS_OUTPUT_NODES = 2;
+const int TACS_OUTPUT_DISPLACEMENTS = 4;
+const int TACS_OUTPUT_STRAINS = 8;
+const int TACS_OUTPUT_STRESSES = 16;
+const int TACS_OUTPUT_EXTRAS = 32;
+const int TACS_OUTPUT_LOADS = 64;
+enum ElementType { TACS_ELEMENT_NONE = 0, TACS_BEAM_OR_SHELL_ELEMENT = 4 };
+enum ElementLayout { TACS_LAYOUT_NONE = 0, TACS_QUAD_ELEMENT = 8 };
+
 // ElementMatrixType (src/elements/TACSElementTypes.h:101-107), same values
 enum ElementMatrixType {
   TACS_JACOBIAN_MATRIX,
@@ -322,6 +334,18 @@ class TACSAssembler : public TACSObject {
 
   // the element model of all elements (0 linear, 1 nonlinear)
   int getModel() const { return model; }
+
+  // host copies of the mesh and the component of each element (output)
+  const std::vector<int> &getConnectivity() const { return conn_h; }
+  const std::vector<TacsScalar> &getNodeLocations() const { return X_h; }
+  void setComponents(int num_components, const int *elem_comp);
+  int getNumComponents() const { return num_comp; }
+  int getComponentNum(int elem) const { return comp_h[elem]; }
+  // TACSAssembler::getElementOutputData (src/TACSAssembler.cpp:4433-4468)
+  // for TACS_BEAM_OR_SHELL_ELEMENT at the current variables: nvals values at
+  // each of the 4 nodes of every element (strains, stresses and extras as
+  // selected by write_flag); returns 0 on success
+  int getElementOutputData(int write_flag, int *len, int *nvals, std::vector<TacsScalar> &data);
   // upload design-dependent data (thickness, temperatures, material) if
   // the element objects or design variables changed
   int syncDesigns();
@@ -336,6 +360,9 @@ class TACSAssembler : public TACSObject {
   std::vector<TacsScalar> up_thick, up_temp;  // values last uploaded
   TACSBVec *vars;
   bool designs_dirty;
+  std::vector<int> conn_h, comp_h;
+  std::vector<TacsScalar> X_h;
+  int num_comp;
   friend class TACSBVec;
 };
 
@@ -399,6 +426,47 @@ class TACSFrequencyAnalysis : public TACSObject {
   int max_lanczos, num_eigvals;
   double eig_tol;
   shellgpu_frequency *freq;
+};
+
+// TACSFH5File writer (src/io/TACSFH5.cpp:22-283), one rank: a header
+// (number of components, then for each the length and text of its name with
+// the terminating zero) followed by zones, each a header of 5 ints (data
+// type, dim1, dim2, zone-name length, variable-names length), the two names
+// with their terminating zeros, and dim1 x dim2 values, all in native byte
+// order.  Data types: FH5_INT, FH5_DOUBLE, FH5_FLOAT.
+class TACSFH5File : public TACSObject {
+ public:
+  enum FH5DataType { FH5_INT = 0, FH5_DOUBLE = 1, FH5_FLOAT = 2 };
+  TACSFH5File();
+  ~TACSFH5File();
+  int createFile(const char *file_name, int num_components, char **component_names);
+  int writeZoneData(char *zone_name, char *var_names, FH5DataType data_name, int dim1, int dim2, void *data);
+  void close();
+
+ private:
+  FILE *fp;
+};
+
+// TACSToFH5 (src/io/TACSToFH5.{h,cpp}) for TACS_BEAM_OR_SHELL_ELEMENT:
+// writeToFile writes the connectivity zones (components, ltypes, ptr,
+// connectivity), the continuous nodal zone (X,Y,Z / u..rotz / fx..mz as
+// FH5_FLOAT) and the element zone (strains, stresses, extras at the element
+// nodes, FH5_FLOAT) selected by write_flag, in the reference's order and
+// names.  Element values come from the GPU (shellgpu_element_output).
+class TACSToFH5 : public TACSObject {
+ public:
+  TACSToFH5(TACSAssembler *assembler, ElementType elem_type, int write_flag);
+  ~TACSToFH5();
+  void setComponentName(int comp_num, const char *comp_name);
+  int writeToFile(const char *filename);
+
+ private:
+  int writeConnectivity(TACSFH5File *file);
+  TACSAssembler *assembler;
+  ElementType elem_type;
+  int write_flag, element_write_flag, nvals;
+  std::vector<std::string> component_names;
+  std::string variable_names;
 };
 
 // TACSMeshLoader (src/io/TACSMeshLoader.{h,cpp}): Nastran bulk-data reader

@@ -94,6 +94,7 @@ def load_library(path=LIB_PATH):
         "shellgpu_buckling_extract": (C.c_int, [_VP, C.c_int, _DP, _VP, _DP]),
         "shellgpu_buckling_info": (C.c_int, [_VP, _IP, _DP, _VP]),
         "shellgpu_buckling_add_eigen_sens": (C.c_int, [_VP, _DP, _VP, _VP]),
+        "shellgpu_element_output": (C.c_int, [_VP, C.c_int, _VP, C.c_double, _VP]),
         "shellgpu_frequency_create": (C.c_int, [_VP, C.c_double, C.c_int, C.c_int, C.c_double, C.POINTER(_VP)]),
         "shellgpu_frequency_destroy": (None, [_VP]),
         "shellgpu_frequency_solve": (C.c_int, [_VP]),
@@ -558,6 +559,17 @@ class Context:
 
     def buckling(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12, model=None):
         return Buckling(self, sigma, nev, max_lanczos, tol, model)
+
+    def element_output(self, u=None, model=LINEAR, ys=1e11):
+        """Strains, stresses, failure and design values at the element
+        nodes, [nbatch, nelems, 4, 31] (getOutputData)."""
+        nb = self.nbatch
+        du = None
+        if u is not None:
+            du = u if isinstance(u, DeviceArray) else self.vector(np.broadcast_to(u, (nb, 6 * self.nnodes)))
+        out = DeviceArray(self, nb * self.nelems * 4 * 31)
+        self._check(self.lib.shellgpu_element_output(self.h, model, du.ptr if du else None, float(ys), out.ptr))
+        return out.download().reshape(nb, self.nelems, 4, 31)
 
     def frequency(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12):
         return Frequency(self, sigma, nev, max_lanczos, tol)

@@ -114,6 +114,16 @@ int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double 
 int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u,
                                double *R, shellgpu_mat *A);
 
+/* TACSAssembler::getElementOutputData(TACS_BEAM_OR_SHELL_ELEMENT,
+ * TACS_OUTPUT_STRAINS | TACS_OUTPUT_STRESSES | TACS_OUTPUT_EXTRAS)
+ * (src/TACSAssembler.cpp:4433-4468) -> TACSShellElement::getOutputData
+ * (src/elements/shell/TACSShellElement.h:1402-1500): out (device, nbatch x
+ * nelems x 4 x 31) receives, at the 4 nodes of every element (in element
+ * order), the strains (9), the stresses (9), failure0..5 and dv1..7 (31
+ * values, the element zone of a TACSToFH5 file) of state u (NULL = 0) for the
+ * given model; ys is the yield stress of the von Mises failure index. */
+int shellgpu_element_output(shellgpu_ctx *ctx, int model, const double *u, double ys, double *out);
+
 /* ---- design sensitivities (thickness t_e per element; device outputs
  * accumulate, nbatch*nelems) ---------------------------------------------- */
 /* TACSAssembler::addAdjointResProducts -> TACSShellElement::addAdjResProduct

@@ -57,6 +57,11 @@ void or_elem_jacobian(int model, const or_material *m, double t, double temp,
 void or_elem_stiffness(int model, const or_material *m, double t, double temp,
                        const double X[12], const double q[24],
                        double mat[576]);
+/* getOutputData(TACS_BEAM_OR_SHELL_ELEMENT, STRAINS | STRESSES | EXTRAS):
+ * 31 values at each of the 4 nodes (strain 9, stress 9, failure 6, design
+ * 7), ys the yield stress.  SE.h:1402-1500; ISC:153-189, 254-297, 503-512 */
+void or_elem_output(int model, const or_material *m, double t, double ys, const double X[12],
+                    const double q[24], double out[124]);
 /* getMatType(TACS_MASS_MATRIX): consistent mass of the element, mat
  * overwritten.  SE.h:613-648, 675-704; SD.h:368-430; ISC:120-129 */
 void or_elem_mass(const or_material *m, double t, const double X[12], double mat[576]);

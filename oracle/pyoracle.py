@@ -60,6 +60,12 @@ def elem_stiffness(model, mat, t, temp, X, q):
     return K.reshape(24, 24)
 
 
+def elem_output(model, mat, t, ys, X, q):
+    out = np.zeros(124)
+    lib().or_elem_output(model, C.byref(mat), C.c_double(t), C.c_double(ys), _d(_f64(X)), _d(_f64(q)), _d(out))
+    return out.reshape(4, 31)
+
+
 def elem_mass(mat, t, X):
     M = np.zeros(576)
     lib().or_elem_mass(C.byref(mat), C.c_double(t), _d(_f64(X)), _d(M))

@@ -7,7 +7,7 @@
 // TACSQuad4Shell / TACSQuad4NonlinearShell compute:
 //
 //   record in  (66 doubles): Xpts[12] vars[24] psi[24] t E nu rho cte temperature
-//   record out (2356 doubles):
+//   record out (2516 doubles):
 //     res_lin[24]   TACSShellElement::addResidual, linear model      (SE.h:303-449)
 //     res_nl[24]    same, nonlinear model
 //     K_lin[576]    getMatType(TACS_STIFFNESS_MATRIX), linear model   (SE.h:675-771)
@@ -18,6 +18,8 @@
 //     dRdt_nl       psi^T dR_nl/dt from a 5-point stencil in t
 //     energy_lin    strain energy from computeEnergies, linear model  (SE.h:202-297)
 //     M[576]        getMatType(TACS_MASS_MATRIX), linear model        (SE.h:675-704)
+//     out_lin[160]  getOutputData(TACS_BEAM_OR_SHELL_ELEMENT, nodes | displacements |
+//                   strains | stresses | extras), linear model: 4 nodes x 40 (SE.h:1402-1500)
 //
 // The residual is a polynomial of degree 3 in the thickness (A ~ t, D ~ t^3,
 // drilling ~ t), so the 5-point central stencil with step 0.1 t is exact up to
@@ -35,7 +37,7 @@
 #include "TACSShellElementTransform.h"
 
 static const int NIN = 66;
-static const int NOUT = 24 + 24 + 3 * 576 + 4 + 576;
+static const int NOUT = 24 + 24 + 3 * 576 + 4 + 576 + 160;
 
 static double psi_dot_res(TACSElement *elem, const double *X, const double *q,
                           const double *psi, const double *zero) {
@@ -132,6 +134,9 @@ int main(int argc, char **argv) {
     lin->computeEnergies(0, 0.0, X, q, zero, &Te, &Ue);
     tail[3] = Ue;
     lin->getMatType(TACS_MASS_MATRIX, 0, 0.0, X, q, tail + 4);
+    const int flag = TACS_OUTPUT_NODES | TACS_OUTPUT_DISPLACEMENTS | TACS_OUTPUT_STRAINS |
+                     TACS_OUTPUT_STRESSES | TACS_OUTPUT_EXTRAS;
+    lin->getOutputData(0, TACS_BEAM_OR_SHELL_ELEMENT, flag, X, q, zero, zero, 40, tail + 4 + 576);
 
     nl->decref();
     lin->decref();

@@ -1147,6 +1147,66 @@ void or_elem_stiffness(int model, const or_material *m, double t, double temp, c
   elem_core(model, m, t, temp, 1.0, X, q, res, mat);
 }
 
+/* getOutputData for TACS_BEAM_OR_SHELL_ELEMENT (SE.h:1402-1500) with the
+ * strain, stress and extra groups (TACSToFH5's element zone): at each of
+ * the 4 nodes, the strain e[9] (u0x, u1x and the tying strain interpolated at
+ * the node, e[8] = drilling strain there), the stress s = C e of
+ * TACSIsoShellConstitutive::evalStress (ISC:153-189, no thermal strain),
+ * failure0 = evalFailure (ISC:254-297: KS, weight 100, of the von Mises
+ * values of the top and bottom surface stresses, TACSMaterialProperties.cpp:
+ * 453-457), failure1..5 = 0, dv1 = t, dv2..7 = 0 (ISC:503-512).
+ * out[31 * node + k].
+ * pinned by: golden vectors of the compiled reference getOutputData */
+#ifndef OR_NO_EXTERN_C /* not part of the operation-counting build (flopcount.cpp) */
+void or_elem_output(int model, const or_material *m, double t, double ys, const double Xpts[12],
+                    const double q[24], double out[124]) {
+  elem_setup es;
+  elem_setup_all(model, Xpts, q, &es);
+  abd_t c;
+  eval_abd(m, t, &c);
+  double Q[6];
+  tangent_2d(m, Q);
+  const double ksWeight = 100.0, tOffset = 0.0;
+  for (int n = 0; n < NN; n++) {
+    double pt[2];
+    node_point(n, pt);
+    qp_data qp;
+    qp_kinematics(pt, 1.0, Xpts, q, &es, &qp);
+    double gty[6], e0ty[6], e[9], sg[9];
+    interp_tying(pt, es.ety, gty);
+    sym_tt(qp.XdinvT, gty, e0ty);
+    eval_strain(model, qp.u0x, qp.u1x, e0ty, e);
+    e[8] = qp.et;
+    compute_stress(&c, e, sg);
+    /* evalFailure: top and bottom surface strains and stresses */
+    const double ht = (0.5 - tOffset) * t, hb = -(0.5 + tOffset) * t;
+    double et[3], eb[3], st[3], sb[3];
+    for (int i = 0; i < 3; i++) {
+      et[i] = e[i] + ht * e[3 + i];
+      eb[i] = e[i] + hb * e[3 + i];
+    }
+    st[0] = Q[0] * et[0] + Q[1] * et[1] + Q[2] * et[2];
+    st[1] = Q[1] * et[0] + Q[3] * et[1] + Q[4] * et[2];
+    st[2] = Q[2] * et[0] + Q[4] * et[1] + Q[5] * et[2];
+    sb[0] = Q[0] * eb[0] + Q[1] * eb[1] + Q[2] * eb[2];
+    sb[1] = Q[1] * eb[0] + Q[3] * eb[1] + Q[4] * eb[2];
+    sb[2] = Q[2] * eb[0] + Q[4] * eb[1] + Q[5] * eb[2];
+    const double top = sqrt(st[0] * st[0] + st[1] * st[1] - st[0] * st[1] + 3.0 * st[2] * st[2]) / ys;
+    const double bottom = sqrt(sb[0] * sb[0] + sb[1] * sb[1] - sb[0] * sb[1] + 3.0 * sb[2] * sb[2]) / ys;
+    const double ksMax = top > bottom ? top : bottom;
+    const double ksSum = exp(ksWeight * (top - ksMax)) + exp(ksWeight * (bottom - ksMax));
+    double *o = &out[31 * n];
+    for (int i = 0; i < 9; i++) {
+      o[i] = e[i];
+      o[9 + i] = sg[i];
+    }
+    for (int i = 0; i < 13; i++) o[18 + i] = 0.0;
+    o[18] = ksMax + log(ksSum) / ksWeight;
+    o[24] = t;
+  }
+}
+#endif
+
 /* getMatType(TACS_MASS_MATRIX) (SE.h:675-704 with gamma = 1, alpha = beta =
  * 0): the inertial terms of addJacobian (SE.h:613-648) -- the translational
  * block gamma detXd m0 I through addInterpFieldsOuterProduct and the

@@ -85,7 +85,8 @@ def main():
         K_nl=out[:, 624:1200].reshape(-1, 24, 24),
         G=out[:, 1200:1776].reshape(-1, 24, 24),
         adj_lin=out[:, 1776], dRdt_lin=out[:, 1777], dRdt_nl=out[:, 1778],
-        energy_lin=out[:, 1779], M=out[:, 1780:2356].reshape(-1, 24, 24))
+        energy_lin=out[:, 1779], M=out[:, 1780:2356].reshape(-1, 24, 24),
+        output_lin=out[:, 2356:2516].reshape(-1, 4, 40), ys=1e11)
     print("wrote", len(recs), "records")
 
 

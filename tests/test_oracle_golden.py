@@ -141,3 +141,17 @@ def test_mass_rigid_motion_energies():
     base = np.array([[4, 2, 2, 1], [2, 4, 1, 2], [2, 1, 4, 2], [1, 2, 2, 4]]) * rho * t * area / 36.0
     Mx = M[0::6, 0::6]
     assert np.abs(Mx - base).max() <= 1e-12 * base.max()
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_output_data_matches_reference(i):
+    """getOutputData: strains, stresses, failure and design values at the
+    4 nodes (the element zone of an .f5 file); the leading node location and
+    displacement groups are the element's own inputs."""
+    m, t, T, X, q, psi = _rec(i)
+    out = O.elem_output(0, m, t, float(GOLD["ys"]), X, q)
+    ref = GOLD["output_lin"][i]
+    assert np.array_equal(ref[:, 0:3], X.reshape(4, 3))
+    assert np.array_equal(ref[:, 3:9], q.reshape(4, 6))
+    assert rel(out, ref[:, 9:40]) < 1e-13
+    assert rel(out[:, 18], ref[:, 27]) < 1e-13  # failure0 on its own scale
