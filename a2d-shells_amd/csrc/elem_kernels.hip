@@ -326,7 +326,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     __syncthreads();
   }
 
-  // ---- P1: node normals + nodal drilling frames; tying-point X,xi ----
+  // ---- P1: node normals; tying-point X,xi ----
   // (TacsShellComputeNodeNormals, TACSShellUtilities.h:301-343)
   if (lane < 4) {
     const int n = lane;
@@ -354,16 +354,6 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     sm.fn[3 * n] = f[0];
     sm.fn[3 * n + 1] = f[1];
     sm.fn[3 * n + 2] = f[2];
-    double Tn[9], Xd[9], Xdi[9], XTn[9];
-    natural_transform(Xxi, f, Tn);
-    frame3(Xxi, f, Xd);
-    inv3(Xd, Xdi);
-    mm3(Xdi, Tn, XTn);
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-      sm.f.Tn[9 * n + i] = Tn[i];
-      sm.f.XTn[9 * n + i] = XTn[i];
-    }
   } else if (lane >= 8 && lane < 17) {
     const int t = lane - 8;
     double u, v, N[4], Na[4], Nb[4];
@@ -382,137 +372,114 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   }
   __syncthreads();
 
-  // ---- P2: drilling rows; quadrature-point frames + tying weights; tying
-  // point state and derivative coefficients ----
-  if (lane < 4) {
-    // TacsShellAddDrillStrainSens map (TACSShellUtilities.h:787-835) with
-    // TACSLinearizedRotation: et = 0.5 (Ct[3] + u0x[3] - Ct[1] - u0x[1])
-    const int n = lane;
-    const double *Tn = &sm.f.Tn[9 * n], *XTn = &sm.f.XTn[9 * n];
-    const double dA[9] = {0, -0.5, 0, 0.5, 0, 0, 0, 0, 0};
-    double tmp[9], dC[9], du0d[9];
-    // dC = Tn dA Tn^T ; du0d = Tn dA XTn^T
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        tmp[3 * i + j] = dA[3 * i] * Tn[3 * j] + dA[3 * i + 1] * Tn[3 * j + 1] +
-                         dA[3 * i + 2] * Tn[3 * j + 2];
-      }
-    mm3(Tn, tmp, dC);
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        tmp[3 * i + j] = dA[3 * i] * XTn[3 * j] + dA[3 * i + 1] * XTn[3 * j + 1] +
-                         dA[3 * i + 2] * XTn[3 * j + 2];
-      }
-    mm3(Tn, tmp, du0d);
-    // rotation dofs of node n enter through C, displacements of every node
-    // through u0x
-    const double rq[3] = {-(dC[7] - dC[5]), -(dC[2] - dC[6]), -(dC[3] - dC[1])};
+  // ---- P1b: frames at the nodes (lanes 0-3: the drilling-strain frames) and
+  // at the evaluation points (lanes 4-7: the quadrature points, or the nodes
+  // in the output modes) in one pass of the same code; quadrature-point
+  // kinematics (TacsShellComputeDispGrad, TACSShellUtilities.h:361-422) and
+  // tying weights; tying-point state and derivative coefficients ----
+  if (lane < 8) {
+    const bool at_node = lane < 4;
+    const int iq = lane & 3;
     double u, v, N[4], Na[4], Nb[4];
-    node_point(n, u, v);
-    shape4(u, v, N, Na, Nb);
-    const double g[6] = {du0d[0], du0d[1], du0d[3], du0d[4], du0d[6], du0d[7]};
-    double et = 0.0;
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-#pragma unroll
-      for (int d = 0; d < 6; d++) {
-        const double val = (d < 3) ? Na[m] * g[2 * d] + Nb[m] * g[2 * d + 1] : (m == n ? rq[d - 3] : 0.0);
-        sm.drot[24 * n + 6 * m + d] = val;
-        et += val * sm.s[6 * m + d];
-      }
-    sm.etn[n] = et;
-  } else if (lane < 8) {
-    // quadrature point kinematics (TacsShellComputeDispGrad,
-    // TACSShellUtilities.h:361-422)
-    const int iq = lane - 4;
-    double u, v, N[4], Na[4], Nb[4];
-    if (MT::kOut)
+    if (at_node || MT::kOut)
       node_point(iq, u, v);
     else
       quad_point(iq, u, v);
     shape4(u, v, N, Na, Nb);
-    double Xxi[6] = {0, 0, 0, 0, 0, 0}, nxi[6] = {0, 0, 0, 0, 0, 0}, n0[3] = {0, 0, 0};
+    // at a node the interpolated normal is that node's normal exactly
+    double Xxi[6] = {0, 0, 0, 0, 0, 0}, n0[3] = {0, 0, 0};
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         Xxi[2 * d] += Na[m] * sm.X[3 * m + d];
         Xxi[2 * d + 1] += Nb[m] * sm.X[3 * m + d];
-        nxi[2 * d] += Na[m] * sm.fn[3 * m + d];
-        nxi[2 * d + 1] += Nb[m] * sm.fn[3 * m + d];
         n0[d] += N[m] * sm.fn[3 * m + d];
       }
-    double T[9], Xd[9], Xdz[9], Xdi[9], XT[9], nXdz[9], XzT[9];
+    double T[9], Xd[9], Xdi[9], XT[9];
     natural_transform(Xxi, n0, T);
     frame3(Xxi, n0, Xd);
-    frame3z(nxi, Xdz);
     const double det = inv3(Xd, Xdi);
-    mm3(Xdi, Xdz, nXdz);
-#pragma unroll
-    for (int i = 0; i < 9; i++) nXdz[i] = -nXdz[i];
     mm3(Xdi, T, XT);
-    mm3(nXdz, XT, XzT);
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-      sm.T[9 * iq + i] = T[i];
-      sm.XT[9 * iq + i] = XT[i];
-      sm.XzT[9 * iq + i] = XzT[i];
-    }
-    sm.wdet[iq] = det;  // Gauss weight 1
-
-    // tying field -> strain weights: e0ty = XT^T gty XT, gty the tying
-    // fields interpolated at the point (mat3x3SymmTransformTranspose,
-    // TACSElementAlgebra.h:1094); the interpolation weights are applied
-    // where the fields are formed (tying_fields)
-    const int kk[5] = {0, 3, 1, 4, 2};
-    const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
-    const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
-#pragma unroll
-    for (int r = 0; r < 5; r++) {
-      const int ka = pa[kk[r]], kb = pb[kk[r]];
-#pragma unroll
-      for (int f = 0; f < 5; f++) {
-        const int sl = field_slot(f);
-        const int c = pa[sl], d = pb[sl];
-        const double M = (c == d) ? XT[3 * c + ka] * XT[3 * c + kb]
-                                  : XT[3 * c + ka] * XT[3 * d + kb] + XT[3 * d + ka] * XT[3 * c + kb];
-        sm.Wf[25 * iq + 5 * r + f] = ff[r] * M;
-      }
-    }
-    if (kState) {
-      // displacement-gradient state at the point: u0x, u1x
-      double d0[3] = {0, 0, 0}, d0xi[6] = {0, 0, 0, 0, 0, 0}, u0xi[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-      for (int m = 0; m < 4; m++) {
-        double dm[3];
-        cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          d0[d] += N[m] * dm[d];
-          d0xi[2 * d] += Na[m] * dm[d];
-          d0xi[2 * d + 1] += Nb[m] * dm[d];
-          u0xi[2 * d] += Na[m] * sm.s[6 * m + d];
-          u0xi[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
-        }
-      }
-      double u0d[9], u1d[9], tmp[9], tmp2[9], o0[9], o1[9];
-      frame3(u0xi, d0, u0d);
-      frame3z(d0xi, u1d);
-      mm3(u1d, XT, tmp);
-      mm3(u0d, XzT, tmp2);
-#pragma unroll
-      for (int i = 0; i < 9; i++) tmp[i] += tmp2[i];
-      mtm3(T, tmp, o1);
-      mm3(u0d, XT, tmp);
-      mtm3(T, tmp, o0);
+    if (at_node) {
 #pragma unroll
       for (int i = 0; i < 9; i++) {
-        sm.u0x[9 * iq + i] = o0[i];
-        sm.u1x[9 * iq + i] = o1[i];
+        sm.f.Tn[9 * iq + i] = T[i];
+        sm.f.XTn[9 * iq + i] = XT[i];
+      }
+    } else {
+      double nxi[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          nxi[2 * d] += Na[m] * sm.fn[3 * m + d];
+          nxi[2 * d + 1] += Nb[m] * sm.fn[3 * m + d];
+        }
+      double Xdz[9], nXdz[9], XzT[9];
+      frame3z(nxi, Xdz);
+      mm3(Xdi, Xdz, nXdz);
+#pragma unroll
+      for (int i = 0; i < 9; i++) nXdz[i] = -nXdz[i];
+      mm3(nXdz, XT, XzT);
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        sm.T[9 * iq + i] = T[i];
+        sm.XT[9 * iq + i] = XT[i];
+        sm.XzT[9 * iq + i] = XzT[i];
+      }
+      sm.wdet[iq] = det;  // Gauss weight 1
+
+      // tying field -> strain weights: e0ty = XT^T gty XT, gty the tying
+      // fields interpolated at the point (mat3x3SymmTransformTranspose,
+      // TACSElementAlgebra.h:1094); the interpolation weights are applied
+      // where the fields are formed (tying_fields)
+      const int kk[5] = {0, 3, 1, 4, 2};
+      const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
+      const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
+  #pragma unroll
+      for (int r = 0; r < 5; r++) {
+        const int ka = pa[kk[r]], kb = pb[kk[r]];
+  #pragma unroll
+        for (int f = 0; f < 5; f++) {
+          const int sl = field_slot(f);
+          const int c = pa[sl], d = pb[sl];
+          const double M = (c == d) ? XT[3 * c + ka] * XT[3 * c + kb]
+                                    : XT[3 * c + ka] * XT[3 * d + kb] + XT[3 * d + ka] * XT[3 * c + kb];
+          sm.Wf[25 * iq + 5 * r + f] = ff[r] * M;
+        }
+      }
+      if (kState) {
+        // displacement-gradient state at the point: u0x, u1x
+        double d0[3] = {0, 0, 0}, d0xi[6] = {0, 0, 0, 0, 0, 0}, u0xi[6] = {0, 0, 0, 0, 0, 0};
+  #pragma unroll
+        for (int m = 0; m < 4; m++) {
+          double dm[3];
+          cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
+  #pragma unroll
+          for (int d = 0; d < 3; d++) {
+            d0[d] += N[m] * dm[d];
+            d0xi[2 * d] += Na[m] * dm[d];
+            d0xi[2 * d + 1] += Nb[m] * dm[d];
+            u0xi[2 * d] += Na[m] * sm.s[6 * m + d];
+            u0xi[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
+          }
+        }
+        double u0d[9], u1d[9], tmp[9], tmp2[9], o0[9], o1[9];
+        frame3(u0xi, d0, u0d);
+        frame3z(d0xi, u1d);
+        mm3(u1d, XT, tmp);
+        mm3(u0d, XzT, tmp2);
+  #pragma unroll
+        for (int i = 0; i < 9; i++) tmp[i] += tmp2[i];
+        mtm3(T, tmp, o1);
+        mm3(u0d, XT, tmp);
+        mtm3(T, tmp, o0);
+  #pragma unroll
+        for (int i = 0; i < 9; i++) {
+          sm.u0x[9 * iq + i] = o0[i];
+          sm.u1x[9 * iq + i] = o1[i];
+        }
       }
     }
   } else if (lane >= 8 && lane < 17) {
@@ -523,14 +490,17 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     double n0[3] = {0, 0, 0}, U[6] = {0, 0, 0, 0, 0, 0}, d0[3] = {0, 0, 0};
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-      double dm[3];
-      cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
 #pragma unroll
-      for (int d = 0; d < 3; d++) {
-        n0[d] += N[m] * sm.fn[3 * m + d];
-        d0[d] += N[m] * dm[d];
-        U[2 * d] += Na[m] * sm.s[6 * m + d];
-        U[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
+      for (int d = 0; d < 3; d++) n0[d] += N[m] * sm.fn[3 * m + d];
+      if (kState) {  // the linear stiffness has no state: U = d0 = 0
+        double dm[3];
+        cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          d0[d] += N[m] * dm[d];
+          U[2 * d] += Na[m] * sm.s[6 * m + d];
+          U[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
+        }
       }
     }
     const double *X = &sm.f.tXxi[6 * t];
@@ -601,6 +571,51 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 #pragma unroll
       for (int i = 0; i < 3; i++) cd[3 * t + i] = dd[i];
     }
+  }
+  __syncthreads();
+
+  // ---- P2: drilling rows at the nodes ----
+  if (lane < 4) {
+    // TacsShellAddDrillStrainSens map (TACSShellUtilities.h:787-835) with
+    // TACSLinearizedRotation: et = 0.5 (Ct[3] + u0x[3] - Ct[1] - u0x[1])
+    const int n = lane;
+    const double *Tn = &sm.f.Tn[9 * n], *XTn = &sm.f.XTn[9 * n];
+    const double dA[9] = {0, -0.5, 0, 0.5, 0, 0, 0, 0, 0};
+    double tmp[9], dC[9], du0d[9];
+    // dC = Tn dA Tn^T ; du0d = Tn dA XTn^T
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        tmp[3 * i + j] = dA[3 * i] * Tn[3 * j] + dA[3 * i + 1] * Tn[3 * j + 1] +
+                         dA[3 * i + 2] * Tn[3 * j + 2];
+      }
+    mm3(Tn, tmp, dC);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        tmp[3 * i + j] = dA[3 * i] * XTn[3 * j] + dA[3 * i + 1] * XTn[3 * j + 1] +
+                         dA[3 * i + 2] * XTn[3 * j + 2];
+      }
+    mm3(Tn, tmp, du0d);
+    // rotation dofs of node n enter through C, displacements of every node
+    // through u0x
+    const double rq[3] = {-(dC[7] - dC[5]), -(dC[2] - dC[6]), -(dC[3] - dC[1])};
+    double u, v, N[4], Na[4], Nb[4];
+    node_point(n, u, v);
+    shape4(u, v, N, Na, Nb);
+    const double g[6] = {du0d[0], du0d[1], du0d[3], du0d[4], du0d[6], du0d[7]};
+    double et = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int d = 0; d < 6; d++) {
+        const double val = (d < 3) ? Na[m] * g[2 * d] + Nb[m] * g[2 * d + 1] : (m == n ? rq[d - 3] : 0.0);
+        sm.drot[24 * n + 6 * m + d] = val;
+        et += val * sm.s[6 * m + d];
+      }
+    sm.etn[n] = et;
   }
   __syncthreads();
 
@@ -678,9 +693,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         // weight 100, of the top and bottom von Mises indices), dv1 = t
         double s9[9];
         apply_c(sec, 1.0, e, s9);
-        const double G = 0.5 * a.mat.E / (1.0 + a.mat.nu);
-        const double Dm = a.mat.E / (1.0 - a.mat.nu * a.mat.nu);
-        const double Q0 = Dm, Q1 = a.mat.nu * Dm, Q5 = G;
+        const double Q0 = a.mat.q11, Q1 = a.mat.q12, Q5 = a.mat.q66;
         const double ht = 0.5 * thick, hb = -0.5 * thick;
         double et[3], eb[3];
 #pragma unroll

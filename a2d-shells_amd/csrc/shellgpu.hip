@@ -374,7 +374,7 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
 int shellgpu_set_material(shellgpu_ctx *ctx, double E, double nu, double rho, double cte, double k_drill) {
   if (!ctx) return 1;
   if (!(E > 0.0) || !(nu > -1.0 && nu < 0.5)) return set_err(ctx, 1, "invalid material");
-  ctx->mat = Material{E, nu, rho, cte, k_drill};
+  ctx->mat = make_material(E, nu, rho, cte, k_drill);
   return 0;
 }
 

@@ -19,7 +19,7 @@ struct shellgpu_ctx {
   int *d_color_elems = nullptr;
   double *d_X = nullptr;
 
-  shellgpu::Material mat{70e9, 0.3, 2700.0, 0.0, 10.0};
+  shellgpu::Material mat = shellgpu::make_material(70e9, 0.3, 2700.0, 0.0, 10.0);
 
   int nbc = 0;
   std::vector<int> h_bc_nodes, h_bc_mask;

@@ -6,9 +6,18 @@
 namespace shellgpu {
 
 // Isotropic material (TACSMaterialProperties, TACSMaterialProperties.cpp:24-63)
+// with its plane-stress stiffness Q = [[q11 q12 0][q12 q11 0][0 0 q66]]
+// (evalTangentStiffness2D, TACSMaterialProperties.cpp:295-313) formed once
 struct Material {
   double E, nu, rho, cte, kdrill;
+  double q11, q12, q66;
 };
+
+inline Material make_material(double E, double nu, double rho, double cte, double kdrill) {
+  const double G = 0.5 * E / (1.0 + nu);
+  const double Dm = E / (1.0 - nu * nu);
+  return Material{E, nu, rho, cte, kdrill, Dm, nu * Dm, G};
+}
 
 // Section stiffness of TACSIsoShellConstitutive with tOffset = 0
 // (TACSIsoShellConstitutive.cpp:192-226): A = t Q, B = 0, D = t^3/12 Q,
@@ -19,9 +28,7 @@ struct Section {
 
 __host__ __device__ inline Section make_section(const Material &m, double t) {
   Section s;
-  const double G = 0.5 * m.E / (1.0 + m.nu);
-  const double Dm = m.E / (1.0 - m.nu * m.nu);
-  const double Q[6] = {Dm, m.nu * Dm, 0.0, Dm, 0.0, G};
+  const double Q[6] = {m.q11, m.q12, 0.0, m.q11, 0.0, m.q66};
   const double I = t * t * t / 12.0;
   for (int i = 0; i < 6; i++) {
     s.D[i] = I * Q[i];
@@ -86,9 +93,8 @@ struct ElemArgs {
 // drill' = kdrill 5/6 G), the derivative of make_section
 __host__ __device__ inline Section make_section_dt(const Material &m, double t) {
   Section s;
-  const double G = 0.5 * m.E / (1.0 + m.nu);
-  const double Dm = m.E / (1.0 - m.nu * m.nu);
-  const double Q[6] = {Dm, m.nu * Dm, 0.0, Dm, 0.0, G};
+  const double G = m.q66;
+  const double Q[6] = {m.q11, m.q12, 0.0, m.q11, 0.0, m.q66};
   for (int i = 0; i < 6; i++) {
     s.A[i] = Q[i];
     s.D[i] = 0.25 * t * t * Q[i];
