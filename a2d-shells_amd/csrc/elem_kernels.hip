@@ -870,6 +870,23 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // lane tile: rows row0..row0+2 (one node's translations or rotations),
   // columns col0..col0+5 (all dofs of node col0 / 6)
   const int row0 = 3 * (lane >> 2), col0 = 6 * (lane & 3);
+  // the lane's 18 entries of its BSR block; the nonlinear modes read them
+  // now so that the read completes during the quadrature loop (no other
+  // element of this launch writes the block), the linear stiffness at the
+  // end (measured faster: fewer live registers through the loop)
+  constexpr bool kEarly = (MODE == kModeKNl || MODE == kModeG);
+  double prev[18];
+  double *dst = nullptr;
+  if (kMat && valid) {
+    const long blk = a.blk[16 * e + 4 * (row0 / 6) + col0 / 6];
+    dst = a.vals + design * a.vals_stride + 36 * blk + 6 * (row0 % 6);
+    if (kEarly) {
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) prev[6 * i + j] = dst[6 * i + j];
+    }
+  }
   double acc[18];
 #pragma unroll
   for (int i = 0; i < 18; i++) acc[i] = 0.0;
@@ -972,14 +989,11 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 
   // ---- P7: scatter (elements of one colour share no node) ----
   if (kMat && valid) {
-    const int ni = row0 / 6, nj = col0 / 6;
-    const int rr = row0 % 6;
-    const long blk = a.blk[16 * e + 4 * ni + nj];
-    double *dst = a.vals + design * a.vals_stride + 36 * blk;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 6; j++) dst[6 * (rr + i) + j] += alpha * acc[6 * i + j];
+      for (int j = 0; j < 6; j++)
+        dst[6 * i + j] = (kEarly ? prev[6 * i + j] : dst[6 * i + j]) + alpha * acc[6 * i + j];
   }
   if (kRes && valid && lane < 24) {
     double *dst = a.res + design * a.res_stride;
