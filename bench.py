@@ -61,6 +61,11 @@ PEAK_HBM_GBPS = 8000.0    # MI355X HBM3E
 KS_RHO = 100.0
 
 
+def progress(leg):
+    """One line on stderr per finished leg (stdout carries only the JSON)."""
+    print(f"[bench] {leg} done", file=sys.stderr, flush=True)
+
+
 def spmv_bytes(nnzb, nnodes):
     """Algorithmic bytes of one 6x6-block SpMV: values (288 B/block), column
     indices (4 B/block), row pointers, x gathered once and y written once
@@ -408,18 +413,17 @@ def main():
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
         },
     }
-    if not args.skip_pipeline:
-        out["buckling_pipeline"] = buckling_pipeline()
-    if not args.skip_frequency:
-        out["frequency"] = frequency_analysis()
-    if not args.skip_boxbeam:
-        out["box_beam"] = box_beam()
-    if not args.skip_batched:
-        out["batched"] = batched_designs(args.batch)
-    if not args.skip_accuracy:
-        out["accuracy"] = accuracy_vs_oracle()
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cyl, thick, args.cpu_seconds)
+    legs = [("buckling_pipeline", not args.skip_pipeline, buckling_pipeline),
+            ("frequency", not args.skip_frequency, frequency_analysis),
+            ("box_beam", not args.skip_boxbeam, box_beam),
+            ("batched", not args.skip_batched, lambda: batched_designs(args.batch)),
+            ("accuracy", not args.skip_accuracy, accuracy_vs_oracle),
+            ("cpu_baseline", not args.no_cpu_baseline, lambda: cpu_baseline(cyl, thick, args.cpu_seconds))]
+    progress("assembly")
+    for name, run, fn in legs:
+        if run:
+            out[name] = fn()
+            progress(name)
     print(json.dumps(out), flush=True)
 
 

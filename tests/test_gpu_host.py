@@ -104,3 +104,29 @@ def test_mech_frequency_driver_matches_dense_oracle(tmp_path):
     h = 1e-6 * t
     fd = (dense(t + h)[0] - dense(t - h)[0]) / (2 * h)
     assert abs(dsum - fd) <= 1e-6 * abs(fd)
+
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_reference_bdf_buckling_matches_oracle_golden(tmp_path):
+    """The reference's own example mesh (examples/cylinder-buckling/
+    mech-cylinder.bdf, kept compressed under tests/golden/): the C++ driver's
+    loads against the oracle's on the same file (tests/golden/
+    make_bdf_golden.py), the first 5 within 1e-8."""
+    import gzip
+    import json
+
+    gold = json.load(open(os.path.join(GOLDEN_DIR, "mech_cylinder_buckling.json")))
+    path = str(tmp_path / "mech-cylinder.bdf")
+    with gzip.open(os.path.join(GOLDEN_DIR, "mech-cylinder.bdf.gz"), "rb") as f:
+        open(path, "wb").write(f.read())
+    out = subprocess.run([DRIVER, path, str(gold["thickness"]), str(gold["sigma"]), str(gold["nev"]),
+                          str(gold["max_lanczos"])], capture_output=True, text=True, timeout=180)
+    print(out.stdout[-2500:], out.stderr[-2000:])
+    assert out.returncode == 0
+    assert f"nodes {gold['nodes']} elements {gold['elements']}" in out.stdout
+    lam = np.array([float(m.group(2)) for m in re.finditer(r"lambda\[(\d+)\] = (\S+)", out.stdout)])
+    ref = np.array(gold["lambda"])
+    print("driver", lam[:5], "oracle", ref[:5])
+    assert np.abs(lam[:5] - ref[:5]).max() <= 1e-8 * np.abs(ref[:5]).max()
