@@ -1609,7 +1609,7 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // shift: sigma > 0 as given; sigma <= 0 places it at 0.95 of a first-load
   // estimate from a short buckling-mode Lanczos on K^{-1}(-G)
   if (B->sigma <= 0.0) {
-    if ((rc = lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-4, B->Q, B->KQ, B->hv, B->res, lo)))
+    if ((rc = lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-2, B->Q, B->KQ, B->hv, B->res, lo)))
       return rc;
     for (int d = 0; d < nb; d++) {
       const int ns = (int)lo.alpha[d].size();
