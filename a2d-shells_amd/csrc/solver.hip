@@ -102,6 +102,9 @@ struct shellgpu_factor {
   long tmp_elems = 0;        // largest m_l * m_{l-1}
   double *buf = nullptr;     // nbatch * per_design
   double *tmp = nullptr;     // 2 * nbatch * tmp_elems (E P and its transpose)
+  double *wbuf = nullptr;    // SPD-inverse scratch: nbatch * wstride
+  std::vector<long> woff;    // per recursion depth, offset in a design's scratch
+  long wstride = 0;
   int *info = nullptr;       // nbatch
   int *d_bcflag = nullptr;   // per level-ordered dof: 1 if a BC dof
   int *d_pos2dof = nullptr;  // level-ordered position -> original dof
@@ -354,32 +357,66 @@ __global__ __launch_bounds__(kCholThreads) void cholinv_base_kernel(double *cons
   if (t == 0 && bad) info[blockIdx.x] = 1;
 }
 
-// P = X^T X (lower triangle) of one n x n lower-triangular block (n <= 96)
-// per workgroup, in place: P_ij = sum_{k >= i} X_ki X_kj for i >= j; the
-// entries above the diagonal are left as they are.
-__global__ __launch_bounds__(kCholThreads) void lauum_base_kernel(double *const *ptrs, int n, int lda) {
+// Inverse of one n x n SPD block (n <= 96) per workgroup (design
+// blockIdx.x, blocks A + blockIdx.x sA), in place and in full (both
+// triangles): the Cholesky-inverse steps above give X = chol(A)^{-1}, then
+// A^{-1} = X^T X, P_ij = sum_{k >= i} X_ki X_kj.  info[blockIdx.x] = 1 for a
+// non-positive pivot.
+__global__ __launch_bounds__(kCholThreads) void spdinv_base_kernel(double *A, long sA, int n, int lda, int *info) {
   constexpr int ld = kCholBase + 1;
   constexpr int nw = kCholThreads / 64;
-  __shared__ double X[kCholBase * ld];
-  __shared__ double P[kCholBase * ld];
-  double *A = ptrs[blockIdx.x];
+  __shared__ double L[kCholBase * ld];
+  __shared__ double Y[kCholBase * ld];
+  __shared__ double piv[kCholBase];
+  A += blockIdx.x * sA;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   for (int e = t; e < n * n; e += kCholThreads) {
     const int i = e % n, j = e / n;
-    X[i * ld + j] = i >= j ? A[i + (long)j * lda] : 0.0;
+    L[i * ld + j] = A[i + (long)j * lda];
+    Y[i * ld + j] = i == j ? 1.0 : 0.0;
   }
   __syncthreads();
+  bool bad = false;
+  for (int j = 0; j < n; j++) {
+    double a = L[j * ld + j];
+    if (!(a > 0.0)) {
+      bad = true;
+      a = 1.0;
+    }
+    if (t == 0) piv[j] = a;
+    const double ra = 1.0 / a;
+    const int k0 = j + 1 + lane, k1 = k0 + 64;
+    const double l0 = k0 < n ? L[k0 * ld + j] : 0.0, l1 = k1 < n ? L[k1 * ld + j] : 0.0;
+    const double y0 = lane <= j ? Y[j * ld + lane] : 0.0;
+    const double y1 = lane + 64 <= j ? Y[j * ld + lane + 64] : 0.0;
+    for (int i = j + 1 + wave; i < n; i += nw) {
+      const double r = L[i * ld + j] * ra;
+      if (k0 <= i) L[i * ld + k0] -= r * l0;
+      if (k1 <= i) L[i * ld + k1] -= r * l1;
+      if (lane <= j) Y[i * ld + lane] -= r * y0;
+      if (lane + 64 <= j) Y[i * ld + lane + 64] -= r * y1;
+    }
+    __syncthreads();
+  }
+  // X = Y scaled by rows (lower), into Y
+  for (int e = t; e < n * n; e += kCholThreads) {
+    const int i = e / n, c = e % n;
+    Y[i * ld + c] = c <= i ? Y[i * ld + c] / sqrt(piv[i]) : 0.0;
+  }
+  __syncthreads();
+  // P = X^T X into L (lower), then written in full
   for (int i = wave; i < n; i += nw)
     for (int j = lane; j <= i; j += 64) {
       double s = 0.0;
-      for (int k = i; k < n; k++) s += X[k * ld + i] * X[k * ld + j];
-      P[i * ld + j] = s;
+      for (int k = i; k < n; k++) s += Y[k * ld + i] * Y[k * ld + j];
+      L[i * ld + j] = s;
     }
   __syncthreads();
   for (int e = t; e < n * n; e += kCholThreads) {
     const int i = e % n, j = e / n;
-    if (i >= j) A[i + (long)j * lda] = P[i * ld + j];
+    A[i + (long)j * lda] = i >= j ? L[i * ld + j] : L[j * ld + i];
   }
+  if (t == 0 && bad) info[blockIdx.x] = 1;
 }
 
 // B = A^T (A: rows x cols, column-major lda; B: cols x rows, ldb), 32 x 32
@@ -403,22 +440,6 @@ __device__ inline void tri_tile(int t, int &I, int &J) {
   while ((I + 1) * (I + 2) / 2 <= t) I++;
   while (I * (I + 1) / 2 > t) I--;
   J = t - I * (I + 1) / 2;
-}
-
-// copy the lower triangle of the n x n column-major A to the upper one,
-// 32 x 32 tiles of the lower triangle (blockIdx.x), batched (blockIdx.y)
-__global__ __launch_bounds__(256) void symmetrize_kernel(int n, double *__restrict__ A, int lda, long sAd) {
-  __shared__ double tile[32][33];
-  A += blockIdx.y * sAd;
-  int bi, bj;
-  tri_tile(blockIdx.x, bi, bj);
-  const int r0 = bi * 32, c0 = bj * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int k = ty; k < 32; k += 8)
-    if (r0 + tx < n && c0 + k < n) tile[k][tx] = A[(r0 + tx) + (long)(c0 + k) * lda];
-  __syncthreads();
-  // entry (c0 + tx, r0 + k) of the upper triangle = (r0 + k, c0 + tx)
-  for (int k = ty; k < 32; k += 8)
-    if (c0 + tx < n && r0 + k < n && (bi > bj || tx < k)) A[(c0 + tx) + (long)(r0 + k) * lda] = tile[tx][k];
 }
 
 // Y[:, 6j + a] = beta Y[:, 6j + a] + alpha sum_q sum_r X[:, 6 col_q + r] E_q(a, r)
@@ -493,40 +514,44 @@ __global__ void couple_bwd_kernel(int nnodes, const int *__restrict__ rowp, cons
   y[d * svd + g] = s;
 }
 
-// y = P x for a symmetric m x m P stored in full (column-major, lda),
-// reading the lower triangle only, in 64 x 64 tiles (I, J), I >= J
-// (blockIdx.x), designs blockIdx.y.  Pass 1: tile (I, J) writes the partial
-// sums P_IJ x_J of block row I into slot J and, for I > J, P_IJ^T x_I of
-// block row J into slot I: every (block row, slot) pair is written once.
-// Lanes take the tile's rows, each wavefront every fourth column; the
-// column sums of the transposed product are reduced over the lanes by a
-// halving exchange (16 values in 17 shuffles).
+// y = P x for a symmetric m x m P stored in full (column-major, lda; m
+// even, 16-byte aligned), reading the lower triangle only, in 128 x 128
+// tiles (I, J), I >= J (blockIdx.x), designs blockIdx.y.  Pass 1: tile
+// (I, J) writes the partial sums P_IJ x_J of block row I into slot J and,
+// for I > J, P_IJ^T x_I of block row J into slot I: every (block row, slot)
+// pair is written once.  Lanes take two rows each (one 16-byte load per
+// column), each wavefront every fourth column; the column sums of the
+// transposed product are reduced over the lanes by a halving exchange (32
+// values in 32 shuffles).
+constexpr int kSymvTile = 128;
 __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
                                                          long sAd, const double *__restrict__ x, long sxd,
                                                          double *__restrict__ part) {
-  __shared__ double red[4][64];
+  __shared__ double2 red[4][64];
   int I, J;
   tri_tile(blockIdx.x, I, J);
   const int d = blockIdx.y;
   A += d * sAd;
   x += d * sxd;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int row = I * 64 + lane;
-  const bool rv = row < m;
-  const double xi = rv ? x[row] : 0.0;
-  double acc = 0.0, tp[16];
+  const int row = I * kSymvTile + 2 * lane;
+  const bool rv = row < m;  // m is even: row + 1 < m as well
+  const double2 xi = rv ? *reinterpret_cast<const double2 *>(x + row) : make_double2(0.0, 0.0);
+  double a0 = 0.0, a1 = 0.0, tp[32];
 #pragma unroll
-  for (int p = 0; p < 16; p++) {
-    const int col = J * 64 + wave + 4 * p;
-    const double v = (rv && col < m) ? A[row + (long)col * lda] : 0.0;
+  for (int p = 0; p < 32; p++) {
+    const int col = J * kSymvTile + wave + 4 * p;
+    const double2 v =
+        (rv && col < m) ? *reinterpret_cast<const double2 *>(A + row + (long)col * lda) : make_double2(0.0, 0.0);
     const double xc = col < m ? x[col] : 0.0;
-    acc += v * xc;
-    tp[p] = v * xi;
+    a0 += v.x * xc;
+    a1 += v.y * xc;
+    tp[p] = v.x * xi.x + v.y * xi.y;
   }
-  red[wave][lane] = acc;
+  red[wave][lane] = make_double2(a0, a1);
   if (I > J) {
 #pragma unroll
-    for (int h = 8, b = 32; h >= 1; h >>= 1, b >>= 1) {
+    for (int h = 16, b = 32; h >= 1; h >>= 1, b >>= 1) {
       const bool hi = (lane & b) != 0;
 #pragma unroll
       for (int k = 0; k < h; k++) {
@@ -535,13 +560,16 @@ __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const do
         tp[k] = keep + __shfl_xor(send, b, 64);
       }
     }
-    tp[0] += __shfl_xor(tp[0], 2, 64);
     tp[0] += __shfl_xor(tp[0], 1, 64);
-    if ((lane & 3) == 0) part[(((long)d * nT + J) * nT + I) * 64 + wave + 4 * ((lane >> 2) & 15)] = tp[0];
+    if ((lane & 1) == 0)
+      part[(((long)d * nT + J) * nT + I) * kSymvTile + wave + 4 * ((lane >> 1) & 31)] = tp[0];
   }
   __syncthreads();
-  if (wave == 0)
-    part[(((long)d * nT + I) * nT + J) * 64 + lane] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  if (wave == 0) {
+    const double2 r0 = red[0][lane], r1 = red[1][lane], r2 = red[2][lane], r3 = red[3][lane];
+    *reinterpret_cast<double2 *>(part + (((long)d * nT + I) * nT + J) * kSymvTile + 2 * lane) =
+        make_double2((r0.x + r1.x) + (r2.x + r3.x), (r0.y + r1.y) + (r2.y + r3.y));
+  }
 }
 
 // Pass 2: y = beta c + alpha sum over the nT slots of the row's block
@@ -550,9 +578,9 @@ __global__ void symv_pass2_kernel(int m, int nT, const double *__restrict__ part
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
   const int d = blockIdx.y;
-  const double *p = part + ((long)d * nT + (row >> 6)) * nT * 64 + (row & 63);
+  const double *p = part + ((long)d * nT + row / kSymvTile) * nT * kSymvTile + row % kSymvTile;
   double s = 0.0;
-  for (int J = 0; J < nT; J++) s += p[J * 64];
+  for (int J = 0; J < nT; J++) s += p[J * kSymvTile];
   y[d * svd + row] = (beta == 0.0 ? 0.0 : beta * c[d * svd + row]) + alpha * s;
 }
 
@@ -749,6 +777,27 @@ int setup_plan(shellgpu_factor *F) {
       if (l > 0) F->tmp_elems = std::max(F->tmp_elems, (long)F->m[l] * F->m[l - 1]);
     }
     F->E_off = off;
+    // SPD-inverse scratch: per recursion depth the largest n2 x n1 block
+    std::vector<long> need;
+    struct Rec {
+      static void go(int n, int depth, std::vector<long> &need) {
+        if (n <= kCholBase) return;
+        int n1 = n / 2;
+        n1 -= n1 & 1;
+        if ((int)need.size() <= depth) need.resize(depth + 1, 0);
+        need[depth] = std::max(need[depth], (long)n1 * (n - n1));
+        go(n1, depth + 1, need);
+        go(n - n1, depth + 1, need);
+      }
+    };
+    for (int l = 0; l < nl; l++) Rec::go(F->m[l], 0, need);
+    F->woff.assign(need.size(), 0);
+    long w = 0;
+    for (size_t k = 0; k < need.size(); k++) {
+      F->woff[k] = w;
+      w += (need[k] + 1) & ~1L;
+    }
+    F->wstride = std::max(2L, w);
   }
   F->noff = noff;
 
@@ -949,22 +998,6 @@ void add_chol_group(shellgpu_factor *F, std::vector<double *> &h, const std::vec
   F->cgroups.push_back(grp);
 }
 
-// the sequential factorisation's Cholesky-inverse groups: one per level
-int build_thomas_pointers(shellgpu_factor *F) {
-  shellgpu_ctx *ctx = F->ctx;
-  const int nb = F->nbatch;
-  std::vector<double *> h;
-  F->cgroups.clear();
-  for (int l = 0; l < F->nlevels; l++) {
-    std::vector<double *> D(nb);
-    for (int d = 0; d < nb; d++) D[d] = F->buf + d * F->per_design + F->S_off[l];
-    add_chol_group(F, h, D, F->m[l], F->m[l]);
-  }
-  HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
-  HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
-  return 0;
-}
-
 // Pointer arrays of every cyclic-reduction round, entry q = k nb + d
 // (level entry k, design d): D_i, B_i, Z_i, D_{i-s}, D_{i+s}, B_{i+s} for
 // i = s (2k + 1); the last three exist for the first nR nb entries only.
@@ -1041,6 +1074,7 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
   hipError_t e = hipMalloc((void **)&F->buf, bytes);
   if (e == hipSuccess && !F->cr)
     e = hipMalloc((void **)&F->tmp, sizeof(double) * 2 * std::max(1L, F->tmp_elems) * nb);
+  if (e == hipSuccess && !F->cr) e = hipMalloc((void **)&F->wbuf, sizeof(double) * F->wstride * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->info, sizeof(int) * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vb, sizeof(double) * (size_t)F->nv * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vz, sizeof(double) * (size_t)F->nv * nb);
@@ -1048,8 +1082,8 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
   if (e == hipSuccess) {
     // GEMV partial sums of cyclic reduction; symmetric-product tile sums of
     // the sequential factorisation (64 x 64 tiles)
-    const long nT = (F->maxm + 63) / 64;
-    const long parts = F->cr ? (long)F->maxm * 32 * nb * F->nkmax : nT * nT * 64 * nb;
+    const long nT = (F->maxm + kSymvTile - 1) / kSymvTile;
+    const long parts = F->cr ? (long)F->maxm * 32 * nb * F->nkmax : nT * nT * kSymvTile * nb;
     e = hipMalloc((void **)&F->part, sizeof(double) * (size_t)parts);
   }
   if (e == hipSuccess) e = hipMalloc((void **)&F->cr_info, sizeof(int) * 2 * nb * (F->nlevels + 1));
@@ -1058,7 +1092,7 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
     return ferr(ctx, 2, std::string("factor allocation (") + std::to_string(bytes >> 20) +
                             " MiB): " + hipGetErrorString(e));
   }
-  if ((rc = F->cr ? build_cr_pointers(F) : build_thomas_pointers(F))) {
+  if (F->cr && (rc = build_cr_pointers(F))) {
     shellgpu_factor_destroy(F);
     return rc;
   }
@@ -1072,7 +1106,7 @@ void shellgpu_factor_destroy(shellgpu_factor *F) {
   void *bufs[] = {F->buf,      F->tmp,       F->info,      F->d_bcflag,  F->d_pos2dof, F->d_blk_base,
                   F->d_blk_ld, F->vb,        F->vz,        F->vx,        F->part,      F->d_mlev,
                   F->d_ptr,    F->cr_info,   F->d_cp_rowp, F->d_cp_col,  F->d_ct_rowp, F->d_ct_src,
-                  F->d_ct_blk};
+                  F->d_ct_blk, F->wbuf};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete F;
@@ -1156,7 +1190,7 @@ static int gemv_batch(shellgpu_factor *F, bool trans, int m, int n, const double
 // sequential factorisation, vectors of stride nv)
 static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const double *c, double beta, double *y) {
   shellgpu_ctx *ctx = F->ctx;
-  const int m = F->m[l], nT = (m + 63) / 64, nb = F->nbatch;
+  const int m = F->m[l], nT = (m + kSymvTile - 1) / kSymvTile, nb = F->nbatch;
   hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, nb), dim3(256), 0, ctx->stream, m, nT,
                      F->buf + F->S_off[l], m, F->per_design, x, F->nv, F->part);
   HIPCALL(ctx, hipGetLastError());
@@ -1202,38 +1236,50 @@ static int cholinv(shellgpu_factor *F, size_t g, int *info) {
   return 0;
 }
 
-// P = X^T X (lower) in place for the cnt lower-triangular blocks of group
-// g, on the Cholesky-inverse's node sequence: leaves in LDS; internal
-// nodes, between the two halves' recursions,
-//   P11 += X21^T X21,  X21 <- X22^T X21.
-static int lauum(shellgpu_factor *F, size_t g) {
+// In-place inverse of the SPD diagonal block [r0, r0 + n) of level l's
+// S_l (every design, strided), by recursive halving down to LDS leaves:
+//   A11 <- A11^{-1},  W = A21 A11,  A22 <- (A22 - W A21^T)^{-1},
+//   A21 <- -A22 W,  A11 <- A11 - W^T A21,  A12 <- A21^T
+// (block elimination of an SPD matrix: every step is a GEMM; W of each
+// recursion depth has its own scratch)
+static int spdinv(shellgpu_factor *F, int l, int r0, int n, int depth, int *info) {
   shellgpu_ctx *ctx = F->ctx;
-  rocblas_handle h = (rocblas_handle)ctx->blas;
-  const shellgpu_factor::CholGroup &grp = F->cgroups[g];
-  const int M = grp.lda, cnt = grp.cnt;
-  const double one = 1.0;
-  for (size_t k = 0; k < grp.nodes.size(); k++) {
-    const CholNode &c = grp.nodes[k];
-    double **P = F->d_ptr + grp.offs[k];
-    double **A11 = P, **A21 = P + cnt, **A22 = P + 2 * (long)cnt;
-    if (c.n2 == 0) {
-      hipLaunchKernelGGL(lauum_base_kernel, dim3(cnt), dim3(kCholThreads), 0, ctx->stream, A11, c.n1, M);
-      HIPCALL(ctx, hipGetLastError());
-    } else if (c.first) {
-      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_transpose, rocblas_operation_none, c.n1, c.n1, c.n2,
-                                          &one, (const double *const *)A21, M, (const double *const *)A21, M, &one,
-                                          A11, M, cnt));
-      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
-                                          rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A22, M,
-                                          (const double *const *)A21, M, A21, M, cnt));
-    }
+  const int nb = F->nbatch, m = F->m[l];
+  const long ps = F->per_design;
+  double *A11 = F->buf + F->S_off[l] + (long)r0 * (m + 1);
+  if (n <= kCholBase) {
+    hipLaunchKernelGGL(spdinv_base_kernel, dim3(nb), dim3(kCholThreads), 0, ctx->stream, A11, ps, n, m, info);
+    HIPCALL(ctx, hipGetLastError());
+    return 0;
   }
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  int n1 = n / 2;
+  n1 -= n1 & 1;
+  const int n2 = n - n1;
+  double *A21 = A11 + n1, *A22 = A11 + (long)n1 * (m + 1), *A12 = A11 + (long)n1 * m;
+  double *W = F->wbuf + F->woff[depth];
+  const long ws = F->wstride;
+  const double one = 1.0, mone = -1.0, zero = 0.0;
+  int rc = 0;
+  if ((rc = spdinv(F, l, r0, n1, depth + 1, info))) return rc;
+  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n2, n1, n1, &one, A21,
+                                              m, ps, A11, m, ps, &zero, W, n2, ws, nb));
+  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, n2, n2, n1,
+                                              &mone, W, n2, ws, A21, m, ps, &one, A22, m, ps, nb));
+  if ((rc = spdinv(F, l, r0 + n1, n2, depth + 1, info))) return rc;
+  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n2, n1, n2, &mone,
+                                              A22, m, ps, W, n2, ws, &zero, A21, m, ps, nb));
+  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, n1, n1, n2,
+                                              &mone, W, n2, ws, A21, m, ps, &one, A11, m, ps, nb));
+  hipLaunchKernelGGL(transpose_kernel, dim3((n2 + 31) / 32, (n1 + 31) / 32, nb), dim3(256), 0, ctx->stream, n2, n1,
+                     A21, m, ps, A12, m, ps);
+  HIPCALL(ctx, hipGetLastError());
   return 0;
 }
 
 // The sequential factorisation after the scatter: per level,
 //   T = P_{l-1} E_l^T,  U = T^T (= E_l P_{l-1}),  S_l = D_l - U E_l^T,
-//   X_l = chol(S_l)^{-1},  P_l = X_l^T X_l (then mirrored to the upper half)
+//   P_l = S_l^{-1} (in full)
 static int factor_schur(shellgpu_factor *F) {
   shellgpu_ctx *ctx = F->ctx;
   const int nb = F->nbatch;
@@ -1257,11 +1303,7 @@ static int factor_schur(shellgpu_factor *F) {
                          ts, S, ml, ps, rowp, F->d_cp_col, E, ps, -1.0, 1.0);
       HIPCALL(ctx, hipGetLastError());
     }
-    if ((rc = cholinv(F, l, F->cr_info + (long)l * nb))) return rc;
-    if ((rc = lauum(F, l))) return rc;
-    const int t32 = (ml + 31) / 32;
-    hipLaunchKernelGGL(symmetrize_kernel, dim3(t32 * (t32 + 1) / 2, nb), dim3(256), 0, ctx->stream, ml, S, ml, ps);
-    HIPCALL(ctx, hipGetLastError());
+    if ((rc = spdinv(F, l, 0, ml, 0, F->cr_info + (long)l * nb))) return rc;
   }
   return 0;
 }
