@@ -112,6 +112,7 @@ struct shellgpu_factor {
   int *d_blk_ld = nullptr;     // per BSR block: leading dimension of its target
   double *vb = nullptr, *vz = nullptr, *vx = nullptr;  // level-ordered work vectors
   double *part = nullptr;    // GEMV partial sums
+  int rhs_cap = 1;           // vectors per design that vb, vz, vx and part hold
   bool factored = false;
   long nv = 0;               // level-ordered vector length per design
   // block cyclic reduction over uniform (padded) levels of M dofs: per
@@ -304,23 +305,25 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double 
 }
 
 // Cholesky factor and its inverse of one n x n SPD block (n <= 96) per
-// workgroup, in LDS: A (column-major, lda) -> X = chol(A)^{-1}, lower, with
-// zeros above the diagonal of the block; info[entry] = 1 for a
-// non-positive pivot.  Right-looking with the scaling of the pivot column
+// workgroup, in LDS.  Right-looking with the scaling of the pivot column
 // deferred: step j subtracts L_ij L_kj / a_j (a_j the pivot) from the
 // trailing triangle and, for the inverse, Y_ic -= (L_ij / a_j) Y_jc (c <= j,
 // Y = I at the start), so one barrier per step suffices; at the end
-// X_ic = Y_ic / sqrt(a_i).  A wavefront takes every eighth row, its lanes
-// the columns.
+// X = chol^{-1} has rows X_ic = Y_ic / sqrt(a_i).  A wavefront takes every
+// sixteenth row, its lanes the columns; within a step a wavefront reads all
+// of its rows before it writes any (the step reads only column j and row j
+// besides the wavefront's own rows), so the LDS latency is paid once per
+// step instead of once per row.
 constexpr int kCholBase = 96;
-constexpr int kCholThreads = 512;
-__global__ __launch_bounds__(kCholThreads) void cholinv_base_kernel(double *const *ptrs, int n, int lda, int *info) {
-  constexpr int ld = kCholBase + 1;  // odd row stride: column walks spread over the banks
-  constexpr int nw = kCholThreads / 64;
-  __shared__ double L[kCholBase * ld];
-  __shared__ double Y[kCholBase * ld];
-  __shared__ double piv[kCholBase];
-  double *A = ptrs[blockIdx.x];
+constexpr int kCholThreads = 1024;
+constexpr int kCholLd = kCholBase + 1;  // odd row stride: column walks spread over the banks
+constexpr int kCholWaves = kCholThreads / 64;
+constexpr int kCholRows = (kCholBase + kCholWaves - 1) / kCholWaves;  // rows per wavefront and step
+
+// A (column-major, lda) -> L (scratch) and Y (rows of chol^{-1} before
+// scaling), piv (pivots); returns true for a non-positive pivot
+__device__ bool chol_inverse_lds(const double *A, int n, int lda, double *L, double *Y, double *piv) {
+  constexpr int ld = kCholLd;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   for (int e = t; e < n * n; e += kCholThreads) {
     const int i = e % n, j = e / n;
@@ -339,77 +342,84 @@ __global__ __launch_bounds__(kCholThreads) void cholinv_base_kernel(double *cons
     const double ra = 1.0 / a;
     const int k0 = j + 1 + lane, k1 = k0 + 64;  // trailing columns of this lane
     const double l0 = k0 < n ? L[k0 * ld + j] : 0.0, l1 = k1 < n ? L[k1 * ld + j] : 0.0;
-    const double y0 = lane <= j ? Y[j * ld + lane] : 0.0;
-    const double y1 = lane + 64 <= j ? Y[j * ld + lane + 64] : 0.0;
-    for (int i = j + 1 + wave; i < n; i += nw) {
-      const double r = L[i * ld + j] * ra;
-      if (k0 <= i) L[i * ld + k0] -= r * l0;
-      if (k1 <= i) L[i * ld + k1] -= r * l1;
-      if (lane <= j) Y[i * ld + lane] -= r * y0;
-      if (lane + 64 <= j) Y[i * ld + lane + 64] -= r * y1;
+    const bool c0 = lane <= j, c1 = lane + 64 <= j;
+    const double y0 = c0 ? Y[j * ld + lane] : 0.0, y1 = c1 ? Y[j * ld + lane + 64] : 0.0;
+    double r[kCholRows], v0[kCholRows], v1[kCholRows], w0[kCholRows], w1[kCholRows];
+#pragma unroll
+    for (int s = 0; s < kCholRows; s++) {
+      const int i = j + 1 + wave + kCholWaves * s;
+      r[s] = v0[s] = v1[s] = w0[s] = w1[s] = 0.0;
+      if (i < n) {
+        r[s] = L[i * ld + j] * ra;
+        if (k0 <= i) v0[s] = L[i * ld + k0];
+        if (k1 <= i) v1[s] = L[i * ld + k1];
+        if (c0) w0[s] = Y[i * ld + lane];
+        if (c1) w1[s] = Y[i * ld + lane + 64];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kCholRows; s++) {
+      const int i = j + 1 + wave + kCholWaves * s;
+      if (i < n) {
+        if (k0 <= i) L[i * ld + k0] = v0[s] - r[s] * l0;
+        if (k1 <= i) L[i * ld + k1] = v1[s] - r[s] * l1;
+        if (c0) Y[i * ld + lane] = w0[s] - r[s] * y0;
+        if (c1) Y[i * ld + lane + 64] = w1[s] - r[s] * y1;
+      }
     }
     __syncthreads();
   }
-  for (int e = t; e < n * n; e += kCholThreads) {
+  return bad;
+}
+
+// X = chol(A)^{-1}, lower, with zeros above the diagonal of the block, in
+// place (A: column-major, lda; blocks ptrs[blockIdx.x]); info[entry] = 1
+// for a non-positive pivot
+__global__ __launch_bounds__(kCholThreads) void cholinv_base_kernel(double *const *ptrs, int n, int lda, int *info) {
+  __shared__ double L[kCholBase * kCholLd];
+  __shared__ double Y[kCholBase * kCholLd];
+  __shared__ double piv[kCholBase];
+  double *A = ptrs[blockIdx.x];
+  const bool bad = chol_inverse_lds(A, n, lda, L, Y, piv);
+  for (int e = threadIdx.x; e < n * n; e += kCholThreads) {
     const int i = e % n, c = e / n;
-    A[i + (long)c * lda] = i >= c ? Y[i * ld + c] / sqrt(piv[i]) : 0.0;
+    A[i + (long)c * lda] = i >= c ? Y[i * kCholLd + c] / sqrt(piv[i]) : 0.0;
   }
-  if (t == 0 && bad) info[blockIdx.x] = 1;
+  if (threadIdx.x == 0 && bad) info[blockIdx.x] = 1;
 }
 
 // Inverse of one n x n SPD block (n <= 96) per workgroup (design
 // blockIdx.x, blocks A + blockIdx.x sA), in place and in full (both
-// triangles): the Cholesky-inverse steps above give X = chol(A)^{-1}, then
-// A^{-1} = X^T X, P_ij = sum_{k >= i} X_ki X_kj.  info[blockIdx.x] = 1 for a
-// non-positive pivot.
+// triangles): X = chol(A)^{-1} as above, then A^{-1} = X^T X,
+// P_ij = sum_{k >= i} X_ki X_kj.  info[blockIdx.x] = 1 for a non-positive
+// pivot.
 __global__ __launch_bounds__(kCholThreads) void spdinv_base_kernel(double *A, long sA, int n, int lda, int *info) {
-  constexpr int ld = kCholBase + 1;
-  constexpr int nw = kCholThreads / 64;
+  constexpr int ld = kCholLd;
   __shared__ double L[kCholBase * ld];
   __shared__ double Y[kCholBase * ld];
   __shared__ double piv[kCholBase];
   A += blockIdx.x * sA;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  for (int e = t; e < n * n; e += kCholThreads) {
-    const int i = e % n, j = e / n;
-    L[i * ld + j] = A[i + (long)j * lda];
-    Y[i * ld + j] = i == j ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  bool bad = false;
-  for (int j = 0; j < n; j++) {
-    double a = L[j * ld + j];
-    if (!(a > 0.0)) {
-      bad = true;
-      a = 1.0;
-    }
-    if (t == 0) piv[j] = a;
-    const double ra = 1.0 / a;
-    const int k0 = j + 1 + lane, k1 = k0 + 64;
-    const double l0 = k0 < n ? L[k0 * ld + j] : 0.0, l1 = k1 < n ? L[k1 * ld + j] : 0.0;
-    const double y0 = lane <= j ? Y[j * ld + lane] : 0.0;
-    const double y1 = lane + 64 <= j ? Y[j * ld + lane + 64] : 0.0;
-    for (int i = j + 1 + wave; i < n; i += nw) {
-      const double r = L[i * ld + j] * ra;
-      if (k0 <= i) L[i * ld + k0] -= r * l0;
-      if (k1 <= i) L[i * ld + k1] -= r * l1;
-      if (lane <= j) Y[i * ld + lane] -= r * y0;
-      if (lane + 64 <= j) Y[i * ld + lane + 64] -= r * y1;
-    }
-    __syncthreads();
-  }
+  const bool bad = chol_inverse_lds(A, n, lda, L, Y, piv);
   // X = Y scaled by rows (lower), into Y
   for (int e = t; e < n * n; e += kCholThreads) {
     const int i = e / n, c = e % n;
     Y[i * ld + c] = c <= i ? Y[i * ld + c] / sqrt(piv[i]) : 0.0;
   }
   __syncthreads();
-  // P = X^T X into L (lower), then written in full
-  for (int i = wave; i < n; i += nw)
+  // P = X^T X into L (lower; four partial sums per entry), then written in full
+  for (int i = wave; i < n; i += kCholWaves)
     for (int j = lane; j <= i; j += 64) {
-      double s = 0.0;
-      for (int k = i; k < n; k++) s += Y[k * ld + i] * Y[k * ld + j];
-      L[i * ld + j] = s;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int k = i;
+      for (; k + 3 < n; k += 4) {
+        s0 += Y[k * ld + i] * Y[k * ld + j];
+        s1 += Y[(k + 1) * ld + i] * Y[(k + 1) * ld + j];
+        s2 += Y[(k + 2) * ld + i] * Y[(k + 2) * ld + j];
+        s3 += Y[(k + 3) * ld + i] * Y[(k + 3) * ld + j];
+      }
+      for (; k < n; k++) s0 += Y[k * ld + i] * Y[k * ld + j];
+      L[i * ld + j] = (s0 + s1) + (s2 + s3);
     }
   __syncthreads();
   for (int e = t; e < n * n; e += kCholThreads) {
@@ -443,75 +453,82 @@ __device__ inline void tri_tile(int t, int &I, int &J) {
 }
 
 // Y[:, 6j + a] = beta Y[:, 6j + a] + alpha sum_q sum_r X[:, 6 col_q + r] E_q(a, r)
-// over the coupling blocks q of node j of one level (blockIdx.y), rows
-// 0..rows-1 of the column-major X and Y; E_q(a, r) = E[36 q + 6 r + a].
-// Per design (blockIdx.z) strides of X, Y and E.
-__global__ __launch_bounds__(256) void couple_right_kernel(int rows, const double *__restrict__ X, int ldx, long sXd,
-                                                           double *__restrict__ Y, int ldy, long sYd,
-                                                           const int *__restrict__ rowp, const int *__restrict__ colx,
-                                                           const double *__restrict__ E, long sEd, double alpha,
-                                                           double beta) {
+// over the coupling blocks q of node j of one level, rows 0..rows-1 of the
+// column-major X and Y; E_q(a, r) = E[36 q + 6 r + a].  A workgroup takes
+// 256 rows of kCoupleNodes consecutive nodes (blockIdx.y), which share most
+// of their neighbour columns of X; per design (blockIdx.z) strides of X, Y
+// and E.
+constexpr int kCoupleNodes = 4;
+__global__ __launch_bounds__(256) void couple_right_kernel(int rows, int nnodes, const double *__restrict__ X,
+                                                           int ldx, long sXd, double *__restrict__ Y, int ldy,
+                                                           long sYd, const int *__restrict__ rowp,
+                                                           const int *__restrict__ colx, const double *__restrict__ E,
+                                                           long sEd, double alpha, double beta) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= rows) return;
-  const int j = blockIdx.y, d = blockIdx.z;
+  const int d = blockIdx.z;
   X += d * sXd + i;
   Y += d * sYd + i;
   E += d * sEd;
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int q = rowp[j]; q < rowp[j + 1]; q++) {
-    const double *xk = X + 6L * colx[q] * ldx;
-    const double *e = E + 36L * q;
-    double xv[6];
+  const int j1 = min(nnodes, (int)(blockIdx.y + 1) * kCoupleNodes);
+  for (int j = blockIdx.y * kCoupleNodes; j < j1; j++) {
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = rowp[j]; q < rowp[j + 1]; q++) {
+      const double *xk = X + 6L * colx[q] * ldx;
+      const double *e = E + 36L * q;
+      double xv[6];
 #pragma unroll
-    for (int r = 0; r < 6; r++) xv[r] = xk[(long)r * ldx];
+      for (int r = 0; r < 6; r++) xv[r] = xk[(long)r * ldx];
 #pragma unroll
-    for (int r = 0; r < 6; r++)
+      for (int r = 0; r < 6; r++)
 #pragma unroll
-      for (int a = 0; a < 6; a++) acc[a] += xv[r] * e[6 * r + a];
+        for (int a = 0; a < 6; a++) acc[a] += xv[r] * e[6 * r + a];
+    }
+    double *y = Y + 6L * j * ldy;
+#pragma unroll
+    for (int a = 0; a < 6; a++) y[(long)a * ldy] = (beta == 0.0 ? 0.0 : beta * y[(long)a * ldy]) + alpha * acc[a];
   }
-  double *y = Y + 6L * j * ldy;
-#pragma unroll
-  for (int a = 0; a < 6; a++) y[(long)a * ldy] = (beta == 0.0 ? 0.0 : beta * y[(long)a * ldy]) + alpha * acc[a];
 }
 
 // z[6i + a] -= sum_q sum_r E_q(a, r) t[6 col_q + r] for the nodes i of one
-// level (t: the previous level's segment), per design (blockIdx.y)
+// level (t: the previous level's segment); vector entry e = blockIdx.y (of
+// stride svd) uses the couplings of design e % nbf
 __global__ void couple_fwd_kernel(int nnodes, const int *__restrict__ rowp, const int *__restrict__ colx,
                                   const double *__restrict__ E, long sEd, const double *__restrict__ t,
-                                  double *__restrict__ z, long svd) {
+                                  double *__restrict__ z, long svd, int nbf) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= 6 * nnodes) return;
-  const int i = g / 6, a = g - 6 * i, d = blockIdx.y;
-  E += d * sEd + a;
-  t += d * svd;
+  const int i = g / 6, a = g - 6 * i, e = blockIdx.y;
+  E += (e % nbf) * sEd + a;
+  t += e * svd;
   double s = 0.0;
   for (int q = rowp[i]; q < rowp[i + 1]; q++) {
     const double *tk = t + 6 * colx[q];
-    const double *e = E + 36L * q;
+    const double *ep = E + 36L * q;
 #pragma unroll
-    for (int r = 0; r < 6; r++) s += e[6 * r] * tk[r];
+    for (int r = 0; r < 6; r++) s += ep[6 * r] * tk[r];
   }
-  z[d * svd + g] -= s;
+  z[e * svd + g] -= s;
 }
 
 // y[6v + r] = sum_e sum_a E_{blk_e}(a, r) x[6 src_e + a] for the nodes v of
-// one level (x: the next level's segment), per design (blockIdx.y)
+// one level (x: the next level's segment); vector entries as above
 __global__ void couple_bwd_kernel(int nnodes, const int *__restrict__ rowp, const int *__restrict__ src,
                                   const int *__restrict__ blk, const double *__restrict__ E, long sEd,
-                                  const double *__restrict__ x, double *__restrict__ y, long svd) {
+                                  const double *__restrict__ x, double *__restrict__ y, long svd, int nbf) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= 6 * nnodes) return;
-  const int v = g / 6, r = g - 6 * v, d = blockIdx.y;
-  E += d * sEd + 6 * r;
-  x += d * svd;
+  const int v = g / 6, r = g - 6 * v, e = blockIdx.y;
+  E += (e % nbf) * sEd + 6 * r;
+  x += e * svd;
   double s = 0.0;
   for (int k = rowp[v]; k < rowp[v + 1]; k++) {
     const double *xu = x + 6 * src[k];
-    const double *e = E + 36L * blk[k];
+    const double *ep = E + 36L * blk[k];
 #pragma unroll
-    for (int a = 0; a < 6; a++) s += e[a] * xu[a];
+    for (int a = 0; a < 6; a++) s += ep[a] * xu[a];
   }
-  y[d * svd + g] = s;
+  y[e * svd + g] = s;
 }
 
 // y = P x for a symmetric m x m P stored in full (column-major, lda; m
@@ -569,6 +586,48 @@ __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const do
     const double2 r0 = red[0][lane], r1 = red[1][lane], r2 = red[2][lane], r3 = red[3][lane];
     *reinterpret_cast<double2 *>(part + (((long)d * nT + I) * nT + J) * kSymvTile + 2 * lane) =
         make_double2((r0.x + r1.x) + (r2.x + r3.x), (r0.y + r1.y) + (r2.y + r3.y));
+  }
+}
+
+// Y_r = P X_r for NR vectors per design (vector entries e = r nbf + d of
+// stride sxe), P read in full once for all of them: 128 x 128 tiles (I, J)
+// over the whole matrix (blockIdx.x = I nT + J), designs blockIdx.y; the
+// partial sums P_IJ x_J of block row I go to slot J, as pass 1 of the
+// symmetric product lays them out.
+template <int NR>
+__global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
+                                                               long sAd, const double *__restrict__ x, long sxe,
+                                                               int nbf, double *__restrict__ part) {
+  __shared__ double2 red[4][NR][64];
+  const int I = blockIdx.x / nT, J = blockIdx.x - I * nT, d = blockIdx.y;
+  A += d * sAd;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int row = I * kSymvTile + 2 * lane;
+  const bool rv = row < m;
+  double2 acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = make_double2(0.0, 0.0);
+#pragma unroll 4
+  for (int p = 0; p < kSymvTile / 4; p++) {
+    const int col = J * kSymvTile + wave + 4 * p;
+    if (col >= m) break;
+    const double2 v = rv ? *reinterpret_cast<const double2 *>(A + row + (long)col * lda) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const double xc = x[((long)r * nbf + d) * sxe + col];
+      acc[r].x += v.x * xc;
+      acc[r].y += v.y * xc;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++) red[wave][r][lane] = acc[r];
+  __syncthreads();
+  for (int q = threadIdx.x; q < 64 * NR; q += 256) {
+    const int ln = q & 63, r = q >> 6;
+    const double2 a0 = red[0][r][ln], a1 = red[1][r][ln], a2 = red[2][r][ln], a3 = red[3][r][ln];
+    const long e = (long)r * nbf + d;
+    *reinterpret_cast<double2 *>(part + ((e * nT + I) * nT + J) * kSymvTile + 2 * ln) =
+        make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
   }
 }
 
@@ -1123,7 +1182,7 @@ int shellgpu_factor_info(shellgpu_factor *F, int *nlevels, int *max_level_dofs, 
 static int factor_cr(shellgpu_factor *F);
 static int apply_cr(shellgpu_factor *F);
 static int factor_schur(shellgpu_factor *F);
-static int apply_schur(shellgpu_factor *F);
+static int apply_schur(shellgpu_factor *F, int nrhs);
 
 int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   shellgpu_ctx *ctx = F->ctx;
@@ -1186,16 +1245,41 @@ static int gemv_batch(shellgpu_factor *F, bool trans, int m, int n, const double
   return 0;
 }
 
-// y = beta c + alpha P x for every design (symmetric P of level l of the
-// sequential factorisation, vectors of stride nv)
-static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const double *c, double beta, double *y) {
+// y = beta c + alpha P x for nrhs vectors of every design (symmetric P of
+// level l of the sequential factorisation; vector entries e = r nb + d of
+// stride nv): one vector per design reads half of P, several read it in
+// full once for all
+constexpr int kMaxRhs = 8;
+static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const double *c, double beta, double *y,
+                int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
   const int m = F->m[l], nT = (m + kSymvTile - 1) / kSymvTile, nb = F->nbatch;
-  hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, nb), dim3(256), 0, ctx->stream, m, nT,
-                     F->buf + F->S_off[l], m, F->per_design, x, F->nv, F->part);
+  const double *P = F->buf + F->S_off[l];
+  const long ps = F->per_design, nv = F->nv;
+  switch (nrhs) {
+#define SYMV_MULTI(R)                                                                                             \
+  case R:                                                                                                         \
+    hipLaunchKernelGGL(symv_multi_pass1_kernel<R>, dim3(nT * nT, nb), dim3(256), 0, ctx->stream, m, nT, P, m, ps, \
+                       x, nv, nb, F->part);                                                                       \
+    break;
+    case 1:
+      hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, nb), dim3(256), 0, ctx->stream, m, nT, P, m, ps,
+                         x, nv, F->part);
+      break;
+      SYMV_MULTI(2)
+      SYMV_MULTI(3)
+      SYMV_MULTI(4)
+      SYMV_MULTI(5)
+      SYMV_MULTI(6)
+      SYMV_MULTI(7)
+      SYMV_MULTI(8)
+#undef SYMV_MULTI
+    default:
+      return ferr(ctx, 1, "too many right-hand sides for one level product");
+  }
   HIPCALL(ctx, hipGetLastError());
-  hipLaunchKernelGGL(symv_pass2_kernel, dim3((m + 255) / 256, nb), dim3(256), 0, ctx->stream, m, nT, F->part, alpha,
-                     c, beta, y, F->nv);
+  hipLaunchKernelGGL(symv_pass2_kernel, dim3((m + 255) / 256, nrhs * nb), dim3(256), 0, ctx->stream, m, nT, F->part,
+                     alpha, c, beta, y, nv);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
@@ -1293,14 +1377,15 @@ static int factor_schur(shellgpu_factor *F) {
     if (l > 0) {
       const int mp = F->m[l - 1];
       const int *rowp = F->d_cp_rowp + F->noff[l];
-      hipLaunchKernelGGL(couple_right_kernel, dim3((mp + 255) / 256, nn, nb), dim3(256), 0, ctx->stream, mp,
+      const int ng = (nn + kCoupleNodes - 1) / kCoupleNodes;
+      hipLaunchKernelGGL(couple_right_kernel, dim3((mp + 255) / 256, ng, nb), dim3(256), 0, ctx->stream, mp, nn,
                          F->buf + F->S_off[l - 1], mp, ps, T, mp, ts, rowp, F->d_cp_col, E, ps, 1.0, 0.0);
       HIPCALL(ctx, hipGetLastError());
       hipLaunchKernelGGL(transpose_kernel, dim3((mp + 31) / 32, (ml + 31) / 32, nb), dim3(256), 0, ctx->stream, mp,
                          ml, T, mp, ts, U, ml, ts);
       HIPCALL(ctx, hipGetLastError());
-      hipLaunchKernelGGL(couple_right_kernel, dim3((ml + 255) / 256, nn, nb), dim3(256), 0, ctx->stream, ml, U, ml,
-                         ts, S, ml, ps, rowp, F->d_cp_col, E, ps, -1.0, 1.0);
+      hipLaunchKernelGGL(couple_right_kernel, dim3((ml + 255) / 256, ng, nb), dim3(256), 0, ctx->stream, ml, nn, U,
+                         ml, ts, S, ml, ps, rowp, F->d_cp_col, E, ps, -1.0, 1.0);
       HIPCALL(ctx, hipGetLastError());
     }
     if ((rc = spdinv(F, l, 0, ml, 0, F->cr_info + (long)l * nb))) return rc;
@@ -1308,10 +1393,11 @@ static int factor_schur(shellgpu_factor *F) {
   return 0;
 }
 
-// x = A^{-1} b by the sequential factorisation (level-ordered vb -> vx)
-static int apply_schur(shellgpu_factor *F) {
+// x = A^{-1} b by the sequential factorisation for nrhs vectors of every
+// design (level-ordered vb -> vx, entries e = r nb + d)
+static int apply_schur(shellgpu_factor *F, int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
-  const int nb = F->nbatch, nl = F->nlevels;
+  const int nb = F->nbatch, nl = F->nlevels, ne = nrhs * nb;
   const long ps = F->per_design, nv = F->nv;
   const double *E = F->buf + F->E_off;
   int rc = 0;
@@ -1320,20 +1406,20 @@ static int apply_schur(shellgpu_factor *F) {
     double *zl = F->vb + F->voff[l];
     if (l > 0) {
       const int nn = F->m[l] / 6;
-      hipLaunchKernelGGL(couple_fwd_kernel, dim3((6 * nn + 255) / 256, nb), dim3(256), 0, ctx->stream, nn,
-                         F->d_cp_rowp + F->noff[l], F->d_cp_col, E, ps, F->vz + F->voff[l - 1], zl, nv);
+      hipLaunchKernelGGL(couple_fwd_kernel, dim3((6 * nn + 255) / 256, ne), dim3(256), 0, ctx->stream, nn,
+                         F->d_cp_rowp + F->noff[l], F->d_cp_col, E, ps, F->vz + F->voff[l - 1], zl, nv, nb);
       HIPCALL(ctx, hipGetLastError());
     }
-    if ((rc = symv(F, l, zl, 1.0, nullptr, 0.0, (l == nl - 1 ? F->vx : F->vz) + F->voff[l]))) return rc;
+    if ((rc = symv(F, l, zl, 1.0, nullptr, 0.0, (l == nl - 1 ? F->vx : F->vz) + F->voff[l], nrhs))) return rc;
   }
   // backward: x_l = t_l - P_l E_{l+1}^T x_{l+1} (E^T x into vb)
   for (int l = nl - 2; l >= 0; l--) {
     const int nn = F->m[l] / 6;
     double *yl = F->vb + F->voff[l];
-    hipLaunchKernelGGL(couple_bwd_kernel, dim3((6 * nn + 255) / 256, nb), dim3(256), 0, ctx->stream, nn,
-                       F->d_ct_rowp + F->noff[l], F->d_ct_src, F->d_ct_blk, E, ps, F->vx + F->voff[l + 1], yl, nv);
+    hipLaunchKernelGGL(couple_bwd_kernel, dim3((6 * nn + 255) / 256, ne), dim3(256), 0, ctx->stream, nn,
+                       F->d_ct_rowp + F->noff[l], F->d_ct_src, F->d_ct_blk, E, ps, F->vx + F->voff[l + 1], yl, nv, nb);
     HIPCALL(ctx, hipGetLastError());
-    if ((rc = symv(F, l, yl, -1.0, F->vz + F->voff[l], 1.0, F->vx + F->voff[l]))) return rc;
+    if ((rc = symv(F, l, yl, -1.0, F->vz + F->voff[l], 1.0, F->vx + F->voff[l], nrhs))) return rc;
   }
   return 0;
 }
@@ -1459,12 +1545,62 @@ int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
   hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv, n,
                      F->d_pos2dof, b, F->vb, 1);
   HIPCALL(ctx, hipGetLastError());
-  if ((rc = F->cr ? apply_cr(F) : apply_schur(F))) return rc;
+  if ((rc = F->cr ? apply_cr(F) : apply_schur(F, 1))) return rc;
   hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv, n,
                      F->d_pos2dof, F->vx, x, 0);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
+
+}  // extern "C"
+
+namespace {
+
+// x = A^{-1} b for nrhs vectors of every design (entry e = r nbatch + d at
+// offset e n of b and x).  The sequential factorisation solves them
+// together, reading each level inverse once per solve; cyclic reduction
+// solves them one after the other.
+int factor_apply_multi(shellgpu_factor *F, const double *b, double *x, int nrhs) {
+  shellgpu_ctx *ctx = F->ctx;
+  const int nb = F->nbatch;
+  const long nv = F->nv, n = F->n;
+  if (nrhs <= 1 || F->cr) {
+    for (int r = 0; r < nrhs; r++) {
+      const int rc = shellgpu_factor_apply(F, b + (size_t)r * nb * n, x + (size_t)r * nb * n);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  if (!F->factored) return ferr(ctx, 1, "factor not computed");
+  if (nrhs > kMaxRhs) return ferr(ctx, 1, "too many right-hand sides for one solve");
+  if (nrhs > F->rhs_cap) {
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    double **bufs[] = {&F->vb, &F->vz, &F->vx};
+    for (double **p : bufs) {
+      HIPCALL(ctx, hipFree(*p));
+      *p = nullptr;
+      HIPCALL(ctx, hipMalloc((void **)p, sizeof(double) * (size_t)nv * nb * nrhs));
+    }
+    const long nT = (F->maxm + kSymvTile - 1) / kSymvTile;
+    HIPCALL(ctx, hipFree(F->part));
+    F->part = nullptr;
+    HIPCALL(ctx, hipMalloc((void **)&F->part, sizeof(double) * (size_t)nT * nT * kSymvTile * nb * nrhs));
+    F->rhs_cap = nrhs;
+  }
+  int rc = 0;
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nrhs * nb), dim3(256), 0, ctx->stream, nv, n,
+                     F->d_pos2dof, b, F->vb, 1);
+  HIPCALL(ctx, hipGetLastError());
+  if ((rc = apply_schur(F, nrhs))) return rc;
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nrhs * nb), dim3(256), 0, ctx->stream, nv, n,
+                     F->d_pos2dof, F->vx, x, 0);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
 
 }  // extern "C"
 
@@ -1668,8 +1804,13 @@ __global__ void shifted_kernel(long n, const double *K, const double *G, const d
 }
 
 struct LanczosOut {
+  int bs = 1;     // block size
+  int steps = 0;  // basis vectors
+  // block size 1: the tridiagonal projection per design
   std::vector<std::vector<double>> alpha, beta;
-  int steps = 0;
+  // block size > 1: per design the projected matrix (steps x steps,
+  // column-major) and the last residual block (bs x bs, column-major)
+  std::vector<std::vector<double>> T, R;
 };
 
 // Lanczos in the K inner product, designs in lock-step, full CGS2
@@ -1773,6 +1914,311 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
   }
   lo.steps = std::min(k, m);
   return 0;
+}
+
+// Ritz pairs of every design's projection after a Lanczos run: w[d] the
+// Ritz values, z[d] the coefficient vectors (column i of an s x s
+// column-major matrix) and res[d][i] the residual norm of pair i (in the
+// inner product of the basis).  Block size 1: implicit QL of the
+// tridiagonal matrix on the host; larger blocks: the dense projection by a
+// batched symmetric eigensolver on the device.
+int proj_eigs(shellgpu_ctx *ctx, const LanczosOut &lo, int s, std::vector<std::vector<double>> &w,
+              std::vector<std::vector<double>> &z, std::vector<std::vector<double>> &res) {
+  const int nb = ctx->nbatch, bs = lo.bs;
+  w.assign(nb, {});
+  z.assign(nb, {});
+  res.assign(nb, {});
+  if (bs == 1) {
+    for (int d = 0; d < nb; d++) {
+      std::vector<double> e(lo.beta[d].begin(), lo.beta[d].begin() + (s - 1));
+      std::vector<double> a(lo.alpha[d].begin(), lo.alpha[d].begin() + s);
+      if (tridiag_ql(s, a, e, w[d], z[d]) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+      res[d].resize(s);
+      for (int i = 0; i < s; i++) res[d][i] = std::fabs(lo.beta[d][s - 1] * z[d][(size_t)i * s + s - 1]);
+    }
+    return 0;
+  }
+  const size_t s2 = (size_t)s * s;
+  std::vector<double> hT(s2 * nb);
+  for (int d = 0; d < nb; d++)
+    for (int j = 0; j < s; j++)
+      for (int i = 0; i < s; i++) hT[d * s2 + (size_t)j * s + i] = lo.T[d][(size_t)j * lo.steps + i];
+  double *dT = nullptr, *dD = nullptr, *dE = nullptr;
+  int *dinfo = nullptr;
+  auto release = [&]() {
+    void *b[] = {dT, dD, dE, dinfo};
+    for (void *p : b)
+      if (p) (void)hipFree(p);
+  };
+  if (hipMalloc((void **)&dT, sizeof(double) * s2 * nb) != hipSuccess ||
+      hipMalloc((void **)&dD, sizeof(double) * (size_t)s * nb) != hipSuccess ||
+      hipMalloc((void **)&dE, sizeof(double) * (size_t)s * nb) != hipSuccess ||
+      hipMalloc((void **)&dinfo, sizeof(int) * nb) != hipSuccess) {
+    release();
+    return ferr(ctx, 2, "projected eigenproblem allocation failed");
+  }
+  std::vector<double> hD((size_t)s * nb);
+  std::vector<int> info(nb);
+  rocblas_status st = rocblas_status_success;
+  hipError_t he = hipMemcpyAsync(dT, hT.data(), sizeof(double) * hT.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (he == hipSuccess)
+    st = rocsolver_dsyevd_strided_batched((rocblas_handle)ctx->blas, rocblas_evect_original, rocblas_fill_lower, s, dT,
+                                          s, (rocblas_stride)s2, dD, s, dE, s, dinfo, nb);
+  if (he == hipSuccess && st == rocblas_status_success)
+    he = hipMemcpyAsync(hT.data(), dT, sizeof(double) * hT.size(), hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess && st == rocblas_status_success)
+    he = hipMemcpyAsync(hD.data(), dD, sizeof(double) * hD.size(), hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess && st == rocblas_status_success)
+    he = hipMemcpyAsync(info.data(), dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+  release();
+  if (he != hipSuccess) return ferr(ctx, 2, std::string("projected eigenproblem: ") + hipGetErrorString(he));
+  if (st != rocblas_status_success) return ferr(ctx, 4, "projected eigenproblem: rocsolver_dsyevd failed");
+  for (int d = 0; d < nb; d++) {
+    if (info[d] != 0) return ferr(ctx, 6, "projected eigenproblem did not converge");
+    w[d].assign(hD.begin() + (size_t)d * s, hD.begin() + (size_t)(d + 1) * s);
+    z[d].assign(hT.begin() + d * s2, hT.begin() + (d + 1) * s2);
+    res[d].resize(s);
+    for (int i = 0; i < s; i++) {
+      const double *zl = z[d].data() + (size_t)i * s + (s - bs);  // last block of coefficient vector i
+      double r2 = 0.0;
+      for (int a = 0; a < bs; a++) {
+        double v = 0.0;
+        for (int b = 0; b < bs; b++) v += lo.R[d][(size_t)b * bs + a] * zl[b];
+        r2 += v * v;
+      }
+      res[d][i] = std::sqrt(r2);
+    }
+  }
+  return 0;
+}
+
+// Block Lanczos with bs vectors per block (same operators, modes, inner
+// product and stopping test as lanczos above): each step applies the
+// operator to a whole block (one multi-vector solve: every level inverse of
+// the sequential factorisation is read once for the block), removes the
+// basis by block classical Gram-Schmidt applied twice and K-orthonormalises
+// the block by Cholesky QR applied twice.  The projection is block
+// tridiagonal: T_jj = A_j (the block's own Gram-Schmidt coefficients),
+// T_{j+1,j} = B_{j+1} (the Cholesky factors).  m bounds the basis size.
+int lanczos_block(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
+                  const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, int bs,
+                  LanczosOut &lo) {
+  const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
+  const size_t V = (size_t)nb * n;  // one basis vector of every design
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  const double one = 1.0, mone = -1.0, zero = 0.0;
+  const int jmax = (m + 1) / bs - 1;  // block steps (basis blocks 0..jmax)
+  if (jmax < 1) return ferr(ctx, 1, "Lanczos basis too small for the block size");
+  const int smax = jmax * bs;
+  lo.bs = bs;
+  lo.T.assign(nb, std::vector<double>((size_t)smax * smax, 0.0));
+  lo.R.assign(nb, std::vector<double>((size_t)bs * bs, 0.0));
+  lo.alpha.clear();
+  lo.beta.clear();
+  double *dH = nullptr, *dM = nullptr;
+  if (hipMalloc((void **)&dH, sizeof(double) * (size_t)nb * (smax + bs) * bs) != hipSuccess ||
+      hipMalloc((void **)&dM, sizeof(double) * (size_t)nb * bs * bs) != hipSuccess) {
+    if (dH) (void)hipFree(dH);
+    return ferr(ctx, 2, "block Lanczos allocation failed");
+  }
+  struct Free {
+    double *a, *b;
+    ~Free() {
+      (void)hipFree(a);
+      (void)hipFree(b);
+    }
+  } fr{dH, dM};
+  int rc = 0;
+  // K-orthonormalise the block W (bs vectors, K W in KW) by Cholesky QR
+  // twice; Rtot (per design, bs x bs column-major upper) = R2 R1
+  std::vector<double> hM((size_t)nb * bs * bs), Rinv((size_t)nb * bs * bs);
+  auto cholqr2 = [&](double *W, double *KW, std::vector<double> &Rtot) -> int {
+    Rtot.assign((size_t)nb * bs * bs, 0.0);
+    for (int d = 0; d < nb; d++)
+      for (int a = 0; a < bs; a++) Rtot[(size_t)d * bs * bs + a * bs + a] = 1.0;
+    for (int pass = 0; pass < 2; pass++) {
+      BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, bs, bs, n,
+                                                  &one, W, (int)V, n, KW, (int)V, n, &zero, dM, bs, bs * bs, nb));
+      HIPCALL(ctx, hipMemcpyAsync(hM.data(), dM, sizeof(double) * hM.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+      for (int d = 0; d < nb; d++) {
+        double *M = hM.data() + (size_t)d * bs * bs, *Ri = Rinv.data() + (size_t)d * bs * bs;
+        // upper Cholesky M = R^T R (column-major, R in M's upper triangle)
+        for (int j = 0; j < bs; j++) {
+          for (int i = 0; i <= j; i++) {
+            double v = 0.5 * (M[j * bs + i] + M[i * bs + j]);
+            for (int k = 0; k < i; k++) v -= M[i * bs + k] * M[j * bs + k];
+            if (i == j) {
+              if (!(v > 0.0)) return ferr(ctx, 6, "block Lanczos breakdown (rank-deficient block)");
+              M[j * bs + j] = std::sqrt(v);
+            } else {
+              M[j * bs + i] = v / M[i * bs + i];
+            }
+          }
+        }
+        // R stored in M(i, j) = M[j * bs + i] for i <= j; Ri = R^{-1} (upper)
+        std::fill(Ri, Ri + bs * bs, 0.0);
+        for (int j = 0; j < bs; j++) {
+          Ri[j * bs + j] = 1.0 / M[j * bs + j];
+          for (int i = j - 1; i >= 0; i--) {
+            double v = 0.0;
+            for (int k = i + 1; k <= j; k++) v += M[k * bs + i] * Ri[j * bs + k];
+            Ri[j * bs + i] = -v / M[i * bs + i];
+          }
+        }
+        // Rtot = R Rtot (both upper)
+        double *Rt = Rtot.data() + (size_t)d * bs * bs;
+        std::vector<double> P((size_t)bs * bs, 0.0);
+        for (int j = 0; j < bs; j++)
+          for (int i = 0; i <= j; i++) {
+            double v = 0.0;
+            for (int k = i; k <= j; k++) v += M[k * bs + i] * Rt[j * bs + k];
+            P[j * bs + i] = v;
+          }
+        std::copy(P.begin(), P.end(), Rt);
+      }
+      HIPCALL(ctx, hipMemcpyAsync(dM, Rinv.data(), sizeof(double) * Rinv.size(), hipMemcpyHostToDevice, ctx->stream));
+      BLASCALL(ctx, rocblas_dtrmm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                                  rocblas_diagonal_non_unit, n, bs, &one, dM, bs, bs * bs, W, (int)V,
+                                                  n, W, (int)V, n, nb));
+      BLASCALL(ctx, rocblas_dtrmm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                                  rocblas_diagonal_non_unit, n, bs, &one, dM, bs, bs * bs, KW, (int)V,
+                                                  n, KW, (int)V, n, nb));
+      // the host copies of Rinv must outlive the transfers above
+      HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return 0;
+  };
+  {
+    std::vector<double> q0((size_t)bs * V);
+    unsigned long long sd = 0x5eedULL;
+    for (auto &v : q0) {
+      sd += 0x9E3779B97F4A7C15ULL;
+      unsigned long long zz = sd;
+      zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBULL;
+      zz ^= zz >> 31;
+      v = 2.0 * ((double)(zz >> 11) / 9007199254740992.0) - 1.0;
+    }
+    HIPCALL(ctx, hipMemcpyAsync(Q, q0.data(), sizeof(double) * q0.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < bs; c++) {
+      if ((rc = shellgpu_vec_zero_bcs(ctx, Q + c * V))) return rc;
+      if ((rc = shellgpu_mat_mult(K, Q + c * V, KQ + c * V))) return rc;
+    }
+    std::vector<double> R0;
+    if ((rc = cholqr2(Q, KQ, R0))) return rc;
+  }
+  std::vector<std::vector<double>> Hsum(nb);
+  std::vector<double> hH;
+  int j, s = 0;
+  for (j = 0; j < jmax; j++) {
+    double *Qj = Q + (size_t)j * bs * V, *KQj = KQ + (size_t)j * bs * V;
+    double *W = Q + (size_t)(j + 1) * bs * V, *KW = KQ + (size_t)(j + 1) * bs * V;
+    if (mode == 0 || mode == 2) {
+      if ((rc = factor_apply_multi(F, KQj, W, bs))) return rc;
+    } else {
+      for (int c = 0; c < bs; c++) {
+        if ((rc = shellgpu_mat_mult(G, Qj + c * V, KW + c * V))) return rc;
+        if ((rc = shellgpu_vec_zero_bcs(ctx, KW + c * V))) return rc;
+      }
+      if ((rc = factor_apply_multi(F, KW, W, bs))) return rc;
+      if ((rc = shellgpu_vec_scale(ctx, bs * V, -1.0, W))) return rc;
+    }
+    for (int c = 0; c < bs; c++)
+      if ((rc = shellgpu_vec_zero_bcs(ctx, W + c * V))) return rc;
+    const int ncol = (j + 1) * bs;
+    for (int d = 0; d < nb; d++) Hsum[d].assign((size_t)ncol * bs, 0.0);
+    hH.resize((size_t)nb * ncol * bs);
+    for (int pass = 0; pass < 2; pass++) {
+      BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, ncol, bs, n,
+                                                  &one, KQ, (int)V, n, W, (int)V, n, &zero, dH, ncol, ncol * bs, nb));
+      BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n, bs, ncol,
+                                                  &mone, Q, (int)V, n, dH, ncol, ncol * bs, &one, W, (int)V, n, nb));
+      HIPCALL(ctx, hipMemcpyAsync(hH.data(), dH, sizeof(double) * hH.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+      for (int d = 0; d < nb; d++)
+        for (size_t q = 0; q < (size_t)ncol * bs; q++) Hsum[d][q] += hH[(size_t)d * ncol * bs + q];
+    }
+    for (int c = 0; c < bs; c++)
+      if ((rc = shellgpu_mat_mult(K, W + c * V, KW + c * V))) return rc;
+    std::vector<double> Rb;
+    if ((rc = cholqr2(W, KW, Rb))) return rc;
+    // projection blocks: T_jj = A_j (symmetrised), T_{j+1,j} = B_{j+1}
+    for (int d = 0; d < nb; d++) {
+      std::vector<double> &T = lo.T[d];
+      const double *Hd = Hsum[d].data();  // ncol x bs, column-major
+      for (int b = 0; b < bs; b++)
+        for (int a = 0; a < bs; a++) {
+          const double v = 0.5 * (Hd[(size_t)b * ncol + j * bs + a] + Hd[(size_t)a * ncol + j * bs + b]);
+          T[(size_t)(j * bs + b) * smax + j * bs + a] = v;
+        }
+      const double *B = Rb.data() + (size_t)d * bs * bs;
+      std::copy(B, B + bs * bs, lo.R[d].begin());
+      if (j + 1 < jmax)
+        for (int b = 0; b < bs; b++)
+          for (int a = 0; a <= b; a++) {
+            const double v = B[(size_t)b * bs + a];  // B(a, b), upper
+            T[(size_t)(j * bs + b) * smax + (j + 1) * bs + a] = v;   // T(j+1 block row a, j block col b)
+            T[(size_t)((j + 1) * bs + a) * smax + j * bs + b] = v;   // transpose
+          }
+    }
+    s = (j + 1) * bs;
+    if (s >= nev) {
+      lo.steps = smax;  // T's leading dimension while iterating
+      std::vector<std::vector<double>> w, z, res;
+      if ((rc = proj_eigs(ctx, lo, s, w, z, res))) return rc;
+      bool all = true;
+      for (int d = 0; d < nb && all; d++) {
+        std::vector<int> idx(s);
+        for (int i = 0; i < s; i++) idx[i] = i;
+        auto key = [&](int i) {
+          const double th = w[d][i];
+          if (mode == 2) return std::fabs(1.0 / th + sigma[d]);
+          return mode == 0 ? std::fabs(-th * sigma[d] / (1.0 - th)) : (th > 0 ? 1.0 / th : 1e300);
+        };
+        std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+        for (int q = 0; q < nev; q++)
+          if (!(res[d][idx[q]] <= tol * std::fabs(w[d][idx[q]]))) all = false;
+      }
+      if (all) break;
+    }
+  }
+  // compact T to the final basis size s
+  for (int d = 0; d < nb; d++) {
+    std::vector<double> Tc((size_t)s * s);
+    for (int c = 0; c < s; c++)
+      for (int r = 0; r < s; r++) Tc[(size_t)c * s + r] = lo.T[d][(size_t)c * smax + r];
+    lo.T[d].swap(Tc);
+  }
+  lo.steps = s;
+  return 0;
+}
+
+// Vectors per Lanczos block: 1 (the single-vector recurrence) unless
+// SHELLGPU_LANCZOS_BLOCK=b (2 ... 8) asks for blocks, which only the
+// sequential factorisation solves together (one pass over its level
+// inverses per block).  Measured on 64 designs of the 100 x 200 cylinder,
+// blocks of 4 read the level inverses 2.5 times less per vector but need a
+// basis as large as the single-vector run's, and the block Gram-Schmidt
+// products (tall, skinny, transposed GEMMs) cost more than the saving.
+int lanczos_block_size(const shellgpu_factor *F) {
+  if (F->cr) return 1;
+  int bs = 1;
+  if (const char *env = getenv("SHELLGPU_LANCZOS_BLOCK")) bs = atoi(env);
+  return std::max(1, std::min(kMaxRhs, bs));
+}
+
+int run_lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
+                const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
+                double *work, LanczosOut &lo, bool single = false) {
+  const int bs = single ? 1 : lanczos_block_size(F);
+  if (bs == 1 || (m + 1) / bs < 3) {
+    lo.bs = 1;
+    return lanczos(ctx, F, K, G, mode, sigma, nev, m, tol, Q, KQ, hv, work, lo);
+  }
+  return lanczos_block(ctx, F, K, G, mode, sigma, nev, m, tol, Q, KQ, bs, lo);
 }
 
 }  // namespace
@@ -1986,13 +2432,13 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // shift: sigma > 0 as given; sigma <= 0 places it at 0.95 of a first-load
   // estimate from a short buckling-mode Lanczos on K^{-1}(-G)
   if (B->sigma <= 0.0) {
-    if ((rc = lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-2, B->Q, B->KQ, B->hv, B->res, lo)))
+    if ((rc = run_lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-2, B->Q, B->KQ, B->hv, B->res,
+                          lo, true)))
       return rc;
+    std::vector<std::vector<double>> w, z, res;
+    if ((rc = proj_eigs(ctx, lo, lo.steps, w, z, res))) return rc;
     for (int d = 0; d < nb; d++) {
-      const int ns = (int)lo.alpha[d].size();
-      std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
-      if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
-      const double tmax = *std::max_element(w.begin(), w.end());
+      const double tmax = *std::max_element(w[d].begin(), w[d].end());
       if (!(tmax > 0.0)) return ferr(ctx, 7, "no positive buckling load under this load case");
       B->sig[d] = 0.95 / tmax;
     }
@@ -2018,17 +2464,19 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
       if (info[d] != 0) B->sig[d] *= (B->sigma <= 0.0 ? 0.8 : 0.5);
   }
 
-  if ((rc = lanczos(ctx, B->F, B->K, B->G, 0, B->sig, nev, m, B->tol, B->Q, B->KQ, B->hv, B->res, lo))) return rc;
+  if ((rc = run_lanczos(ctx, B->F, B->K, B->G, 0, B->sig, nev, m, B->tol, B->Q, B->KQ, B->hv, B->res, lo)))
+    return rc;
   B->steps = lo.steps;
   B->lam.assign(nb, {});
   B->err.assign(nb, {});
   B->ritz.assign(nb, {});
   B->ns.assign(nb, 0);
+  const int ns = lo.steps;
+  if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
+  std::vector<std::vector<double>> wv, zv, rv;
+  if ((rc = proj_eigs(ctx, lo, ns, wv, zv, rv))) return rc;
   for (int d = 0; d < nb; d++) {
-    const int ns = (int)lo.alpha[d].size();
-    if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
-    std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
-    if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+    const std::vector<double> &w = wv[d], &z = zv[d];
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
     const double sd = B->sig[d];
@@ -2039,7 +2487,7 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
     for (int j = 0; j < nev; j++) {
       const int i = idx[j];
       B->lam[d].push_back(lam_of(i));
-      B->err[d].push_back(std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]));
+      B->err[d].push_back(rv[d][i] / std::fabs(w[i]));
       B->ritz[d].insert(B->ritz[d].end(), z.begin() + (size_t)i * ns, z.begin() + (size_t)(i + 1) * ns);
     }
   }
@@ -2310,18 +2758,19 @@ int shellgpu_frequency_solve(shellgpu_frequency *F) {
     }
     ctx->err.clear();
   }
-  if ((rc = lanczos(ctx, F->F, F->M, nullptr, 2, F->sig, nev, F->m, F->tol, F->Q, F->MQ, F->hv, F->work, lo)))
+  if ((rc = run_lanczos(ctx, F->F, F->M, nullptr, 2, F->sig, nev, F->m, F->tol, F->Q, F->MQ, F->hv, F->work, lo)))
     return rc;
   F->steps = lo.steps;
   F->lam.assign(nb, {});
   F->err.assign(nb, {});
   F->ritz.assign(nb, {});
   F->ns.assign(nb, 0);
+  const int ns = lo.steps;
+  if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
+  std::vector<std::vector<double>> wv, zv, rv;
+  if ((rc = proj_eigs(ctx, lo, ns, wv, zv, rv))) return rc;
   for (int d = 0; d < nb; d++) {
-    const int ns = (int)lo.alpha[d].size();
-    if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
-    std::vector<double> w, z, e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
-    if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+    const std::vector<double> &w = wv[d], &z = zv[d];
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
     const double sd = F->sig[d];
@@ -2333,7 +2782,7 @@ int shellgpu_frequency_solve(shellgpu_frequency *F) {
     for (int j = 0; j < nev; j++) {
       const int i = idx[j];
       F->lam[d].push_back(lam_of(i));
-      F->err[d].push_back(std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]) / std::fabs(w[i]));
+      F->err[d].push_back(rv[d][i] / std::fabs(w[i]));
       F->ritz[d].insert(F->ritz[d].end(), z.begin() + (size_t)i * ns, z.begin() + (size_t)(i + 1) * ns);
     }
   }

@@ -61,8 +61,11 @@ def test_mass_matrix_matches_oracle():
         assert bsr_row_floor_ratio(rowp, A.values()[0], ref) < 1e-12
 
 
-@pytest.mark.parametrize("sigma_factor", [0.0, 0.8, 3.0])
-def test_frequencies_match_dense_oracle(sigma_factor):
+@pytest.mark.parametrize("sigma_factor,block", [(0.0, False), (0.8, False), (3.0, False), (0.0, True)])
+def test_frequencies_match_dense_oracle(sigma_factor, block, monkeypatch):
+    if block:  # the sequential factorisation with block Lanczos (blocks of 4)
+        monkeypatch.setenv("SHELLGPU_FACTOR", "thomas")
+        monkeypatch.setenv("SHELLGPU_LANCZOS_BLOCK", "4")
     cyl = Cylinder(10, 20)
     thick = np.random.default_rng(4).uniform(0.0015, 0.0025, cyl.nelems)
     ctx = make_ctx(cyl.conn, cyl.Xpts, thick, cyl.bcs(), cyl.ring_nodes(0))

@@ -12,6 +12,14 @@ from shellgpu.problems import Cylinder
 pytestmark = pytest.mark.gpu
 
 
+def set_factor(monkeypatch, factor):
+    """cr: block cyclic reduction; thomas: the sequential level factorisation;
+    thomas-block: the latter with block Lanczos (blocks of 4 vectors)"""
+    monkeypatch.setenv("SHELLGPU_FACTOR", factor.split("-")[0])
+    if factor.endswith("-block"):
+        monkeypatch.setenv("SHELLGPU_LANCZOS_BLOCK", "4")
+
+
 def make_ctx(cyl, thick):
     ctx = sg.Context(0)
     ctx.set_mesh(cyl.conn, cyl.Xpts)
@@ -58,10 +66,10 @@ def _subspace_angle(V, W, rowp, cols, Kvals):
     return float(np.arccos(min(1.0, s.min())))
 
 
-@pytest.mark.parametrize("factor", ["cr", "thomas"])
+@pytest.mark.parametrize("factor", ["cr", "thomas", "thomas-block"])
 @pytest.mark.parametrize("na,nc", [(20, 40)])
 def test_buckling_matches_oracle(na, nc, factor, monkeypatch):
-    monkeypatch.setenv("SHELLGPU_FACTOR", factor)
+    set_factor(monkeypatch, factor)
     cyl = Cylinder(na, nc)
     thick = np.full(cyl.nelems, 0.002)
     sigma = 0.9 * cyl.classical_load()
@@ -92,9 +100,9 @@ def test_buckling_matches_oracle(na, nc, factor, monkeypatch):
         i = j
 
 
-@pytest.mark.parametrize("factor", ["cr", "thomas"])
+@pytest.mark.parametrize("factor", ["cr", "thomas", "thomas-block"])
 def test_batched_designs_match_single_designs(factor, monkeypatch):
-    monkeypatch.setenv("SHELLGPU_FACTOR", factor)
+    set_factor(monkeypatch, factor)
     cyl = Cylinder(8, 16, imperfection=(2e-4, 2, 3))
     rng = np.random.default_rng(9)
     thick = rng.uniform(0.0015, 0.0025, (3, cyl.nelems))
