@@ -398,6 +398,13 @@ int shellgpu_set_designs(shellgpu_ctx *ctx, int nbatch, const double *thick, con
   if (!ctx || nbatch <= 0 || !thick) return set_err(ctx, 1, "invalid design arguments");
   if (ctx->nelems <= 0) return set_err(ctx, 1, "set the mesh before the designs");
   ctx->nbatch = nbatch;
+  {
+    const size_t ne = ctx->nelems;
+    bool same = true;
+    for (int d = 1; d < nbatch && same; d++)
+      same = std::equal(thick, thick + ne, thick + d * ne) && (!temp || std::equal(temp, temp + ne, temp + d * ne));
+    ctx->same_designs = same;
+  }
   int rc = 0;
   if ((rc = upload(ctx, &ctx->d_thick, thick, (size_t)nbatch * ctx->nelems))) return rc;
   if (temp) {

@@ -27,6 +27,7 @@ struct shellgpu_ctx {
   double *d_bc_vals = nullptr;
 
   double *d_thick = nullptr, *d_temp = nullptr;
+  bool same_designs = false;  // every design has the same thickness and temperature
   double *d_coef = nullptr;  // 4 * nbatch per-design kernel coefficients
   // timing
   bool timing = false;

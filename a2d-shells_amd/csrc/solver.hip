@@ -784,22 +784,23 @@ int setup_plan(shellgpu_factor *F) {
 
   // Block cyclic reduction when the padded factor (three M x M blocks per
   // level and design) fits in 85 % of the free device memory, padding the
-  // levels to M adds at most half the unknowns, and the batch is small;
-  // otherwise the sequential level Cholesky.  SHELLGPU_FACTOR=thomas / =cr
-  // selects either (cr only where it fits).
+  // levels to M adds at most half the unknowns, and few vectors are solved
+  // at a time (the context's designs: a shared factor solves one per
+  // design); otherwise the sequential factorisation.  SHELLGPU_FACTOR=thomas
+  // / =cr selects either (cr only where it fits).
   {
     const long M = F->maxm;
     size_t freeb = 0, totalb = 0;
     HIPCALL(ctx, hipMemGetInfo(&freeb, &totalb));
-    const double need = 3.0 * nl * (double)M * (double)M * 8.0 * ctx->nbatch;
+    const double need = 3.0 * nl * (double)M * (double)M * 8.0 * F->nbatch;
     const char *env = getenv("SHELLGPU_FACTOR");
     const bool fits = nl >= 3 && need <= 0.85 * (double)freeb;
     const bool uniform = (double)nl * M <= 1.5 * F->n;
-    // cyclic reduction pays off while one level's batched work is too small
-    // to fill the GPU (few designs); with many designs the sequential
-    // factorisation is already parallel and its factor is smaller (two
-    // blocks per level instead of three, and no padding: fewer bytes per
-    // solve)
+    // cyclic reduction pays off while one level's work is too small to fill
+    // the GPU and the level chain is long (its rounds batch the levels);
+    // with many vectors per solve the sequential factorisation is parallel
+    // enough and reads fewer bytes per solve (half of one inverse per
+    // level, where cyclic reduction reads three padded blocks)
     F->cr = fits && uniform && (long)ctx->nbatch * M <= 8192;
     if (env && strcmp(env, "thomas") == 0) F->cr = false;
     if (env && strcmp(env, "cr") == 0) F->cr = fits;
@@ -1116,14 +1117,20 @@ int shellgpu_set_ordering_seed(shellgpu_ctx *ctx, int n, const int *nodes) {
   return 0;
 }
 
-int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
+}  // extern "C"
+
+namespace {
+
+// a factor of nbatch designs: the context's (nbatch = ctx->nbatch) or, for
+// identical designs, one shared by all (nbatch = 1)
+int factor_create(shellgpu_ctx *ctx, int nbatch, shellgpu_factor **out) {
   if (!ctx || !out) return 1;
   if (ctx->nnzb <= 0 || ctx->nbatch <= 0) return ferr(ctx, 1, "set the mesh and designs before the factor");
   int rc = ensure_blas(ctx);
   if (rc) return rc;
   shellgpu_factor *F = new shellgpu_factor();
   F->ctx = ctx;
-  F->nbatch = ctx->nbatch;
+  F->nbatch = nbatch;
   if ((rc = setup_plan(F))) {
     delete F;
     return rc;
@@ -1159,6 +1166,15 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
+  if (!ctx) return 1;
+  return factor_create(ctx, ctx->nbatch, out);
+}
+
 void shellgpu_factor_destroy(shellgpu_factor *F) {
   if (!F) return;
   (void)hipStreamSynchronize(F->ctx->stream);
@@ -1186,7 +1202,8 @@ static int apply_schur(shellgpu_factor *F, int nrhs);
 
 int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   shellgpu_ctx *ctx = F->ctx;
-  if (A->nbatch != F->nbatch) return ferr(ctx, 1, "matrix and factor batch sizes differ");
+  // a shared factor (nbatch 1) takes the first design's matrix
+  if (A->nbatch != F->nbatch && F->nbatch != 1) return ferr(ctx, 1, "matrix and factor batch sizes differ");
   int rc = ensure_blas(ctx);
   if (rc) return rc;
   const int nb = F->nbatch;
@@ -1536,42 +1553,32 @@ static int apply_cr(shellgpu_factor *F) {
   return 0;
 }
 
-int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
-  shellgpu_ctx *ctx = F->ctx;
-  if (!F->factored) return ferr(ctx, 1, "factor not computed");
-  const int nb = F->nbatch;
-  const long nv = F->nv, n = F->n;
-  int rc = 0;
-  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv, n,
-                     F->d_pos2dof, b, F->vb, 1);
-  HIPCALL(ctx, hipGetLastError());
-  if ((rc = F->cr ? apply_cr(F) : apply_schur(F, 1))) return rc;
-  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv, n,
-                     F->d_pos2dof, F->vx, x, 0);
-  HIPCALL(ctx, hipGetLastError());
-  return 0;
-}
-
 }  // extern "C"
 
 namespace {
 
-// x = A^{-1} b for nrhs vectors of every design (entry e = r nbatch + d at
-// offset e n of b and x).  The sequential factorisation solves them
-// together, reading each level inverse once per solve; cyclic reduction
-// solves them one after the other.
-int factor_apply_multi(shellgpu_factor *F, const double *b, double *x, int nrhs) {
+// x = A^{-1} b for nrhs vectors per factored design (entry e = r nbatch + d
+// of the factor at offset e n of b and x).  The sequential factorisation
+// solves them together, reading each level inverse once per solve; cyclic
+// reduction solves them one after the other.
+int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
   const int nb = F->nbatch;
   const long nv = F->nv, n = F->n;
-  if (nrhs <= 1 || F->cr) {
+  const dim3 pgrid((unsigned)((nv + 255) / 256), nb);
+  int rc = 0;
+  if (F->cr || nrhs == 1) {
     for (int r = 0; r < nrhs; r++) {
-      const int rc = shellgpu_factor_apply(F, b + (size_t)r * nb * n, x + (size_t)r * nb * n);
-      if (rc) return rc;
+      const double *br = b + (size_t)r * nb * n;
+      double *xr = x + (size_t)r * nb * n;
+      hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, br, F->vb, 1);
+      HIPCALL(ctx, hipGetLastError());
+      if ((rc = F->cr ? apply_cr(F) : apply_schur(F, 1))) return rc;
+      hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, xr, 0);
+      HIPCALL(ctx, hipGetLastError());
     }
     return 0;
   }
-  if (!F->factored) return ferr(ctx, 1, "factor not computed");
   if (nrhs > kMaxRhs) return ferr(ctx, 1, "too many right-hand sides for one solve");
   if (nrhs > F->rhs_cap) {
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
@@ -1587,20 +1594,41 @@ int factor_apply_multi(shellgpu_factor *F, const double *b, double *x, int nrhs)
     HIPCALL(ctx, hipMalloc((void **)&F->part, sizeof(double) * (size_t)nT * nT * kSymvTile * nb * nrhs));
     F->rhs_cap = nrhs;
   }
-  int rc = 0;
-  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nrhs * nb), dim3(256), 0, ctx->stream, nv, n,
-                     F->d_pos2dof, b, F->vb, 1);
+  const dim3 mgrid((unsigned)((nv + 255) / 256), nrhs * nb);
+  hipLaunchKernelGGL(permute_kernel, mgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, b, F->vb, 1);
   HIPCALL(ctx, hipGetLastError());
   if ((rc = apply_schur(F, nrhs))) return rc;
-  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nv + 255) / 256), nrhs * nb), dim3(256), 0, ctx->stream, nv, n,
-                     F->d_pos2dof, F->vx, x, 0);
+  hipLaunchKernelGGL(permute_kernel, mgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, x, 0);
   HIPCALL(ctx, hipGetLastError());
   return 0;
+}
+
+// block Lanczos: nrhs vectors of every design
+int factor_apply_multi(shellgpu_factor *F, const double *b, double *x, int nrhs) {
+  if (!F->factored) return ferr(F->ctx, 1, "factor not computed");
+  return apply_block(F, b, x, nrhs);
 }
 
 }  // namespace
 
 extern "C" {
+
+// x = A^{-1} b for every design of the context.  A factor of one design
+// shared by all (identical designs: load cases of one structure) solves the
+// designs' vectors as right-hand sides of that design, kMaxRhs at a time.
+int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->factored) return ferr(ctx, 1, "factor not computed");
+  if (F->nbatch == ctx->nbatch) return apply_block(F, b, x, 1);
+  if (F->nbatch != 1) return ferr(ctx, 1, "factor batch size does not match the context's designs");
+  const long n = F->n;
+  for (int r0 = 0; r0 < ctx->nbatch; r0 += kMaxRhs) {
+    const int rc = apply_block(F, b + (size_t)r0 * n, x + (size_t)r0 * n, std::min(kMaxRhs, ctx->nbatch - r0));
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 
 }  // extern "C"
 
@@ -2204,7 +2232,7 @@ int lanczos_block(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellg
 // basis as large as the single-vector run's, and the block Gram-Schmidt
 // products (tall, skinny, transposed GEMMs) cost more than the saving.
 int lanczos_block_size(const shellgpu_factor *F) {
-  if (F->cr) return 1;
+  if (F->cr || F->nbatch != F->ctx->nbatch) return 1;
   int bs = 1;
   if (const char *env = getenv("SHELLGPU_LANCZOS_BLOCK")) bs = atoi(env);
   return std::max(1, std::min(kMaxRhs, bs));
@@ -2397,7 +2425,8 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // assembled at zero state, the path is the static solution K u = f (with
   // the BC-driven residual) unless u0 is given
   if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, B->K))) return rc;
-  if (!B->FK && shellgpu_factor_create(ctx, &B->FK) != 0) {
+  // identical designs (load cases of one structure) share one factor of K
+  if (!B->FK && factor_create(ctx, ctx->same_designs ? 1 : nb, &B->FK) != 0) {
     B->FK = B->F;  // no room for a second factor
     ctx->err.clear();
     (void)hipGetLastError();  // drop the failed allocation's error state

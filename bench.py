@@ -183,10 +183,11 @@ def ks_of_positive_loads(lam, npos=5):
     return np.array(ks), coef
 
 
-def box_beam(ncases=16, chunk=4):
+def box_beam(ncases=16, chunk=16):
     """Stiffened box beam (128,000 quads, 746,352 dof): 16 load cases (tip
-    force / bottom-skin pressure mixes) batched `chunk` at a time: static
-    path, buckling (first loads), KS thickness gradient per load case."""
+    force / bottom-skin pressure mixes) batched `chunk` at a time (one
+    batch: the load cases share one factor of K): static path, buckling
+    (first loads), KS thickness gradient per load case."""
     b = BoxBeam()
     t = b.panel_thickness(4000)
     t0 = time.perf_counter()
