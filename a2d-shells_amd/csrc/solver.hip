@@ -591,15 +591,19 @@ __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const do
 
 // Y_r = P X_r for NR vectors per design (vector entries e = r nbf + d of
 // stride sxe), P read in full once for all of them: 128 x 128 tiles (I, J)
-// over the whole matrix (blockIdx.x = I nT + J), designs blockIdx.y; the
-// partial sums P_IJ x_J of block row I go to slot J, as pass 1 of the
-// symmetric product lays them out.
-template <int NR>
+// over the whole matrix, each split into CS column slabs of 128 / CS
+// columns (blockIdx.x = (I nT + J) CS + slab), designs blockIdx.y; the
+// partial sums of block row I go to slot J CS + slab of nT CS slots.
+// Splitting keeps the launch wider than the GPU when one design is
+// factored.
+constexpr int kMaxSplit = 4;
+template <int NR, int CS>
 __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
                                                                long sAd, const double *__restrict__ x, long sxe,
                                                                int nbf, double *__restrict__ part) {
   __shared__ double2 red[4][NR][64];
-  const int I = blockIdx.x / nT, J = blockIdx.x - I * nT, d = blockIdx.y;
+  const int tile = blockIdx.x / CS, slab = blockIdx.x - tile * CS;
+  const int I = tile / nT, J = tile - I * nT, d = blockIdx.y, ns = nT * CS;
   A += d * sAd;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int row = I * kSymvTile + 2 * lane;
@@ -607,9 +611,10 @@ __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, co
   double2 acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) acc[r] = make_double2(0.0, 0.0);
+  const int cbase = J * kSymvTile + slab * (kSymvTile / CS);
 #pragma unroll 4
-  for (int p = 0; p < kSymvTile / 4; p++) {
-    const int col = J * kSymvTile + wave + 4 * p;
+  for (int p = 0; p < kSymvTile / (4 * CS); p++) {
+    const int col = cbase + wave + 4 * p;
     if (col >= m) break;
     const double2 v = rv ? *reinterpret_cast<const double2 *>(A + row + (long)col * lda) : make_double2(0.0, 0.0);
 #pragma unroll
@@ -626,20 +631,20 @@ __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, co
     const int ln = q & 63, r = q >> 6;
     const double2 a0 = red[0][r][ln], a1 = red[1][r][ln], a2 = red[2][r][ln], a3 = red[3][r][ln];
     const long e = (long)r * nbf + d;
-    *reinterpret_cast<double2 *>(part + ((e * nT + I) * nT + J) * kSymvTile + 2 * ln) =
+    *reinterpret_cast<double2 *>(part + ((e * nT + I) * ns + J * CS + slab) * kSymvTile + 2 * ln) =
         make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
   }
 }
 
-// Pass 2: y = beta c + alpha sum over the nT slots of the row's block
-__global__ void symv_pass2_kernel(int m, int nT, const double *__restrict__ part, double alpha,
+// Pass 2: y = beta c + alpha sum over the ns slots of the row's block
+__global__ void symv_pass2_kernel(int m, int nT, int ns, const double *__restrict__ part, double alpha,
                                   const double *__restrict__ c, double beta, double *__restrict__ y, long svd) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
   const int d = blockIdx.y;
-  const double *p = part + ((long)d * nT + row / kSymvTile) * nT * kSymvTile + row % kSymvTile;
+  const double *p = part + ((long)d * nT + row / kSymvTile) * ns * kSymvTile + row % kSymvTile;
   double s = 0.0;
-  for (int J = 0; J < nT; J++) s += p[J * kSymvTile];
+  for (int J = 0; J < ns; J++) s += p[J * kSymvTile];
   y[d * svd + row] = (beta == 0.0 ? 0.0 : beta * c[d * svd + row]) + alpha * s;
 }
 
@@ -1273,13 +1278,19 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
   const int m = F->m[l], nT = (m + kSymvTile - 1) / kSymvTile, nb = F->nbatch;
   const double *P = F->buf + F->S_off[l];
   const long ps = F->per_design, nv = F->nv;
-  switch (nrhs) {
-#define SYMV_MULTI(R)                                                                                             \
-  case R:                                                                                                         \
-    hipLaunchKernelGGL(symv_multi_pass1_kernel<R>, dim3(nT * nT, nb), dim3(256), 0, ctx->stream, m, nT, P, m, ps, \
-                       x, nv, nb, F->part);                                                                       \
+  // column slabs per tile when the tiles alone would not fill the GPU
+  const int cs = (nrhs > 1 && (long)nT * nT * nb < 512) ? kMaxSplit : 1;
+  switch (nrhs * 8 + (cs > 1)) {
+#define SYMV_MULTI(R)                                                                                               \
+  case R * 8:                                                                                                       \
+    hipLaunchKernelGGL((symv_multi_pass1_kernel<R, 1>), dim3(nT * nT, nb), dim3(256), 0, ctx->stream, m, nT, P, m,  \
+                       ps, x, nv, nb, F->part);                                                                     \
+    break;                                                                                                          \
+  case R * 8 + 1:                                                                                                   \
+    hipLaunchKernelGGL((symv_multi_pass1_kernel<R, kMaxSplit>), dim3(nT * nT * kMaxSplit, nb), dim3(256), 0,       \
+                       ctx->stream, m, nT, P, m, ps, x, nv, nb, F->part);                                          \
     break;
-    case 1:
+    case 8:
       hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, nb), dim3(256), 0, ctx->stream, m, nT, P, m, ps,
                          x, nv, F->part);
       break;
@@ -1295,8 +1306,8 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
       return ferr(ctx, 1, "too many right-hand sides for one level product");
   }
   HIPCALL(ctx, hipGetLastError());
-  hipLaunchKernelGGL(symv_pass2_kernel, dim3((m + 255) / 256, nrhs * nb), dim3(256), 0, ctx->stream, m, nT, F->part,
-                     alpha, c, beta, y, nv);
+  hipLaunchKernelGGL(symv_pass2_kernel, dim3((m + 255) / 256, nrhs * nb), dim3(256), 0, ctx->stream, m, nT, nT * cs,
+                     F->part, alpha, c, beta, y, nv);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
@@ -1591,7 +1602,8 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
     const long nT = (F->maxm + kSymvTile - 1) / kSymvTile;
     HIPCALL(ctx, hipFree(F->part));
     F->part = nullptr;
-    HIPCALL(ctx, hipMalloc((void **)&F->part, sizeof(double) * (size_t)nT * nT * kSymvTile * nb * nrhs));
+    HIPCALL(ctx,
+            hipMalloc((void **)&F->part, sizeof(double) * (size_t)nT * nT * kMaxSplit * kSymvTile * nb * nrhs));
     F->rhs_cap = nrhs;
   }
   const dim3 mgrid((unsigned)((nv + 255) / 256), nrhs * nb);
