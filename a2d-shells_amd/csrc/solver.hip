@@ -2355,6 +2355,12 @@ int shellgpu_buckling_create(shellgpu_ctx *ctx, int model, double sigma, int nev
   if ((rc = shellgpu_mat_create(ctx, nb, &B->G))) goto fail;
   if ((rc = shellgpu_mat_create(ctx, nb, &B->A))) goto fail;
   if ((rc = shellgpu_factor_create(ctx, &B->F))) goto fail;
+  // the factor of K (shared by identical designs), when memory allows
+  if (factor_create(ctx, ctx->same_designs ? 1 : nb, &B->FK) != 0) {
+    B->FK = B->F;
+    ctx->err.clear();
+    (void)hipGetLastError();  // drop the failed allocation's error state
+  }
   {
     const size_t v = sizeof(double) * (size_t)n * nb;
     double **vecs[] = {&B->u, &B->res, &B->f0, &B->ph, &B->gv, &B->psi};
@@ -2396,6 +2402,7 @@ int shellgpu_buckling_set_operators(shellgpu_buckling *B, shellgpu_mat *K, shell
   const int nb = ctx->nbatch;
   if (K->nbatch != nb || G->nbatch != nb || A->nbatch != nb)
     return ferr(ctx, 1, "matrix batch size does not match the context's designs");
+  if (B->FK == B->F) B->FK = nullptr;  // the K factor fell back to the replaced factor
   if (B->own_ops) {
     shellgpu_mat_destroy(B->K);
     shellgpu_mat_destroy(B->G);
@@ -2437,8 +2444,14 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // assembled at zero state, the path is the static solution K u = f (with
   // the BC-driven residual) unless u0 is given
   if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_STIFFNESS, nullptr, B->K))) return rc;
-  // identical designs (load cases of one structure) share one factor of K
-  if (!B->FK && factor_create(ctx, ctx->same_designs ? 1 : nb, &B->FK) != 0) {
+  // identical designs (load cases of one structure) share one factor of K;
+  // the designs may have changed since the factor was made
+  const int fk_nb = ctx->same_designs ? 1 : nb;
+  if (B->FK && B->FK != B->F && B->FK->nbatch != fk_nb) {
+    shellgpu_factor_destroy(B->FK);
+    B->FK = nullptr;
+  }
+  if (!B->FK && factor_create(ctx, fk_nb, &B->FK) != 0) {
     B->FK = B->F;  // no room for a second factor
     ctx->err.clear();
     (void)hipGetLastError();  // drop the failed allocation's error state

