@@ -2254,7 +2254,10 @@ int run_lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu
                 const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
                 double *work, LanczosOut &lo, bool single = false) {
   const int bs = single ? 1 : lanczos_block_size(F);
-  if (bs == 1 || (m + 1) / bs < 3) {
+  // the block recurrence hands rocBLAS the basis with a leading dimension of
+  // nbatch * n (an int)
+  const bool ld_fits = (long)ctx->nbatch * 6 * ctx->nnodes <= 0x7fffffffL;
+  if (bs == 1 || (m + 1) / bs < 3 || !ld_fits) {
     lo.bs = 1;
     return lanczos(ctx, F, K, G, mode, sigma, nev, m, tol, Q, KQ, hv, work, lo);
   }
