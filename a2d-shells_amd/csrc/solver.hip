@@ -113,6 +113,8 @@ struct shellgpu_factor {
   double *vb = nullptr, *vz = nullptr, *vx = nullptr;  // level-ordered work vectors
   double *part = nullptr;    // GEMV partial sums
   int rhs_cap = 1;           // vectors per design that vb, vz, vx and part hold
+  hipGraphExec_t solve_graph = nullptr;  // the single-vector solve body, captured
+  bool graph_off = false;
   bool factored = false;
   long nv = 0;               // level-ordered vector length per design
   // block cyclic reduction over uniform (padded) levels of M dofs: per
@@ -1183,6 +1185,7 @@ int shellgpu_factor_create(shellgpu_ctx *ctx, shellgpu_factor **out) {
 void shellgpu_factor_destroy(shellgpu_factor *F) {
   if (!F) return;
   (void)hipStreamSynchronize(F->ctx->stream);
+  if (F->solve_graph) (void)hipGraphExecDestroy(F->solve_graph);
   void *bufs[] = {F->buf,      F->tmp,       F->info,      F->d_bcflag,  F->d_pos2dof, F->d_blk_base,
                   F->d_blk_ld, F->vb,        F->vz,        F->vx,        F->part,      F->d_mlev,
                   F->d_ptr,    F->cr_info,   F->d_cp_rowp, F->d_cp_col,  F->d_ct_rowp, F->d_ct_src,
@@ -1572,6 +1575,42 @@ namespace {
 // of the factor at offset e n of b and x).  The sequential factorisation
 // solves them together, reading each level inverse once per solve; cyclic
 // reduction solves them one after the other.
+// The single-vector solve body (level-ordered vb -> vx) launches kernels
+// only, on buffers fixed for the factor's life (until a multi-vector solve
+// regrows them), with the factor values read at run time: it is captured
+// into a HIP graph at its first use and replayed, which removes the host
+// cost of its hundreds of launches per solve.  SHELLGPU_GRAPH=0 launches it
+// directly.
+int solve_body(shellgpu_factor *F) { return F->cr ? apply_cr(F) : apply_schur(F, 1); }
+
+int run_solve_body(shellgpu_factor *F) {
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->solve_graph && !F->graph_off) {
+    const char *env = getenv("SHELLGPU_GRAPH");
+    if (env && strcmp(env, "0") == 0) {
+      F->graph_off = true;
+    } else {
+      hipGraph_t g = nullptr;
+      HIPCALL(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = solve_body(F);
+      const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc != 0 || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : ferr(ctx, 2, std::string("solve capture: ") + hipGetErrorString(e));
+      }
+      const hipError_t ei = hipGraphInstantiate(&F->solve_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        F->solve_graph = nullptr;
+        return ferr(ctx, 2, std::string("solve graph: ") + hipGetErrorString(ei));
+      }
+    }
+  }
+  if (F->graph_off) return solve_body(F);
+  HIPCALL(ctx, hipGraphLaunch(F->solve_graph, ctx->stream));
+  return 0;
+}
+
 int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
   const int nb = F->nbatch;
@@ -1584,7 +1623,7 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
       double *xr = x + (size_t)r * nb * n;
       hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, br, F->vb, 1);
       HIPCALL(ctx, hipGetLastError());
-      if ((rc = F->cr ? apply_cr(F) : apply_schur(F, 1))) return rc;
+      if ((rc = run_solve_body(F))) return rc;
       hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, xr, 0);
       HIPCALL(ctx, hipGetLastError());
     }
@@ -1593,6 +1632,10 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
   if (nrhs > kMaxRhs) return ferr(ctx, 1, "too many right-hand sides for one solve");
   if (nrhs > F->rhs_cap) {
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    if (F->solve_graph) {  // captured on the buffers about to move
+      HIPCALL(ctx, hipGraphExecDestroy(F->solve_graph));
+      F->solve_graph = nullptr;
+    }
     double **bufs[] = {&F->vb, &F->vz, &F->vx};
     for (double **p : bufs) {
       HIPCALL(ctx, hipFree(*p));
