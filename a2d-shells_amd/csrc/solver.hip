@@ -1577,17 +1577,18 @@ namespace {
 // reduction solves them one after the other.
 // The single-vector solve body (level-ordered vb -> vx) launches kernels
 // only, on buffers fixed for the factor's life (until a multi-vector solve
-// regrows them), with the factor values read at run time: it is captured
-// into a HIP graph at its first use and replayed, which removes the host
-// cost of its hundreds of launches per solve.  SHELLGPU_GRAPH=0 launches it
-// directly.
+// regrows them), with the factor values read at run time.  With
+// SHELLGPU_GRAPH=1 it is captured into a HIP graph at its first use and
+// replayed.  It is opt-in: the solves are bound by the GPU, not by their
+// launches (no wall-time change measured), and rocprofv3's kernel tracing
+// crashes in the capture.
 int solve_body(shellgpu_factor *F) { return F->cr ? apply_cr(F) : apply_schur(F, 1); }
 
 int run_solve_body(shellgpu_factor *F) {
   shellgpu_ctx *ctx = F->ctx;
   if (!F->solve_graph && !F->graph_off) {
     const char *env = getenv("SHELLGPU_GRAPH");
-    if (env && strcmp(env, "0") == 0) {
+    if (!env || strcmp(env, "1") != 0) {
       F->graph_off = true;
     } else {
       hipGraph_t g = nullptr;

@@ -117,9 +117,9 @@ def test_batched_designs_match_single_designs(factor, monkeypatch):
 
 @pytest.mark.parametrize("factor", ["cr", "thomas"])
 def test_solve_graph_replays_match_direct_launches(factor, monkeypatch):
-    """The captured solve graph, replayed with eager work between replays and
-    after a refactorisation with new values, gives bit-identical results to
-    the same solve launched kernel by kernel (SHELLGPU_GRAPH=0)."""
+    """The captured solve graph (SHELLGPU_GRAPH=1), replayed with eager work
+    between replays and after a refactorisation with new values, gives
+    bit-identical results to the same solve launched kernel by kernel."""
     set_factor(monkeypatch, factor)
     cyl = Cylinder(10, 20)
     rng = np.random.default_rng(11)
@@ -127,12 +127,15 @@ def test_solve_graph_replays_match_direct_launches(factor, monkeypatch):
     ctx = make_ctx(cyl, thick)
     K = ctx.matrix()
     ctx.assemble_mattype(sg.STIFFNESS, K)
-    Fg = ctx.factor()  # graph (default)
-    Fg.compute(K)
-    monkeypatch.setenv("SHELLGPU_GRAPH", "0")
-    Fd = ctx.factor()  # direct launches
-    Fd.compute(K)
     n = 2 * 6 * cyl.nnodes
+    # a factor decides at its first solve: capture (graph) or direct launches
+    monkeypatch.setenv("SHELLGPU_GRAPH", "1")
+    Fg = ctx.factor()
+    Fg.compute(K)
+    Fg.apply(ctx.vector(rng.standard_normal(n)), ctx.vector())  # captures the graph
+    monkeypatch.setenv("SHELLGPU_GRAPH", "0")
+    Fd = ctx.factor()
+    Fd.compute(K)
     for rep in range(6):
         if rep == 3:  # new values in the same buffers: the graph reads them at run time
             ctx.set_designs(thick * 1.1)
