@@ -1912,8 +1912,9 @@ struct LanczosOut {
 // matrix M and A = K - sigma M (generalised shift-invert, M inner product).
 // Ritz value theta relates to the eigenvalue by lambda = -theta sigma /
 // (1 - theta) (mode 0), lambda = 1 / theta (mode 1) or lambda = 1 / theta +
-// sigma (mode 2).  Stops when the nev wanted Ritz values have relative
-// residual <= tol.
+// sigma (mode 2).  Mode 3 is mode 1 with the wanted values the nev smallest
+// |lambda| of either sign (mode 1 wants the smallest positive loads).  Stops
+// when the nev wanted Ritz values have relative residual <= tol.
 int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
             const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
             double *work, LanczosOut &lo) {
@@ -1989,6 +1990,7 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
         for (int i = 0; i < ns; i++) idx[i] = i;
         auto key = [&](int i) {
           if (mode == 2) return std::fabs(1.0 / w[i] + sigma[d]);
+          if (mode == 3) return w[i] != 0.0 ? std::fabs(1.0 / w[i]) : 1e300;
           return mode == 0 ? std::fabs(-w[i] * sigma[d] / (1.0 - w[i])) : (w[i] > 0 ? 1.0 / w[i] : 1e300);
         };
         std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
@@ -2268,6 +2270,7 @@ int lanczos_block(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellg
         auto key = [&](int i) {
           const double th = w[d][i];
           if (mode == 2) return std::fabs(1.0 / th + sigma[d]);
+          if (mode == 3) return th != 0.0 ? std::fabs(1.0 / th) : 1e300;
           return mode == 0 ? std::fabs(-th * sigma[d] / (1.0 - th)) : (th > 0 ? 1.0 / th : 1e300);
         };
         std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
@@ -2538,49 +2541,85 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   if ((rc = shellgpu_vec_set_bcs(ctx, B->u))) return rc;
   if ((rc = shellgpu_assemble_mattype(ctx, SHELLGPU_GEOMETRIC, B->u, B->G))) return rc;
 
-  // shift: sigma > 0 as given; sigma <= 0 places it at 0.98 of a lower
-  // bound of the first load from a short buckling-mode Lanczos on
-  // K^{-1}(-G): its largest Ritz value theta (residual r) lies within r of
-  // the largest eigenvalue 1 / lambda_1, so lambda_1 >= 1 / (theta + r).
-  // Closer shifts converge the main run in fewer steps (45 against 65 at
-  // 0.95 on the 200 x 400 cylinder); a shift above lambda_1 is caught by
-  // the Cholesky and lowered.
-  if (B->sigma <= 0.0) {
-    if ((rc = run_lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-2, B->Q, B->KQ, B->hv, B->res,
-                          lo, true)))
+  // Load cases of one structure (identical designs, one shared factor of K)
+  // with no shift given first run buckling mode on that factor, w =
+  // K^{-1}(-G q): it needs no factorisation per load case, where
+  // shift-invert factors K + sigma G for every case.  Its spectrum near the
+  // wanted loads is less separated, so it can run out of steps (clustered
+  // cylinder loads do); a design with an unconverged wanted pair sends the
+  // whole batch down the shift-invert path, with the shift taken from this
+  // run's Ritz values.
+  bool buck = false, have_shift = false;
+  if (B->sigma <= 0.0 && nb > 1 && B->FK != B->F && B->FK->nbatch == 1) {
+    if ((rc = run_lanczos(ctx, B->FK, B->K, B->G, 3, B->sig, nev, m, B->tol, B->Q, B->KQ, B->hv, B->res, lo, true)))
       return rc;
     std::vector<std::vector<double>> w, z, res;
     if ((rc = proj_eigs(ctx, lo, lo.steps, w, z, res))) return rc;
-    for (int d = 0; d < nb; d++) {
-      const int i = (int)(std::max_element(w[d].begin(), w[d].end()) - w[d].begin());
-      const double tmax = w[d][i];
-      if (!(tmax > 0.0)) return ferr(ctx, 7, "no positive buckling load under this load case");
-      B->sig[d] = 0.98 / (tmax + res[d][i]);
+    buck = lo.steps >= nev;
+    for (int d = 0; d < nb && buck; d++) {
+      std::vector<int> idx(lo.steps);
+      for (int i = 0; i < lo.steps; i++) idx[i] = i;
+      auto key = [&](int i) { return w[d][i] != 0.0 ? std::fabs(1.0 / w[d][i]) : 1e300; };
+      std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+      for (int j = 0; j < nev; j++)
+        if (!(res[d][idx[j]] <= B->tol * std::fabs(w[d][idx[j]]))) buck = false;
     }
-  }
-  // A = K + sigma G with BCs; factor, lowering the shift of any design whose
-  // shifted matrix is indefinite (shift above its first load)
-  for (int attempt = 0;; attempt++) {
-    HIPCALL(ctx, hipMemcpyAsync(B->dsig, B->sig.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
-    {
-      const long nv = 36L * ctx->nnzb;
-      hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
-                         shellgpu_mat_values(B->K), shellgpu_mat_values(B->G), B->dsig, shellgpu_mat_values(B->A));
-      HIPCALL(ctx, hipGetLastError());
+    if (!buck) {
+      for (int d = 0; d < nb; d++) {
+        const int i = (int)(std::max_element(w[d].begin(), w[d].end()) - w[d].begin());
+        if (!(w[d][i] > 0.0)) return ferr(ctx, 7, "no positive buckling load under this load case");
+        B->sig[d] = 0.98 / (w[d][i] + res[d][i]);
+      }
+      have_shift = true;
     }
-    if ((rc = shellgpu_mat_apply_bcs(B->A, 1.0))) return rc;
-    if (B->FK == B->F) B->fk_valid = false;
-    rc = shellgpu_factor_compute(B->F, B->A);
-    if (rc == 0) break;
-    if (rc != 5 || attempt >= 8) return rc;
-    std::vector<int> info(nb);
-    HIPCALL(ctx, hipMemcpy(info.data(), B->F->info, sizeof(int) * nb, hipMemcpyDeviceToHost));
-    for (int d = 0; d < nb; d++)
-      if (info[d] != 0) B->sig[d] *= (B->sigma <= 0.0 ? 0.8 : 0.5);
   }
 
-  if ((rc = run_lanczos(ctx, B->F, B->K, B->G, 0, B->sig, nev, m, B->tol, B->Q, B->KQ, B->hv, B->res, lo)))
-    return rc;
+  if (!buck) {
+    // shift: sigma > 0 as given; sigma <= 0 places it at 0.98 of a lower
+    // bound of the first load from a short buckling-mode Lanczos on
+    // K^{-1}(-G): its largest Ritz value theta (residual r) lies within r
+    // of the largest eigenvalue 1 / lambda_1, so lambda_1 >= 1 / (theta +
+    // r).  Closer shifts converge the main run in fewer steps (45 against
+    // 65 at 0.95 on the 200 x 400 cylinder); a shift above lambda_1 is
+    // caught by the Cholesky and lowered.
+    if (B->sigma <= 0.0 && !have_shift) {
+      if ((rc = run_lanczos(ctx, B->FK, B->K, B->G, 1, B->sig, 1, std::min(m, 40), 1e-2, B->Q, B->KQ, B->hv,
+                            B->res, lo, true)))
+        return rc;
+      std::vector<std::vector<double>> w, z, res;
+      if ((rc = proj_eigs(ctx, lo, lo.steps, w, z, res))) return rc;
+      for (int d = 0; d < nb; d++) {
+        const int i = (int)(std::max_element(w[d].begin(), w[d].end()) - w[d].begin());
+        const double tmax = w[d][i];
+        if (!(tmax > 0.0)) return ferr(ctx, 7, "no positive buckling load under this load case");
+        B->sig[d] = 0.98 / (tmax + res[d][i]);
+      }
+    }
+    // A = K + sigma G with BCs; factor, lowering the shift of any design
+    // whose shifted matrix is indefinite (shift above its first load)
+    for (int attempt = 0;; attempt++) {
+      HIPCALL(ctx, hipMemcpyAsync(B->dsig, B->sig.data(), sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+      {
+        const long nv = 36L * ctx->nnzb;
+        hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
+                           shellgpu_mat_values(B->K), shellgpu_mat_values(B->G), B->dsig, shellgpu_mat_values(B->A));
+        HIPCALL(ctx, hipGetLastError());
+      }
+      if ((rc = shellgpu_mat_apply_bcs(B->A, 1.0))) return rc;
+      if (B->FK == B->F) B->fk_valid = false;
+      rc = shellgpu_factor_compute(B->F, B->A);
+      if (rc == 0) break;
+      if (rc != 5 || attempt >= 8) return rc;
+      std::vector<int> info(nb);
+      HIPCALL(ctx, hipMemcpy(info.data(), B->F->info, sizeof(int) * nb, hipMemcpyDeviceToHost));
+      for (int d = 0; d < nb; d++)
+        if (info[d] != 0) B->sig[d] *= (B->sigma <= 0.0 ? 0.8 : 0.5);
+    }
+    if ((rc = run_lanczos(ctx, B->F, B->K, B->G, 0, B->sig, nev, m, B->tol, B->Q, B->KQ, B->hv, B->res, lo)))
+      return rc;
+  } else {
+    B->sig.assign(nb, 0.0);  // reported shift 0: buckling mode
+  }
   B->steps = lo.steps;
   B->lam.assign(nb, {});
   B->err.assign(nb, {});
@@ -2595,7 +2634,9 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
     const double sd = B->sig[d];
-    auto lam_of = [&](int i) { return -w[i] * sd / (1.0 - w[i]); };
+    // theta = 1 / lambda (buckling mode), lambda / (lambda - sigma)
+    // (shift-invert)
+    auto lam_of = [&](int i) { return buck ? 1.0 / w[i] : -w[i] * sd / (1.0 - w[i]); };
     // sortEigenvalues, SMALLEST_MAGNITUDE (src/bpmat/GSEP.cpp:693-748)
     std::sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(lam_of(a)) < std::fabs(lam_of(b)); });
     B->ns[d] = ns;

@@ -220,7 +220,10 @@ int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, in
  * max_lanczos steps at most, stopping when the nev eigenvalues of smallest
  * magnitude have relative Ritz residual <= tol.  sigma > 0 is the shift as
  * in the reference (halved while K + sigma G is indefinite); sigma <= 0
- * selects 0.95 x a first-load estimate per design.  Outputs:
+ * selects 0.98 x a lower bound of the first positive load per design, or,
+ * for identical designs (load cases of one structure, nbatch > 1), first
+ * tries buckling-mode Lanczos on their shared factor of K and reports
+ * sigma_used 0 when it converges.  Outputs:
  * lambda[nbatch*nev] (host), phi (device, nbatch*nev*6*nnodes,
  * K-normalised; may be NULL), path (device, nbatch*6*nnodes; may be NULL),
  * Lanczos steps, relative Ritz residuals resid[nbatch*nev] (host, may be
@@ -264,7 +267,8 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
  * may be NULL). */
 int shellgpu_buckling_extract(shellgpu_buckling *B, int n, double *lambda, double *phi,
                               double *error);
-/* Lanczos steps taken, shift per design (host, nbatch), path (device,
+/* Lanczos steps taken, shift per design (host, nbatch; 0 when the
+ * buckling-mode run on a shared factor of K converged), path (device,
  * nbatch*6*nnodes); any may be NULL. */
 int shellgpu_buckling_info(shellgpu_buckling *B, int *nsteps, double *sigma_used,
                            double *path);
