@@ -611,7 +611,9 @@ template <int NR, int CS>
 __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
                                                                long sAd, const double *__restrict__ x, long sxe,
                                                                int nbf, double *__restrict__ part) {
-  __shared__ double2 red[4][NR][64];
+  // the 4 waves' sums are added through LDS, 8 vectors at a time
+  constexpr int RH = NR < 8 ? NR : 8;
+  __shared__ double2 red[4][RH][64];
   const int tile = blockIdx.x / CS, slab = blockIdx.x - tile * CS;
   const int I = tile / nT, J = tile - I * nT, d = blockIdx.y, ns = nT * CS;
   A += d * sAd;
@@ -635,27 +637,45 @@ __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, co
     }
   }
 #pragma unroll
-  for (int r = 0; r < NR; r++) red[wave][r][lane] = acc[r];
-  __syncthreads();
-  for (int q = threadIdx.x; q < 64 * NR; q += 256) {
-    const int ln = q & 63, r = q >> 6;
-    const double2 a0 = red[0][r][ln], a1 = red[1][r][ln], a2 = red[2][r][ln], a3 = red[3][r][ln];
-    const long e = (long)r * nbf + d;
-    *reinterpret_cast<double2 *>(part + ((e * nT + I) * ns + J * CS + slab) * kSymvTile + 2 * ln) =
-        make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+  for (int h = 0; h < NR; h += RH) {
+    if (h > 0) __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RH; r++) red[wave][r][lane] = acc[h + r];
+    __syncthreads();
+    for (int q = threadIdx.x; q < 64 * RH; q += 256) {
+      const int ln = q & 63, r = q >> 6;
+      const double2 a0 = red[0][r][ln], a1 = red[1][r][ln], a2 = red[2][r][ln], a3 = red[3][r][ln];
+      const long e = (long)(h + r) * nbf + d;
+      *reinterpret_cast<double2 *>(part + ((e * nT + I) * ns + J * CS + slab) * kSymvTile + 2 * ln) =
+          make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+    }
   }
 }
 
-// Pass 2: y = beta c + alpha sum over the ns slots of the row's block
-__global__ void symv_pass2_kernel(int m, int nT, int ns, const double *__restrict__ part, double alpha,
-                                  const double *__restrict__ c, double beta, double *__restrict__ y, long svd) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= m) return;
-  const int d = blockIdx.y;
-  const double *p = part + ((long)d * nT + row / kSymvTile) * ns * kSymvTile + row % kSymvTile;
+// Pass 2: y = beta c + alpha (sum over the ns slots of the row's tile).
+// 64 rows per workgroup; the slots are dealt round-robin to G waves, whose
+// sums are added in LDS in a fixed order, so the result does not depend on
+// timing.  Spreading the slots keeps a launch of few rows (one factored
+// design, many slots) from being a chain of dependent loads per thread.
+template <int G>
+__global__ __launch_bounds__(64 * G) void symv_pass2_kernel(int m, int nT, int ns, const double *__restrict__ part,
+                                                            double alpha, const double *__restrict__ c, double beta,
+                                                            double *__restrict__ y, long svd) {
+  __shared__ double red[G][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + lane, d = blockIdx.y;
   double s = 0.0;
-  for (int J = 0; J < ns; J++) s += p[J * kSymvTile];
-  y[d * svd + row] = (beta == 0.0 ? 0.0 : beta * c[d * svd + row]) + alpha * s;
+  if (row < m) {
+    const double *p = part + ((long)d * nT + row / kSymvTile) * ns * kSymvTile + row % kSymvTile;
+    for (int J = g; J < ns; J += G) s += p[(long)J * kSymvTile];
+  }
+  if (G > 1) {
+    red[g][lane] = s;
+    __syncthreads();
+    if (g != 0) return;
+    for (int k = 1; k < G; k++) s += red[k][lane];
+  }
+  if (row < m) y[(long)d * svd + row] = (beta == 0.0 ? 0.0 : beta * c[(long)d * svd + row]) + alpha * s;
 }
 
 // identity on the padded diagonal entries of the cyclic-reduction levels
@@ -1281,8 +1301,11 @@ static int gemv_batch(shellgpu_factor *F, bool trans, int m, int n, const double
 // y = beta c + alpha P x for nrhs vectors of every design (symmetric P of
 // level l of the sequential factorisation; vector entries e = r nb + d of
 // stride nv): one vector per design reads half of P, several read it in
-// full once for all
-constexpr int kMaxRhs = 8;
+// full once for all (nrhs <= 8 or 16)
+constexpr int kMaxRhs = 16;
+// vectors per pass when a factor of one design solves n vectors: 16 while
+// there are that many (P read once for 16), else up to 8
+inline int rhs_chunk(int n) { return n >= kMaxRhs ? kMaxRhs : std::min(8, n); }
 static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const double *c, double beta, double *y,
                 int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
@@ -1312,13 +1335,26 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
       SYMV_MULTI(6)
       SYMV_MULTI(7)
       SYMV_MULTI(8)
+      SYMV_MULTI(16)
 #undef SYMV_MULTI
     default:
       return ferr(ctx, 1, "too many right-hand sides for one level product");
   }
   HIPCALL(ctx, hipGetLastError());
-  hipLaunchKernelGGL(symv_pass2_kernel, dim3((m + 255) / 256, nrhs * nb), dim3(256), 0, ctx->stream, m, nT, nT * cs,
-                     F->part, alpha, c, beta, y, nv);
+  // about 4 slots per thread
+  const int ns = nT * cs;
+  const dim3 g2((m + 63) / 64, nrhs * nb);
+  if (ns >= 64)
+    hipLaunchKernelGGL(symv_pass2_kernel<16>, g2, dim3(1024), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y,
+                       nv);
+  else if (ns >= 32)
+    hipLaunchKernelGGL(symv_pass2_kernel<8>, g2, dim3(512), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+  else if (ns >= 16)
+    hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+  else if (ns >= 8)
+    hipLaunchKernelGGL(symv_pass2_kernel<2>, g2, dim3(128), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+  else
+    hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
@@ -1638,7 +1674,7 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
     }
     return 0;
   }
-  if (nrhs > kMaxRhs) return ferr(ctx, 1, "too many right-hand sides for one solve");
+  if (nrhs > kMaxRhs || (nrhs > 8 && nrhs != kMaxRhs)) return ferr(ctx, 1, "unsupported number of right-hand sides");
   if (nrhs > F->rhs_cap) {
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
     if (F->solve_graph) {  // captured on the buffers about to move
@@ -1679,16 +1715,19 @@ extern "C" {
 
 // x = A^{-1} b for every design of the context.  A factor of one design
 // shared by all (identical designs: load cases of one structure) solves the
-// designs' vectors as right-hand sides of that design, kMaxRhs at a time.
+// designs' vectors as right-hand sides of that design, 16 (or at most 8)
+// at a time.
 int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
   shellgpu_ctx *ctx = F->ctx;
   if (!F->factored) return ferr(ctx, 1, "factor not computed");
   if (F->nbatch == ctx->nbatch) return apply_block(F, b, x, 1);
   if (F->nbatch != 1) return ferr(ctx, 1, "factor batch size does not match the context's designs");
   const long n = F->n;
-  for (int r0 = 0; r0 < ctx->nbatch; r0 += kMaxRhs) {
-    const int rc = apply_block(F, b + (size_t)r0 * n, x + (size_t)r0 * n, std::min(kMaxRhs, ctx->nbatch - r0));
+  for (int r0 = 0; r0 < ctx->nbatch;) {
+    const int nr = rhs_chunk(ctx->nbatch - r0);
+    const int rc = apply_block(F, b + (size_t)r0 * n, x + (size_t)r0 * n, nr);
     if (rc) return rc;
+    r0 += nr;
   }
   return 0;
 }
@@ -2302,7 +2341,7 @@ int lanczos_block_size(const shellgpu_factor *F) {
   if (F->cr || F->nbatch != F->ctx->nbatch) return 1;
   int bs = 1;
   if (const char *env = getenv("SHELLGPU_LANCZOS_BLOCK")) bs = atoi(env);
-  return std::max(1, std::min(kMaxRhs, bs));
+  return std::max(1, std::min(8, bs));
 }
 
 int run_lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
