@@ -605,40 +605,51 @@ __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const do
 // columns (blockIdx.x = (I nT + J) CS + slab), designs blockIdx.y; the
 // partial sums of block row I go to slot J CS + slab of nT CS slots.
 // Splitting keeps the launch wider than the GPU when one design is
-// factored.
-constexpr int kMaxSplit = 4;
+// factored.  The slab's NR x (128 / CS) vector entries are staged in LDS
+// first (read there as wave-uniform broadcasts); columns past m read the
+// last column against zero entries, so every wave issues all its loads
+// without a bound test.
+constexpr int kMaxSplit = 2;
 template <int NR, int CS>
 __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
                                                                long sAd, const double *__restrict__ x, long sxe,
                                                                int nbf, double *__restrict__ part) {
-  // the 4 waves' sums are added through LDS, 8 vectors at a time
-  constexpr int RH = NR < 8 ? NR : 8;
-  __shared__ double2 red[4][RH][64];
+  // the 4 waves' sums are added through LDS, 8 vectors at a time, in the
+  // space that held the staged entries
+  constexpr int RH = NR < 8 ? NR : 8, NC = kSymvTile / CS;
+  constexpr int kRed = 4 * RH * 64 * 2, kX = NC * NR;
+  __shared__ double lds[kRed > kX ? kRed : kX];
+  double2(*red)[RH][64] = reinterpret_cast<double2(*)[RH][64]>(lds);
   const int tile = blockIdx.x / CS, slab = blockIdx.x - tile * CS;
   const int I = tile / nT, J = tile - I * nT, d = blockIdx.y, ns = nT * CS;
   A += d * sAd;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int row = I * kSymvTile + 2 * lane;
   const bool rv = row < m;
+  const int cbase = J * kSymvTile + slab * NC;
+  for (int q = threadIdx.x; q < kX; q += 256) {
+    const int c = q % NC, r = q / NC;
+    lds[c * NR + r] = cbase + c < m ? x[((long)r * nbf + d) * sxe + cbase + c] : 0.0;
+  }
+  __syncthreads();
   double2 acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) acc[r] = make_double2(0.0, 0.0);
-  const int cbase = J * kSymvTile + slab * (kSymvTile / CS);
-#pragma unroll 4
-  for (int p = 0; p < kSymvTile / (4 * CS); p++) {
-    const int col = cbase + wave + 4 * p;
-    if (col >= m) break;
+#pragma unroll 2
+  for (int p = 0; p < NC / 4; p++) {
+    const int c = wave + 4 * p;
+    const int col = min(cbase + c, m - 1);
     const double2 v = rv ? load_stream2(A + row + (long)col * lda) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      const double xc = x[((long)r * nbf + d) * sxe + col];
+      const double xc = lds[c * NR + r];
       acc[r].x += v.x * xc;
       acc[r].y += v.y * xc;
     }
   }
 #pragma unroll
   for (int h = 0; h < NR; h += RH) {
-    if (h > 0) __syncthreads();
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < RH; r++) red[wave][r][lane] = acc[h + r];
     __syncthreads();
