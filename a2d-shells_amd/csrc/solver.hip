@@ -635,16 +635,28 @@ __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, co
   double2 acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) acc[r] = make_double2(0.0, 0.0);
-#pragma unroll 2
-  for (int p = 0; p < NC / 4; p++) {
-    const int c = wave + 4 * p;
-    const int col = min(cbase + c, m - 1);
-    const double2 v = rv ? load_stream2(A + row + (long)col * lda) : make_double2(0.0, 0.0);
+  // a wave's columns w, w + 4, ...: CH loads issued before their products
+  constexpr int CPW = NC / 4, CH = CPW < 8 ? CPW : 8;
+#pragma unroll 1
+  for (int p0 = 0; p0 < CPW; p0 += CH) {
+    double2 v[CH];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-      const double xc = lds[c * NR + r];
-      acc[r].x += v.x * xc;
-      acc[r].y += v.y * xc;
+    for (int q = 0; q < CH; q++) {
+      const int col = min(cbase + wave + 4 * (p0 + q), m - 1);
+      v[q] = rv ? load_stream2(A + row + (long)col * lda) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int c = wave + 4 * (p0 + q);
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const double xc = lds[c * NR + r];
+        acc[r].x += v[q].x * xc;
+        acc[r].y += v[q].y * xc;
+      }
+      // keep each column's LDS reads in its own iteration (hoisted, they
+      // would hold every column's entries in registers at once)
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 #pragma unroll
