@@ -877,9 +877,14 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   constexpr bool kEarly = (MODE == kModeKNl || MODE == kModeG);
   double prev[18];
   double *dst = nullptr;
+  // first touch of the block in this assembly: written, not added to (the
+  // matrix is not zeroed first)
+  bool fresh = false;
   if (kMat && valid) {
-    const long blk = a.blk[16 * e + 4 * (row0 / 6) + col0 / 6];
+    const int slot = 4 * (row0 / 6) + col0 / 6;
+    const long blk = a.blk[16 * e + slot];
     dst = a.vals + design * a.vals_stride + 36 * blk + 6 * (row0 % 6);
+    fresh = a.first && ((a.first[e] >> slot) & 1);
     if (kEarly) {
 #pragma unroll
       for (int i = 0; i < 3; i++)
@@ -992,8 +997,12 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 6; j++)
-        dst[6 * i + j] = (kEarly ? prev[6 * i + j] : dst[6 * i + j]) + alpha * acc[6 * i + j];
+      for (int j = 0; j < 6; j++) {
+        // read even where the value is discarded: a load only for blocks
+        // already reached measured slower than loading them all
+        const double old = kEarly ? prev[6 * i + j] : dst[6 * i + j];
+        dst[6 * i + j] = (fresh ? 0.0 : old) + alpha * acc[6 * i + j];
+      }
   }
   if (kRes && valid && lane < 24) {
     double *dst = a.res + design * a.res_stride;

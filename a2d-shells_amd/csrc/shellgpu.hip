@@ -257,7 +257,7 @@ void shellgpu_destroy(shellgpu_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   void *bufs[] = {ctx->d_conn, ctx->d_rowp,     ctx->d_cols,    ctx->d_blk,
                   ctx->d_color_elems, ctx->d_X, ctx->d_bc_nodes, ctx->d_bc_mask,
-                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp, ctx->d_coef};
+                  ctx->d_bc_vals, ctx->d_thick, ctx->d_temp, ctx->d_coef, ctx->d_first};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   drop_events(ctx->elem_ev);
@@ -360,7 +360,31 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
     for (int e = 0; e < nelems; e++) ctx->h_color_elems[pos[ctx->h_color[e]]++] = e;
   }
 
+  // first-touch bits in colour launch order (elements of one colour share no
+  // node, so no block is reached twice within a colour)
+  std::vector<int> first(nelems, 0);
+  bool distinct = true;
+  for (int e = 0; e < nelems && distinct; e++)
+    for (int i = 0; i < 4; i++)
+      for (int j = i + 1; j < 4; j++) distinct = distinct && conn[4 * e + i] != conn[4 * e + j];
+  if (distinct) {
+    std::vector<char> seen(ctx->nnzb, 0);
+    for (int e : ctx->h_color_elems)
+      for (int k = 0; k < 16; k++) {
+        const int b = ctx->h_blk[16L * e + k];
+        if (!seen[b]) {
+          seen[b] = 1;
+          first[e] |= 1 << k;
+        }
+      }
+  }
+
   int rc = 0;
+  if (ctx->d_first) {
+    (void)hipFree(ctx->d_first);
+    ctx->d_first = nullptr;
+  }
+  if (distinct && (rc = upload(ctx, &ctx->d_first, first.data(), first.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_conn, conn, 4L * nelems))) return rc;
   if ((rc = upload(ctx, &ctx->d_X, Xpts, 3L * nnodes))) return rc;
   if ((rc = upload(ctx, &ctx->d_rowp, ctx->h_rowp.data(), ctx->h_rowp.size()))) return rc;
@@ -535,9 +559,16 @@ int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y) {
 int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, shellgpu_mat *A) {
   int rc = check_batch(ctx, A);
   if (rc) return rc;
-  HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC && mattype != SHELLGPU_GEOMETRIC_FD &&
+      mattype != SHELLGPU_TANGENT_NONLINEAR && mattype != SHELLGPU_MASS)
+    return set_err(ctx, 1, "unknown matrix type");
   ElemArgs a = base_args(ctx, u);
   a.vals = A->vals;
+  // the stiffness-type modes write each block at its first touch, which
+  // replaces zeroing the matrix; the mass kernel adds
+  a.first = mattype == SHELLGPU_MASS ? nullptr : ctx->d_first;
+  if (!a.first)
+    HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
   if (mattype == SHELLGPU_STIFFNESS) {
     if ((rc = run_colored(ctx, kModeKLin, a))) return rc;
   } else if (mattype == SHELLGPU_GEOMETRIC) {
@@ -546,13 +577,12 @@ int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, s
     a.fd_sign = 1;
     if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
     a.fd_sign = -1;
+    a.first = nullptr;  // the second difference adds to the first
     if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
   } else if (mattype == SHELLGPU_TANGENT_NONLINEAR) {
     if ((rc = run_colored(ctx, kModeKNl, a))) return rc;
-  } else if (mattype == SHELLGPU_MASS) {
-    if ((rc = run_colored(ctx, kModeMass, a))) return rc;
   } else {
-    return set_err(ctx, 1, "unknown matrix type");
+    if ((rc = run_colored(ctx, kModeMass, a))) return rc;
   }
   return shellgpu_mat_apply_bcs(A, 1.0);
 }
@@ -587,9 +617,11 @@ int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u, do
   int rc = check_batch(ctx, A);
   if (rc) return rc;
   if (R && (rc = shellgpu_assemble_res(ctx, model, u, R))) return rc;
-  HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
   ElemArgs a = base_args(ctx, u);
   a.vals = A->vals;
+  a.first = ctx->d_first;
+  if (!a.first)
+    HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
   if ((rc = run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeKNl : kModeKLin, a))) return rc;
   return shellgpu_mat_apply_bcs(A, 1.0);
 }

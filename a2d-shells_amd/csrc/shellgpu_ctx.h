@@ -17,6 +17,11 @@ struct shellgpu_ctx {
   std::vector<int> h_conn, h_rowp, h_cols, h_color, h_color_ptr, h_color_elems, h_blk;
   int *d_conn = nullptr, *d_rowp = nullptr, *d_cols = nullptr, *d_blk = nullptr;
   int *d_color_elems = nullptr;
+  // per element, bit 4 i + j: this element is the first, in colour launch
+  // order, to reach BSR block (node i, node j), so a matrix assembly can
+  // write that block instead of adding to a zeroed one; null when an element
+  // repeats a node (two of its blocks coincide)
+  int *d_first = nullptr;
   double *d_X = nullptr;
 
   shellgpu::Material mat = shellgpu::make_material(70e9, 0.3, 2700.0, 0.0, 10.0);

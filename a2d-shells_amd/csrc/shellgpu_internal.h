@@ -72,6 +72,9 @@ struct ElemArgs {
                  // geometric stiffness step (TACSShellElement.h:705-766), 0 = off
   double *vals;  // BSR values
   long vals_stride;
+  // matrix modes: per element the blocks it writes rather than adds to
+  // (first-touch bits, see shellgpu_ctx::d_first); null: add to every block
+  const int *first;
   double *res;
   long res_stride;
   // thickness-derivative modes (' = d/dt_e, all outputs accumulate):

@@ -135,3 +135,31 @@ def test_spmv_matches_oracle_and_batches():
         assert bsr_row_floor_ratio(rowp, vals[b], ref) < TOL_K
         yr = O.bsr_matvec(rowp, cols, vals[b], x.reshape(3, -1)[b])
         assert np.abs(y[b] - yr).max() <= 1e-13 * np.abs(yr).max()
+
+
+def test_assembly_overwrites_previous_values():
+    """Matrix assembly writes each block at its first touch instead of adding
+    to a zeroed matrix: into a matrix holding arbitrary values, every matrix
+    type (and the Jacobian entry point) gives bit for bit what it gives into
+    a fresh one, two designs at a time."""
+    conn, X = random_mesh(5, 4, 2)
+    bcs = (np.array([0, 1], np.int32), np.array([0x3F, 0x3F], np.int32))
+    rng = np.random.default_rng(17)
+    thick = rng.uniform(0.0015, 0.0025, (2, conn.shape[0]))
+    ctx = setup_ctx(conn, X, thick, bcs)
+    u = ctx.vector(1e-3 * rng.standard_normal(2 * 6 * X.shape[0]))
+    junk = 1e3 * rng.standard_normal((2, ctx.nnzb, 6, 6))
+    for mattype in (sg.STIFFNESS, sg.GEOMETRIC, sg.GEOMETRIC_FD, sg.TANGENT_NONLINEAR, sg.MASS):
+        fresh, used = ctx.matrix(), ctx.matrix()
+        used.set_values(junk)
+        ctx.assemble_mattype(mattype, fresh, u)
+        ctx.assemble_mattype(mattype, used, u)
+        ctx.assemble_mattype(mattype, used, u)  # twice: no accumulation
+        assert np.array_equal(fresh.values(), used.values()), mattype
+    for model in (sg.LINEAR, sg.NONLINEAR):
+        fresh, used = ctx.matrix(), ctx.matrix()
+        used.set_values(junk)
+        ctx.assemble_jacobian(fresh, u=u, model=model)
+        ctx.assemble_jacobian(used, u=u, model=model)
+        assert np.array_equal(fresh.values(), used.values()), model
+    ctx.close()
