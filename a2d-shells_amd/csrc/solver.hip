@@ -115,6 +115,10 @@ struct shellgpu_factor {
   int rhs_cap = 1;           // vectors per design that vb, vz, vx and part hold
   hipGraphExec_t solve_graph = nullptr;  // the single-vector solve body, captured
   bool graph_off = false;
+  // single-vector solves of the sequential form: only designs d_act[0 ..
+  // nact) when nact >= 0 (Lanczos stops solving for converged designs)
+  int *d_act = nullptr;
+  int nact = -1;
   bool factored = false;
   long nv = 0;               // level-ordered vector length per design
   // block cyclic reduction over uniform (padded) levels of M dofs: per
@@ -506,10 +510,10 @@ __global__ __launch_bounds__(256) void couple_right_kernel(int rows, int nnodes,
 // stride svd) uses the couplings of design e % nbf
 __global__ void couple_fwd_kernel(int nnodes, const int *__restrict__ rowp, const int *__restrict__ colx,
                                   const double *__restrict__ E, long sEd, const double *__restrict__ t,
-                                  double *__restrict__ z, long svd, int nbf) {
+                                  double *__restrict__ z, long svd, int nbf, const int *__restrict__ act) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= 6 * nnodes) return;
-  const int i = g / 6, a = g - 6 * i, e = blockIdx.y;
+  const int i = g / 6, a = g - 6 * i, e = act ? act[blockIdx.y] : blockIdx.y;
   E += (e % nbf) * sEd + a;
   t += e * svd;
   double s = 0.0;
@@ -526,10 +530,11 @@ __global__ void couple_fwd_kernel(int nnodes, const int *__restrict__ rowp, cons
 // one level (x: the next level's segment); vector entries as above
 __global__ void couple_bwd_kernel(int nnodes, const int *__restrict__ rowp, const int *__restrict__ src,
                                   const int *__restrict__ blk, const double *__restrict__ E, long sEd,
-                                  const double *__restrict__ x, double *__restrict__ y, long svd, int nbf) {
+                                  const double *__restrict__ x, double *__restrict__ y, long svd, int nbf,
+                                  const int *__restrict__ act) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= 6 * nnodes) return;
-  const int v = g / 6, r = g - 6 * v, e = blockIdx.y;
+  const int v = g / 6, r = g - 6 * v, e = act ? act[blockIdx.y] : blockIdx.y;
   E += (e % nbf) * sEd + 6 * r;
   x += e * svd;
   double s = 0.0;
@@ -554,11 +559,11 @@ __global__ void couple_bwd_kernel(int nnodes, const int *__restrict__ rowp, cons
 constexpr int kSymvTile = 128;
 __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
                                                          long sAd, const double *__restrict__ x, long sxd,
-                                                         double *__restrict__ part) {
+                                                         double *__restrict__ part, const int *__restrict__ act) {
   __shared__ double2 red[4][64];
   int I, J;
   tri_tile(blockIdx.x, I, J);
-  const int d = blockIdx.y;
+  const int d = act ? act[blockIdx.y] : blockIdx.y;
   A += d * sAd;
   x += d * sxd;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -683,10 +688,11 @@ __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, co
 template <int G>
 __global__ __launch_bounds__(64 * G) void symv_pass2_kernel(int m, int nT, int ns, const double *__restrict__ part,
                                                             double alpha, const double *__restrict__ c, double beta,
-                                                            double *__restrict__ y, long svd) {
+                                                            double *__restrict__ y, long svd,
+                                                            const int *__restrict__ act) {
   __shared__ double red[G][64];
   const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int row = blockIdx.x * 64 + lane, d = blockIdx.y;
+  const int row = blockIdx.x * 64 + lane, d = act ? act[blockIdx.y] : blockIdx.y;
   double s = 0.0;
   if (row < m) {
     const double *p = part + ((long)d * nT + row / kSymvTile) * ns * kSymvTile + row % kSymvTile;
@@ -711,13 +717,14 @@ __global__ void pad_identity_kernel(int M, long M2, const int *__restrict__ mlev
 }
 
 // level order <-> dof order; pos2dof[pos] < 0 marks a padding position
-// (zero in level order)
+// (zero in level order); vector blockIdx.y, or act[blockIdx.y]
 __global__ void permute_kernel(long nv, long n, const int *__restrict__ pos2dof, const double *__restrict__ src,
-                               double *__restrict__ dst, int to_level) {
+                               double *__restrict__ dst, int to_level, const int *__restrict__ act) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const int p = pos2dof[i];
-  const long lo = (long)blockIdx.y * nv, doff = (long)blockIdx.y * n;
+  const long v = act ? act[blockIdx.y] : blockIdx.y;
+  const long lo = v * nv, doff = v * n;
   if (to_level) {
     dst[lo + i] = p >= 0 ? src[doff + p] : 0.0;
   } else if (p >= 0) {
@@ -1043,12 +1050,17 @@ int ensure_blas(shellgpu_ctx *ctx) {
 }
 
 // ---- symmetric tridiagonal eigenproblem (implicit QL with Wilkinson
-// shifts, the EISPACK tql2 algorithm); z[k*n + i] = component i of vector k
+// shifts, the EISPACK tql2 algorithm); z[k*n + i] = component i of vector k.
+// last_only: z[k] = component n - 1 of vector k only (each component is
+// rotated independently, so the last one alone costs O(n^2): enough for
+// the Ritz residual estimates)
 int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<double> &w,
-               std::vector<double> &z) {
+               std::vector<double> &z, bool last_only = false) {
   e.resize(n + 1, 0.0);
-  z.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++) z[(size_t)i * n + i] = 1.0;
+  const int nz = last_only ? 1 : n, k0 = last_only ? n - 1 : 0;
+  z.assign((size_t)n * nz, 0.0);
+  for (int i = 0; i < n; i++)
+    if (!last_only || i == n - 1) z[(size_t)i * nz + i - k0] = 1.0;
   for (int l = 0; l < n; l++) {
     int iter = 0, mm;
     do {
@@ -1079,10 +1091,10 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
           p = s * r;
           d[i + 1] = g + p;
           g = c * r - b;
-          for (int k = 0; k < n; k++) {
-            const double t = z[(size_t)(i + 1) * n + k];
-            z[(size_t)(i + 1) * n + k] = s * z[(size_t)i * n + k] + c * t;
-            z[(size_t)i * n + k] = c * z[(size_t)i * n + k] - s * t;
+          for (int k = 0; k < nz; k++) {
+            const double t = z[(size_t)(i + 1) * nz + k];
+            z[(size_t)(i + 1) * nz + k] = s * z[(size_t)i * nz + k] + c * t;
+            z[(size_t)i * nz + k] = c * z[(size_t)i * nz + k] - s * t;
           }
         }
         if (r == 0.0 && i >= l) continue;
@@ -1200,6 +1212,7 @@ int factor_create(shellgpu_ctx *ctx, int nbatch, shellgpu_factor **out) {
     e = hipMalloc((void **)&F->tmp, sizeof(double) * 2 * std::max(1L, F->tmp_elems) * nb);
   if (e == hipSuccess && !F->cr) e = hipMalloc((void **)&F->wbuf, sizeof(double) * F->wstride * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->info, sizeof(int) * nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&F->d_act, sizeof(int) * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vb, sizeof(double) * (size_t)F->nv * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vz, sizeof(double) * (size_t)F->nv * nb);
   if (e == hipSuccess) e = hipMalloc((void **)&F->vx, sizeof(double) * (size_t)F->nv * nb);
@@ -1240,7 +1253,7 @@ void shellgpu_factor_destroy(shellgpu_factor *F) {
   void *bufs[] = {F->buf,      F->tmp,       F->info,      F->d_bcflag,  F->d_pos2dof, F->d_blk_base,
                   F->d_blk_ld, F->vb,        F->vz,        F->vx,        F->part,      F->d_mlev,
                   F->d_ptr,    F->cr_info,   F->d_cp_rowp, F->d_cp_col,  F->d_ct_rowp, F->d_ct_src,
-                  F->d_ct_blk, F->wbuf};
+                  F->d_ct_blk, F->wbuf,      F->d_act};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete F;
@@ -1335,6 +1348,8 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
   const int m = F->m[l], nT = (m + kSymvTile - 1) / kSymvTile, nb = F->nbatch;
   const double *P = F->buf + F->S_off[l];
   const long ps = F->per_design, nv = F->nv;
+  // active designs of a single-vector solve (Lanczos skipping converged ones)
+  const int *act = (nrhs == 1 && F->nact >= 0) ? F->d_act : nullptr;
   // column slabs per tile when the tiles alone would not fill the GPU
   const int cs = (nrhs > 1 && (long)nT * nT * nb < 512) ? kMaxSplit : 1;
   switch (nrhs * 8 + (cs > 1)) {
@@ -1348,8 +1363,8 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
                        ctx->stream, m, nT, P, m, ps, x, nv, nb, F->part);                                          \
     break;
     case 8:
-      hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, nb), dim3(256), 0, ctx->stream, m, nT, P, m, ps,
-                         x, nv, F->part);
+      hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, act ? F->nact : nb), dim3(256), 0, ctx->stream,
+                         m, nT, P, m, ps, x, nv, F->part, act);
       break;
       SYMV_MULTI(2)
       SYMV_MULTI(3)
@@ -1366,18 +1381,22 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
   HIPCALL(ctx, hipGetLastError());
   // about 4 slots per thread
   const int ns = nT * cs;
-  const dim3 g2((m + 63) / 64, nrhs * nb);
+  const dim3 g2((m + 63) / 64, act ? F->nact : nrhs * nb);
   if (ns >= 64)
     hipLaunchKernelGGL(symv_pass2_kernel<16>, g2, dim3(1024), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y,
-                       nv);
+                       nv, act);
   else if (ns >= 32)
-    hipLaunchKernelGGL(symv_pass2_kernel<8>, g2, dim3(512), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+    hipLaunchKernelGGL(symv_pass2_kernel<8>, g2, dim3(512), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+                       act);
   else if (ns >= 16)
-    hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+    hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+                       act);
   else if (ns >= 8)
-    hipLaunchKernelGGL(symv_pass2_kernel<2>, g2, dim3(128), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+    hipLaunchKernelGGL(symv_pass2_kernel<2>, g2, dim3(128), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+                       act);
   else
-    hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv);
+    hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+                       act);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
@@ -1495,7 +1514,9 @@ static int factor_schur(shellgpu_factor *F) {
 // design (level-ordered vb -> vx, entries e = r nb + d)
 static int apply_schur(shellgpu_factor *F, int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
-  const int nb = F->nbatch, nl = F->nlevels, ne = nrhs * nb;
+  const int nb = F->nbatch, nl = F->nlevels;
+  const int *act = (nrhs == 1 && F->nact >= 0) ? F->d_act : nullptr;
+  const int ne = act ? F->nact : nrhs * nb;
   const long ps = F->per_design, nv = F->nv;
   const double *E = F->buf + F->E_off;
   int rc = 0;
@@ -1505,7 +1526,7 @@ static int apply_schur(shellgpu_factor *F, int nrhs) {
     if (l > 0) {
       const int nn = F->m[l] / 6;
       hipLaunchKernelGGL(couple_fwd_kernel, dim3((6 * nn + 255) / 256, ne), dim3(256), 0, ctx->stream, nn,
-                         F->d_cp_rowp + F->noff[l], F->d_cp_col, E, ps, F->vz + F->voff[l - 1], zl, nv, nb);
+                         F->d_cp_rowp + F->noff[l], F->d_cp_col, E, ps, F->vz + F->voff[l - 1], zl, nv, nb, act);
       HIPCALL(ctx, hipGetLastError());
     }
     if ((rc = symv(F, l, zl, 1.0, nullptr, 0.0, (l == nl - 1 ? F->vx : F->vz) + F->voff[l], nrhs))) return rc;
@@ -1515,7 +1536,8 @@ static int apply_schur(shellgpu_factor *F, int nrhs) {
     const int nn = F->m[l] / 6;
     double *yl = F->vb + F->voff[l];
     hipLaunchKernelGGL(couple_bwd_kernel, dim3((6 * nn + 255) / 256, ne), dim3(256), 0, ctx->stream, nn,
-                       F->d_ct_rowp + F->noff[l], F->d_ct_src, F->d_ct_blk, E, ps, F->vx + F->voff[l + 1], yl, nv, nb);
+                       F->d_ct_rowp + F->noff[l], F->d_ct_src, F->d_ct_blk, E, ps, F->vx + F->voff[l + 1], yl, nv, nb,
+                       act);
     HIPCALL(ctx, hipGetLastError());
     if ((rc = symv(F, l, yl, -1.0, F->vz + F->voff[l], 1.0, F->vx + F->voff[l], nrhs))) return rc;
   }
@@ -1683,16 +1705,21 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
   shellgpu_ctx *ctx = F->ctx;
   const int nb = F->nbatch;
   const long nv = F->nv, n = F->n;
-  const dim3 pgrid((unsigned)((nv + 255) / 256), nb);
+  // a Lanczos run may restrict single-vector solves of the sequential form
+  // to the designs it has not converged yet (the others keep x unchanged)
+  const int *act = (!F->cr && nrhs == 1 && F->nact >= 0) ? F->d_act : nullptr;
+  if (act && F->nact == 0) return 0;
+  const dim3 pgrid((unsigned)((nv + 255) / 256), act ? F->nact : nb);
   int rc = 0;
   if (F->cr || nrhs == 1) {
     for (int r = 0; r < nrhs; r++) {
       const double *br = b + (size_t)r * nb * n;
       double *xr = x + (size_t)r * nb * n;
-      hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, br, F->vb, 1);
+      hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, br, F->vb, 1, act);
       HIPCALL(ctx, hipGetLastError());
-      if ((rc = run_solve_body(F))) return rc;
-      hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, xr, 0);
+      // the captured body covers every design
+      if ((rc = act ? solve_body(F) : run_solve_body(F))) return rc;
+      hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, xr, 0, act);
       HIPCALL(ctx, hipGetLastError());
     }
     return 0;
@@ -1718,10 +1745,10 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
     F->rhs_cap = nrhs;
   }
   const dim3 mgrid((unsigned)((nv + 255) / 256), nrhs * nb);
-  hipLaunchKernelGGL(permute_kernel, mgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, b, F->vb, 1);
+  hipLaunchKernelGGL(permute_kernel, mgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, b, F->vb, 1, nullptr);
   HIPCALL(ctx, hipGetLastError());
   if ((rc = apply_schur(F, nrhs))) return rc;
-  hipLaunchKernelGGL(permute_kernel, mgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, x, 0);
+  hipLaunchKernelGGL(permute_kernel, mgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, x, 0, nullptr);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
@@ -1960,6 +1987,9 @@ __global__ void shifted_kernel(long n, const double *K, const double *G, const d
 struct LanczosOut {
   int bs = 1;     // block size
   int steps = 0;  // basis vectors
+  // block size 1: basis vectors per design (a design stops at its own
+  // convergence; empty: steps for every design)
+  std::vector<int> ns;
   // block size 1: the tridiagonal projection per design
   std::vector<std::vector<double>> alpha, beta;
   // block size > 1: per design the projected matrix (steps x steps,
@@ -1976,11 +2006,22 @@ struct LanczosOut {
 // (1 - theta) (mode 0), lambda = 1 / theta (mode 1) or lambda = 1 / theta +
 // sigma (mode 2).  Mode 3 is mode 1 with the wanted values the nev smallest
 // |lambda| of either sign (mode 1 wants the smallest positive loads).  Stops
-// when the nev wanted Ritz values have relative residual <= tol.
+// when the nev wanted Ritz values have relative residual <= tol.  With a
+// factor of the sequential form per design, a design whose wanted values
+// have converged keeps its basis size (lo.ns) and its later solves are
+// skipped; its later basis vectors and projection entries are not used.
 int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
             const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
             double *work, LanczosOut &lo) {
   const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
+  const bool skip = nb > 1 && !F->cr && F->nbatch == nb;
+  struct ActiveReset {
+    shellgpu_factor *F;
+    ~ActiveReset() { F->nact = -1; }
+  } active_reset{F};
+  std::vector<char> done(nb, 0);
+  std::vector<int> act;
+  lo.ns.assign(nb, 0);
   rocblas_handle h = (rocblas_handle)ctx->blas;
   const double one = 1.0, mone = -1.0, zero = 0.0;
   VecOps vo;
@@ -2042,12 +2083,15 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
     if ((rc = vo.scale(qn, tmp))) return rc;
     if ((rc = vo.scale(Kqn, tmp))) return rc;
     if (k + 1 >= nev && ((k + 1) % 5 == 0 || k + 1 == m)) {
-      bool all = true;
-      for (int d = 0; d < nb && all; d++) {
+      bool all = true, newly = false;
+      for (int d = 0; d < nb; d++) {
+        if (done[d]) continue;
         std::vector<double> w, z;
         const int ns = k + 1;
         std::vector<double> e(lo.beta[d].begin(), lo.beta[d].begin() + (ns - 1));
-        if (tridiag_ql(ns, lo.alpha[d], e, w, z) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+        std::vector<double> a(lo.alpha[d].begin(), lo.alpha[d].begin() + ns);
+        // z: the last component of each Ritz vector
+        if (tridiag_ql(ns, a, e, w, z, true) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
         std::vector<int> idx(ns);
         for (int i = 0; i < ns; i++) idx[i] = i;
         auto key = [&](int i) {
@@ -2056,19 +2100,38 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
           return mode == 0 ? std::fabs(-w[i] * sigma[d] / (1.0 - w[i])) : (w[i] > 0 ? 1.0 / w[i] : 1e300);
         };
         std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+        bool conv = true;
         for (int j = 0; j < nev; j++) {
           const int i = idx[j];
-          const double err = std::fabs(lo.beta[d][ns - 1] * z[(size_t)i * ns + ns - 1]);
-          if (!(err <= tol * std::fabs(w[i]))) all = false;
+          const double err = std::fabs(lo.beta[d][ns - 1] * z[i]);
+          if (!(err <= tol * std::fabs(w[i]))) conv = false;
+        }
+        if (conv) {
+          done[d] = 1;
+          lo.ns[d] = ns;
+          newly = true;
+        } else {
+          all = false;
         }
       }
       if (all) {
         k++;
         break;
       }
+      if (skip && newly) {
+        act.clear();
+        for (int d = 0; d < nb; d++)
+          if (!done[d]) act.push_back(d);
+        HIPCALL(ctx, hipMemcpyAsync(F->d_act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+        F->nact = (int)act.size();
+      }
     }
   }
   lo.steps = std::min(k, m);
+  for (int d = 0; d < nb; d++)
+    if (!done[d]) lo.ns[d] = lo.steps;
   return 0;
 }
 
@@ -2085,12 +2148,14 @@ int proj_eigs(shellgpu_ctx *ctx, const LanczosOut &lo, int s, std::vector<std::v
   z.assign(nb, {});
   res.assign(nb, {});
   if (bs == 1) {
+    // a design's own basis size when it stopped before the others
     for (int d = 0; d < nb; d++) {
-      std::vector<double> e(lo.beta[d].begin(), lo.beta[d].begin() + (s - 1));
-      std::vector<double> a(lo.alpha[d].begin(), lo.alpha[d].begin() + s);
-      if (tridiag_ql(s, a, e, w[d], z[d]) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
-      res[d].resize(s);
-      for (int i = 0; i < s; i++) res[d][i] = std::fabs(lo.beta[d][s - 1] * z[d][(size_t)i * s + s - 1]);
+      const int sd = lo.ns.empty() ? s : std::min(s, lo.ns[d]);
+      std::vector<double> e(lo.beta[d].begin(), lo.beta[d].begin() + (sd - 1));
+      std::vector<double> a(lo.alpha[d].begin(), lo.alpha[d].begin() + sd);
+      if (tridiag_ql(sd, a, e, w[d], z[d]) != 0) return ferr(ctx, 6, "tridiagonal eigensolver failed");
+      res[d].resize(sd);
+      for (int i = 0; i < sd; i++) res[d][i] = std::fabs(lo.beta[d][sd - 1] * z[d][(size_t)i * sd + sd - 1]);
     }
     return 0;
   }
@@ -2378,6 +2443,7 @@ int run_lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu
     lo.bs = 1;
     return lanczos(ctx, F, K, G, mode, sigma, nev, m, tol, Q, KQ, hv, work, lo);
   }
+  lo.ns.clear();
   return lanczos_block(ctx, F, K, G, mode, sigma, nev, m, tol, Q, KQ, bs, lo);
 }
 
@@ -2619,8 +2685,9 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
     if ((rc = proj_eigs(ctx, lo, lo.steps, w, z, res))) return rc;
     buck = lo.steps >= nev;
     for (int d = 0; d < nb && buck; d++) {
-      std::vector<int> idx(lo.steps);
-      for (int i = 0; i < lo.steps; i++) idx[i] = i;
+      const int sd = (int)w[d].size();
+      std::vector<int> idx(sd);
+      for (int i = 0; i < sd; i++) idx[i] = i;
       auto key = [&](int i) { return w[d][i] != 0.0 ? std::fabs(1.0 / w[d][i]) : 1e300; };
       std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
       for (int j = 0; j < nev; j++)
@@ -2693,6 +2760,8 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   if ((rc = proj_eigs(ctx, lo, ns, wv, zv, rv))) return rc;
   for (int d = 0; d < nb; d++) {
     const std::vector<double> &w = wv[d], &z = zv[d];
+    const int ns = (int)w.size();  // this design's basis size
+    if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
     const double sd = B->sig[d];
@@ -2989,6 +3058,8 @@ int shellgpu_frequency_solve(shellgpu_frequency *F) {
   if ((rc = proj_eigs(ctx, lo, ns, wv, zv, rv))) return rc;
   for (int d = 0; d < nb; d++) {
     const std::vector<double> &w = wv[d], &z = zv[d];
+    const int ns = (int)w.size();  // this design's basis size
+    if (ns < nev) return ferr(ctx, 8, "Lanczos basis smaller than the number of eigenvalues");
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; i++) idx[i] = i;
     const double sd = F->sig[d];
