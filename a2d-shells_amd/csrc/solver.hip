@@ -2010,6 +2010,8 @@ struct LanczosOut {
 // factor of the sequential form per design, a design whose wanted values
 // have converged keeps its basis size (lo.ns) and its later solves are
 // skipped; its later basis vectors and projection entries are not used.
+// Otherwise (one factor, or cyclic reduction) every design keeps the whole
+// basis.
 int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat *G, int mode,
             const std::vector<double> &sigma, int nev, int m, double tol, double *Q, double *KQ, double *hv,
             double *work, LanczosOut &lo) {
@@ -2130,8 +2132,9 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
     }
   }
   lo.steps = std::min(k, m);
+  // without skipped solves every design's whole basis is valid: use it
   for (int d = 0; d < nb; d++)
-    if (!done[d]) lo.ns[d] = lo.steps;
+    if (!skip || !done[d]) lo.ns[d] = lo.steps;
   return 0;
 }
 
