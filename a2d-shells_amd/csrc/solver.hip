@@ -2108,12 +2108,13 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
           const double err = std::fabs(lo.beta[d][ns - 1] * z[i]);
           if (!(err <= tol * std::fabs(w[i]))) conv = false;
         }
-        if (conv) {
-          done[d] = 1;
+        if (!conv) {
+          all = false;
+          if (!skip) break;  // every design is checked again at the next check
+        } else if (skip) {
+          done[d] = 1;  // frozen: its wanted pairs are final
           lo.ns[d] = ns;
           newly = true;
-        } else {
-          all = false;
         }
       }
       if (all) {
@@ -2132,9 +2133,8 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
     }
   }
   lo.steps = std::min(k, m);
-  // without skipped solves every design's whole basis is valid: use it
   for (int d = 0; d < nb; d++)
-    if (!skip || !done[d]) lo.ns[d] = lo.steps;
+    if (!done[d]) lo.ns[d] = lo.steps;
   return 0;
 }
 
