@@ -115,6 +115,36 @@ def test_batched_designs_match_single_designs(factor, monkeypatch):
         assert np.abs(out["lambda"][b] - lam).max() <= 1e-8 * np.abs(lam).max()
 
 
+def test_batched_designs_converging_at_different_steps(monkeypatch):
+    """Designs of one batch whose wanted pairs converge at different Lanczos
+    steps (thickness x1, x1.4, x2 under one shift): with the sequential
+    factorisation each design stops solving at its own convergence, and its
+    loads and Ritz residuals still match the oracle's and its own
+    single-design run's."""
+    set_factor(monkeypatch, "thomas")
+    cyl = Cylinder(8, 16, imperfection=(2e-4, 2, 3))
+    base = np.random.default_rng(29).uniform(0.0015, 0.0025, cyl.nelems)
+    thick = np.stack([base, 1.4 * base, 2.0 * base])
+    sigma = 0.8 * cyl.classical_load()
+    f = cyl.edge_load(1.0)
+    ctx = make_ctx(cyl, thick)
+    out = ctx.linear_buckling(sigma, f, nev=4, max_lanczos=120, tol=1e-12)
+    ctx.close()
+    steps = []
+    for b in range(3):
+        c1 = make_ctx(cyl, thick[b])
+        one = c1.linear_buckling(sigma, f, nev=4, max_lanczos=120, tol=1e-12)
+        c1.close()
+        steps.append(one["steps"])
+        lam = O.linear_buckling(cyl.Xpts, cyl.conn, thick[b], cyl.bcs(), f, sigma, nev=4, max_lanczos=120)[0]
+        print("design", b, "batch", out["lambda"][b], "single", one["lambda"][0], "oracle", lam, "steps", one["steps"])
+        assert np.abs(out["lambda"][b] - lam).max() <= 1e-8 * np.abs(lam).max()
+        assert np.abs(out["lambda"][b] - one["lambda"][0]).max() <= 1e-10 * np.abs(lam).max()
+        assert out["resid"][b].max() <= 1e-12
+    # the designs alone converge at different steps, so the batch froze some
+    assert len(set(steps)) > 1, steps
+
+
 @pytest.mark.parametrize("factor", ["cr", "thomas"])
 def test_solve_graph_replays_match_direct_launches(factor, monkeypatch):
     """The captured solve graph (SHELLGPU_GRAPH=1), replayed with eager work
