@@ -2084,7 +2084,9 @@ int lanczos(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellgpu_mat
     }
     if ((rc = vo.scale(qn, tmp))) return rc;
     if ((rc = vo.scale(Kqn, tmp))) return rc;
-    if (k + 1 >= nev && ((k + 1) % 5 == 0 || k + 1 == m)) {
+    // convergence checks: every step for one design (its last-row QL is
+    // cheap next to a solve), every 5 steps for a batch
+    if (k + 1 >= nev && ((k + 1) % (nb == 1 ? 1 : 5) == 0 || k + 1 == m)) {
       bool all = true, newly = false;
       for (int d = 0; d < nb; d++) {
         if (done[d]) continue;
