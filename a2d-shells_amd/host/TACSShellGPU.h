@@ -133,7 +133,7 @@ class TACSQuad4NonlinearShell : public TACSElement {
   TACSShellTransform *transform;
 };
 // TacsCreateShellByName (src/elements/shell/TACSShellElementDefs.cpp):
-// "CQUAD4" / "CQUADR" -> TACSQuad4Shell; NULL for element types this build
+// "CQUAD4" / "CQUADR" -> TACSQuad4Shell; NULL for element types this library
 // does not implement
 TACSElement *TacsCreateShellByName(const char *name, TACSShellTransform *transform,
                                    TACSShellConstitutive *con);

@@ -368,8 +368,8 @@ def main():
                          "3 x 36 x nnzb doubles of the workload's matrices) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU decomposition is a "
-                                   "design note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py measures one GPU only; a decomposition across GPUs is not a "
+                                   "measured path of this library", "n_gpus": args.gpus}))
         sys.exit(2)
 
     cyl = Cylinder(200, 400, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))

@@ -1,6 +1,6 @@
 // Writes one .f5 file of fixed content through a TACSFH5File: the
 // reference's own writer (compiled with -DREF_FH5 against
-// src/io/TACSFH5.cpp by oracle/Makefile, target ref) or this build's
+// src/io/TACSFH5.cpp by oracle/Makefile, target ref) or this library's
 // (host/TACSShellGPU.h).  The CPU tests compare the two files byte by byte.
 // Usage: fh5_check <out.f5>
 #ifdef REF_FH5

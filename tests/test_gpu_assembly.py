@@ -12,7 +12,7 @@ from shellgpu.problems import Cylinder
 
 pytestmark = pytest.mark.gpu
 
-TOL_K = 1e-10   # assembled K / tangent / residual entries
+TOL_K = 1e-10   # assembled K / tangent / residual entries (row-floor relative)
 TOL_G = 1e-10   # exact G against the reference's central-difference G
 
 
