@@ -17,6 +17,7 @@ using namespace shellgpu;
 
 namespace {
 
+
 int set_err(shellgpu_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -244,6 +245,7 @@ int shellgpu_create(shellgpu_ctx **out, int device) {
   if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
   if (e != hipSuccess) {
     fprintf(stderr, "shellgpu_create: %s\n", hipGetErrorString(e));
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 2;
   }

@@ -747,21 +747,41 @@ __global__ void axpy_batched_kernel(int n, const double *a, const double *x, dou
   y[o] += a[blockIdx.y] * x[o];
 }
 
-// partial dot products of x[b] . y[b] -> out[b] (one block per (chunk, b))
-__global__ void dot_batched_kernel(int n, const double *x, long xs, const double *y, long ys,
-                                   double *out) {
-  __shared__ double red[256];
-  const int b = blockIdx.y;
-  double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    acc += x[b * xs + i] * y[b * ys + i];
-  red[threadIdx.x] = acc;
+// Batched dot products x[b] . y[b] in two passes with a fixed summation
+// order (bit-reproducible from run to run): pass 1 writes one partial sum
+// per (chunk, b) to part[b * gridDim.x + chunk], pass 2 adds a design's
+// partials in chunk order.  Both passes use 256-thread blocks and the same
+// pairwise tree.
+constexpr int kDotThreads = 256;
+constexpr int kDotChunks = 256;  // largest pass-1 grid.x
+
+__device__ __forceinline__ double block_tree_sum(double v, double *red) {
+  red[threadIdx.x] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(&out[b], red[0]);
+  return red[0];
+}
+
+__global__ __launch_bounds__(kDotThreads) void dot_batched_kernel(int n, const double *x, long xs, const double *y,
+                                                                  long ys, double *part) {
+  __shared__ double red[kDotThreads];
+  const int b = blockIdx.y;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kDotThreads + threadIdx.x; i < n; i += gridDim.x * kDotThreads)
+    acc += x[b * xs + i] * y[b * ys + i];
+  const double s = block_tree_sum(acc, red);
+  if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kDotThreads) void dot_finish_kernel(int nchunks, const double *part, double *out) {
+  __shared__ double red[kDotThreads];
+  const int b = blockIdx.x;
+  const double v = threadIdx.x < nchunks ? part[(long)b * nchunks + threadIdx.x] : 0.0;
+  const double s = block_tree_sum(v, red);
+  if (threadIdx.x == 0) out[b] = s;
 }
 
 // recursive halving of an n-block at r0 down to leaves of <= kCholBase
@@ -1701,7 +1721,9 @@ int run_solve_body(shellgpu_factor *F) {
   return 0;
 }
 
-int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
+// may_alloc = false: no capture of the solve graph and no growth of the
+// vector buffers (an error instead)
+int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs, bool may_alloc = true) {
   shellgpu_ctx *ctx = F->ctx;
   const int nb = F->nbatch;
   const long nv = F->nv, n = F->n;
@@ -1718,7 +1740,8 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
       hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, br, F->vb, 1, act);
       HIPCALL(ctx, hipGetLastError());
       // the captured body covers every design
-      if ((rc = act ? solve_body(F) : run_solve_body(F))) return rc;
+      const bool direct = act || (!may_alloc && !F->solve_graph);
+      if ((rc = direct ? solve_body(F) : run_solve_body(F))) return rc;
       hipLaunchKernelGGL(permute_kernel, pgrid, dim3(256), 0, ctx->stream, nv, n, F->d_pos2dof, F->vx, xr, 0, act);
       HIPCALL(ctx, hipGetLastError());
     }
@@ -1726,6 +1749,7 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs) {
   }
   if (nrhs > kMaxRhs || (nrhs > 8 && nrhs != kMaxRhs)) return ferr(ctx, 1, "unsupported number of right-hand sides");
   if (nrhs > F->rhs_cap) {
+    if (!may_alloc) return ferr(ctx, 1, "the factor's vector buffers would have to grow");
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
     if (F->solve_graph) {  // captured on the buffers about to move
       HIPCALL(ctx, hipGraphExecDestroy(F->solve_graph));
@@ -1759,6 +1783,22 @@ int factor_apply_multi(shellgpu_factor *F, const double *b, double *x, int nrhs)
   return apply_block(F, b, x, nrhs);
 }
 
+// shellgpu_factor_apply without device allocations (see apply_block)
+int factor_apply_noalloc(shellgpu_factor *F, const double *b, double *x) {
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->factored) return ferr(ctx, 1, "factor not computed");
+  if (F->nbatch == ctx->nbatch) return apply_block(F, b, x, 1, false);
+  if (F->nbatch != 1) return ferr(ctx, 1, "factor batch size does not match the context's designs");
+  const long n = F->n;
+  for (int r0 = 0; r0 < ctx->nbatch;) {
+    const int nr = rhs_chunk(ctx->nbatch - r0);
+    const int rc = apply_block(F, b + (size_t)r0 * n, x + (size_t)r0 * n, nr, false);
+    if (rc) return rc;
+    r0 += nr;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1790,26 +1830,40 @@ int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// device doubles VecOps needs for nb designs: the dot results, per-design
+// coefficients and the partial sums of the two-pass dot
+inline long vecops_doubles(int nb) { return 2L * (nb + 1) + (long)kDotChunks * nb; }
+
 struct VecOps {
   shellgpu_ctx *ctx;
   int n, nb;
-  double *dwork = nullptr;  // nb doubles
-  int init(shellgpu_ctx *c, int n_, int nb_) {
+  double *dwork = nullptr;  // vecops_doubles(nb)
+  bool owned = false;
+  // ws: caller-provided workspace of vecops_doubles(nb) doubles, or null
+  // to allocate one
+  int init(shellgpu_ctx *c, int n_, int nb_, double *ws = nullptr) {
     ctx = c;
     n = n_;
     nb = nb_;
-    HIPCALL(ctx, hipMalloc((void **)&dwork, sizeof(double) * (nb + 1) * 2));
+    if (ws) {
+      dwork = ws;
+      return 0;
+    }
+    HIPCALL(ctx, hipMalloc((void **)&dwork, sizeof(double) * vecops_doubles(nb)));
+    owned = true;
     return 0;
   }
   ~VecOps() {
-    if (dwork) (void)hipFree(dwork);
+    if (owned && dwork) (void)hipFree(dwork);
   }
   // out[b] = x[b] . y[b]
   int dot(const double *x, const double *y, std::vector<double> &out) {
-    HIPCALL(ctx, hipMemsetAsync(dwork, 0, sizeof(double) * nb, ctx->stream));
-    const int blocks = std::min(256, (n + 255) / 256);
-    hipLaunchKernelGGL(dot_batched_kernel, dim3(blocks, nb), dim3(256), 0, ctx->stream, n, x, (long)n, y, (long)n,
-                       dwork);
+    const int blocks = std::min(kDotChunks, (n + kDotThreads - 1) / kDotThreads);
+    double *part = dwork + 2L * (nb + 1);
+    hipLaunchKernelGGL(dot_batched_kernel, dim3(blocks, nb), dim3(kDotThreads), 0, ctx->stream, n, x, (long)n, y,
+                       (long)n, part);
+    HIPCALL(ctx, hipGetLastError());
+    hipLaunchKernelGGL(dot_finish_kernel, dim3(nb), dim3(kDotThreads), 0, ctx->stream, blocks, part, dwork);
     HIPCALL(ctx, hipGetLastError());
     out.resize(nb);
     HIPCALL(ctx, hipMemcpyAsync(out.data(), dwork, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
@@ -1833,21 +1887,33 @@ struct VecOps {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b, double *x, int msub,
-                   int nrestart, double rtol, double atol, int *iters, double *resnorm) {
-  if (!ctx || !A) return 1;
+// GMRES workspace of nb designs of n dofs: msub + 1 Krylov vectors and one
+// work vector (each rounded up to 32 doubles), then the VecOps buffer
+inline long gmres_vec_stride(int n, int nb) { return ((long)n * nb + 31) / 32 * 32; }
+inline long gmres_doubles(int n, int nb, int msub) {
+  return (long)(msub + 2) * gmres_vec_stride(n, nb) + vecops_doubles(nb);
+}
+
+// Right-preconditioned restarted GMRES (GMRES::solve, src/bpmat/KSM.cpp:
+// 785-955) on the workspace ws (gmres_doubles).  may_alloc = false: the
+// preconditioner must not allocate (no capture of its solve graph, no
+// growth of a shared factor's vector buffers; the caller checks the
+// latter).
+int gmres_run(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b, double *x, int msub,
+              int nrestart, double rtol, double atol, int *iters, double *resnorm, double *ws, bool may_alloc) {
   const int nb = ctx->nbatch, n = 6 * ctx->nnodes;
-  if (A->nbatch != nb) return ferr(ctx, 1, "matrix batch size");
-  msub = std::max(1, msub);
+  const long vs = gmres_vec_stride(n, nb);
   VecOps vo;
-  int rc = vo.init(ctx, n, nb);
+  int rc = vo.init(ctx, n, nb, ws + (msub + 2) * vs);
   if (rc) return rc;
   std::vector<double *> W(msub + 1);
-  double *work = nullptr;
-  for (auto &w : W) HIPCALL(ctx, hipMalloc((void **)&w, sizeof(double) * (size_t)n * nb));
-  HIPCALL(ctx, hipMalloc((void **)&work, sizeof(double) * (size_t)n * nb));
+  for (int i = 0; i <= msub; i++) W[i] = ws + i * vs;
+  double *work = ws + (msub + 1) * vs;
+  auto precond = [&](const double *in, double *out) {
+    return may_alloc ? shellgpu_factor_apply(F, in, out) : factor_apply_noalloc(F, in, out);
+  };
   // per design Hessenberg (column major, (msub+1) x msub), Givens, residuals
   std::vector<std::vector<double>> H(nb, std::vector<double>((size_t)(msub + 1) * msub, 0.0));
   std::vector<std::vector<double>> cs(nb, std::vector<double>(msub)), sn(nb, std::vector<double>(msub)),
@@ -1858,16 +1924,16 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
   HIPCALL(ctx, hipMemsetAsync(x, 0, sizeof(double) * (size_t)n * nb, ctx->stream));
   for (int count = 0; count <= nrestart; count++) {
     // W0 = b - A x
-    if ((rc = shellgpu_mat_mult(A, x, W[0]))) goto out;
+    if ((rc = shellgpu_mat_mult(A, x, W[0]))) return rc;
     {
       std::vector<double> m1(nb, -1.0);
-      if ((rc = vo.scale(W[0], m1))) goto out;
+      if ((rc = vo.scale(W[0], m1))) return rc;
     }
     {
       std::vector<double> p1(nb, 1.0);
-      if ((rc = vo.axpy(p1, b, W[0]))) goto out;
+      if ((rc = vo.axpy(p1, b, W[0]))) return rc;
     }
-    if ((rc = vo.dot(W[0], W[0], tmp))) goto out;
+    if ((rc = vo.dot(W[0], W[0], tmp))) return rc;
     for (int d = 0; d < nb; d++) {
       cur[d] = std::sqrt(tmp[d]);
       if (count == 0) rhs_norm[d] = cur[d];
@@ -1877,31 +1943,31 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
       tmp[d] = (cur[d] > 0.0) ? 1.0 / cur[d] : 0.0;
     }
     if (std::all_of(done.begin(), done.end(), [](int v) { return v; })) break;
-    if ((rc = vo.scale(W[0], tmp))) goto out;
+    if ((rc = vo.scale(W[0], tmp))) return rc;
     int niters = 0;
     for (int i = 0; i < msub; i++) {
       if (F) {
-        if ((rc = shellgpu_factor_apply(F, W[i], work))) goto out;
-        if ((rc = shellgpu_mat_mult(A, work, W[i + 1]))) goto out;
+        if ((rc = precond(W[i], work))) return rc;
+        if ((rc = shellgpu_mat_mult(A, work, W[i + 1]))) return rc;
       } else {
-        if ((rc = shellgpu_mat_mult(A, W[i], W[i + 1]))) goto out;
+        if ((rc = shellgpu_mat_mult(A, W[i], W[i + 1]))) return rc;
       }
       // modified Gram-Schmidt (GMRES::orthogonalize)
       for (int j = 0; j <= i; j++) {
-        if ((rc = vo.dot(W[i + 1], W[j], tmp))) goto out;
+        if ((rc = vo.dot(W[i + 1], W[j], tmp))) return rc;
         for (int d = 0; d < nb; d++) H[d][(size_t)i * (msub + 1) + j] = tmp[d];
         std::vector<double> m(nb);
         for (int d = 0; d < nb; d++) m[d] = -tmp[d];
-        if ((rc = vo.axpy(m, W[j], W[i + 1]))) goto out;
+        if ((rc = vo.axpy(m, W[j], W[i + 1]))) return rc;
       }
-      if ((rc = vo.dot(W[i + 1], W[i + 1], tmp))) goto out;
+      if ((rc = vo.dot(W[i + 1], W[i + 1], tmp))) return rc;
       std::vector<double> inv(nb);
       for (int d = 0; d < nb; d++) {
         const double hn = std::sqrt(tmp[d]);
         H[d][(size_t)i * (msub + 1) + i + 1] = hn;
         inv[d] = hn > 0.0 ? 1.0 / hn : 0.0;
       }
-      if ((rc = vo.scale(W[i + 1], inv))) goto out;
+      if ((rc = vo.scale(W[i + 1], inv))) return rc;
       bool all = true;
       for (int d = 0; d < nb; d++) {
         double *Hc = &H[d][(size_t)i * (msub + 1)];
@@ -1941,15 +2007,14 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
       for (int i = 0; i < niters; i++) {
         std::vector<double> c(nb);
         for (int d = 0; d < nb; d++) c[d] = y[d][i];
-        if ((rc = vo.axpy(c, W[i], work))) goto out;
+        if ((rc = vo.axpy(c, W[i], work))) return rc;
       }
+      std::vector<double> p1(nb, 1.0);
       if (F) {
-        if ((rc = shellgpu_factor_apply(F, work, W[0]))) goto out;
-        std::vector<double> p1(nb, 1.0);
-        if ((rc = vo.axpy(p1, W[0], x))) goto out;
+        if ((rc = precond(work, W[0]))) return rc;
+        if ((rc = vo.axpy(p1, W[0], x))) return rc;
       } else {
-        std::vector<double> p1(nb, 1.0);
-        if ((rc = vo.axpy(p1, work, x))) goto out;
+        if ((rc = vo.axpy(p1, work, x))) return rc;
       }
     }
     bool all = true;
@@ -1963,9 +2028,33 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
     for (int d = 0; d < nb; d++) worst = std::max(worst, rhs_norm[d] > 0 ? cur[d] / rhs_norm[d] : 0.0);
     *resnorm = worst;
   }
-out:
-  for (auto &w : W) (void)hipFree(w);
-  (void)hipFree(work);
+  return 0;
+}
+
+int gmres_check(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b, double *x, int msub) {
+  if (!ctx) return 1;
+  if (!A || !b || !x) return ferr(ctx, 1, "A, b and x are required");
+  if (A->nbatch != ctx->nbatch) return ferr(ctx, 1, "matrix batch size");
+  if (msub < 1) return ferr(ctx, 1, "msub must be at least 1");
+  if (F && F->ctx != ctx) return ferr(ctx, 1, "factor of another context");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b, double *x, int msub,
+                   int nrestart, double rtol, double atol, int *iters, double *resnorm) {
+  int rc = gmres_check(ctx, A, F, b, x, std::max(1, msub));
+  if (rc) return rc;
+  msub = std::max(1, msub);
+  double *ws = nullptr;
+  HIPCALL(ctx, hipMalloc((void **)&ws, sizeof(double) * gmres_doubles(6 * ctx->nnodes, ctx->nbatch, msub)));
+  rc = gmres_run(ctx, A, F, b, x, msub, nrestart, rtol, atol, iters, resnorm, ws, true);
+  // the stream may still read ws
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(ws);
   return rc;
 }
 

@@ -187,6 +187,17 @@ class Matrix:
         self.ctx._check(self.ctx.lib.shellgpu_mat_get_values(self.h, _dp(out)))
         return out.reshape(self.nbatch, self.ctx.nnzb, 6, 6)
 
+    def design_values(self, d):
+        """Values of design d only, (nnzb, 6, 6)."""
+        if not 0 <= d < self.nbatch:
+            raise ShellGPUError(f"design {d} out of range")
+        n = self.ctx.nnzb * 36
+        out = np.empty(n)
+        self.ctx.synchronize()
+        self.ctx._check(self.ctx.lib.shellgpu_memcpy_d2h(self.ctx.h, out.ctypes.data, self.device_values + 8 * n * d,
+                                                         8 * n))
+        return out.reshape(self.ctx.nnzb, 6, 6)
+
     def set_values(self, vals):
         vals = np.ascontiguousarray(vals, dtype=np.float64).ravel()
         self.ctx._check(self.ctx.lib.shellgpu_mat_set_values(self.h, _dp(vals)))
@@ -496,6 +507,7 @@ class Context:
         self._check(self.lib.shellgpu_gmres(self.h, A.h, F.h if F else None, b.ptr, x.ptr, msub, nrestart, rtol,
                                             atol, C.byref(its), C.byref(rn)))
         return its.value, rn.value
+
 
     def linear_buckling(self, sigma, force, nev=5, max_lanczos=100, tol=1e-12, want_vectors=False, want_path=False):
         """TACSLinearBuckling::solve for every design: returns lambda

@@ -181,3 +181,33 @@ def test_solve_graph_replays_match_direct_launches(factor, monkeypatch):
     Fg.destroy()
     Fd.destroy()
     ctx.close()
+
+
+def test_gmres_is_bit_reproducible():
+    # the batched dot products reduce in a fixed order (two passes, no
+    # atomics): two identical solves agree bit for bit
+    cyl = Cylinder(10, 20)
+    thick = np.random.default_rng(4).uniform(0.0015, 0.0025, (3, cyl.nelems))
+    with make_ctx(cyl, thick) as ctx:
+        K = ctx.matrix()
+        ctx.assemble_mattype(sg.STIFFNESS, K)
+        F = ctx.factor()
+        F.compute(K)
+        b = ctx.vector(np.random.default_rng(5).standard_normal((3, 6 * cyl.nnodes)))
+        out = []
+        for _ in range(2):
+            x = ctx.vector()
+            r = ctx.gmres(K, b, x, F=F, msub=10, nrestart=3, rtol=1e-14)
+            out.append((r, x.download()))
+        assert out[0][0] == out[1][0]
+        assert np.array_equal(out[0][1].view(np.int64), out[1][1].view(np.int64))
+        # unpreconditioned: many dot products per solve
+        out = []
+        for _ in range(2):
+            x = ctx.vector()
+            r = ctx.gmres(K, b, x, msub=40, nrestart=0)
+            out.append((r, x.download()))
+        assert out[0][0] == out[1][0]
+        assert np.array_equal(out[0][1].view(np.int64), out[1][1].view(np.int64))
+        K.destroy()
+        F.destroy()
