@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,8 +16,35 @@
 
 using namespace shellgpu;
 
+// Workspace of the entry points that allocate nothing
+// (shellgpu_gmres_noalloc): a fixed 64 MiB array in device memory.  Every
+// GPU the library runs on holds its own copy, at its own address, until the
+// process ends.  Host code must not use the name `scratch` as a device
+// address: shellgpu_create maps it once per device with
+// hipGetSymbolAddress and every context keeps that pointer.  All calls on
+// one device share the array, so calls that overlap on one GPU race.
+constexpr size_t kScratchDoubles = size_t(8) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
+constexpr int kMaxDevices = 256;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch` (device must be current)
+hipError_t map_scratch(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 int set_err(shellgpu_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->err = msg;
@@ -243,6 +271,10 @@ int shellgpu_create(shellgpu_ctx **out, int device) {
   ctx->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+  if (e == hipSuccess) {
+    e = map_scratch(device, &ctx->scratch);
+    ctx->scratch_doubles = kScratchDoubles;
+  }
   if (e != hipSuccess) {
     fprintf(stderr, "shellgpu_create: %s\n", hipGetErrorString(e));
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -39,6 +39,11 @@ struct shellgpu_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
   hipEvent_t t0 = nullptr, t1 = nullptr;
 
+  // the device's copy of the library's static scratch array (mapped at
+  // shellgpu_create; shared by every context on the device) and its size
+  double *scratch = nullptr;
+  size_t scratch_doubles = 0;
+
   // solver state
   std::vector<int> order_seed;  // BFS seed nodes of the level ordering
   void *blas = nullptr;         // rocblas_handle bound to `stream`

@@ -1799,6 +1799,18 @@ int factor_apply_noalloc(shellgpu_factor *F, const double *b, double *x) {
   return 0;
 }
 
+// 0 when factor_apply_noalloc can run on F without allocating
+int factor_noalloc_ready(shellgpu_factor *F) {
+  shellgpu_ctx *ctx = F->ctx;
+  if (!F->factored) return ferr(ctx, 1, "factor not computed");
+  if (F->nbatch == ctx->nbatch || F->cr) return 0;
+  if (F->nbatch != 1) return ferr(ctx, 1, "factor batch size does not match the context's designs");
+  if (rhs_chunk(ctx->nbatch) > F->rhs_cap)
+    return ferr(ctx, 1, "a factor shared by " + std::to_string(ctx->nbatch) +
+                            " designs needs larger vector buffers: apply it once with shellgpu_factor_apply first");
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2056,6 +2068,20 @@ int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(ws);
   return rc;
+}
+
+int shellgpu_gmres_noalloc(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b, double *x,
+                           int msub, int nrestart, double rtol, double atol, int *iters, double *resnorm) {
+  int rc = gmres_check(ctx, A, F, b, x, msub);
+  if (rc) return rc;
+  if (!ctx->scratch) return ferr(ctx, 2, "the device scratch array is not mapped on this device");
+  const long need = gmres_doubles(6 * ctx->nnodes, ctx->nbatch, msub);
+  if (need > (long)ctx->scratch_doubles)
+    return ferr(ctx, 1, "GMRES workspace of " + std::to_string(need * 8) + " bytes exceeds the " +
+                            std::to_string(ctx->scratch_doubles * 8) +
+                            "-byte device scratch array (fewer designs, a smaller msub or shellgpu_gmres)");
+  if (F && (rc = factor_noalloc_ready(F))) return rc;
+  return gmres_run(ctx, A, F, b, x, msub, nrestart, rtol, atol, iters, resnorm, ctx->scratch, false);
 }
 
 }  // extern "C"

@@ -85,6 +85,8 @@ def load_library(path=LIB_PATH):
         "shellgpu_factor_compute": (C.c_int, [_VP, _VP]),
         "shellgpu_factor_apply": (C.c_int, [_VP, _VP, _VP]),
         "shellgpu_gmres": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, _IP, _DP]),
+        "shellgpu_gmres_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, _IP,
+                                             _DP]),
         "shellgpu_linear_buckling": (C.c_int, [_VP, C.c_double, _VP, C.c_int, C.c_int, C.c_double, _DP, _VP, _VP,
                                                _IP, _DP, _DP]),
         "shellgpu_buckling_create": (C.c_int, [_VP, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
@@ -508,6 +510,13 @@ class Context:
                                             atol, C.byref(its), C.byref(rn)))
         return its.value, rn.value
 
+    def gmres_noalloc(self, A, b, x, F=None, msub=10, nrestart=15, rtol=1e-12, atol=1e-30):
+        """gmres() on the library's fixed device scratch array (no device
+        allocation; the workspace must fit in 64 MiB)"""
+        its, rn = C.c_int(), C.c_double()
+        self._check(self.lib.shellgpu_gmres_noalloc(self.h, A.h, F.h if F else None, b.ptr, x.ptr, msub, nrestart,
+                                                    rtol, atol, C.byref(its), C.byref(rn)))
+        return its.value, rn.value
 
     def linear_buckling(self, sigma, force, nev=5, max_lanczos=100, tol=1e-12, want_vectors=False, want_path=False):
         """TACSLinearBuckling::solve for every design: returns lambda

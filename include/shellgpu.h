@@ -197,6 +197,22 @@ int shellgpu_factor_apply(shellgpu_factor *F, const double *b, double *x);
 int shellgpu_gmres(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b,
                    double *x, int msub, int nrestart, double rtol, double atol, int *iters,
                    double *resnorm);
+/* shellgpu_gmres without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops that reuse one factor, harnesses that forbid hipMalloc).  The
+ * Krylov basis, the work vector and the reduction partials live in the
+ * library's fixed device scratch array of 64 MiB (8,388,608 doubles), one
+ * per GPU, held until the process ends: (msub + 2) vectors of
+ * nbatch * 6 * nnodes doubles (each rounded up to a multiple of 32) plus
+ * 258 * nbatch + 2 doubles must fit in it, or the call fails with an error.
+ * F (may be NULL) must be factored; a factor of one design shared by
+ * several designs must have been applied once with shellgpu_factor_apply
+ * (which sizes its vector buffers).  Results equal shellgpu_gmres's bit for
+ * bit.  Every call on a GPU uses the same array: calls that overlap on one
+ * GPU (other threads, other contexts) race. */
+int shellgpu_gmres_noalloc(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *F, const double *b,
+                           double *x, int msub, int nrestart, double rtol, double atol, int *iters,
+                           double *resnorm);
 
 /* ---- nonlinear static path ------------------------------------------ */
 /* Load-stepped Newton on the nonlinear model (TACSQuad4NonlinearShell).
