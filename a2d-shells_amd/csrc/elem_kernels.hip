@@ -39,16 +39,21 @@ namespace shellgpu {
 // What each kernel mode computes.
 template <int MODE>
 struct ModeTraits {
-  static constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG);
+  // residual and alpha K + gamma M in one pass (addJacobian)
+  static constexpr bool kJac = (MODE == kModeJacLin || MODE == kModeJacNl);
+  // (alpha K + gamma M) x per element (addJacobianVecProduct)
+  static constexpr bool kJvp = (MODE == kModeJvpLin || MODE == kModeJvpNl);
+  static constexpr bool kMassTerms = kJac || kJvp;
+  static constexpr bool kMat = (MODE == kModeKLin || MODE == kModeKNl || MODE == kModeG || kJac || kJvp);
   static constexpr bool kState = (MODE != kModeKLin);
-  static constexpr bool kNonlin =
-      (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl || MODE == kModeOutNl);
+  static constexpr bool kNonlin = (MODE == kModeKNl || MODE == kModeRNl || MODE == kModeAdjResNl ||
+                                   MODE == kModeOutNl || MODE == kModeJacNl || MODE == kModeJvpNl);
   // strains, stresses and failure values at the element nodes instead of
   // the quadrature points (getOutputData)
   static constexpr bool kOut = (MODE == kModeOutLin || MODE == kModeOutNl);
-  static constexpr bool kRes =
-      (MODE == kModeRLin || MODE == kModeRNl || MODE == kModeAdjResLin || MODE == kModeAdjResNl);
-  static constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG);
+  static constexpr bool kRes = (MODE == kModeRLin || MODE == kModeRNl || MODE == kModeAdjResLin ||
+                                MODE == kModeAdjResNl || kJac);
+  static constexpr bool kGeo = (MODE == kModeKNl || MODE == kModeG || MODE == kModeJacNl || MODE == kModeJvpNl);
   // B1 = dB/dq along the state is formed too (geometric stiffness and the
   // thickness adjoint), which doubles the tying Jacobian and B storage
   static constexpr bool kTwo = (MODE == kModeG || MODE == kModeAdjDV);
@@ -67,6 +72,7 @@ struct ElemSmem {
   double X[12];
   double s[24];
   double v2[MT::kAdj ? 24 : 1];  // second state of the adjoint modes: path u or psi
+  double dd[MT::kJac ? 24 : 1];  // second time derivative of the state (addJacobian)
   double fn[12];
   double T[36], XT[36], XzT[36], wdet[4];
   double ety[MT::kState ? 9 : 1];
@@ -270,6 +276,52 @@ __device__ __forceinline__ double tying_geo(const SM &sm, int k, int l) {
   return h;
 }
 
+// Consistent mass of the linearized-rotation shell between nodes ni and nj
+// (the derivation is given at mass_kernel below): with S = sum_qp detXd
+// N_ni N_nj, translations couple by m0 S I and rotations by
+// m2 S ((t_i . t_j) I - t_j t_i^T), t = node normals.  MassTile holds the
+// factors of one 3 x 6 tile (rows rr..rr+2 of node ni, the 6 dofs of nj).
+struct MassTile {
+  double S, m0, m2, tt;
+  const double *ti, *tj;
+  int rr;
+};
+
+// S = sum_qp detXd N_ni N_nj (Gauss weights 1)
+__device__ __forceinline__ double mass_weight(const double wdet[4], int ni, int nj) {
+  const double ha = (ni & 1) ? 0.5 : -0.5, hb = (ni & 2) ? 0.5 : -0.5;
+  const double ga = (nj & 1) ? 0.5 : -0.5, gb = (nj & 2) ? 0.5 : -0.5;
+  double S = 0.0;
+#pragma unroll
+  for (int iq = 0; iq < 4; iq++) {
+    double u, v;
+    quad_point(iq, u, v);
+    // N of nodes ni and nj at the point, (0.5 + h u)(0.5 + h v) as in shape4
+    const double Ni = (0.5 + ha * u) * (0.5 + hb * v), Nj = (0.5 + ga * u) * (0.5 + gb * v);
+    S += wdet[iq] * (Ni * Nj);
+  }
+  return S;
+}
+
+__device__ __forceinline__ MassTile mass_tile(const double wdet[4], const double fn[12], int ni, int nj, int rr,
+                                              double m0, double m2) {
+  MassTile m;
+  m.S = mass_weight(wdet, ni, nj);
+  m.m0 = m0;
+  m.m2 = m2;
+  m.ti = &fn[3 * ni];
+  m.tj = &fn[3 * nj];
+  m.tt = m.ti[0] * m.tj[0] + m.ti[1] * m.tj[1] + m.ti[2] * m.tj[2];
+  m.rr = rr;
+  return m;
+}
+
+// entry (i, j) of the tile, i < 3, j < 6
+__device__ __forceinline__ double mass_entry(const MassTile &m, int i, int j) {
+  if (m.rr == 0) return (j == i) ? m.m0 * m.S : 0.0;
+  return (j < 3) ? 0.0 : m.m2 * m.S * ((j - 3 == i ? m.tt : 0.0) - m.tj[i] * m.ti[j - 3]);
+}
+
 // Two elements per wavefront: lanes 0-31 work on element 2 blockIdx.x of the
 // launch list, lanes 32-63 on the next one, each half with its own LDS.  An
 // odd list leaves the last upper half computing a copy of the lower half's
@@ -307,6 +359,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
       sm.v2[lane] = a.u2 ? a.u2[design * a.u2_stride + 6 * cn[lane / 6] + lane % 6] : 0.0;
     if (MODE == kModeAdjResLin || MODE == kModeAdjResNl)
       sm.v2[lane] = a.psi[design * a.psi_stride + 6 * cn[lane / 6] + lane % 6];
+    if (MT::kJac) sm.dd[lane] = a.ddu ? a.ddu[design * a.ddu_stride + 6 * cn[lane / 6] + lane % 6] : 0.0;
   }
   __syncthreads();
 
@@ -322,7 +375,7 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     __syncthreads();
     if (lane < 24) sm.s[lane] = (a.fd_sign * dh) * sm.s[lane] / nrm;
     temp = temp + (a.fd_sign * dh) * temp / nrm;
-    alpha = a.fd_sign * (0.5 * nrm / dh);
+    alpha = a.alpha * (a.fd_sign * (0.5 * nrm / dh));
     __syncthreads();
   }
 
@@ -874,13 +927,13 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   // now so that the read completes during the quadrature loop (no other
   // element of this launch writes the block), the linear stiffness at the
   // end (measured faster: fewer live registers through the loop)
-  constexpr bool kEarly = (MODE == kModeKNl || MODE == kModeG);
+  constexpr bool kEarly = (MODE == kModeKNl || MODE == kModeG || MODE == kModeJacNl);
   double prev[18];
   double *dst = nullptr;
   // first touch of the block in this assembly: written, not added to (the
   // matrix is not zeroed first)
   bool fresh = false;
-  if (kMat && valid) {
+  if (kMat && !MT::kJvp && valid) {
     const int slot = 4 * (row0 / 6) + col0 / 6;
     const long blk = a.blk[16 * e + slot];
     dst = a.vals + design * a.vals_stride + 36 * blk + 6 * (row0 % 6);
@@ -992,6 +1045,84 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
     }
   }
 
+  // ---- inertial terms of addJacobian (TACSShellElement.h:613-648 and the
+  // gamma d2Tdotd part of TACSLinearizedRotation::addDirectorJacobian,
+  // TACSDirector.h:368-430): gamma M into the matrix and M u'' into the
+  // residual, with the mass moments m0 = rho t, m1 = 0, m2 = rho t^3 / 12
+  // (TACSIsoShellConstitutive.cpp:120-129).  The residual is not scaled by
+  // alpha or gamma. ----
+  if constexpr (MT::kMassTerms) {
+    const double m0 = a.mat.rho * thick, m2 = a.mat.rho * thick * thick * thick * (1.0 / 12.0);
+    if (MT::kJac && a.ddu && lane < 24) {
+      const int n = lane / 6, r = lane % 6;
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const MassTile mt = mass_tile(sm.wdet, sm.fn, n, m, r < 3 ? 0 : 3, m0, m2);
+#pragma unroll
+        for (int j = 0; j < 6; j++) racc += mass_entry(mt, r % 3, j) * sm.dd[6 * m + j];
+      }
+    }
+    // the tile becomes alpha K_e + gamma M_e
+    const MassTile mt = mass_tile(sm.wdet, sm.fn, row0 / 6, col0 / 6, row0 % 6, m0, m2);
+    const double gamma = a.gamma;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 6; j++) acc[6 * i + j] = alpha * acc[6 * i + j] + gamma * mass_entry(mt, i, j);
+  }
+
+  // ---- element matrix times a vector instead of the scatter
+  // (TACSAssembler::addJacobianVecProduct, TACSAssembler.cpp:4334-4390) ----
+  if constexpr (MT::kJvp) {
+    const double *x = a.mvx + design * a.mv_stride;
+    double *y = a.mvy + design * a.mv_stride;
+    const int nr = cn[row0 / 6], nc = cn[col0 / 6], rr = row0 % 6;
+    if (!a.mv_trans) {
+      // rows row0..row0+2: sum over the 4 column nodes (lanes 4k..4k+3)
+      double xc[6], p[3];
+#pragma unroll
+      for (int j = 0; j < 6; j++) xc[j] = x[6 * nc + j];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        p[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) p[i] += acc[6 * i + j] * xc[j];
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        p[i] += __shfl_xor(p[i], 1);
+        p[i] += __shfl_xor(p[i], 2);
+      }
+      if (valid && (lane & 3) == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) y[6 * nr + rr + i] += a.mv_scale * p[i];
+      }
+    } else {
+      // columns of node col0 / 6: sum over the 8 row tiles (lanes with the
+      // same lane & 3)
+      double xr[3], p[6];
+#pragma unroll
+      for (int i = 0; i < 3; i++) xr[i] = x[6 * nr + rr + i];
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        p[j] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) p[j] += acc[6 * i + j] * xr[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        p[j] += __shfl_xor(p[j], 4);
+        p[j] += __shfl_xor(p[j], 8);
+        p[j] += __shfl_xor(p[j], 16);
+      }
+      if (valid && lane < 4) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) y[6 * nc + j] += a.mv_scale * p[j];
+      }
+    }
+    return;
+  }
+
   // ---- P7: scatter (elements of one colour share no node) ----
   if (kMat && valid) {
 #pragma unroll
@@ -1001,12 +1132,14 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
         // read even where the value is discarded: a load only for blocks
         // already reached measured slower than loading them all
         const double old = kEarly ? prev[6 * i + j] : dst[6 * i + j];
-        dst[6 * i + j] = (fresh ? 0.0 : old) + alpha * acc[6 * i + j];
+        // (the mass-term modes have applied alpha already)
+        dst[6 * i + j] = (fresh ? 0.0 : old) + (MT::kMassTerms ? acc[6 * i + j] : alpha * acc[6 * i + j]);
       }
   }
-  if (kRes && valid && lane < 24) {
+  // the residual is unscaled (alpha scales only the matrix)
+  if (kRes && valid && lane < 24 && a.res) {
     double *dst = a.res + design * a.res_stride;
-    dst[6 * cn[lane / 6] + lane % 6] += alpha * racc;
+    dst[6 * cn[lane / 6] + lane % 6] += racc;
   }
 }
 
@@ -1091,36 +1224,15 @@ __global__ __launch_bounds__(64) void mass_kernel(ElemArgs a) {
 
   const int row0 = 3 * (lane >> 2), col0 = 6 * (lane & 3);
   const int ni = row0 / 6, nj = col0 / 6, rr = row0 % 6;
-  double S = 0.0;
-#pragma unroll
-  for (int iq = 0; iq < 4; iq++) {
-    double u, v, N[4], Na[4], Nb[4];
-    quad_point(iq, u, v);
-    shape4(u, v, N, Na, Nb);
-    const double ha = (ni & 1) ? 0.5 : -0.5, hb = (ni & 2) ? 0.5 : -0.5;
-    const double ga = (nj & 1) ? 0.5 : -0.5, gb = (nj & 2) ? 0.5 : -0.5;
-    // N of nodes ni and nj at the point, (0.5 + h u)(0.5 + h v) as in shape4
-    const double Ni = (0.5 + ha * u) * (0.5 + hb * v), Nj = (0.5 + ga * u) * (0.5 + gb * v);
-    S += sm.wdet[iq] * (Ni * Nj);
-  }
   const double rho = a.mat.rho;
   const double m0 = (MODE == kModeMass) ? rho * t : rho;
   const double m2 = (MODE == kModeMass) ? rho * t * t * t * (1.0 / 12.0) : rho * t * t * 0.25;
+  const MassTile mt = mass_tile(sm.wdet, sm.fn, ni, nj, rr, m0, m2);
   double acc[18];
-  if (rr == 0) {
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+  for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 6; j++) acc[6 * i + j] = (j == i) ? m0 * S : 0.0;
-  } else {
-    const double *ti = &sm.fn[3 * ni], *tj = &sm.fn[3 * nj];
-    const double tt = ti[0] * tj[0] + ti[1] * tj[1] + ti[2] * tj[2];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 6; j++)
-        acc[6 * i + j] = (j < 3) ? 0.0 : m2 * S * ((j - 3 == i ? tt : 0.0) - tj[i] * ti[j - 3]);
-  }
+    for (int j = 0; j < 6; j++) acc[6 * i + j] = mass_entry(mt, i, j);
 
   if (MODE == kModeMass) {
     if (valid) {
@@ -1186,6 +1298,14 @@ hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nba
       return launch_mode<kModeOutLin>(args, count, nbatch, stream);
     case kModeOutNl:
       return launch_mode<kModeOutNl>(args, count, nbatch, stream);
+    case kModeJacLin:
+      return launch_mode<kModeJacLin>(args, count, nbatch, stream);
+    case kModeJacNl:
+      return launch_mode<kModeJacNl>(args, count, nbatch, stream);
+    case kModeJvpLin:
+      return launch_mode<kModeJvpLin>(args, count, nbatch, stream);
+    case kModeJvpNl:
+      return launch_mode<kModeJvpNl>(args, count, nbatch, stream);
     case kModeMass:
     case kModeMassDV: {
       if (count <= 0 || nbatch <= 0) return hipSuccess;

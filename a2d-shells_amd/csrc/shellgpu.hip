@@ -130,6 +130,59 @@ __global__ __launch_bounds__(192) void bsr_spmv6_kernel(int nrows, const int *__
   y[6L * row + r] = acc;
 }
 
+// y = A^T x, 6x6 blocks: row j of A^T gathers block (i, j) of A, transposed,
+// for every block (j, i) of row j (the pattern is structurally symmetric)
+__global__ __launch_bounds__(192) void bsr_spmv6_t_kernel(int nrows, const int *__restrict__ rowp,
+                                                          const int *__restrict__ cols,
+                                                          const int *__restrict__ tpos,
+                                                          const double *__restrict__ vals, long vstride,
+                                                          const double *__restrict__ x, double *__restrict__ y,
+                                                          long xstride) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(tid / 6), r = (int)(tid % 6);
+  if (row >= nrows) return;
+  vals += (long)blockIdx.y * vstride;
+  x += (long)blockIdx.y * xstride;
+  y += (long)blockIdx.y * xstride;
+  double acc = 0.0;
+  for (int p = rowp[row]; p < rowp[row + 1]; p++) {
+    const double *a = vals + 36L * tpos[p] + r;  // column r of block (cols[p], row)
+    const double *xx = x + 6L * cols[p];
+    acc += a[0] * xx[0] + a[6] * xx[1] + a[12] * xx[2] + a[18] * xx[3] + a[24] * xx[4] + a[30] * xx[5];
+  }
+  y[6L * row + r] = acc;
+}
+
+__global__ void scale_kernel(long n, double alpha, double *x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] *= alpha;
+}
+
+// the 6 diagonal entries of every diagonal block += alpha (BCSRMat::addDiag)
+__global__ void add_diag_kernel(int nnodes, const int *diag, double *vals, long vstride, double alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * nnodes) return;
+  vals[(long)blockIdx.y * vstride + 36L * diag[i / 6] + 7 * (i % 6)] += alpha;
+}
+
+// dst block blk[s] += the sum, in order, of the 36-value blocks
+// vals[ptr[s] .. ptr[s + 1]) (one thread per block entry)
+__global__ void add_blocks_kernel(int nseg, const int *ptr, const int *blk, const double *vals, double *dst) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 36L * nseg) return;
+  const int sgm = (int)(t / 36), e = (int)(t % 36);
+  double acc = 0.0;
+  for (int k = ptr[sgm]; k < ptr[sgm + 1]; k++) acc += vals[36L * k + e];
+  dst[36L * blk[sgm] + e] += acc;
+}
+
+// w+ = a + b, w- = a - b
+__global__ void sum_diff_kernel(long n, const double *a, const double *b, double *wp, double *wm) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    wp[i] = a[i] + b[i];
+    wm[i] = a[i] - b[i];
+  }
+}
+
 __global__ void axpy_kernel(long n, double alpha, const double *x, double *y) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] += alpha * x[i];
@@ -289,7 +342,7 @@ void shellgpu_destroy(shellgpu_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  void *bufs[] = {ctx->d_conn, ctx->d_rowp,     ctx->d_cols,    ctx->d_blk,
+  void *bufs[] = {ctx->d_conn, ctx->d_rowp,     ctx->d_cols,    ctx->d_blk, ctx->d_diag, ctx->d_tpos,
                   ctx->d_color_elems, ctx->d_X, ctx->d_bc_nodes, ctx->d_bc_mask,
                   ctx->d_bc_vals, ctx->d_thick, ctx->d_temp, ctx->d_coef, ctx->d_first};
   for (void *b : bufs)
@@ -383,6 +436,18 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
       }
     }
 
+  // diagonal block of every node; transposed position of every block
+  std::vector<int> diag(nnodes), tpos(ctx->nnzb);
+  for (int i = 0; i < nnodes; i++) {
+    const int *c0 = &ctx->h_cols[ctx->h_rowp[i]], *c1 = &ctx->h_cols[ctx->h_rowp[i + 1]];
+    diag[i] = (int)(std::lower_bound(c0, c1, i) - ctx->h_cols.data());
+    for (int p = ctx->h_rowp[i]; p < ctx->h_rowp[i + 1]; p++) {
+      const int j = ctx->h_cols[p];
+      const int *d0 = &ctx->h_cols[ctx->h_rowp[j]], *d1 = &ctx->h_cols[ctx->h_rowp[j + 1]];
+      tpos[p] = (int)(std::lower_bound(d0, d1, i) - ctx->h_cols.data());
+    }
+  }
+
   // colouring
   color_elements(nnodes, nelems, ctx->h_conn, ctx->h_color, ctx->ncolors);
   ctx->h_color_ptr.assign(ctx->ncolors + 1, 0);
@@ -424,6 +489,8 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
   if ((rc = upload(ctx, &ctx->d_rowp, ctx->h_rowp.data(), ctx->h_rowp.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_cols, ctx->h_cols.data(), ctx->h_cols.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_blk, ctx->h_blk.data(), ctx->h_blk.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_diag, diag.data(), diag.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_tpos, tpos.data(), tpos.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_color_elems, ctx->h_color_elems.data(), ctx->h_color_elems.size())))
     return rc;
   return 0;
@@ -569,6 +636,97 @@ int shellgpu_mat_axpy(shellgpu_mat *Y, double alpha, const shellgpu_mat *X) {
   return 0;
 }
 
+int shellgpu_mat_scale(shellgpu_mat *A, double alpha) {
+  shellgpu_ctx *ctx = A->ctx;
+  const long n = 36L * ctx->nnzb * A->nbatch;
+  hipLaunchKernelGGL(scale_kernel, dim3(launch_grid(n)), dim3(256), 0, ctx->stream, n, alpha, A->vals);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_mat_add_diag(shellgpu_mat *A, double alpha) {
+  shellgpu_ctx *ctx = A->ctx;
+  hipLaunchKernelGGL(add_diag_kernel, dim3((6 * ctx->nnodes + 255) / 256, A->nbatch), dim3(256), 0, ctx->stream,
+                     ctx->nnodes, ctx->d_diag, A->vals, 36L * ctx->nnzb, alpha);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_mat_add_blocks(shellgpu_mat *A, int design, int nblk, const int *brow, const int *bcol,
+                            const double *vals) {
+  shellgpu_ctx *ctx = A ? A->ctx : nullptr;
+  if (!A) return 1;
+  if (design < 0 || design >= A->nbatch) return set_err(ctx, 1, "design out of range");
+  if (nblk < 0 || (nblk > 0 && (!brow || !bcol || !vals))) return set_err(ctx, 1, "invalid block arguments");
+  // target block of every input block (-1: not in the pattern, skipped)
+  std::vector<int> tgt(nblk, -1);
+  long missing = 0;
+  int first_r = -1, first_c = -1;
+  for (int k = 0; k < nblk; k++) {
+    const int r = brow[k], c = bcol[k];
+    if (r < 0 || c < 0) continue;  // negative indices are skipped, as in BCSRMat::addRowValues
+    if (r < ctx->nnodes && c < ctx->nnodes) {
+      const int *c0 = &ctx->h_cols[ctx->h_rowp[r]], *c1 = &ctx->h_cols[ctx->h_rowp[r + 1]];
+      const int *it = std::lower_bound(c0, c1, c);
+      if (it != c1 && *it == c) tgt[k] = (int)(it - ctx->h_cols.data());
+    }
+    if (tgt[k] < 0 && missing++ == 0) {
+      first_r = r;
+      first_c = c;
+    }
+  }
+  // stable order by target block: every block's additions are summed in
+  // input order, so the result does not depend on the launch
+  std::vector<int> order;
+  for (int k = 0; k < nblk; k++)
+    if (tgt[k] >= 0) order.push_back(k);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tgt[x] < tgt[y]; });
+  std::vector<double> sv(36 * order.size());
+  std::vector<int> ptr{0}, blk;
+  for (size_t q = 0; q < order.size(); q++) {
+    std::copy(vals + 36L * order[q], vals + 36L * order[q] + 36, sv.begin() + 36 * q);
+    if (q > 0 && tgt[order[q]] != tgt[order[q - 1]]) ptr.push_back((int)q);
+    if (q == 0 || tgt[order[q]] != tgt[order[q - 1]]) blk.push_back(tgt[order[q]]);
+  }
+  ptr.push_back((int)order.size());
+  const int nseg = (int)blk.size();
+  if (nseg > 0) {
+    double *dv = nullptr;
+    int *dp = nullptr, *db = nullptr;
+    hipError_t e = hipMalloc((void **)&dv, sizeof(double) * sv.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&dp, sizeof(int) * ptr.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&db, sizeof(int) * blk.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(db, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(add_blocks_kernel, dim3((unsigned)((36L * nseg + 255) / 256)), dim3(256), 0, ctx->stream,
+                         nseg, dp, db, dv, A->vals + (long)design * 36L * ctx->nnzb);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dv);
+    (void)hipFree(dp);
+    (void)hipFree(db);
+    if (e != hipSuccess) return set_err(ctx, 2, std::string("add blocks: ") + hipGetErrorString(e));
+  }
+  if (missing > 0)
+    return set_err(ctx, 7, std::to_string(missing) + " block(s) not in the pattern (first: row " +
+                               std::to_string(first_r) + ", column " + std::to_string(first_c) +
+                               "); the others were added");
+  return 0;
+}
+
+int shellgpu_mat_mult_transpose(shellgpu_mat *A, const double *x, double *y) {
+  shellgpu_ctx *ctx = A->ctx;
+  const long nthreads = 6L * ctx->nnodes;
+  const int blocks = (int)((nthreads + 191) / 192);
+  hipLaunchKernelGGL(bsr_spmv6_t_kernel, dim3(blocks, A->nbatch), dim3(192), 0, ctx->stream, ctx->nnodes,
+                     ctx->d_rowp, ctx->d_cols, ctx->d_tpos, A->vals, 36L * ctx->nnzb, x, y, 6L * ctx->nnodes);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
 int shellgpu_mat_apply_bcs(shellgpu_mat *A, double diag) {
   shellgpu_ctx *ctx = A->ctx;
   if (ctx->nbc == 0) return 0;
@@ -590,18 +748,21 @@ int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y) {
   return record_end(ctx, ctx->spmv_ev);
 }
 
-int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, shellgpu_mat *A) {
-  int rc = check_batch(ctx, A);
-  if (rc) return rc;
+// one matrix type written (accumulate = false: the previous values are
+// overwritten) or added (accumulate = true) into A with factor alpha; no BCs
+static int assemble_type(shellgpu_ctx *ctx, int mattype, const double *u, shellgpu_mat *A, double alpha,
+                         bool accumulate) {
   if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC && mattype != SHELLGPU_GEOMETRIC_FD &&
       mattype != SHELLGPU_TANGENT_NONLINEAR && mattype != SHELLGPU_MASS)
     return set_err(ctx, 1, "unknown matrix type");
+  int rc = 0;
   ElemArgs a = base_args(ctx, u);
   a.vals = A->vals;
+  a.alpha = alpha;
   // the stiffness-type modes write each block at its first touch, which
   // replaces zeroing the matrix; the mass kernel adds
-  a.first = mattype == SHELLGPU_MASS ? nullptr : ctx->d_first;
-  if (!a.first)
+  a.first = (accumulate || mattype == SHELLGPU_MASS) ? nullptr : ctx->d_first;
+  if (!accumulate && !a.first)
     HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
   if (mattype == SHELLGPU_STIFFNESS) {
     if ((rc = run_colored(ctx, kModeKLin, a))) return rc;
@@ -618,7 +779,48 @@ int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, s
   } else {
     if ((rc = run_colored(ctx, kModeMass, a))) return rc;
   }
+  return 0;
+}
+
+int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u, shellgpu_mat *A) {
+  int rc = check_batch(ctx, A);
+  if (rc) return rc;
+  if ((rc = assemble_type(ctx, mattype, u, A, 1.0, false))) return rc;
   return shellgpu_mat_apply_bcs(A, 1.0);
+}
+
+int shellgpu_assemble_mat_combo(shellgpu_ctx *ctx, int nmats, const int *mattypes, const double *scale,
+                                const double *u, shellgpu_mat *A) {
+  int rc = check_batch(ctx, A);
+  if (rc) return rc;
+  if (nmats < 0 || (nmats > 0 && (!mattypes || !scale))) return set_err(ctx, 1, "invalid matrix combination");
+  HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  for (int k = 0; k < nmats; k++)
+    if ((rc = assemble_type(ctx, mattypes[k], u, A, scale[k], true))) return rc;
+  return shellgpu_mat_apply_bcs(A, 1.0);
+}
+
+int shellgpu_add_jacobian_vec_product(shellgpu_ctx *ctx, int model, double scale, double alpha, double beta,
+                                      double gamma, const double *u, const double *dudt, const double *d2udt2,
+                                      const double *x, double *y, int transpose) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!x || !y) return set_err(ctx, 1, "x and y are required");
+  // no velocity terms (TACSDirector.h:368-432); u'' enters only the residual
+  (void)beta;
+  (void)dudt;
+  (void)d2udt2;
+  ElemArgs a = base_args(ctx, u);
+  a.alpha = alpha;
+  a.gamma = gamma;
+  a.mvx = x;
+  a.mvy = y;
+  a.mv_stride = 6L * ctx->nnodes;
+  a.mv_scale = scale;
+  a.mv_trans = transpose ? 1 : 0;
+  const bool nonlin = model == SHELLGPU_NONLINEAR;
+  if ((rc = run_colored(ctx, nonlin ? kModeJvpNl : kModeJvpLin, a))) return rc;
+  return shellgpu_vec_zero_bcs(ctx, y);
 }
 
 int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double *R) {
@@ -647,16 +849,43 @@ int shellgpu_element_output(shellgpu_ctx *ctx, int model, const double *u, doubl
   return run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeOutNl : kModeOutLin, a);
 }
 
-int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u, double *R, shellgpu_mat *A) {
+int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, double alpha, double beta, double gamma,
+                               const double *u, const double *dudt, const double *d2udt2, double *R,
+                               shellgpu_mat *A) {
   int rc = check_batch(ctx, A);
   if (rc) return rc;
-  if (R && (rc = shellgpu_assemble_res(ctx, model, u, R))) return rc;
+  if (!A) return set_err(ctx, 1, "A is required");
+  // the linearized-rotation director has no velocity-dependent terms
+  // (TACSLinearizedRotation::addDirectorJacobian, TACSDirector.h:368-432):
+  // beta and du/dt do not enter
+  (void)beta;
+  (void)dudt;
   ElemArgs a = base_args(ctx, u);
   a.vals = A->vals;
+  a.alpha = alpha;
+  a.gamma = gamma;
+  a.ddu = d2udt2;
+  a.ddu_stride = 6L * ctx->nnodes;
   a.first = ctx->d_first;
   if (!a.first)
     HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
-  if ((rc = run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeKNl : kModeKLin, a))) return rc;
+  const bool nonlin = model == SHELLGPU_NONLINEAR;
+  if (!R && gamma == 0.0) {
+    // alpha K (or alpha K_t) alone
+    if ((rc = run_colored(ctx, nonlin ? kModeKNl : kModeKLin, a))) return rc;
+  } else {
+    // the residual (with M u'') and alpha K + gamma M in one pass
+    if (R) {
+      HIPCALL(ctx, hipMemsetAsync(R, 0, sizeof(double) * 6L * ctx->nnodes * ctx->nbatch, ctx->stream));
+      a.res = R;
+    }
+    if ((rc = run_colored(ctx, nonlin ? kModeJacNl : kModeJacLin, a))) return rc;
+    if (R && ctx->nbc > 0) {
+      hipLaunchKernelGGL(vec_bc_kernel, dim3((6 * ctx->nbc + 255) / 256, ctx->nbatch), dim3(256), 0, ctx->stream,
+                         R, u, 6L * ctx->nnodes, ctx->nbc, ctx->d_bc_nodes, ctx->d_bc_mask, ctx->d_bc_vals, 2);
+      HIPCALL(ctx, hipGetLastError());
+    }
+  }
   return shellgpu_mat_apply_bcs(A, 1.0);
 }
 
@@ -689,11 +918,32 @@ int shellgpu_add_adj_res_product(shellgpu_ctx *ctx, int model, double scale, con
   return run_colored(ctx, model == SHELLGPU_NONLINEAR ? kModeAdjResNl : kModeAdjResLin, a);
 }
 
-int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale, const double *phi,
-                                           const double *u, double *dfdt) {
-  int rc = check_batch(ctx, nullptr);
-  if (rc) return rc;
-  if (!phi || !dfdt) return set_err(ctx, 1, "phi and dfdt are required");
+// psi^T A phi for a symmetric A (K, G(u), M and their derivatives) by
+// polarization, psi^T A phi = (w+^T A w+ - w-^T A w-) / 4 with
+// w+- = psi +- phi, so that products with psi != phi reuse the symmetric
+// kernels: runs fn(w, c) for (phi, 1) when psi is null or phi, else for
+// (w+, 1/4) and (w-, -1/4).
+}  // extern "C"
+template <class Fn>
+static int polarized(shellgpu_ctx *ctx, const double *psi, const double *phi, Fn fn) {
+  if (!psi || psi == phi) return fn(phi, 1.0);
+  const long n = 6L * ctx->nnodes * ctx->nbatch;
+  double *w = nullptr;
+  HIPCALL(ctx, hipMalloc((void **)&w, sizeof(double) * 2 * n));
+  hipLaunchKernelGGL(sum_diff_kernel, dim3(launch_grid(n)), dim3(256), 0, ctx->stream, n, psi, phi, w, w + n);
+  int rc = hipGetLastError() == hipSuccess ? 0 : set_err(ctx, 2, "sum/difference launch failed");
+  if (!rc) rc = fn(w, 0.25);
+  if (!rc) rc = fn(w + n, -0.25);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(w);
+  return rc;
+}
+extern "C" {
+
+// dfdt += scale phi^T (dA/dt) phi
+static int mat_dv_sym(shellgpu_ctx *ctx, int mattype, double scale, const double *phi, const double *u,
+                      double *dfdt) {
+  int rc = 0;
   if (mattype == SHELLGPU_MASS) {
     // one coefficient per design
     std::vector<double> h(ctx->nbatch, scale);
@@ -705,8 +955,6 @@ int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, doubl
     a.dv = dfdt;
     return run_colored(ctx, kModeMassDV, a);
   }
-  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC)
-    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS, SHELLGPU_GEOMETRIC or SHELLGPU_MASS");
   const bool geo = mattype == SHELLGPU_GEOMETRIC;
   if ((rc = set_coef(ctx, geo ? 0.0 : scale, geo ? scale : 0.0))) return rc;
   ElemArgs a = base_args(ctx, phi);
@@ -717,16 +965,31 @@ int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, doubl
   return run_colored(ctx, kModeAdjDV, a);
 }
 
-int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *phi, double *dfdu) {
+int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale, const double *psi,
+                                           const double *phi, const double *u, double *dfdt) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  if (!phi || !dfdt) return set_err(ctx, 1, "phi and dfdt are required");
+  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC && mattype != SHELLGPU_MASS)
+    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS, SHELLGPU_GEOMETRIC or SHELLGPU_MASS");
+  return polarized(ctx, psi, phi,
+                   [&](const double *w, double c) { return mat_dv_sym(ctx, mattype, c * scale, w, u, dfdt); });
+}
+
+int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *psi, const double *phi,
+                                            double *dfdu) {
   int rc = check_batch(ctx, nullptr);
   if (rc) return rc;
   if (!phi || !dfdu) return set_err(ctx, 1, "phi and dfdu are required");
   HIPCALL(ctx, hipMemsetAsync(dfdu, 0, sizeof(double) * 6L * ctx->nnodes * ctx->nbatch, ctx->stream));
-  if ((rc = set_coef(ctx, 0.0, 1.0))) return rc;
-  ElemArgs a = base_args(ctx, phi);
-  a.coef = ctx->d_coef;
-  a.res = dfdu;
-  return run_colored(ctx, kModeAdjDV, a);
+  return polarized(ctx, psi, phi, [&](const double *w, double c) {
+    int r = set_coef(ctx, 0.0, c);
+    if (r) return r;
+    ElemArgs a = base_args(ctx, w);
+    a.coef = ctx->d_coef;
+    a.res = dfdu;
+    return run_colored(ctx, kModeAdjDV, a);
+  });
 }
 
 // per-design coefficient quadruples (c0, c1, c2, 0) into ctx->d_coef
@@ -742,16 +1005,19 @@ static int set_coef4(shellgpu_ctx *ctx, double c0, double c1, double c2) {
   return 0;
 }
 
-int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale, const double *phi,
-                                            const double *u, double *dfdX) {
+int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale, const double *psi,
+                                            const double *phi, const double *u, double *dfdX) {
   int rc = check_batch(ctx, nullptr);
   if (rc) return rc;
   if (!phi || !dfdX) return set_err(ctx, 1, "phi and dfdX are required");
   if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC)
     return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS or SHELLGPU_GEOMETRIC");
   const bool geo = mattype == SHELLGPU_GEOMETRIC;
-  if ((rc = set_coef4(ctx, geo ? 0.0 : scale, geo ? scale : 0.0, 0.0))) return rc;
-  return run_xpt(ctx, phi, geo ? u : nullptr, nullptr, ctx->d_coef, 0, dfdX);
+  return polarized(ctx, psi, phi, [&](const double *w, double c) {
+    int r = set_coef4(ctx, geo ? 0.0 : c * scale, geo ? c * scale : 0.0, 0.0);
+    if (r) return r;
+    return run_xpt(ctx, w, geo ? u : nullptr, nullptr, ctx->d_coef, 0, dfdX);
+  });
 }
 
 int shellgpu_add_adj_res_xpt_product(shellgpu_ctx *ctx, int model, double scale, const double *psi,
@@ -778,6 +1044,19 @@ int shellgpu_vec_set_bcs(shellgpu_ctx *ctx, double *x) {
                      ctx->stream, x, (const double *)nullptr, 6L * ctx->nnodes, ctx->nbc, ctx->d_bc_nodes,
                      ctx->d_bc_mask, ctx->d_bc_vals, 1);
   HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch, int arch_len, char *pci_bus_id,
+                         int pci_len, int *compute_units, size_t *hbm_bytes) {
+  if (!ctx) return 1;
+  hipDeviceProp_t prop;
+  HIPCALL(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  if (name && name_len > 0) snprintf(name, name_len, "%s", prop.name);
+  if (arch && arch_len > 0) snprintf(arch, arch_len, "%s", prop.gcnArchName);
+  if (pci_bus_id && pci_len > 0) HIPCALL(ctx, hipDeviceGetPCIBusId(pci_bus_id, pci_len, ctx->device));
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
   return 0;
 }
 

@@ -16,6 +16,9 @@ struct shellgpu_ctx {
   int nnodes = 0, nelems = 0, nnzb = 0, ncolors = 0, nbatch = 0;
   std::vector<int> h_conn, h_rowp, h_cols, h_color, h_color_ptr, h_color_elems, h_blk;
   int *d_conn = nullptr, *d_rowp = nullptr, *d_cols = nullptr, *d_blk = nullptr;
+  // per node the block index of its diagonal block; per block the index of
+  // the transposed block (the pattern is structurally symmetric)
+  int *d_diag = nullptr, *d_tpos = nullptr;
   int *d_color_elems = nullptr;
   // per element, bit 4 i + j: this element is the first, in colour launch
   // order, to reach BSR block (node i, node j), so a matrix assembly can

@@ -53,6 +53,10 @@ enum ElemMode {
   kModeMassDV = 9,    // dv[e] += coef[d] phi^T dM_e/dt phi (state = phi)
   kModeOutLin = 10,   // output data at the element nodes, linear model (getOutputData)
   kModeOutNl = 11,    // the same, nonlinear model
+  kModeJacLin = 12,   // addJacobian(alpha, beta, gamma): R and alpha K + gamma M, linear model
+  kModeJacNl = 13,    // the same, nonlinear model (alpha K_t)
+  kModeJvpLin = 14,   // (alpha K + gamma M) x per element, nothing assembled, linear model
+  kModeJvpNl = 15,    // the same, nonlinear model
 };
 
 struct ElemArgs {
@@ -68,6 +72,11 @@ struct ElemArgs {
   int nelems;
   Material mat;
   double alpha;
+  // kModeJac*: coefficient of the mass matrix and the second time
+  // derivative of the state (may be null: zero), 6 per node
+  double gamma;
+  const double *ddu;
+  long ddu_stride;
   int fd_sign;   // kModeKNl only: +-1 selects the reference's central-difference
                  // geometric stiffness step (TACSShellElement.h:705-766), 0 = off
   double *vals;  // BSR values
@@ -90,6 +99,14 @@ struct ElemArgs {
   // output modes: out[((d nelems + e) 4 + node) 31 + k], ys = yield stress
   double *out;
   double ys;
+  // kModeJvp*: no matrix is written; instead mvy += mv_scale A_e mvx
+  // (mv_trans = 0) or A_e^T mvx (1) per element, A_e = alpha K_e + gamma
+  // M_e (6 per node, stride mv_stride per design)
+  const double *mvx;
+  double *mvy;
+  long mv_stride;
+  double mv_scale;
+  int mv_trans;
 };
 
 // d/dt of the section stiffness (A' = Q, D' = t^2/4 Q, As' = 5/6 G,
@@ -130,5 +147,26 @@ hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream
 
 hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream);
+
+// Batched FP64 GEMM on the matrix cores (gemm_f64.hip): C_b = alpha op(A_b)
+// op(B_b) + beta C_b, column-major, op(A) M x K, op(B) K x N; entry b's
+// matrices from the pointer arrays Ap/Bp/Cp when set, else at A + b sA etc.
+struct GemmArgs {
+  int M, N, K;
+  double alpha, beta;
+  const double *A;
+  long sA;
+  const double *const *Ap;
+  int lda;
+  const double *B;
+  long sB;
+  const double *const *Bp;
+  int ldb;
+  double *C;
+  long sC;
+  double *const *Cp;
+  int ldc;
+};
+hipError_t launch_gemm_f64(bool transA, bool transB, const GemmArgs &g, int batch, hipStream_t stream);
 
 }  // namespace shellgpu

@@ -166,6 +166,46 @@ int ferr(shellgpu_ctx *ctx, int code, const std::string &msg) {
       return ferr(ctx, 4, std::string(#call) + ": " + rocblas_status_to_string(_s)); \
   } while (0)
 
+// Dense updates of the factorisations: rocBLAS, or with SHELLGPU_GEMM=own
+// the project's FP64 matrix-core GEMM (gemm_f64.hip).
+bool own_gemm() {
+  static const bool own = [] {
+    const char *e = getenv("SHELLGPU_GEMM");
+    return e && strcmp(e, "own") == 0;
+  }();
+  return own;
+}
+
+// strided batch: C_b = alpha op(A_b) op(B_b) + beta C_b
+int gemm_sb(shellgpu_ctx *ctx, bool ta, bool tb, int M, int N, int K, double alpha, const double *A, int lda, long sA,
+            const double *B, int ldb, long sB, double beta, double *C, int ldc, long sC, int batch) {
+  if (own_gemm()) {
+    GemmArgs g{M, N, K, alpha, beta, A, sA, nullptr, lda, B, sB, nullptr, ldb, C, sC, nullptr, ldc};
+    HIPCALL(ctx, launch_gemm_f64(ta, tb, g, batch, ctx->stream));
+    return 0;
+  }
+  BLASCALL(ctx, rocblas_dgemm_strided_batched((rocblas_handle)ctx->blas,
+                                              ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                              tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K,
+                                              &alpha, A, lda, sA, B, ldb, sB, &beta, C, ldc, sC, batch));
+  return 0;
+}
+
+// pointer-array batch
+int gemm_pb(shellgpu_ctx *ctx, bool ta, bool tb, int M, int N, int K, double alpha, const double *const *A, int lda,
+            const double *const *B, int ldb, double beta, double *const *C, int ldc, int batch) {
+  if (own_gemm()) {
+    GemmArgs g{M, N, K, alpha, beta, nullptr, 0, A, lda, nullptr, 0, B, ldb, nullptr, 0, C, ldc};
+    HIPCALL(ctx, launch_gemm_f64(ta, tb, g, batch, ctx->stream));
+    return 0;
+  }
+  BLASCALL(ctx, rocblas_dgemm_batched((rocblas_handle)ctx->blas,
+                                      ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                      tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A,
+                                      lda, B, ldb, &beta, C, ldc, batch));
+  return 0;
+}
+
 // BSR values -> dense level blocks (D_l and L_l; blocks above the diagonal
 // level are not stored).  BC columns are zeroed (the BSR rows of BC dofs are
 // already zero with a unit diagonal), giving the symmetric elimination.
@@ -1442,9 +1482,9 @@ static int cholinv(shellgpu_factor *F, size_t g, int *info) {
       BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
                                           rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A11, M,
                                           (const double *const *)A21, M, A21, M, cnt));
-      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_none, rocblas_operation_transpose, c.n2, c.n2, c.n1,
-                                          &mone, (const double *const *)A21, M, (const double *const *)A21, M, &one,
-                                          A22, M, cnt));
+      int rc = gemm_pb(ctx, false, true, c.n2, c.n2, c.n1, mone, (const double *const *)A21, M,
+                       (const double *const *)A21, M, one, A22, M, cnt);
+      if (rc) return rc;
     } else {
       BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
                                           rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A11, M,
@@ -1473,7 +1513,6 @@ static int spdinv(shellgpu_factor *F, int l, int r0, int n, int depth, int *info
     HIPCALL(ctx, hipGetLastError());
     return 0;
   }
-  rocblas_handle h = (rocblas_handle)ctx->blas;
   int n1 = n / 2;
   n1 -= n1 & 1;
   const int n2 = n - n1;
@@ -1483,15 +1522,11 @@ static int spdinv(shellgpu_factor *F, int l, int r0, int n, int depth, int *info
   const double one = 1.0, mone = -1.0, zero = 0.0;
   int rc = 0;
   if ((rc = spdinv(F, l, r0, n1, depth + 1, info))) return rc;
-  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n2, n1, n1, &one, A21,
-                                              m, ps, A11, m, ps, &zero, W, n2, ws, nb));
-  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, n2, n2, n1,
-                                              &mone, W, n2, ws, A21, m, ps, &one, A22, m, ps, nb));
+  if ((rc = gemm_sb(ctx, false, false, n2, n1, n1, one, A21, m, ps, A11, m, ps, zero, W, n2, ws, nb))) return rc;
+  if ((rc = gemm_sb(ctx, false, true, n2, n2, n1, mone, W, n2, ws, A21, m, ps, one, A22, m, ps, nb))) return rc;
   if ((rc = spdinv(F, l, r0 + n1, n2, depth + 1, info))) return rc;
-  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n2, n1, n2, &mone,
-                                              A22, m, ps, W, n2, ws, &zero, A21, m, ps, nb));
-  BLASCALL(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, n1, n1, n2,
-                                              &mone, W, n2, ws, A21, m, ps, &one, A11, m, ps, nb));
+  if ((rc = gemm_sb(ctx, false, false, n2, n1, n2, mone, A22, m, ps, W, n2, ws, zero, A21, m, ps, nb))) return rc;
+  if ((rc = gemm_sb(ctx, true, false, n1, n1, n2, mone, W, n2, ws, A21, m, ps, one, A11, m, ps, nb))) return rc;
   hipLaunchKernelGGL(transpose_kernel, dim3((n2 + 31) / 32, (n1 + 31) / 32, nb), dim3(256), 0, ctx->stream, n2, n1,
                      A21, m, ps, A12, m, ps);
   HIPCALL(ctx, hipGetLastError());
@@ -1601,18 +1636,18 @@ static int factor_cr(shellgpu_factor *F) {
                                           (const double *const *)pBr, M, pZ, M, cntR));
     // D_l -= Y_i^T Y_i (full GEMM: rocBLAS's batched SYRK runs its
     // diagonal tiles in a slow non-MFMA kernel at these sizes)
-    BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_transpose, rocblas_operation_none, M, M, M, &mone,
-                                        (const double *const *)pB, M, (const double *const *)pB, M, &one, pDl, M,
-                                        cnt));
+    if ((rc = gemm_pb(ctx, true, false, M, M, M, mone, (const double *const *)pB, M, (const double *const *)pB, M,
+                      one, pDl, M, cnt)))
+      return rc;
     if (cntR > 0) {
       // D_r -= W_i W_i^T
-      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_none, rocblas_operation_transpose, M, M, M, &mone,
-                                          (const double *const *)pZ, M, (const double *const *)pZ, M, &one, pDr, M,
-                                          cntR));
+      if ((rc = gemm_pb(ctx, false, true, M, M, M, mone, (const double *const *)pZ, M, (const double *const *)pZ, M,
+                        one, pDr, M, cntR)))
+        return rc;
       // B_r = -W_i Y_i (coupling of r to l)
-      BLASCALL(ctx, rocblas_dgemm_batched(h, rocblas_operation_none, rocblas_operation_none, M, M, M, &mone,
-                                          (const double *const *)pZ, M, (const double *const *)pB, M, &zero, pBr, M,
-                                          cntR));
+      if ((rc = gemm_pb(ctx, false, false, M, M, M, mone, (const double *const *)pZ, M, (const double *const *)pB,
+                        M, zero, pBr, M, cntR)))
+        return rc;
     }
     ioff += cnt;
   }
@@ -2978,7 +3013,7 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
   if ((rc = shellgpu_vec_zero_bcs(ctx, B->gv))) return rc;
   shellgpu_mat *T = B->K;
   if (B->model == SHELLGPU_NONLINEAR) {
-    if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, B->u, nullptr, B->A))) return rc;
+    if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, 1.0, 0.0, 0.0, B->u, nullptr, nullptr, nullptr, B->A))) return rc;
     T = B->A;
   }
   {
@@ -3321,7 +3356,7 @@ int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, in
     const double s = (double)k / nsteps;
     for (int it = 0;; it++) {
       // R(u) - s f with BC rows u - ubar; tangent of the nonlinear model
-      if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, u, R, A))) goto out;
+      if ((rc = shellgpu_assemble_jacobian(ctx, SHELLGPU_NONLINEAR, 1.0, 0.0, 0.0, u, nullptr, nullptr, R, A))) goto out;
       {
         std::vector<double> ms(nb, -s);
         if ((rc = vo.axpy(ms, f0, R))) goto out;

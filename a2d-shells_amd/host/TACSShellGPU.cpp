@@ -116,17 +116,77 @@ TACSMat::TACSMat(TACSAssembler *assembler) : assembler(assembler), mat(NULL) {
 TACSMat::~TACSMat() {
   if (mat) shellgpu_mat_destroy(mat);
 }
+void TACSMat::flush() {
+  if (pend_row.empty()) return;
+  shellgpu_ctx *ctx = assembler->getContext();
+  report(ctx, shellgpu_mat_add_blocks(mat, 0, (int)pend_row.size(), pend_row.data(), pend_col.data(), pend_vals.data()),
+         "TACSMat::addValues");
+  pend_row.clear();
+  pend_col.clear();
+  pend_vals.clear();
+}
+shellgpu_mat *TACSMat::getHandle() {
+  flush();
+  return mat;
+}
 void TACSMat::mult(TACSBVec *x, TACSBVec *y) {
+  flush();
   report(assembler->getContext(), shellgpu_mat_mult(mat, x->getDeviceArray(), y->getDeviceArray()), "TACSMat::mult");
 }
-void TACSMat::zeroEntries() { report(assembler->getContext(), shellgpu_mat_zero(mat), "TACSMat::zeroEntries"); }
+void TACSMat::multTranspose(TACSBVec *x, TACSBVec *y) {
+  flush();
+  report(assembler->getContext(), shellgpu_mat_mult_transpose(mat, x->getDeviceArray(), y->getDeviceArray()),
+         "TACSMat::multTranspose");
+}
+void TACSMat::zeroEntries() {
+  // additions staged before the zeroing would have been zeroed with the rest
+  pend_row.clear();
+  pend_col.clear();
+  pend_vals.clear();
+  report(assembler->getContext(), shellgpu_mat_zero(mat), "TACSMat::zeroEntries");
+}
 void TACSMat::copyValues(TACSMat *m) {
-  report(assembler->getContext(), shellgpu_mat_copy(mat, m->mat), "TACSMat::copyValues");
+  pend_row.clear();
+  pend_col.clear();
+  pend_vals.clear();
+  report(assembler->getContext(), shellgpu_mat_copy(mat, m->getHandle()), "TACSMat::copyValues");
 }
 void TACSMat::axpy(TacsScalar alpha, TACSMat *m) {
-  report(assembler->getContext(), shellgpu_mat_axpy(mat, alpha, m->mat), "TACSMat::axpy");
+  flush();
+  report(assembler->getContext(), shellgpu_mat_axpy(mat, alpha, m->getHandle()), "TACSMat::axpy");
 }
-void TACSMat::applyBCs() { report(assembler->getContext(), shellgpu_mat_apply_bcs(mat, 1.0), "TACSMat::applyBCs"); }
+void TACSMat::scale(TacsScalar alpha) {
+  flush();
+  report(assembler->getContext(), shellgpu_mat_scale(mat, alpha), "TACSMat::scale");
+}
+void TACSMat::addDiag(TacsScalar alpha) {
+  flush();
+  report(assembler->getContext(), shellgpu_mat_add_diag(mat, alpha), "TACSMat::addDiag");
+}
+void TACSMat::applyBCs() {
+  flush();
+  report(assembler->getContext(), shellgpu_mat_apply_bcs(mat, 1.0), "TACSMat::applyBCs");
+}
+void TACSMat::addValues(int nrow, const int *row, int ncol, const int *col, int, int mv, const TacsScalar *values) {
+  for (int i = 0; i < nrow; i++) {
+    if (row[i] < 0) continue;
+    for (int j = 0; j < ncol; j++) {
+      if (col[j] < 0) continue;
+      pend_row.push_back(row[i]);
+      pend_col.push_back(col[j]);
+      // block (i, j): rows 6 i .. 6 i + 5, columns 6 j .. 6 j + 5 of values
+      for (int ii = 0; ii < 6; ii++)
+        for (int jj = 0; jj < 6; jj++) pend_vals.push_back(values[(size_t)(6 * i + ii) * mv + 6 * j + jj]);
+    }
+  }
+}
+void TACSMat::endAssembly() { flush(); }
+void TACSMat::getSize(int *nr, int *nc) {
+  const int n = 6 * assembler->getNumNodes();
+  if (nr) *nr = n;
+  if (nc) *nc = n;
+}
+TACSMat *TACSMat::createDuplicate() { return new TACSMat(assembler); }
 
 TACSSchurPc::TACSSchurPc(TACSMat *mat, int, double, int) : mat(mat), fac(NULL) {
   mat->incref();
@@ -189,7 +249,8 @@ int GMRES::solve(TACSBVec *b, TACSBVec *x, int zero_guess) {
 
 TACSAssembler::TACSAssembler(int num_nodes, int num_elements, const int *conn, const TacsScalar *Xpts, int device)
     : ctx(NULL), num_nodes(num_nodes), num_elements(num_elements), model(0), elements(num_elements, NULL),
-      thick(num_elements, 1.0), vars(NULL), designs_dirty(true), conn_h(conn, conn + 4 * (size_t)num_elements),
+      thick(num_elements, 1.0), vars(NULL), dvars(NULL), ddvars(NULL), ddvars_zero(true), designs_dirty(true),
+      conn_h(conn, conn + 4 * (size_t)num_elements),
       comp_h(num_elements, 0), X_h(Xpts, Xpts + 3 * (size_t)num_nodes), num_comp(1) {
   int rc = shellgpu_create(&ctx, device);
   if (rc) {
@@ -202,6 +263,8 @@ TACSAssembler::TACSAssembler(int num_nodes, int num_elements, const int *conn, c
 }
 TACSAssembler::~TACSAssembler() {
   if (vars) vars->decref();
+  if (dvars) dvars->decref();
+  if (ddvars) ddvars->decref();
   for (TACSElement *e : elements)
     if (e) e->decref();
   if (ctx) shellgpu_destroy(ctx);
@@ -314,49 +377,106 @@ void TACSAssembler::setDesignVars(TACSBVec *x) {
   syncDesigns();
 }
 void TACSAssembler::zeroVariables() { vars->zeroEntries(); }
-void TACSAssembler::setVariables(TACSBVec *u) { vars->copyValues(u); }
-void TACSAssembler::getVariables(TACSBVec *u) { u->copyValues(vars); }
+void TACSAssembler::zeroDotVariables() {
+  if (dvars) dvars->zeroEntries();
+}
+void TACSAssembler::zeroDDotVariables() {
+  if (ddvars) ddvars->zeroEntries();
+  ddvars_zero = true;
+}
+void TACSAssembler::setVariables(TACSBVec *q, TACSBVec *qdot, TACSBVec *qddot) {
+  if (q) vars->copyValues(q);
+  if (qdot) {
+    if (!dvars) {
+      dvars = createVec();
+      dvars->incref();
+    }
+    dvars->copyValues(qdot);
+  }
+  if (qddot) {
+    if (!ddvars) {
+      ddvars = createVec();
+      ddvars->incref();
+    }
+    ddvars->copyValues(qddot);
+    ddvars_zero = false;
+  }
+}
+void TACSAssembler::getVariables(TACSBVec *q, TACSBVec *qdot, TACSBVec *qddot) {
+  if (q) q->copyValues(vars);
+  if (qdot) {
+    if (dvars)
+      qdot->copyValues(dvars);
+    else
+      qdot->zeroEntries();
+  }
+  if (qddot) {
+    if (ddvars)
+      qddot->copyValues(ddvars);
+    else
+      qddot->zeroEntries();
+  }
+}
 void TACSAssembler::applyBCs(TACSBVec *vec) { vec->applyBCs(); }
 void TACSAssembler::applyBCs(TACSMat *mat) { mat->applyBCs(); }
 void TACSAssembler::setBCs(TACSBVec *vec) { vec->setBCs(); }
 
-void TACSAssembler::assembleRes(TACSBVec *residual) {
+void TACSAssembler::assembleRes(TACSBVec *residual, const TacsScalar) {
   syncDesigns();
-  check(shellgpu_assemble_res(ctx, model, vars->getDeviceArray(), residual->getDeviceArray()),
-        "TACSAssembler::assembleRes");
+  if (check(shellgpu_assemble_res(ctx, model, vars->getDeviceArray(), residual->getDeviceArray()),
+            "TACSAssembler::assembleRes"))
+    return;
+  if (!ddvars_zero && ddvars) {
+    // the inertial forces M q'' (zero on BC dofs; the BC rows keep u - ubar)
+    TACSBVec *t = createVec();
+    t->incref();
+    check(shellgpu_add_jacobian_vec_product(ctx, model, 1.0, 0.0, 0.0, 1.0, vars->getDeviceArray(), NULL, NULL,
+                                            ddvars->getDeviceArray(), t->getDeviceArray(), 0),
+          "TACSAssembler::assembleRes (inertia)");
+    residual->axpy(1.0, t);
+    t->decref();
+  }
 }
 
 void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, TACSBVec *residual,
-                                     TACSMat *A) {
-  if (beta != 0.0 || gamma != 0.0) {
-    fprintf(stderr, "TACSAssembler::assembleJacobian: only static Jacobians (beta = gamma = 0) are available\n");
-    return;
-  }
+                                     TACSMat *A, MatrixOrientation, const TacsScalar) {
   syncDesigns();
-  if (check(shellgpu_assemble_jacobian(ctx, model, vars->getDeviceArray(),
-                                       residual ? residual->getDeviceArray() : NULL, A->getHandle()),
-            "TACSAssembler::assembleJacobian"))
-    return;
-  if (alpha != 1.0) {
-    int nnodes, nelems, nnzb, ncolors, nbatch;
-    shellgpu_get_sizes(ctx, &nnodes, &nelems, &nnzb, &ncolors, &nbatch);
-    check(shellgpu_vec_scale(ctx, 36 * (size_t)nnzb, alpha, shellgpu_mat_values(A->getHandle())), "scale");
-    A->applyBCs();
-  }
+  check(shellgpu_assemble_jacobian(ctx, model, alpha, beta, gamma, vars->getDeviceArray(),
+                                   dvars ? dvars->getDeviceArray() : NULL,
+                                   (ddvars && !ddvars_zero) ? ddvars->getDeviceArray() : NULL,
+                                   residual ? residual->getDeviceArray() : NULL, A->getHandle()),
+        "TACSAssembler::assembleJacobian");
 }
 
-void TACSAssembler::assembleMatType(ElementMatrixType matType, TACSMat *A) {
+// the shellgpu matrix type of getMatType(matType) for the assembler's model
+static int mat_kind(ElementMatrixType matType, int model) {
+  if (matType == TACS_GEOMETRIC_STIFFNESS_MATRIX) return SHELLGPU_GEOMETRIC;
+  if (matType == TACS_MASS_MATRIX) return SHELLGPU_MASS;
+  // getMatType(STIFFNESS) is the element's own Jacobian at the variables
+  return model ? SHELLGPU_TANGENT_NONLINEAR : SHELLGPU_STIFFNESS;
+}
+
+void TACSAssembler::assembleMatCombo(ElementMatrixType matTypes[], TacsScalar scale[], int nmats, TACSMat *A,
+                                     MatrixOrientation, const TacsScalar) {
   syncDesigns();
-  int kind;
-  if (matType == TACS_GEOMETRIC_STIFFNESS_MATRIX) {
-    kind = SHELLGPU_GEOMETRIC;
-  } else if (matType == TACS_MASS_MATRIX) {
-    kind = SHELLGPU_MASS;
-  } else {
-    // getMatType(STIFFNESS) is the element's own Jacobian at the variables
-    kind = model ? SHELLGPU_TANGENT_NONLINEAR : SHELLGPU_STIFFNESS;
-  }
-  check(shellgpu_assemble_mattype(ctx, kind, vars->getDeviceArray(), A->getHandle()), "TACSAssembler::assembleMatType");
+  std::vector<int> kinds(nmats);
+  for (int k = 0; k < nmats; k++) kinds[k] = mat_kind(matTypes[k], model);
+  check(shellgpu_assemble_mat_combo(ctx, nmats, kinds.data(), scale, vars->getDeviceArray(), A->getHandle()),
+        "TACSAssembler::assembleMatCombo");
+}
+
+void TACSAssembler::addJacobianVecProduct(TacsScalar scale, TacsScalar alpha, TacsScalar beta, TacsScalar gamma,
+                                          TACSBVec *x, TACSBVec *y, MatrixOrientation matOr, const TacsScalar) {
+  syncDesigns();
+  check(shellgpu_add_jacobian_vec_product(ctx, model, scale, alpha, beta, gamma, vars->getDeviceArray(), NULL, NULL,
+                                          x->getDeviceArray(), y->getDeviceArray(), matOr == TACS_MAT_TRANSPOSE),
+        "TACSAssembler::addJacobianVecProduct");
+}
+
+void TACSAssembler::assembleMatType(ElementMatrixType matType, TACSMat *A, MatrixOrientation, const TacsScalar) {
+  syncDesigns();
+  check(shellgpu_assemble_mattype(ctx, mat_kind(matType, model), vars->getDeviceArray(), A->getHandle()),
+        "TACSAssembler::assembleMatType");
 }
 
 void TACSAssembler::addAdjointResProducts(TacsScalar scale, int n, TACSBVec **adjoint, TACSBVec **dfdx) {
@@ -377,44 +497,37 @@ void TACSAssembler::addAdjointResXptSensProducts(TacsScalar scale, int n, TACSBV
 
 void TACSAssembler::addMatDVSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
                                              TACSBVec *phi, TACSBVec *dfdx) {
-  if (psi != phi) {
-    fprintf(stderr, "TACSAssembler::addMatDVSensInnerProduct: psi and phi must be the same vector\n");
-    return;
-  }
   syncDesigns();
   const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC
                    : matType == TACS_MASS_MATRIX              ? SHELLGPU_MASS
                                                               : SHELLGPU_STIFFNESS;
-  check(shellgpu_add_mat_dv_sens_inner_product(ctx, kind, scale, phi->getDeviceArray(), vars->getDeviceArray(),
-                                               dfdx->getDeviceArray()),
+  check(shellgpu_add_mat_dv_sens_inner_product(ctx, kind, scale, psi->getDeviceArray(), phi->getDeviceArray(),
+                                               vars->getDeviceArray(), dfdx->getDeviceArray()),
         "TACSAssembler::addMatDVSensInnerProduct");
 }
 
 void TACSAssembler::addMatXptSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
                                               TACSBVec *phi, TACSBVec *dfdX) {
-  if (psi != phi) {
-    fprintf(stderr, "TACSAssembler::addMatXptSensInnerProduct: psi and phi must be the same vector\n");
-    return;
-  }
   syncDesigns();
   if (matType == TACS_MASS_MATRIX) {
     fprintf(stderr, "TACSAssembler::addMatXptSensInnerProduct: no coordinate derivative of the mass matrix\n");
     return;
   }
   const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC : SHELLGPU_STIFFNESS;
-  check(shellgpu_add_mat_xpt_sens_inner_product(ctx, kind, scale, phi->getDeviceArray(), vars->getDeviceArray(),
-                                                dfdX->getDeviceArray()),
+  check(shellgpu_add_mat_xpt_sens_inner_product(ctx, kind, scale, psi->getDeviceArray(), phi->getDeviceArray(),
+                                                vars->getDeviceArray(), dfdX->getDeviceArray()),
         "TACSAssembler::addMatXptSensInnerProduct");
 }
 
 void TACSAssembler::evalMatSVSensInnerProduct(ElementMatrixType matType, TACSBVec *psi, TACSBVec *phi,
                                               TACSBVec *res) {
-  if (matType != TACS_GEOMETRIC_STIFFNESS_MATRIX || psi != phi) {
-    fprintf(stderr, "TACSAssembler::evalMatSVSensInnerProduct: geometric stiffness with psi == phi only\n");
+  if (matType != TACS_GEOMETRIC_STIFFNESS_MATRIX) {
+    fprintf(stderr, "TACSAssembler::evalMatSVSensInnerProduct: only the geometric stiffness depends on the state\n");
     return;
   }
   syncDesigns();
-  check(shellgpu_eval_mat_sv_sens_inner_product(ctx, phi->getDeviceArray(), res->getDeviceArray()),
+  check(shellgpu_eval_mat_sv_sens_inner_product(ctx, psi->getDeviceArray(), phi->getDeviceArray(),
+                                                res->getDeviceArray()),
         "TACSAssembler::evalMatSVSensInnerProduct");
 }
 

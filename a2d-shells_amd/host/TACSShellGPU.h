@@ -18,6 +18,16 @@
 
 #include "shellgpu.h"
 
+// MPI, when its header is on the include path: the reference's drivers call
+// MPI_Init / MPI_Comm_rank / MPI_Finalize and pass MPI_COMM_WORLD to the
+// mesh loader.  This library runs one rank; the communicator overloads
+// below check that.
+#if defined(__has_include)
+#if __has_include(<mpi.h>)
+#include <mpi.h>
+#endif
+#endif
+
 namespace shellgpu {
 
 typedef double TacsScalar;
@@ -34,6 +44,20 @@ const int TACS_OUTPUT_EXTRAS = 32;
 const int TACS_OUTPUT_LOADS = 64;
 enum ElementType { TACS_ELEMENT_NONE = 0, TACS_BEAM_OR_SHELL_ELEMENT = 4 };
 enum ElementLayout { TACS_LAYOUT_NONE = 0, TACS_QUAD_ELEMENT = 8 };
+
+// MatrixOrientation (src/bpmat/KSM.h:34), same values
+enum MatrixOrientation { TACS_MAT_NORMAL, TACS_MAT_TRANSPOSE };
+
+#ifdef MPI_VERSION
+// the one-rank check of the communicator overloads
+inline void tacs_check_one_rank(MPI_Comm comm, const char *who) {
+  int size = 1;
+  MPI_Comm_size(comm, &size);
+  if (size != 1)
+    fprintf(stderr, "%s: this library runs one rank per GPU context; the communicator has %d ranks\n", who,
+            size);
+}
+#endif
 
 // ElementMatrixType (src/elements/TACSElementTypes.h:101-107), same values
 enum ElementMatrixType {
@@ -180,16 +204,38 @@ class TACSMat : public TACSObject {
   explicit TACSMat(TACSAssembler *assembler);
   ~TACSMat();
   void mult(TACSBVec *x, TACSBVec *y);
+  // y = A^T x (TACSMat::multTranspose, KSM.h:176)
+  void multTranspose(TACSBVec *x, TACSBVec *y);
   void zeroEntries();
   void copyValues(TACSMat *mat);
   void axpy(TacsScalar alpha, TACSMat *mat);
+  // A = alpha A; diagonal += alpha (KSM.h:178-180)
+  void scale(TacsScalar alpha);
+  void addDiag(TacsScalar alpha);
   void applyBCs();
-  shellgpu_mat *getHandle() { return mat; }
+  // TACSMat::addValues (KSM.h:153-154, TACSSchurMat::addValues,
+  // TACSSchurMat.cpp:453-526): values is the dense (6 nrow) x mv row-major
+  // matrix of the blocks (row[i], col[j]) (block (i, j) at row 6 i, column
+  // 6 j); negative indices are skipped, blocks outside the pattern are
+  // reported on stderr.  Additions are staged on the host and applied in one
+  // launch at endAssembly, or before any other use of the matrix.
+  void addValues(int nrow, const int *row, int ncol, const int *col, int nv, int mv, const TacsScalar *values);
+  void beginAssembly() {}
+  void endAssembly();
+  // scalar rows and columns
+  void getSize(int *nr, int *nc);
+  // a matrix with the same pattern (values zero)
+  TACSMat *createDuplicate();
+  // the device matrix (staged additions applied first)
+  shellgpu_mat *getHandle();
   TACSAssembler *getAssembler() { return assembler; }
 
  private:
+  void flush();
   TACSAssembler *assembler;
   shellgpu_mat *mat;
+  std::vector<int> pend_row, pend_col;
+  std::vector<TacsScalar> pend_vals;
 };
 class TACSSchurMat : public TACSMat {
  public:
@@ -304,27 +350,42 @@ class TACSAssembler : public TACSObject {
 
   void getDesignVars(TACSBVec *x);
   void setDesignVars(TACSBVec *x);
+  // the state and its first and second time derivatives
+  // (TACSAssembler.h:183-191)
   void zeroVariables();
-  void setVariables(TACSBVec *u);
-  void getVariables(TACSBVec *u);
+  void zeroDotVariables();
+  void zeroDDotVariables();
+  void setVariables(TACSBVec *q, TACSBVec *qdot = NULL, TACSBVec *qddot = NULL);
+  void getVariables(TACSBVec *q, TACSBVec *qdot = NULL, TACSBVec *qddot = NULL);
   TACSBVec *getVariables() { return vars; }
   void applyBCs(TACSBVec *vec);
   void applyBCs(TACSMat *mat);
   void setBCs(TACSBVec *vec);
 
-  // element loops at the current variables (TACSAssembler.cpp:4000-4250)
-  void assembleRes(TACSBVec *residual);
-  void assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, TACSBVec *residual,
-                        TACSMat *A);
-  void assembleMatType(ElementMatrixType matType, TACSMat *A);
+  // element loops at the current variables (TACSAssembler.h:213-228,
+  // TACSAssembler.cpp:4000-4390).  lambda scales the contributions of
+  // auxiliary elements in the reference; this path has none (loads are
+  // force vectors), so it does not enter.  The shell's element matrices
+  // (K, K_t, G, M) are symmetric, so TACS_MAT_TRANSPOSE assembles the same
+  // matrices as TACS_MAT_NORMAL; addJacobianVecProduct forms the transposed
+  // element products.
+  void assembleRes(TACSBVec *residual, const TacsScalar lambda = 1.0);
+  void assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, TACSBVec *residual, TACSMat *A,
+                        MatrixOrientation matOr = TACS_MAT_NORMAL, const TacsScalar lambda = 1.0);
+  void assembleMatType(ElementMatrixType matType, TACSMat *A, MatrixOrientation matOr = TACS_MAT_NORMAL,
+                       const TacsScalar lambda = 1.0);
+  void assembleMatCombo(ElementMatrixType matTypes[], TacsScalar scale[], int nmats, TACSMat *A,
+                        MatrixOrientation matOr = TACS_MAT_NORMAL, const TacsScalar lambda = 1.0);
+  void addJacobianVecProduct(TacsScalar scale, TacsScalar alpha, TacsScalar beta, TacsScalar gamma, TACSBVec *x,
+                             TACSBVec *y, MatrixOrientation matOr = TACS_MAT_NORMAL,
+                             const TacsScalar lambda = 1.0);
 
   // sensitivities (the reference's archived assembler API)
   void addAdjointResProducts(TacsScalar scale, int num_adjoints, TACSBVec **adjoint,
                              TACSBVec **dfdx);
   void addAdjointResXptSensProducts(TacsScalar scale, int num_adjoints, TACSBVec **adjoint,
                                     TACSBVec **dfdX);
-  // psi must be phi: the symmetric products are what the buckling
-  // sensitivities use
+  // psi != phi through the symmetry of the element matrices (polarization)
   void addMatDVSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
                                 TACSBVec *phi, TACSBVec *dfdx);
   void addMatXptSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
@@ -358,7 +419,8 @@ class TACSAssembler : public TACSObject {
   std::vector<TACSElement *> elements;
   std::vector<TacsScalar> thick;
   std::vector<TacsScalar> up_thick, up_temp;  // values last uploaded
-  TACSBVec *vars;
+  TACSBVec *vars, *dvars, *ddvars;  // dvars, ddvars created on first use
+  bool ddvars_zero;                 // ddvars is known to be zero
   bool designs_dirty;
   std::vector<int> conn_h, comp_h;
   std::vector<TacsScalar> X_h;
@@ -478,6 +540,10 @@ class TACSToFH5 : public TACSObject {
 class TACSMeshLoader : public TACSObject {
  public:
   TACSMeshLoader();
+#ifdef MPI_VERSION
+  // TACSMeshLoader(MPI_Comm) (src/io/TACSMeshLoader.h:63), one rank
+  explicit TACSMeshLoader(MPI_Comm comm) : TACSMeshLoader() { tacs_check_one_rank(comm, "TACSMeshLoader"); }
+#endif
   ~TACSMeshLoader();
   int scanBDFFile(const char *file_name);
   int getNumComponents() const { return num_components; }

@@ -68,13 +68,19 @@ def load_library(path=LIB_PATH):
         "shellgpu_mat_axpy": (C.c_int, [_VP, C.c_double, _VP]),
         "shellgpu_mat_apply_bcs": (C.c_int, [_VP, C.c_double]),
         "shellgpu_mat_mult": (C.c_int, [_VP, _VP, _VP]),
+        "shellgpu_mat_mult_transpose": (C.c_int, [_VP, _VP, _VP]),
+        "shellgpu_mat_scale": (C.c_int, [_VP, C.c_double]),
+        "shellgpu_mat_add_diag": (C.c_int, [_VP, C.c_double]),
+        "shellgpu_mat_add_blocks": (C.c_int, [_VP, C.c_int, C.c_int, _IP, _IP, _DP]),
+        "shellgpu_assemble_mat_combo": (C.c_int, [_VP, C.c_int, _IP, _DP, _VP, _VP]),
+        "shellgpu_add_jacobian_vec_product": (C.c_int, [_VP, C.c_int] + [C.c_double] * 4 + [_VP] * 5 + [C.c_int]),
         "shellgpu_assemble_mattype": (C.c_int, [_VP, C.c_int, _VP, _VP]),
         "shellgpu_assemble_res": (C.c_int, [_VP, C.c_int, _VP, _VP]),
-        "shellgpu_assemble_jacobian": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+        "shellgpu_assemble_jacobian": (C.c_int, [_VP, C.c_int] + [C.c_double] * 3 + [_VP] * 5),
         "shellgpu_add_adj_res_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
-        "shellgpu_add_mat_dv_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
-        "shellgpu_eval_mat_sv_sens_inner_product": (C.c_int, [_VP, _VP, _VP]),
-        "shellgpu_add_mat_xpt_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
+        "shellgpu_add_mat_dv_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP, _VP]),
+        "shellgpu_eval_mat_sv_sens_inner_product": (C.c_int, [_VP, _VP, _VP, _VP]),
+        "shellgpu_add_mat_xpt_sens_inner_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP, _VP]),
         "shellgpu_add_adj_res_xpt_product": (C.c_int, [_VP, C.c_int, C.c_double, _VP, _VP, _VP]),
         "shellgpu_vec_zero_bcs": (C.c_int, [_VP, _VP]),
         "shellgpu_vec_set_bcs": (C.c_int, [_VP, _VP]),
@@ -105,6 +111,8 @@ def load_library(path=LIB_PATH):
         "shellgpu_frequency_add_eigen_sens": (C.c_int, [_VP, _DP, _VP]),
         "shellgpu_ks_aggregate": (C.c_int, [C.c_int, C.c_double, _DP, _DP, _DP]),
         "shellgpu_newton_solve": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_double, _VP, _IP, _DP]),
+        "shellgpu_device_info": (C.c_int, [_VP, C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int, _IP,
+                                           C.POINTER(C.c_size_t)]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
         "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
         "shellgpu_timer_start": (C.c_int, [_VP]),
@@ -218,6 +226,24 @@ class Matrix:
 
     def mult(self, x, y):
         self.ctx._check(self.ctx.lib.shellgpu_mat_mult(self.h, x.ptr, y.ptr))
+
+    def mult_transpose(self, x, y):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_mult_transpose(self.h, x.ptr, y.ptr))
+
+    def scale(self, alpha):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_scale(self.h, float(alpha)))
+
+    def add_diag(self, alpha):
+        self.ctx._check(self.ctx.lib.shellgpu_mat_add_diag(self.h, float(alpha)))
+
+    def add_blocks(self, rows, cols, vals, design=0):
+        """TACSMat::addValues as blocks: vals[k] (6x6, row-major) added to
+        block (rows[k], cols[k]) of design `design`."""
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1, 36)
+        self.ctx._check(self.ctx.lib.shellgpu_mat_add_blocks(self.h, int(design), rows.size, _ip(rows), _ip(cols),
+                                                             _dp(vals)))
 
     def destroy(self):
         if self.h and self.ctx.h:
@@ -436,6 +462,15 @@ class Context:
     def synchronize(self):
         self._check(self.lib.shellgpu_synchronize(self.h))
 
+    def device_info(self):
+        """The context's GPU: name, architecture, PCI bus id, compute units,
+        HBM bytes."""
+        name, arch, pci = C.create_string_buffer(256), C.create_string_buffer(64), C.create_string_buffer(64)
+        cu, mem = C.c_int(), C.c_size_t()
+        self._check(self.lib.shellgpu_device_info(self.h, name, 256, arch, 64, pci, 64, C.byref(cu), C.byref(mem)))
+        return {"name": name.value.decode(), "arch": arch.value.decode(), "pci_bus_id": pci.value.decode(),
+                "compute_units": cu.value, "hbm_bytes": mem.value}
+
     def set_mesh(self, conn, Xpts):
         conn = np.ascontiguousarray(conn, dtype=np.int32)
         Xpts = np.ascontiguousarray(Xpts, dtype=np.float64)
@@ -493,9 +528,26 @@ class Context:
     def assemble_res(self, R, u=None, model=LINEAR):
         self._check(self.lib.shellgpu_assemble_res(self.h, int(model), u.ptr if u else None, R.ptr))
 
-    def assemble_jacobian(self, A, R=None, u=None, model=LINEAR):
+    def assemble_jacobian(self, A, R=None, u=None, model=LINEAR, alpha=1.0, beta=0.0, gamma=0.0, dudt=None,
+                          d2udt2=None):
+        """assembleJacobian: A = alpha dR/du + gamma M, R = R(u) + M d2udt2
+        (BCs applied to both)."""
         self._check(self.lib.shellgpu_assemble_jacobian(
-            self.h, int(model), u.ptr if u else None, R.ptr if R else None, A.h))
+            self.h, int(model), float(alpha), float(beta), float(gamma), u.ptr if u else None,
+            dudt.ptr if dudt else None, d2udt2.ptr if d2udt2 else None, R.ptr if R else None, A.h))
+
+    def assemble_mat_combo(self, A, mattypes, scale, u=None):
+        t = np.ascontiguousarray(mattypes, dtype=np.int32)
+        c = np.ascontiguousarray(scale, dtype=np.float64)
+        self._check(self.lib.shellgpu_assemble_mat_combo(self.h, t.size, _ip(t), _dp(c), u.ptr if u else None, A.h))
+
+    def add_jacobian_vec_product(self, y, x, scale=1.0, alpha=1.0, beta=0.0, gamma=0.0, u=None, model=LINEAR,
+                                 transpose=False):
+        """y += scale (alpha dR/du + gamma M) x (or its transpose), then
+        y[bc] = 0 (addJacobianVecProduct)."""
+        self._check(self.lib.shellgpu_add_jacobian_vec_product(
+            self.h, int(model), float(scale), float(alpha), float(beta), float(gamma), u.ptr if u else None,
+            None, None, x.ptr, y.ptr, int(bool(transpose))))
 
     def set_ordering_seed(self, nodes):
         nodes = np.ascontiguousarray(nodes, dtype=np.int32)
@@ -548,24 +600,30 @@ class Context:
         self._check(self.lib.shellgpu_add_adj_res_product(self.h, model, float(scale), psi.ptr,
                                                           u.ptr if u else None, dfdt.ptr))
 
-    def add_mat_dv_sens_inner_product(self, dfdt, mattype, phi, u=None, scale=1.0):
-        self._check(self.lib.shellgpu_add_mat_dv_sens_inner_product(self.h, mattype, float(scale), phi.ptr,
+    def add_mat_dv_sens_inner_product(self, dfdt, mattype, phi, u=None, scale=1.0, psi=None):
+        """dfdt += scale psi^T (dA/dt) phi (psi None: psi = phi)"""
+        self._check(self.lib.shellgpu_add_mat_dv_sens_inner_product(self.h, mattype, float(scale),
+                                                                    psi.ptr if psi else None, phi.ptr,
                                                                     u.ptr if u else None, dfdt.ptr))
 
     def coord_vector(self):
         """Zeroed device array of per-design nodal coordinates [nbatch*3*nnodes]."""
         return DeviceArray(self, self.nbatch * 3 * self.nnodes).zero()
 
-    def add_mat_xpt_sens_inner_product(self, dfdX, mattype, phi, u=None, scale=1.0):
-        self._check(self.lib.shellgpu_add_mat_xpt_sens_inner_product(self.h, mattype, float(scale), phi.ptr,
+    def add_mat_xpt_sens_inner_product(self, dfdX, mattype, phi, u=None, scale=1.0, psi=None):
+        """dfdX += scale d(psi^T A phi)/dX (psi None: psi = phi)"""
+        self._check(self.lib.shellgpu_add_mat_xpt_sens_inner_product(self.h, mattype, float(scale),
+                                                                     psi.ptr if psi else None, phi.ptr,
                                                                      u.ptr if u else None, dfdX.ptr))
 
     def add_adj_res_xpt_product(self, dfdX, psi, u=None, scale=1.0, model=LINEAR):
         self._check(self.lib.shellgpu_add_adj_res_xpt_product(self.h, model, float(scale), psi.ptr,
                                                               u.ptr if u else None, dfdX.ptr))
 
-    def eval_mat_sv_sens_inner_product(self, phi, dfdu):
-        self._check(self.lib.shellgpu_eval_mat_sv_sens_inner_product(self.h, phi.ptr, dfdu.ptr))
+    def eval_mat_sv_sens_inner_product(self, phi, dfdu, psi=None):
+        """dfdu = d(psi^T G(u) phi)/du (psi None: psi = phi)"""
+        self._check(self.lib.shellgpu_eval_mat_sv_sens_inner_product(self.h, psi.ptr if psi else None, phi.ptr,
+                                                                     dfdu.ptr))
 
     def newton_solve(self, force, nsteps=5, max_iters=20, rtol=1e-10, u0=None):
         """Load-stepped Newton on the nonlinear model: returns (u device

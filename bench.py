@@ -271,6 +271,33 @@ def batched_designs(nb=64):
             "batched_vs_single_grad_max_rel_diff": float(np.abs(gtb[0] - gt1[0]).max() / np.abs(gt1[0]).max())}
 
 
+def spmv_hbm(nb=64, reps=20):
+    """6x6-block SpMV over 64 matrices of the 100 x 200 cylinder (one batched
+    launch, 64 x 52 MB = 3.3 GB of values: far beyond the 256 MB MALL, so
+    the matrix streams from HBM)."""
+    cyl = Cylinder(100, 200, R=1.0, L=2.0)
+    thick = np.concatenate([seeded_thickness(cyl.nelems, 2000 + d) for d in range(nb)])
+    ctx = make_context(cyl, thick)
+    K = ctx.matrix()
+    ctx.assemble_mattype(sg.STIFFNESS, K)
+    x = ctx.vector(np.random.default_rng(8).standard_normal(nb * 6 * cyl.nnodes))
+    y = ctx.vector()
+    for _ in range(3):
+        K.mult(x, y)
+    ctx.set_timing(True)
+    for _ in range(reps):
+        K.mult(x, y)
+    _, _, ms, n = ctx.get_timing()
+    ctx.set_timing(False)
+    per = spmv_bytes(ctx.nnzb, ctx.nnodes) * nb
+    out = {"workload": f"cylinder 100x200, {nb} stiffness matrices (one batched launch)",
+           "bytes_per_launch": per, "avg_launch_ms": ms / n, "GBps": per / (ms / n * 1e-3) / 1e9,
+           "frac_of_hbm_peak": per / (ms / n * 1e-3) / 1e9 / PEAK_HBM_GBPS}
+    K.destroy()
+    ctx.close()
+    return out
+
+
 def accuracy_vs_oracle():
     """First 5 loads of the 20 x 40 cylinder (t = 0.002) against the oracle;
     both run the same shift-invert Lanczos problem."""
@@ -292,62 +319,123 @@ def accuracy_vs_oracle():
             "gpu_s": gpu_s, "oracle_cpu_s": cpu_s, "oracle_cores": 1}
 
 
-def cpu_baseline(cyl, thick, seconds):
-    """The reference's own element code on the host, when its build is
-    present (oracle/_ref/ref_timer: TACSQuad4Shell::getMatType(K) compiled
-    from the reference's sources), else the oracle (scalar C restatement of
-    the reference element loop); both over K of the same 2,000 elements for
-    ~`seconds` on one core.  The oracle's rate is reported beside the
-    reference's."""
+def host_cpu():
+    """Host CPU model and logical CPU count (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def cpu_workers():
+    """Host worker processes for the CPU baselines: every logical CPU, at
+    most 16 (the CPU share of one GPU on the measurement hosts)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _oracle_k_worker(args):
+    """One worker of the oracle's element K loop (BSR scatter included)."""
+    Xpts, conn, thick, nnodes, seconds = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
-
     m = O.material()
-    sample = 2000
-    conn = cyl.conn[:sample]
-    rowp, cols = O.node_csr(cyl.nnodes, conn)
+    rowp, cols = O.node_csr(nnodes, conn)
     n_done, t_used = 0, 0.0
     while t_used < seconds:
         t0 = time.perf_counter()
-        O.assemble_mat(O.KIND_K, 0, m, cyl.Xpts, conn, thick[0, :sample], rowp, cols)
+        O.assemble_mat(O.KIND_K, 0, m, Xpts, conn, thick, rowp, cols)
         t_used += time.perf_counter() - t0
-        n_done += sample
+        n_done += conn.shape[0]
+    return n_done, t_used
+
+
+def cpu_baseline(cyl, thick, seconds):
+    """The reference's own element code on the host's cores, when its build
+    is present (oracle/_ref/ref_timer: TACSQuad4Shell::getMatType(K) compiled
+    from the reference's sources), one process per worker; the oracle's
+    element loop with its BSR scatter (a scalar C restatement of the
+    reference) is reported beside it, and is the baseline where the
+    reference build is absent.  Each worker repeats K of the same 2,000
+    elements for ~`seconds`; the rate is the sum over workers."""
+    import multiprocessing as mp
+    import subprocess
+    import tempfile
+
+    nw = cpu_workers()
+    sample = 2000
+    conn = np.ascontiguousarray(cyl.conn[:sample])
+    with mp.get_context("fork").Pool(nw) as pool:
+        res = pool.map(_oracle_k_worker, [(cyl.Xpts, conn, thick[0, :sample], cyl.nnodes, seconds)] * nw)
     port = {
-        "value": n_done / t_used,
+        "value": sum(d / t for d, t in res),
         "unit": "elem Jacobians/s",
-        "cores": 1,
+        "cores": nw,
         "kind": "port",
         "sample": f"K of the first {sample} elements of the 200x400 cylinder assembled into BSR by the oracle, "
-                  f"repeated for {t_used:.1f} s ({n_done} element Jacobians) on one host core",
+                  f"repeated for ~{seconds:.0f} s in each of {nw} processes ({sum(d for d, _ in res)} element "
+                  "Jacobians)",
+        "host": host_cpu(),
     }
     timer = os.path.join(ROOT, "oracle", "_ref", "ref_timer")
     if not os.path.exists(timer):
         return port
-    import subprocess
-    import tempfile
     rec = np.concatenate([cyl.Xpts[conn].reshape(sample, 12), thick[0, :sample, None]], axis=1)
     with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as fh:
         rec.astype(np.float64).tofile(fh)
         path = fh.name
     try:
-        out = subprocess.run([timer, path, str(seconds)], capture_output=True, text=True, timeout=seconds + 120)
-        done, used, _ = out.stdout.split()
-        done, used = int(done), float(used)
-    except (subprocess.SubprocessError, ValueError, OSError):
+        procs = [subprocess.Popen([timer, path, str(seconds)], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                                  text=True) for _ in range(nw)]
+        outs = [p.communicate(timeout=seconds + 120)[0].split() for p in procs]
+        rates = [int(o[0]) / float(o[1]) for o in outs]
+        done = sum(int(o[0]) for o in outs)
+    except (subprocess.SubprocessError, ValueError, OSError, IndexError):
         return port
     finally:
         os.unlink(path)
     return {
-        "value": done / used,
+        "value": sum(rates),
         "unit": "elem Jacobians/s",
-        "cores": 1,
+        "cores": nw,
         "kind": "reference",
         "sample": f"TACSQuad4Shell::getMatType(TACS_STIFFNESS_MATRIX) (the reference's element code, compiled "
                   f"from its sources) on the first {sample} elements of the 200x400 cylinder, repeated for "
-                  f"{used:.1f} s ({done} element Jacobians) on one host core; its assembler (needs METIS) is "
-                  "not built, so the scatter into BSR is not in this figure",
-        "port": {"value": port["value"], "sample": port["sample"]},
+                  f"~{seconds:.0f} s in each of {nw} processes ({done} element Jacobians; one process per core); "
+                  "its assembler (needs METIS) is not built, so the scatter into BSR is not in this figure",
+        "per_core": min(rates),
+        "host": host_cpu(),
+        "port": {"value": port["value"], "cores": nw, "sample": port["sample"]},
     }
+
+
+def _oracle_analysis_worker(_):
+    """One oracle analysis of the 20 x 40 cylinder (static path + buckling)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    cyl = Cylinder(20, 40)
+    thick = np.full(cyl.nelems, 0.002)
+    t0 = time.perf_counter()
+    O.linear_buckling(cyl.Xpts, cyl.conn, thick, cyl.bcs(), cyl.edge_load(1.0), 0.9 * cyl.classical_load(), nev=8,
+                      max_lanczos=120)
+    return time.perf_counter() - t0
+
+
+def cpu_analysis_throughput():
+    """The oracle's whole 20 x 40 analysis on every worker at once: analyses
+    per second of the host (to set beside the GPU's single-analysis time)."""
+    import multiprocessing as mp
+    nw = cpu_workers()
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(nw) as pool:
+        per = pool.map(_oracle_analysis_worker, range(nw))
+    wall = time.perf_counter() - t0
+    return {"workers": nw, "analyses_per_s": nw / wall, "s_per_analysis_one_core": float(np.median(per)),
+            "host": host_cpu()}
 
 
 def main():
@@ -374,6 +462,16 @@ def main():
 
     cyl = Cylinder(200, 400, R=1.0, L=2.0, imperfection=seeded_imperfection(1235))
     thick = seeded_thickness(cyl.nelems, 1234)
+    # host baselines first: their worker processes fork a process that has
+    # not touched the GPU
+    cpu = None
+    cpu_analysis = None
+    if args.mode == "assembly" and not args.no_cpu_baseline:
+        cpu = cpu_baseline(cyl, thick, args.cpu_seconds)
+        progress("cpu_baseline")
+        if not args.skip_accuracy:
+            cpu_analysis = cpu_analysis_throughput()
+            progress("cpu_analysis")
     ctx = make_context(cyl, thick)
     K = ctx.matrix()
 
@@ -417,6 +515,7 @@ def main():
     ctx.set_timing(False)
     spmv_gbps = spmv_bytes(ctx.nnzb, ctx.nnodes) * spmv_n / (spmv_ms * 1e-3) / 1e9
     nnzb, nnodes, ncolors = ctx.nnzb, ctx.nnodes, ctx.ncolors
+    device = ctx.device_info()
     K.destroy()
     x.free()
     y.free()
@@ -437,9 +536,14 @@ def main():
         "data": "synthetic (seeded cylinder mesh, thickness and imperfection)",
         "config": {"workload": "cylinder 200x400 (80000 quads, 482400 dof): K assembly", "nelems": cyl.nelems,
                    "nnodes": nnodes, "nnzb": nnzb, "colors": ncolors, "parallelism": "single GPU"},
+        "device": device,
         "bsr_spmv_GBps": spmv_gbps,
+        "bsr_spmv_note": "200x400 K (207.7 MB) fits the 256 MB MALL: consecutive products partly hit it; "
+                         "bsr_spmv_hbm is the HBM-resident figure",
         "roofline": {
-            "bound": "mfma",
+            # vector FP64 FMAs (the element tile is far below an efficient
+            # FP64 MFMA shape); the peak is MI355X's FP64 rate
+            "bound": "fp64-valu",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -450,17 +554,24 @@ def main():
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
         },
     }
-    legs = [("buckling_pipeline", not args.skip_pipeline, buckling_pipeline),
+    legs = [("bsr_spmv_hbm", True, spmv_hbm),
+            ("buckling_pipeline", not args.skip_pipeline, buckling_pipeline),
             ("frequency", not args.skip_frequency, frequency_analysis),
             ("box_beam", not args.skip_boxbeam, box_beam),
             ("batched", not args.skip_batched, lambda: batched_designs(args.batch)),
-            ("accuracy", not args.skip_accuracy, accuracy_vs_oracle),
-            ("cpu_baseline", not args.no_cpu_baseline, lambda: cpu_baseline(cyl, thick, args.cpu_seconds))]
+            ("accuracy", not args.skip_accuracy, accuracy_vs_oracle)]
     progress("assembly")
     for name, run, fn in legs:
         if run:
             out[name] = fn()
             progress(name)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    if cpu_analysis is not None and "accuracy" in out:
+        acc = out["accuracy"]
+        acc["oracle_all_cores"] = cpu_analysis
+        acc["gpu_analyses_per_s"] = 1.0 / acc["gpu_s"]
+        acc["gpu_vs_all_cores"] = acc["gpu_analyses_per_s"] / cpu_analysis["analyses_per_s"]
     print(json.dumps(out), flush=True)
 
 

@@ -100,19 +100,57 @@ int shellgpu_mat_apply_bcs(shellgpu_mat *A, double diag);
 /* TACSMat::mult -> BCSRMatVecMult6 (BCSRMatMult6.cpp:82-125): y = A x for
  * every matrix of the batch; x, y device, nbatch*6*nnodes */
 int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y);
+/* TACSMat::multTranspose (src/bpmat/KSM.h:176): y = A^T x for every
+ * matrix of the batch (the node pattern is structurally symmetric) */
+int shellgpu_mat_mult_transpose(shellgpu_mat *A, const double *x, double *y);
+/* TACSMat::scale (src/bpmat/KSM.h:178): A = alpha A */
+int shellgpu_mat_scale(shellgpu_mat *A, double alpha);
+/* TACSMat::addDiag (src/bpmat/KSM.h:180, BCSRMat::addDiag): every diagonal
+ * entry += alpha */
+int shellgpu_mat_add_diag(shellgpu_mat *A, double alpha);
+/* TACSMat::addValues (src/bpmat/KSM.h:153-154) -> BCSRMat::addRowValues
+ * (src/bpmat/BCSRMat.cpp:1778-1855) for design `design` of the batch: block
+ * k (host vals[36 k .. 36 k + 36), row-major 6x6) is added to block
+ * (brow[k], bcol[k]).  Negative indices are skipped.  Blocks outside the
+ * pattern are reported (non-zero return) after the others are added; the
+ * additions to one block are summed in input order. */
+int shellgpu_mat_add_blocks(shellgpu_mat *A, int design, int nblk, const int *brow, const int *bcol,
+                            const double *vals);
 
 /* ---- assembly (device state vectors, nbatch*6*nnodes, NULL = zero) ----- */
 /* TACSAssembler::assembleMatType (src/TACSAssembler.cpp:4186-4250): zero,
  * element loop, applyBCs (unit diagonal) */
 int shellgpu_assemble_mattype(shellgpu_ctx *ctx, int mattype, const double *u,
                               shellgpu_mat *A);
+/* TACSAssembler::assembleMatCombo (src/TACSAssembler.cpp:4264-4310):
+ * A = sum_k scale[k] (matrix of type mattypes[k] at u), then BCs (unit
+ * diagonal) */
+int shellgpu_assemble_mat_combo(shellgpu_ctx *ctx, int nmats, const int *mattypes,
+                                const double *scale, const double *u, shellgpu_mat *A);
+/* TACSAssembler::addJacobianVecProduct (src/TACSAssembler.cpp:4334-4390):
+ * y += scale J x with J = alpha dR/du + gamma M at u (element matrices,
+ * nothing assembled), J^T when transpose != 0, then y[bc] = 0.  beta,
+ * dudt and d2udt2 do not enter the shell's matrices. */
+int shellgpu_add_jacobian_vec_product(shellgpu_ctx *ctx, int model, double scale, double alpha,
+                                      double beta, double gamma, const double *u,
+                                      const double *dudt, const double *d2udt2, const double *x,
+                                      double *y, int transpose);
 /* TACSAssembler::assembleRes (src/TACSAssembler.cpp:4000-4070): R = element
  * residuals, then R[bc] = u[bc] - ubar[bc] (TACSBVec::applyBCs) */
 int shellgpu_assemble_res(shellgpu_ctx *ctx, int model, const double *u, double *R);
-/* TACSAssembler::assembleJacobian (alpha = 1, beta = gamma = 0)
- * (src/TACSAssembler.cpp:4084-4178): R (may be NULL) and A with BCs */
-int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, const double *u,
-                               double *R, shellgpu_mat *A);
+/* TACSAssembler::assembleJacobian(alpha, beta, gamma, res, A)
+ * (src/TACSAssembler.cpp:4084-4178) -> TACSShellElement::addJacobian
+ * (src/elements/shell/TACSShellElement.h:455-672) at the state u (NULL =
+ * 0) with time derivatives dudt and d2udt2 (the reference's vars, dvars,
+ * ddvars; NULL = 0; nbatch*6*nnodes device vectors each):
+ *   A = alpha dR/du + gamma M,   R = R(u) + M d2udt2,
+ * then R[bc] = u[bc] - ubar[bc] and BC rows of A zeroed with a unit
+ * diagonal.  The residual and the matrix come from one element pass.  The
+ * linearized-rotation director has no velocity terms, so beta and dudt do
+ * not enter (TACSDirector.h:368-432).  R may be NULL. */
+int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, double alpha, double beta,
+                               double gamma, const double *u, const double *dudt,
+                               const double *d2udt2, double *R, shellgpu_mat *A);
 
 /* TACSAssembler::getElementOutputData(TACS_BEAM_OR_SHELL_ELEMENT,
  * TACS_OUTPUT_STRAINS | TACS_OUTPUT_STRESSES | TACS_OUTPUT_EXTRAS)
@@ -131,27 +169,33 @@ int shellgpu_element_output(shellgpu_ctx *ctx, int model, const double *u, doubl
  * scale psi^T dR_e/dt_e at state u (NULL = zero) for the given model */
 int shellgpu_add_adj_res_product(shellgpu_ctx *ctx, int model, double scale, const double *psi,
                                  const double *u, double *dfdt);
-/* TACSAssembler::addMatDVSensInnerProduct(scale, matType, phi, phi, dfdx)
+/* TACSAssembler::addMatDVSensInnerProduct(scale, matType, psi, phi, dfdx)
  * (element default src/elements/TACSElement.cpp:355-432, central
- * differences there, exact here): dfdt += scale phi^T (dA/dt_e) phi with
+ * differences there, exact here): dfdt += scale psi^T (dA/dt_e) phi with
  * A = K (SHELLGPU_STIFFNESS, u ignored), G(u) (SHELLGPU_GEOMETRIC) or M
- * (SHELLGPU_MASS, u ignored) */
+ * (SHELLGPU_MASS, u ignored).  psi NULL means psi = phi; psi != phi uses
+ * the symmetry of A: psi^T A' phi = ((psi+phi)^T A' (psi+phi) -
+ * (psi-phi)^T A' (psi-phi)) / 4. */
 int shellgpu_add_mat_dv_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale,
-                                           const double *phi, const double *u, double *dfdt);
-/* TACSAssembler::evalMatSVSensInnerProduct(GEOMETRIC, phi, phi, dfdu)
+                                           const double *psi, const double *phi, const double *u,
+                                           double *dfdt);
+/* TACSAssembler::evalMatSVSensInnerProduct(GEOMETRIC, psi, phi, dfdu)
  * (element default src/elements/TACSElement.cpp:509-560): dfdu =
- * d(phi^T G(u) phi)/du, which is independent of u (G is linear in the
- * path); no BCs applied */
-int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *phi, double *dfdu);
+ * d(psi^T G(u) phi)/du, which is independent of u (G is linear in the
+ * path); psi NULL means psi = phi; no BCs applied */
+int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *psi, const double *phi,
+                                            double *dfdu);
 
 /* ---- coordinate sensitivities (device outputs accumulate,
  * nbatch*3*nnodes, x/y/z per node) --------------------------------------- */
-/* TACSAssembler::addMatXptSensInnerProduct(scale, matType, phi, phi, dfdX)
+/* TACSAssembler::addMatXptSensInnerProduct(scale, matType, psi, phi, dfdX)
  * (element default src/elements/TACSElement.cpp:434-507, central
  * differences there, exact forward-mode derivatives here): dfdX += scale
- * d(phi^T A phi)/dX with A = K (SHELLGPU_STIFFNESS, u ignored) or G(u) */
+ * d(psi^T A phi)/dX with A = K (SHELLGPU_STIFFNESS, u ignored) or G(u);
+ * psi NULL means psi = phi (psi != phi by polarization, as above) */
 int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale,
-                                            const double *phi, const double *u, double *dfdX);
+                                            const double *psi, const double *phi, const double *u,
+                                            double *dfdX);
 /* TACSAssembler::addAdjointResXptSensProducts -> addAdjResXptProduct
  * (element default src/elements/TACSElement.cpp:270-332): dfdX += scale
  * d(psi^T R(u))/dX at state u (NULL = zero) for the given model */
@@ -336,6 +380,13 @@ int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef,
  * src/utils/TacsUtilities.cpp:971-1011): ks = m + log(sum exp(rho (f_i -
  * m))) / rho, m = max f; dks[i] = d ks / d f_i (may be NULL).  Host only. */
 int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double *dks);
+
+/* ---- device identification (no reference counterpart: measurement
+ * plumbing) ---------------------------------------------------------------- */
+/* the context's GPU: marketing name and architecture (e.g. "gfx950"), PCI bus
+ * id ("0000:05:00.0"), compute units and HBM bytes; any output may be NULL */
+int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch, int arch_len,
+                         char *pci_bus_id, int pci_len, int *compute_units, size_t *hbm_bytes);
 
 /* ---- timing (no reference counterpart: measurement plumbing) ---------- */
 /* With timing enabled, HIP events on the context's stream bracket every

@@ -129,6 +129,8 @@ void or_assemble_mat(int kind, int model, const or_material *m, int nnodes, int 
       or_elem_geometric_step(m, thick[e], T, X, q, 1e5, Ke);
     } else if (kind == 5) {
       or_elem_mass(m, thick[e], X, Ke);
+    } else if (kind == 6) {
+      or_elem_geometric_step_ld(m, thick[e], T, X, q, 1.0, Ke);
     } else {
       memset(Ke, 0, sizeof(Ke));
       memset(res, 0, sizeof(res));

@@ -74,6 +74,13 @@ void or_elem_geometric_step(const or_material *m, double t, double temp,
                             const double X[12], const double u[24], double dh,
                             double mat[576]);
 
+/* or_elem_geometric_step evaluated in long double (shell_elem_ld.cpp): the
+ * accurate geometric stiffness where the double evaluation's round-off
+ * matters (fine meshes) */
+void or_elem_geometric_step_ld(const or_material *m, double t, double temp,
+                               const double X[12], const double u[24], double dh,
+                               double mat[576]);
+
 /* dfdt += scale * psi^T dR_e/dt (thickness adjoint-residual product).
  * SE.h:774-868 with ISC:229-251. */
 void or_elem_adj_res_dv(int model, const or_material *m, double t, double temp,

@@ -80,6 +80,13 @@ def elem_jacobian(model, mat, t, temp, X, q, alpha=1.0):
     return res, K.reshape(24, 24)
 
 
+def elem_geometric_step_ld(mat, t, temp, X, u, dh=1.0):
+    G = np.zeros(576)
+    lib().or_elem_geometric_step_ld(C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(u)),
+                                    C.c_double(dh), _d(G))
+    return G.reshape(24, 24)
+
+
 def elem_geometric_fd(mat, t, temp, X, u):
     G = np.zeros(576)
     lib().or_elem_geometric_fd(C.byref(mat), C.c_double(t), C.c_double(temp), _d(_f64(X)), _d(_f64(u)), _d(G))
@@ -124,6 +131,8 @@ def first_appearance(nnodes, conn):
 
 
 KIND_K, KIND_G_FD, KIND_KNL, KIND_G, KIND_G_THERMAL, KIND_M = 0, 1, 2, 3, 4, 5
+# the accurate geometric stiffness (KIND_G's formula) evaluated in long double
+KIND_G_LD = 6
 
 
 def assemble_mat(kind, model, mat, Xpts, conn, thick, rowp, cols, u=None, temp=None):

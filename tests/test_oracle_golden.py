@@ -155,3 +155,22 @@ def test_output_data_matches_reference(i):
     assert np.array_equal(ref[:, 3:9], q.reshape(4, 6))
     assert rel(out, ref[:, 9:40]) < 1e-13
     assert rel(out[:, 18], ref[:, 27]) < 1e-13  # failure0 on its own scale
+
+
+@pytest.mark.parametrize("i", range(NREC))
+def test_geometric_long_double_matches_reference(i):
+    """The long-double evaluation of the oracle's geometric stiffness against
+    the reference's own central difference (its step 1e-4, so its round-off
+    sets the tolerance as above), against the double evaluation at the
+    accurate step 1, and free of the step: the tangent is exactly quadratic
+    in the state, so every step gives the same matrix once round-off is
+    gone."""
+    m, t, T, X, q, psi = _rec(i)
+    Gl = O.elem_geometric_step_ld(m, t, T, X, q, 1.0)
+    tol = 1e-11 if T == 0.0 else 1e-7
+    assert rel(Gl, GOLD["G"][i]) < tol
+    # with a temperature the difference direction includes it (a larger
+    # perturbation of the stresses): more round-off on both evaluations
+    assert rel(O.elem_geometric_step(m, t, T, X, q, 1.0), Gl) < (1e-12 if T == 0.0 else 1e-11)
+    for dh in (0.1, 10.0):
+        assert rel(O.elem_geometric_step_ld(m, t, T, X, q, dh), Gl) < (1e-15 if T == 0.0 else 1e-13)
