@@ -326,8 +326,12 @@ __device__ __forceinline__ double mass_entry(const MassTile &m, int i, int j) {
 // launch list, lanes 32-63 on the next one, each half with its own LDS.  An
 // odd list leaves the last upper half computing a copy of the lower half's
 // element with its writes suppressed.
-template <int MODE>
-__global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
+//
+// VAR selects a compiled schedule of the same arithmetic (chosen per mode
+// and mesh by shellgpu_tune): bit 0 unrolls the quadrature loop of the
+// matrix tile (the shape values become constants, the code 4 times longer).
+template <int MODE, int VAR>
+__device__ __forceinline__ void elem_body(const ElemArgs &a) {
   using MT = ModeTraits<MODE>;
   constexpr bool kTwo = MT::kTwo;
   constexpr int kNB = ElemSmem<MODE>::kNB;
@@ -950,6 +954,8 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   for (int i = 0; i < 18; i++) acc[i] = 0.0;
   double racc = 0.0;
 
+  constexpr int kUnrollQ = (VAR & 1) ? 4 : 1;
+#pragma unroll kUnrollQ
   for (int iq = 0; iq < 4; iq++) {
     if (lane < 24) {
       double b[9], y[9];
@@ -1143,6 +1149,19 @@ __global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
   }
 }
 
+template <int MODE, int VAR>
+__global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
+  elem_body<MODE, VAR>(a);
+}
+
+// the same body within the register budget of two wavefronts per SIMD (the
+// compiler spills to scratch memory instead of holding one wavefront with
+// accumulation registers)
+template <int MODE, int VAR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void elem_kernel_w2(ElemArgs a) {
+  elem_body<MODE, VAR>(a);
+}
+
 // ---- mass matrix ----
 // getMatType(TACS_MASS_MATRIX) (TACSShellElement.h:675-704, gamma = 1) keeps
 // only the inertial terms of addJacobian (TACSShellElement.h:613-648) and the
@@ -1271,19 +1290,43 @@ static hipError_t launch_mode(const ElemArgs &args, int count, int nbatch, hipSt
   if (count <= 0 || nbatch <= 0) return hipSuccess;
   ElemArgs a = args;
   a.count = count;
-  hipLaunchKernelGGL(elem_kernel<MODE>, dim3((count + 1) / 2, nbatch), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL((elem_kernel<MODE, 0>), dim3((count + 1) / 2, nbatch), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nbatch,
+// the matrix modes in every schedule: variant bit 0 = VAR bit 0 (unrolled
+// quadrature loop), bit 1 = the two-wavefront register budget
+template <int MODE>
+static hipError_t launch_tuned(const ElemArgs &args, int count, int nbatch, int variant, hipStream_t stream) {
+  if (count <= 0 || nbatch <= 0) return hipSuccess;
+  ElemArgs a = args;
+  a.count = count;
+  const dim3 grid((count + 1) / 2, nbatch), block(64);
+  switch (variant & 3) {
+    case 1:
+      hipLaunchKernelGGL((elem_kernel<MODE, 1>), grid, block, 0, stream, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((elem_kernel_w2<MODE, 0>), grid, block, 0, stream, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((elem_kernel_w2<MODE, 1>), grid, block, 0, stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((elem_kernel<MODE, 0>), grid, block, 0, stream, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_elem_kernel(int mode, int variant, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream) {
   switch (mode) {
     case kModeKLin:
-      return launch_mode<kModeKLin>(args, count, nbatch, stream);
+      return launch_tuned<kModeKLin>(args, count, nbatch, variant, stream);
     case kModeKNl:
-      return launch_mode<kModeKNl>(args, count, nbatch, stream);
+      return launch_tuned<kModeKNl>(args, count, nbatch, variant, stream);
     case kModeG:
-      return launch_mode<kModeG>(args, count, nbatch, stream);
+      return launch_tuned<kModeG>(args, count, nbatch, variant, stream);
     case kModeRLin:
       return launch_mode<kModeRLin>(args, count, nbatch, stream);
     case kModeRNl:
@@ -1299,9 +1342,9 @@ hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nba
     case kModeOutNl:
       return launch_mode<kModeOutNl>(args, count, nbatch, stream);
     case kModeJacLin:
-      return launch_mode<kModeJacLin>(args, count, nbatch, stream);
+      return launch_tuned<kModeJacLin>(args, count, nbatch, variant, stream);
     case kModeJacNl:
-      return launch_mode<kModeJacNl>(args, count, nbatch, stream);
+      return launch_tuned<kModeJacNl>(args, count, nbatch, variant, stream);
     case kModeJvpLin:
       return launch_mode<kModeJvpLin>(args, count, nbatch, stream);
     case kModeJvpNl:

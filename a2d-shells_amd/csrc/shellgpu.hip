@@ -190,11 +190,13 @@ __global__ void axpy_kernel(long n, double alpha, const double *x, double *y) {
 
 int launch_grid(long n) { return (int)std::min<long>((n + 255) / 256, 65535L * 4); }
 
-// greedy first-fit colouring of the element dual graph in element order
-// (the scheme of TacsComputeSerialMultiColor, src/utils/TacsUtilities.cpp:
-// 430-487, applied to elements that share a node)
+// greedy first-fit colouring of the element dual graph (the scheme of
+// TacsComputeSerialMultiColor, src/utils/TacsUtilities.cpp:430-487, applied
+// to elements that share a node): strategy 0 visits the elements in their
+// order, strategy 1 those with the most neighbouring elements first (ties in
+// element order), which can need fewer colours on meshes with junctions
 void color_elements(int nnodes, int nelems, const std::vector<int> &conn, std::vector<int> &color,
-                    int &ncolors) {
+                    int &ncolors, int strategy = 0) {
   std::vector<int> nep(nnodes + 1, 0);
   for (int e = 0; e < nelems; e++)
     for (int k = 0; k < 4; k++) nep[conn[4 * e + k] + 1]++;
@@ -202,10 +204,22 @@ void color_elements(int nnodes, int nelems, const std::vector<int> &conn, std::v
   std::vector<int> n2e(nep[nnodes]), fill(nep.begin(), nep.end() - 1);
   for (int e = 0; e < nelems; e++)
     for (int k = 0; k < 4; k++) n2e[fill[conn[4 * e + k]]++] = e;
+  std::vector<int> order(nelems);
+  for (int e = 0; e < nelems; e++) order[e] = e;
+  if (strategy == 1) {
+    // neighbour counts (with repeats over shared nodes: a proxy of the degree)
+    std::vector<int> deg(nelems, 0);
+    for (int e = 0; e < nelems; e++)
+      for (int k = 0; k < 4; k++) {
+        const int n = conn[4 * e + k];
+        deg[e] += nep[n + 1] - nep[n] - 1;
+      }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return deg[x] > deg[y]; });
+  }
   color.assign(nelems, -1);
   ncolors = 0;
   std::vector<int> mark;
-  for (int e = 0; e < nelems; e++) {
+  for (int e : order) {
     mark.assign(ncolors + 1, 0);
     for (int k = 0; k < 4; k++) {
       const int n = conn[4 * e + k];
@@ -252,6 +266,52 @@ void drop_events(std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev) {
   ev.clear();
 }
 
+// colour the elements with the given strategy (color_elements) and upload
+// the colour lists and the first-touch bits of the colour launch order
+int apply_coloring(shellgpu_ctx *ctx, int strategy) {
+  const int nelems = ctx->nelems;
+  const std::vector<int> &conn = ctx->h_conn;
+  color_elements(ctx->nnodes, nelems, conn, ctx->h_color, ctx->ncolors, strategy);
+  ctx->coloring = strategy;
+  ctx->h_color_ptr.assign(ctx->ncolors + 1, 0);
+  for (int e = 0; e < nelems; e++) ctx->h_color_ptr[ctx->h_color[e] + 1]++;
+  for (int c = 0; c < ctx->ncolors; c++) ctx->h_color_ptr[c + 1] += ctx->h_color_ptr[c];
+  ctx->h_color_elems.assign(nelems, 0);
+  {
+    std::vector<int> pos(ctx->h_color_ptr.begin(), ctx->h_color_ptr.end() - 1);
+    for (int e = 0; e < nelems; e++) ctx->h_color_elems[pos[ctx->h_color[e]]++] = e;
+  }
+
+  // first-touch bits in colour launch order (elements of one colour share no
+  // node, so no block is reached twice within a colour)
+  std::vector<int> first(nelems, 0);
+  bool distinct = true;
+  for (int e = 0; e < nelems && distinct; e++)
+    for (int i = 0; i < 4; i++)
+      for (int j = i + 1; j < 4; j++) distinct = distinct && conn[4 * e + i] != conn[4 * e + j];
+  if (distinct) {
+    std::vector<char> seen(ctx->nnzb, 0);
+    for (int e : ctx->h_color_elems)
+      for (int k = 0; k < 16; k++) {
+        const int b = ctx->h_blk[16L * e + k];
+        if (!seen[b]) {
+          seen[b] = 1;
+          first[e] |= 1 << k;
+        }
+      }
+  }
+  int rc = 0;
+  HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->d_first) {
+    (void)hipFree(ctx->d_first);
+    ctx->d_first = nullptr;
+  }
+  if (distinct && (rc = upload(ctx, &ctx->d_first, first.data(), first.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_color_elems, ctx->h_color_elems.data(), ctx->h_color_elems.size())))
+    return rc;
+  return 0;
+}
+
 }  // namespace
 
 namespace shellgpu {
@@ -279,7 +339,7 @@ int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
     a.elems = ctx->d_color_elems + ctx->h_color_ptr[c];
     const int nb = ctx->h_color_ptr[c + 1] - ctx->h_color_ptr[c];
     if ((rc = record_begin(ctx, ctx->elem_ev))) return rc;
-    HIPCALL(ctx, launch_elem_kernel(mode, a, nb, ctx->nbatch, ctx->stream));
+    HIPCALL(ctx, launch_elem_kernel(mode, ctx->elem_variant[mode & 31], a, nb, ctx->nbatch, ctx->stream));
     if ((rc = record_end(ctx, ctx->elem_ev))) return rc;
   }
   return 0;
@@ -448,42 +508,8 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
     }
   }
 
-  // colouring
-  color_elements(nnodes, nelems, ctx->h_conn, ctx->h_color, ctx->ncolors);
-  ctx->h_color_ptr.assign(ctx->ncolors + 1, 0);
-  for (int e = 0; e < nelems; e++) ctx->h_color_ptr[ctx->h_color[e] + 1]++;
-  for (int c = 0; c < ctx->ncolors; c++) ctx->h_color_ptr[c + 1] += ctx->h_color_ptr[c];
-  ctx->h_color_elems.assign(nelems, 0);
-  {
-    std::vector<int> pos(ctx->h_color_ptr.begin(), ctx->h_color_ptr.end() - 1);
-    for (int e = 0; e < nelems; e++) ctx->h_color_elems[pos[ctx->h_color[e]]++] = e;
-  }
-
-  // first-touch bits in colour launch order (elements of one colour share no
-  // node, so no block is reached twice within a colour)
-  std::vector<int> first(nelems, 0);
-  bool distinct = true;
-  for (int e = 0; e < nelems && distinct; e++)
-    for (int i = 0; i < 4; i++)
-      for (int j = i + 1; j < 4; j++) distinct = distinct && conn[4 * e + i] != conn[4 * e + j];
-  if (distinct) {
-    std::vector<char> seen(ctx->nnzb, 0);
-    for (int e : ctx->h_color_elems)
-      for (int k = 0; k < 16; k++) {
-        const int b = ctx->h_blk[16L * e + k];
-        if (!seen[b]) {
-          seen[b] = 1;
-          first[e] |= 1 << k;
-        }
-      }
-  }
-
-  int rc = 0;
-  if (ctx->d_first) {
-    (void)hipFree(ctx->d_first);
-    ctx->d_first = nullptr;
-  }
-  if (distinct && (rc = upload(ctx, &ctx->d_first, first.data(), first.size()))) return rc;
+  int rc = apply_coloring(ctx, ctx->coloring);
+  if (rc) return rc;
   if ((rc = upload(ctx, &ctx->d_conn, conn, 4L * nelems))) return rc;
   if ((rc = upload(ctx, &ctx->d_X, Xpts, 3L * nnodes))) return rc;
   if ((rc = upload(ctx, &ctx->d_rowp, ctx->h_rowp.data(), ctx->h_rowp.size()))) return rc;
@@ -491,8 +517,6 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
   if ((rc = upload(ctx, &ctx->d_blk, ctx->h_blk.data(), ctx->h_blk.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_diag, diag.data(), diag.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_tpos, tpos.data(), tpos.size()))) return rc;
-  if ((rc = upload(ctx, &ctx->d_color_elems, ctx->h_color_elems.data(), ctx->h_color_elems.size())))
-    return rc;
   return 0;
 }
 
@@ -1058,6 +1082,115 @@ int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch
   if (compute_units) *compute_units = prop.multiProcessorCount;
   if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
   return 0;
+}
+
+// the element-kernel classes with compiled schedules (shellgpu_tune)
+static const int kTunedModes[] = {kModeKLin, kModeKNl, kModeG, kModeJacLin, kModeJacNl};
+static const char *kTunedNames[] = {"stiffness", "tangent", "geometric", "jacobian_linear", "jacobian_nonlinear"};
+constexpr int kTunedCount = 5;
+
+int shellgpu_set_elem_schedule(shellgpu_ctx *ctx, int kernel_class, int variant) {
+  if (!ctx) return 1;
+  if (kernel_class < 0 || kernel_class >= kTunedCount || variant < 0 || variant >= kElemVariants)
+    return set_err(ctx, 1, "kernel class or schedule out of range");
+  ctx->elem_variant[kTunedModes[kernel_class]] = variant;
+  return 0;
+}
+
+int shellgpu_tune(shellgpu_ctx *ctx, int flags) {
+  int rc = check_batch(ctx, nullptr);
+  if (rc) return rc;
+  const size_t nv = 6L * ctx->nnodes * ctx->nbatch;
+  shellgpu_mat *A = nullptr;
+  double *u = nullptr, *R = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::string rep = "{";
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  // a smooth state of small displacements and rotations, for the state-
+  // dependent modes (the timing does not depend on its values)
+  std::vector<double> hu(nv);
+  for (size_t i = 0; i < nv; i++) hu[i] = 1e-6 * std::sin(0.001 * (double)i);
+  auto trial = [&](int mode, float &best_ms) -> int {
+    ElemArgs a = base_args(ctx, u);
+    a.vals = A->vals;
+    if (mode == kModeJacLin || mode == kModeJacNl) a.res = R;
+    int r = run_colored(ctx, mode, a);  // warm-up
+    if (r) return r;
+    HIPCALL(ctx, hipEventRecord(e0, ctx->stream));
+    for (int k = 0; k < 3; k++)
+      if ((r = run_colored(ctx, mode, a))) return r;
+    HIPCALL(ctx, hipEventRecord(e1, ctx->stream));
+    HIPCALL(ctx, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCALL(ctx, hipEventElapsedTime(&ms, e0, e1));
+    best_ms = ms / 3.0f;
+    return 0;
+  };
+  if ((rc = shellgpu_mat_create(ctx, ctx->nbatch, &A))) goto out;
+  if (hipMalloc((void **)&u, sizeof(double) * nv) != hipSuccess ||
+      hipMalloc((void **)&R, sizeof(double) * nv) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = set_err(ctx, 2, "tuner allocation failed");
+    goto out;
+  }
+  if (hipMemcpy(u, hu.data(), sizeof(double) * nv, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = set_err(ctx, 2, "tuner state upload failed");
+    goto out;
+  }
+  // colouring (flags bit 0): the stiffness assembly under each strategy
+  if (flags & 1) {
+    char buf[160];
+    float t[2] = {0.0f, 0.0f};
+    int nc[2] = {0, 0};
+    for (int st = 0; st < 2; st++) {
+      if ((rc = apply_coloring(ctx, st))) goto out;
+      nc[st] = ctx->ncolors;
+      if ((rc = trial(kModeKLin, t[st]))) goto out;
+    }
+    const int best = t[1] < t[0] ? 1 : 0;
+    if ((rc = apply_coloring(ctx, best))) goto out;
+    snprintf(buf, sizeof(buf),
+             "\"coloring\": {\"chosen\": \"%s\", \"colors\": %d, \"us\": {\"element_order\": %.1f, "
+             "\"most_neighbours_first\": %.1f}}, ",
+             best ? "most_neighbours_first" : "element_order", nc[best], 1e3 * t[0], 1e3 * t[1]);
+    rep += buf;
+  }
+  // compiled schedule of every matrix mode
+  rep += "\"elem\": {";
+  for (int c = 0; c < kTunedCount; c++) {
+    const int mode = kTunedModes[c];
+    float t[kElemVariants];
+    int best = 0;
+    for (int v = 0; v < kElemVariants; v++) {
+      ctx->elem_variant[mode] = v;
+      if ((rc = trial(mode, t[v]))) goto out;
+      if (t[v] < t[best]) best = v;
+    }
+    ctx->elem_variant[mode] = best;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s\"%s\": {\"schedule\": %d, \"us\": [%.1f, %.1f, %.1f, %.1f]}", c ? ", " : "",
+             kTunedNames[c], best, 1e3 * t[0], 1e3 * t[1], 1e3 * t[2], 1e3 * t[3]);
+    rep += buf;
+  }
+  rep += "}, \"schedules\": [\"rolled quadrature loop\", \"unrolled quadrature loop\", "
+         "\"rolled, 2 wavefronts/SIMD register budget\", \"unrolled, 2 wavefronts/SIMD register budget\"]}";
+  ctx->schedule = rep;
+out:
+  ctx->timing = timing;
+  (void)hipStreamSynchronize(ctx->stream);
+  if (A) shellgpu_mat_destroy(A);
+  if (u) (void)hipFree(u);
+  if (R) (void)hipFree(R);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int shellgpu_get_schedule(shellgpu_ctx *ctx, char *buf, int len) {
+  if (!ctx || !buf || len <= 0) return 1;
+  snprintf(buf, len, "%s", ctx->schedule.empty() ? "{}" : ctx->schedule.c_str());
+  return (int)ctx->schedule.size() >= len ? set_err(ctx, 1, "buffer too small for the schedule") : 0;
 }
 
 int shellgpu_set_timing(shellgpu_ctx *ctx, int enable) {

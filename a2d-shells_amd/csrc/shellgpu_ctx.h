@@ -27,6 +27,14 @@ struct shellgpu_ctx {
   int *d_first = nullptr;
   double *d_X = nullptr;
 
+  // element colouring: 0 greedy first fit in element order, 1 greedy first
+  // fit with the elements of most neighbours first (shellgpu_tune chooses)
+  int coloring = 0;
+  // compiled schedule of each element-kernel mode (launch_elem_kernel's
+  // variant; shellgpu_tune chooses) and the tuner's report
+  int elem_variant[32] = {0};
+  std::string schedule;
+
   shellgpu::Material mat = shellgpu::make_material(70e9, 0.3, 2700.0, 0.0, 10.0);
 
   int nbc = 0;

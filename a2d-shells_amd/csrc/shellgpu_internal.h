@@ -145,7 +145,11 @@ struct XptArgs {
 };
 hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream_t stream);
 
-hipError_t launch_elem_kernel(int mode, const ElemArgs &args, int count, int nbatch,
+// variant: the compiled schedule of the matrix modes (kModeKLin, kModeKNl,
+// kModeG, kModeJac*): bit 0 unrolled quadrature loop, bit 1 two-wavefront
+// register budget; other modes ignore it
+constexpr int kElemVariants = 4;
+hipError_t launch_elem_kernel(int mode, int variant, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream);
 
 // Batched FP64 GEMM on the matrix cores (gemm_f64.hip): C_b = alpha op(A_b)

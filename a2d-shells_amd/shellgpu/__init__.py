@@ -113,6 +113,9 @@ def load_library(path=LIB_PATH):
         "shellgpu_newton_solve": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_double, _VP, _IP, _DP]),
         "shellgpu_device_info": (C.c_int, [_VP, C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int, _IP,
                                            C.POINTER(C.c_size_t)]),
+        "shellgpu_tune": (C.c_int, [_VP, C.c_int]),
+        "shellgpu_get_schedule": (C.c_int, [_VP, C.c_char_p, C.c_int]),
+        "shellgpu_set_elem_schedule": (C.c_int, [_VP, C.c_int, C.c_int]),
         "shellgpu_set_timing": (C.c_int, [_VP, C.c_int]),
         "shellgpu_get_timing": (C.c_int, [_VP, _DP, _IP, _DP, _IP]),
         "shellgpu_timer_start": (C.c_int, [_VP]),
@@ -461,6 +464,22 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.shellgpu_synchronize(self.h))
+
+    def tune(self, coloring=True):
+        """Autotune the element-kernel schedules (and the colouring) on this
+        context's mesh and designs; returns the schedule report."""
+        self._check(self.lib.shellgpu_tune(self.h, 1 if coloring else 0))
+        self._refresh()
+        return self.schedule()
+
+    def schedule(self):
+        import json
+        buf = C.create_string_buffer(8192)
+        self._check(self.lib.shellgpu_get_schedule(self.h, buf, 8192))
+        return json.loads(buf.value.decode())
+
+    def set_elem_schedule(self, kernel_class, variant):
+        self._check(self.lib.shellgpu_set_elem_schedule(self.h, int(kernel_class), int(variant)))
 
     def device_info(self):
         """The context's GPU: name, architecture, PCI bus id, compute units,

@@ -381,6 +381,22 @@ int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef,
  * m))) / rho, m = max f; dks[i] = d ks / d f_i (may be NULL).  Host only. */
 int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double *dks);
 
+/* ---- schedules (no reference counterpart: the reference's element loop
+ * has one schedule, TACSAssembler_thread.cpp:27-252) ---------------------- */
+/* Autotuner: times, on the context's mesh and designs, every compiled
+ * schedule of the matrix-assembly element kernels (stiffness, tangent,
+ * geometric stiffness, Jacobian of either model: rolled or unrolled
+ * quadrature loop, one- or two-wavefront register budget) and, with flags
+ * bit 0, the two element colourings (greedy first fit in element order, or
+ * the elements of most neighbours first), and keeps the fastest of each for
+ * the context.  Results are the same arithmetic under every schedule. */
+int shellgpu_tune(shellgpu_ctx *ctx, int flags);
+/* the tuner's choices and timings as a JSON object ("{}" before tuning) */
+int shellgpu_get_schedule(shellgpu_ctx *ctx, char *buf, int len);
+/* force a schedule (0-3) of one kernel class (0 stiffness, 1 tangent,
+ * 2 geometric, 3 Jacobian linear, 4 Jacobian nonlinear) */
+int shellgpu_set_elem_schedule(shellgpu_ctx *ctx, int kernel_class, int variant);
+
 /* ---- device identification (no reference counterpart: measurement
  * plumbing) ---------------------------------------------------------------- */
 /* the context's GPU: marketing name and architecture (e.g. "gfx950"), PCI bus
