@@ -16,10 +16,12 @@
 // stored k-row by k-row with 80 doubles per row: the four k-rows read by one
 // matrix instruction then fall in distinct LDS banks.
 //
-// Operand layout of v_mfma_f64_16x16x4f64 (one 64-lane wavefront): lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16] of the 16 x 4 and 4 x 16
-// operands, and holds D[4 (l / 16) + i][l % 16], i = 0..3, of the 16 x 16
-// result.
+// Operand layout of v_mfma_f64_16x16x4f64 (one 64-lane wavefront), as
+// measured on gfx950: lane l supplies X[l % 16][l / 16] and Y[l / 16][l % 16]
+// of the 16 x 4 and 4 x 16 operands of D = X Y, and holds D[4 i + l / 16][l % 16],
+// i = 0..3.  The kernel computes the transposed block, X from B and Y from A
+// (D = (A B)^T), so lanes 0-15 of a quarter hold 16 consecutive rows of one
+// column of C and the stores coalesce.
 #include "shellgpu_internal.h"
 
 namespace shellgpu {
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
     }
     if (more) store(cur ^ 1);
     __syncthreads();
@@ -110,20 +112,20 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
 
   // C = alpha acc + beta C
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 2; i++) {
+    const int m = m0 + wm + 16 * i + lr;
+    if (m >= M) continue;
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int n = n0 + wn + 16 * j + lr;
-      if (n >= N) continue;
+    for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int m = m0 + wm + 16 * i + 4 * lk + e;
-        if (m >= M) continue;
+        const int n = n0 + wn + 16 * j + 4 * e + lk;
+        if (n >= N) continue;
         double *c = C + m + (long)n * g.ldc;
         const double v = g.alpha * acc[i][j][e];
         *c = g.beta == 0.0 ? v : v + g.beta * *c;
       }
-    }
+  }
 }
 
 }  // namespace

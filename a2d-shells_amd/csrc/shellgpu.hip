@@ -1076,7 +1076,11 @@ int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch
   if (!ctx) return 1;
   hipDeviceProp_t prop;
   HIPCALL(ctx, hipGetDeviceProperties(&prop, ctx->device));
-  if (name && name_len > 0) snprintf(name, name_len, "%s", prop.name);
+  if (name && name_len > 0) {
+    snprintf(name, name_len, "%s", prop.name);
+    // some drivers report no marketing name in the properties
+    if (!name[0] && hipDeviceGetName(name, name_len, ctx->device) != hipSuccess) name[0] = 0;
+  }
   if (arch && arch_len > 0) snprintf(arch, arch_len, "%s", prop.gcnArchName);
   if (pci_bus_id && pci_len > 0) HIPCALL(ctx, hipDeviceGetPCIBusId(pci_bus_id, pci_len, ctx->device));
   if (compute_units) *compute_units = prop.multiProcessorCount;
