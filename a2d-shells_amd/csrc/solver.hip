@@ -73,6 +73,20 @@ struct FactorRound {
   long off;       // first pointer of this round in the device pointer arrays
 };
 
+// One product of a batch of dense-times-coupling products (first round of
+// cyclic reduction): Y = beta Y + alpha X E^T (RIGHT: E couples level ylev
+// to ylev - 1, Y's columns are ylev's nodes, X's columns ylev - 1's) or
+// Y = beta Y + alpha X E (LEFT: E couples level ylev + 1 to ylev, Y's
+// columns are ylev's nodes, X's columns ylev + 1's), rows 0 .. rows - 1;
+// E points at the design's coupling blocks.  Transposes and copies of the
+// same round use X, Y, rows and cols.
+struct CoupleJob {
+  const double *X;
+  double *Y;
+  const double *E;
+  int rows, cols, ylev, pad;
+};
+
 // one step of the recursive Cholesky-inverse of a diagonal block, in
 // execution order: a leaf (n2 == 0: factor and invert rows/cols [r0, r0 +
 // n1) in LDS), the update after the leading half (first == 1) or the
@@ -144,6 +158,12 @@ struct shellgpu_factor {
     std::vector<long> offs;  // per node: first pointer (A11, A21, A22 arrays of cnt)
   };
   std::vector<CholGroup> cgroups;
+  // first round of cyclic reduction on the sparse couplings: the level
+  // node offsets on the device and the job lists of its batched products
+  // (offsets into d_jobs, entries q = k nb + d of the round)
+  int *d_noff = nullptr;
+  CoupleJob *d_jobs = nullptr;
+  long job_cp = 0, job_q = 0, job_t1 = 0, job_dl = 0, job_v = 0, job_t2 = 0, job_dr = 0, job_br = 0;
 };
 
 namespace {
@@ -545,6 +565,108 @@ __global__ __launch_bounds__(256) void couple_right_kernel(int rows, int nnodes,
   }
 }
 
+// Y[:, 6j + a] over the nodes j of level ylev of job blockIdx.z, a
+// workgroup taking 256 rows of kCoupleNodes consecutive nodes (blockIdx.y);
+// X and Y have leading dimension ld
+template <bool LEFT>
+__global__ __launch_bounds__(256) void couple_job_kernel(const CoupleJob *__restrict__ jobs, int ld,
+                                                         const int *__restrict__ noff, const int *__restrict__ rowp,
+                                                         const int *__restrict__ colx, const int *__restrict__ blkx,
+                                                         double alpha, double beta) {
+  const CoupleJob J = jobs[blockIdx.z];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= J.rows) return;
+  const int u0 = noff[J.ylev], nn = noff[J.ylev + 1] - u0;
+  const double *X = J.X + i;
+  double *Y = J.Y + i;
+  const int j1 = min(nn, (int)(blockIdx.y + 1) * kCoupleNodes);
+  for (int j = blockIdx.y * kCoupleNodes; j < j1; j++) {
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = rowp[u0 + j]; q < rowp[u0 + j + 1]; q++) {
+      const double *xk = X + 6L * colx[q] * ld;
+      const double *e = J.E + 36L * (LEFT ? blkx[q] : q);
+      double xv[6];
+#pragma unroll
+      for (int r = 0; r < 6; r++) xv[r] = xk[(long)r * ld];
+      // RIGHT: block (row a, column r) = e[6 r + a]; LEFT: the transpose
+#pragma unroll
+      for (int r = 0; r < 6; r++)
+#pragma unroll
+        for (int a = 0; a < 6; a++) acc[a] += xv[r] * (LEFT ? e[6 * a + r] : e[6 * r + a]);
+    }
+    double *y = Y + 6L * j * ld;
+#pragma unroll
+    for (int a = 0; a < 6; a++) y[(long)a * ld] = (beta == 0.0 ? 0.0 : beta * y[(long)a * ld]) + alpha * acc[a];
+  }
+}
+
+// Y = X^T (X: rows x cols, Y: cols x rows, leading dimension ld) for job
+// blockIdx.z, 32 x 32 tiles through LDS
+__global__ __launch_bounds__(256) void transpose_job_kernel(const CoupleJob *__restrict__ jobs, int ld) {
+  __shared__ double tile[32][33];
+  const CoupleJob J = jobs[blockIdx.z];
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (r0 >= J.rows || c0 >= J.cols) return;
+  for (int k = ty; k < 32; k += 8)
+    if (r0 + tx < J.rows && c0 + k < J.cols) tile[k][tx] = J.X[(r0 + tx) + (long)(c0 + k) * ld];
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8)
+    if (c0 + tx < J.cols && r0 + k < J.rows) J.Y[(c0 + tx) + (long)(r0 + k) * ld] = tile[tx][k];
+}
+
+// Y = lower triangle of X, zeros above (rows x cols, leading dimension ld)
+// for job blockIdx.y
+__global__ __launch_bounds__(256) void copy_lower_job_kernel(const CoupleJob *__restrict__ jobs, int ld) {
+  const CoupleJob J = jobs[blockIdx.y];
+  const long n = (long)J.rows * J.cols;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const long c = e / J.rows, r = e - c * J.rows;
+    J.Y[r + c * ld] = r >= c ? J.X[r + c * ld] : 0.0;
+  }
+}
+
+// dst_k = beta dst_k + alpha (E_k src_{k-1} + E_{k+1}^T src_{k+1}) over the
+// node dofs of levels k = k0 + 2 blockIdx.y (level k at offset k M of a
+// vector; designs blockIdx.z with vector stride svd and coupling stride sEd)
+__global__ __launch_bounds__(256) void cr_gather_kernel(int k0, int nl, int M, const int *__restrict__ noff,
+                                                        const int *__restrict__ cp_rowp,
+                                                        const int *__restrict__ cp_col,
+                                                        const int *__restrict__ ct_rowp,
+                                                        const int *__restrict__ ct_src,
+                                                        const int *__restrict__ ct_blk, const double *__restrict__ E,
+                                                        long sEd, const double *__restrict__ src,
+                                                        double *__restrict__ dst, long svd, double alpha,
+                                                        double beta) {
+  const int k = k0 + 2 * blockIdx.y, d = blockIdx.z;
+  const int u0 = noff[k], nn = noff[k + 1] - u0;
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= 6 * nn) return;
+  const int j = g / 6, a = g - 6 * j, u = u0 + j;
+  E += d * sEd;
+  src += d * svd;
+  double s = 0.0;
+  if (k > 0) {
+    const double *t = src + (long)(k - 1) * M;
+    for (int q = cp_rowp[u]; q < cp_rowp[u + 1]; q++) {
+      const double *tk = t + 6 * cp_col[q];
+      const double *ep = E + 36L * q + a;
+#pragma unroll
+      for (int r = 0; r < 6; r++) s += ep[6 * r] * tk[r];
+    }
+  }
+  if (k + 1 < nl) {
+    const double *t = src + (long)(k + 1) * M;
+    for (int e = ct_rowp[u]; e < ct_rowp[u + 1]; e++) {
+      const double *tk = t + 6 * ct_src[e];
+      const double *ep = E + 36L * ct_blk[e] + 6 * a;
+#pragma unroll
+      for (int r = 0; r < 6; r++) s += ep[r] * tk[r];
+    }
+  }
+  double *y = dst + d * svd + (long)k * M + g;
+  *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * s;
+}
+
 // z[6i + a] -= sum_q sum_r E_q(a, r) t[6 col_q + r] for the nodes i of one
 // level (t: the previous level's segment); vector entry e = blockIdx.y (of
 // stride svd) uses the couplings of design e % nbf
@@ -935,6 +1057,7 @@ int setup_plan(shellgpu_factor *F) {
     F->M = (int)M;
     F->nv = (long)nl * M;
     F->per_design = 3L * nl * M2;
+    F->E_off = F->per_design;  // the sparse couplings follow the dense blocks
     F->rounds.clear();
     F->nkmax = 1;
     for (int st = 1; st < nl; st *= 2) {
@@ -986,11 +1109,12 @@ int setup_plan(shellgpu_factor *F) {
   }
   F->noff = noff;
 
-  // coupling blocks of the sequential factorisation, numbered in level
-  // order of their row node: CSR over level-ordered nodes (to the previous
-  // level) and its transpose (from the next level)
+  // coupling blocks (the sequential factorisation and the first round of
+  // cyclic reduction), numbered in level order of their row node: CSR over
+  // level-ordered nodes (to the previous level) and its transpose (from the
+  // next level)
   std::vector<int> blockq(ctx->nnzb, -1);
-  if (!F->cr) {
+  {
     std::vector<int> pos2node(nn);
     for (int i = 0; i < nn; i++) pos2node[node_pos[i]] = i;
     std::vector<int> cp_rowp(nn + 1, 0), cp_col, ct_cnt(nn + 1, 0);
@@ -1067,12 +1191,8 @@ int setup_plan(shellgpu_factor *F) {
       if (lj == li) {
         base[p] = F->cr ? (long)li * M2 + cl * F->M + rl : F->S_off[li] + cl * F->m[li] + rl;
       } else if (lj == li - 1) {
-        if (F->cr) {
-          base[p] = (long)(nl + li) * M2 + cl * F->M + rl;
-        } else {
-          base[p] = F->E_off + 36L * blockq[p];  // a 6 x 6 column-major block
-          ld[p] = 6;
-        }
+        base[p] = F->E_off + 36L * blockq[p];  // a 6 x 6 column-major block
+        ld[p] = 6;
       } else if (lj == li + 1) {
         base[p] = -1;  // upper block: the transpose of the next level's coupling
       } else {
@@ -1083,6 +1203,8 @@ int setup_plan(shellgpu_factor *F) {
   if (F->cr) {
     HIPCALL(ctx, hipMalloc((void **)&F->d_mlev, sizeof(int) * nl));
     HIPCALL(ctx, hipMemcpy(F->d_mlev, F->m.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+    HIPCALL(ctx, hipMalloc((void **)&F->d_noff, sizeof(int) * (nl + 1)));
+    HIPCALL(ctx, hipMemcpy(F->d_noff, noff.data(), sizeof(int) * (nl + 1), hipMemcpyHostToDevice));
   }
   HIPCALL(ctx, hipMalloc((void **)&F->d_bcflag, sizeof(int) * ctx->nnzb));
   HIPCALL(ctx, hipMemcpyAsync(F->d_bcflag, colbc.data(), sizeof(int) * ctx->nnzb, hipMemcpyHostToDevice,
@@ -1234,6 +1356,47 @@ int build_cr_pointers(shellgpu_factor *F) {
   HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
   HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
   F->root_off = root;
+
+  // jobs of the first round (odd levels i, left neighbour l = i - 1, right
+  // r = i + 1): the D_i slot holds X_i, then the transposes; Z_i holds P_i;
+  // the B_i slot holds Q_i = P_i E_i, then V_i = P_i E_r^T
+  const FactorRound &r1 = F->rounds[0];
+  std::vector<CoupleJob> jobs;
+  auto add = [&](long &off, bool right_only, auto make) {
+    off = (long)jobs.size();
+    for (int k = 0; k < (right_only ? r1.nR : r1.nE); k++) {
+      const int i = 2 * k + 1;
+      for (int d = 0; d < nb; d++) {
+        double *base = F->buf + d * PD;
+        double *Di = base + (long)i * M2, *Bi = base + (long)(nl + i) * M2, *Zi = base + (long)(2 * nl + i) * M2;
+        double *Dl = base + (long)(i - 1) * M2;
+        double *Dr = right_only ? base + (long)(i + 1) * M2 : nullptr;
+        double *Br = right_only ? base + (long)(nl + i + 1) * M2 : nullptr;
+        const double *E = base + F->E_off;
+        const int mi = F->m[i], ml = F->m[i - 1], mr = right_only ? F->m[i + 1] : 0;
+        jobs.push_back(make(i, Di, Bi, Zi, Dl, Dr, Br, E, mi, ml, mr));
+      }
+    }
+  };
+  const int M = F->M;
+  add(F->job_cp, false, [&](int, double *Di, double *, double *Zi, double *, double *, double *, const double *E,
+                            int, int, int) { return CoupleJob{Di, Zi, E, M, M, 0, 0}; });
+  add(F->job_q, false, [&](int i, double *, double *Bi, double *Zi, double *, double *, double *, const double *E,
+                           int mi, int ml, int) { return CoupleJob{Zi, Bi, E, mi, ml, i - 1, 0}; });
+  add(F->job_t1, false, [&](int, double *Di, double *Bi, double *, double *, double *, double *, const double *E,
+                            int mi, int ml, int) { return CoupleJob{Bi, Di, E, mi, ml, 0, 0}; });
+  add(F->job_dl, false, [&](int i, double *Di, double *, double *, double *Dl, double *, double *, const double *E,
+                            int mi, int ml, int) { return CoupleJob{Di, Dl, E, ml, mi, i - 1, 0}; });
+  add(F->job_v, true, [&](int i, double *, double *Bi, double *Zi, double *, double *, double *, const double *E,
+                          int mi, int, int mr) { return CoupleJob{Zi, Bi, E, mi, mr, i + 1, 0}; });
+  add(F->job_t2, true, [&](int, double *Di, double *Bi, double *, double *, double *, double *, const double *E,
+                           int mi, int, int mr) { return CoupleJob{Bi, Di, E, mi, mr, 0, 0}; });
+  add(F->job_dr, true, [&](int i, double *Di, double *, double *, double *, double *Dr, double *, const double *E,
+                           int mi, int, int mr) { return CoupleJob{Di, Dr, E, mr, mi, i + 1, 0}; });
+  add(F->job_br, true, [&](int i, double *Di, double *, double *, double *, double *, double *Br, const double *E,
+                           int mi, int, int mr) { return CoupleJob{Di, Br, E, mr, mi, i - 1, 0}; });
+  HIPCALL(ctx, hipMalloc((void **)&F->d_jobs, sizeof(CoupleJob) * jobs.size()));
+  HIPCALL(ctx, hipMemcpy(F->d_jobs, jobs.data(), sizeof(CoupleJob) * jobs.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1313,7 +1476,7 @@ void shellgpu_factor_destroy(shellgpu_factor *F) {
   void *bufs[] = {F->buf,      F->tmp,       F->info,      F->d_bcflag,  F->d_pos2dof, F->d_blk_base,
                   F->d_blk_ld, F->vb,        F->vz,        F->vx,        F->part,      F->d_mlev,
                   F->d_ptr,    F->cr_info,   F->d_cp_rowp, F->d_cp_col,  F->d_ct_rowp, F->d_ct_src,
-                  F->d_ct_blk, F->wbuf,      F->d_act};
+                  F->d_ct_blk, F->wbuf,      F->d_act,     F->d_noff,    F->d_jobs};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete F;
@@ -1599,9 +1762,70 @@ static int apply_schur(shellgpu_factor *F, int nrhs) {
   return 0;
 }
 
+// First round of cyclic reduction (the odd levels i, neighbours l = i - 1
+// and r = i + 1), on the sparse couplings E_i (level i to l) and E_r (r to
+// i) as they come from the mesh:
+//   X_i = chol(D_i)^{-1},  P_i = X_i^T X_i = D_i^{-1},
+//   D_l -= E_i^T P_i E_i,  D_r -= E_r P_i E_r^T,  B_r = -E_r P_i E_i,
+// the products with E at the cost of its few 6 x 6 blocks per node (two
+// dense-times-coupling products and a transpose per update, as in the
+// sequential factorisation), where the dense form spends 7 M^3 flops per
+// level on the couplings.  B_r, the coupling of r to l, is dense: the later
+// rounds run on the even levels in the dense form.
+static int factor_first_round(shellgpu_factor *F, int *info) {
+  shellgpu_ctx *ctx = F->ctx;
+  rocblas_handle h = (rocblas_handle)ctx->blas;
+  const FactorRound &r = F->rounds[0];
+  const int nb = F->nbatch, M = F->M, cnt = nb * r.nE, cntR = nb * r.nR;
+  const int maxnn = F->maxm / 6, ng = (maxnn + kCoupleNodes - 1) / kCoupleNodes;
+  const double one = 1.0;
+  double **P = F->d_ptr + r.off;
+  double **pD = P, **pZ = P + 2 * cnt;
+  const CoupleJob *J = F->d_jobs;
+  int rc = 0;
+  if ((rc = cholinv(F, 0, info))) return rc;
+  // P_i = X_i^T X_i in the Z_i slot (above the diagonal of its internal
+  // nodes X_i still holds D_i: the copy takes the lower triangle)
+  hipLaunchKernelGGL(copy_lower_job_kernel, dim3(std::min(1024L, ((long)M * M + 255) / 256), cnt), dim3(256), 0,
+                     ctx->stream, J + F->job_cp, M);
+  HIPCALL(ctx, hipGetLastError());
+  BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                                      rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
+                                      (const double *const *)pZ, M, pZ, M, cnt));
+  const dim3 gc((M + 255) / 256, ng), gt((M + 31) / 32, (M + 31) / 32);
+  auto couple = [&](bool left, long off, int njobs, double alpha, double beta) -> int {
+    if (njobs <= 0) return 0;
+    const dim3 g(gc.x, gc.y, njobs);
+    if (left)
+      hipLaunchKernelGGL(couple_job_kernel<true>, g, dim3(256), 0, ctx->stream, J + off, M, F->d_noff, F->d_ct_rowp,
+                         F->d_ct_src, F->d_ct_blk, alpha, beta);
+    else
+      hipLaunchKernelGGL(couple_job_kernel<false>, g, dim3(256), 0, ctx->stream, J + off, M, F->d_noff,
+                         F->d_cp_rowp, F->d_cp_col, F->d_cp_col, alpha, beta);
+    HIPCALL(ctx, hipGetLastError());
+    return 0;
+  };
+  auto transpose = [&](long off, int njobs) -> int {
+    if (njobs <= 0) return 0;
+    hipLaunchKernelGGL(transpose_job_kernel, dim3(gt.x, gt.y, njobs), dim3(256), 0, ctx->stream, J + off, M);
+    HIPCALL(ctx, hipGetLastError());
+    return 0;
+  };
+  // D_l -= (P_i E_i)^T E_i
+  if ((rc = couple(true, F->job_q, cnt, 1.0, 0.0)) || (rc = transpose(F->job_t1, cnt)) ||
+      (rc = couple(true, F->job_dl, cnt, -1.0, 1.0)))
+    return rc;
+  // V_i = P_i E_r^T;  D_r -= V_i^T E_r^T;  B_r = -V_i^T E_i
+  if ((rc = couple(false, F->job_v, cntR, 1.0, 0.0)) || (rc = transpose(F->job_t2, cntR)) ||
+      (rc = couple(false, F->job_dr, cntR, -1.0, 1.0)) || (rc = couple(true, F->job_br, cntR, -1.0, 0.0)))
+    return rc;
+  return 0;
+}
+
 // Block cyclic reduction of the level-block-tridiagonal matrix in F->buf
-// (after the scatter).  Round with stride s eliminates levels i = s, 3s,
-// ... with active neighbours l = i - s and r = i + s:
+// (after the scatter).  The first round runs on the sparse couplings
+// (factor_first_round); a later round with stride s eliminates levels
+// i = s, 3s, ... with active neighbours l = i - s and r = i + s:
 //   C_i C_i^T = D_i,  X_i = C_i^{-1},  Y_i = X_i B_i,  W_i = B_r X_i^T,
 //   D_l -= Y_i^T Y_i,  D_r -= W_i W_i^T,  B_r <- -W_i Y_i,
 // all entries of a round (levels x designs) in one batched call each;
@@ -1625,6 +1849,11 @@ static int factor_cr(shellgpu_factor *F) {
     double **P = F->d_ptr + r.off;
     double **pD = P, **pB = P + cnt, **pZ = P + 2 * cnt, **pDl = P + 3 * cnt, **pDr = P + 4 * cnt,
            **pBr = P + 5 * cnt;
+    if (r.s == 1) {
+      if ((rc = factor_first_round(F, F->cr_info + ioff))) return rc;
+      ioff += cnt;
+      continue;
+    }
     if ((rc = cholinv(F, &r - F->rounds.data(), F->cr_info + ioff))) return rc;
     // Y_i = X_i B_i (in place)
     BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
@@ -1671,14 +1900,32 @@ static int factor_cr(shellgpu_factor *F) {
 }
 
 // x = A^{-1} b by the cyclic-reduction factors: reduction sweep, root,
-// back-substitution sweep (level-ordered vectors vb, vz, vx)
+// back-substitution sweep (level-ordered vectors vb, vz, vx).  The first
+// round's levels i hold P_i: forward t_i = P_i b_i (into x_i), b_k -= E_k
+// t_{k-1} + E_{k+1}^T t_{k+1} on the even levels k; backward x_i = t_i -
+// P_i (E_i x_{i-1} + E_{i+1}^T x_{i+1}).
 static int apply_cr(shellgpu_factor *F) {
+  shellgpu_ctx *ctx = F->ctx;
   const int nb = F->nbatch, nl = F->nlevels, M = F->M;
   const long M2 = (long)M * M, PD = F->per_design, NV = F->nv;
   double *D0 = F->buf, *B0 = F->buf + (long)nl * M2, *Z0 = F->buf + 2L * nl * M2;
+  const double *E = F->buf + F->E_off;
+  const int gx = (F->maxm + 255) / 256, cnt1 = nb * F->rounds[0].nE;
+  const GemvBatch g1{nb, PD, 2 * M2, NV, 2 * M, NV, 2 * M};
+  auto gather = [&](int k0, const double *src, double *dst, double alpha, double beta) -> int {
+    const int nk = (nl - k0 + 1) / 2;
+    hipLaunchKernelGGL(cr_gather_kernel, dim3(gx, nk, nb), dim3(256), 0, ctx->stream, k0, nl, M, F->d_noff,
+                       F->d_cp_rowp, F->d_cp_col, F->d_ct_rowp, F->d_ct_src, F->d_ct_blk, E, PD, src, dst, NV,
+                       alpha, beta);
+    HIPCALL(ctx, hipGetLastError());
+    return 0;
+  };
   int rc = 0;
+  if ((rc = gemv_batch(F, false, M, M, Z0 + M2, M, F->vb + M, 1.0, 0.0, F->vx + M, 0, g1, cnt1))) return rc;
+  if ((rc = gather(0, F->vx, F->vb, -1.0, 1.0))) return rc;
   for (const FactorRound &r : F->rounds) {
     const long s = r.s;
+    if (s == 1) continue;
     const GemvBatch g{nb, PD, 2 * s * M2, NV, 2 * s * M, NV, 2 * s * M};
     const int cnt = nb * r.nE, cntR = nb * r.nR;
     // c_i = X_i b_i
@@ -1697,6 +1944,7 @@ static int apply_cr(shellgpu_factor *F) {
   }
   for (auto it = F->rounds.rbegin(); it != F->rounds.rend(); ++it) {
     const long s = it->s;
+    if (s == 1) continue;
     const GemvBatch g{nb, PD, 2 * s * M2, NV, 2 * s * M, NV, 2 * s * M};
     const int cnt = nb * it->nE, cntR = nb * it->nR;
     // c_i -= Y_i x_l
@@ -1708,7 +1956,8 @@ static int apply_cr(shellgpu_factor *F) {
     if ((rc = gemv_batch(F, true, M, M, D0 + s * M2, M, F->vz + s * M, 1.0, 0.0, F->vx + s * M, 1, g, cnt)))
       return rc;
   }
-  return 0;
+  if ((rc = gather(1, F->vx, F->vb, 1.0, 0.0))) return rc;
+  return gemv_batch(F, false, M, M, Z0 + M2, M, F->vb + M, -1.0, 1.0, F->vx + M, 0, g1, cnt1);
 }
 
 }  // extern "C"
