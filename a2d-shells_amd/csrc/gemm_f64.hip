@@ -5,7 +5,10 @@
 //   C_b = alpha op(A_b) op(B_b) + beta C_b,   b = 0 .. batch-1,
 //
 // column-major with leading dimensions, op = identity or transpose, the
-// matrices of entry b given by pointer arrays or by a base and a stride.
+// matrices of entry b given by pointer arrays or by a base and a stride;
+// op(A) or op(B) may be triangular (the K loop of a tile runs over the
+// range where both are nonzero) and C may be computed on and below the
+// diagonal only (the symmetric updates of a factorisation).
 //
 // Work split: a workgroup of 4 wavefronts computes a 64 x 64 tile of C, each
 // wavefront a 32 x 32 quarter as 2 x 2 blocks of v_mfma_f64_16x16x4f64
@@ -43,6 +46,13 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
   double *C = g.Cp ? g.Cp[bz] : g.C + bz * g.sC;
   const int M = g.M, N = g.N, K = g.K;
   const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
+  if (g.lower_c && n0 >= m0 + kTM) return;  // tile strictly above the diagonal
+  // K range holding the tile's nonzero products of triangular operands
+  int kbeg = 0, kend = K;
+  if (g.tri_a == kGemmLower) kend = min(kend, m0 + kTM);
+  if (g.tri_a == kGemmUpper) kbeg = max(kbeg, m0);
+  if (g.tri_b == kGemmLower) kbeg = max(kbeg, n0);
+  if (g.tri_b == kGemmUpper) kend = min(kend, n0 + kTN);
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
 
@@ -53,15 +63,18 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
   const int b_c = TB ? ((t & 15) << 2) : (t >> 2), b_k = TB ? (t >> 4) : ((t & 3) << 2);
   double ra[4], rb[4];
 
+  const int tra = g.tri_a, trb = g.tri_b;
   auto load = [&](int k0) {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       // A element (m, k) of op(A)
       const int m = m0 + a_r + (TA ? 0 : j), k = k0 + a_k + (TA ? j : 0);
-      ra[j] = (m < M && k < K) ? (TA ? A[k + (long)m * g.lda] : A[m + (long)k * g.lda]) : 0.0;
+      const bool ina = m < M && k < K && !(tra == kGemmLower && k > m) && !(tra == kGemmUpper && k < m);
+      ra[j] = ina ? (TA ? A[k + (long)m * g.lda] : A[m + (long)k * g.lda]) : 0.0;
       // B element (k, n) of op(B)
       const int n = n0 + b_c + (TB ? j : 0), kb = k0 + b_k + (TB ? 0 : j);
-      rb[j] = (n < N && kb < K) ? (TB ? B[n + (long)kb * g.ldb] : B[kb + (long)n * g.ldb]) : 0.0;
+      const bool inb = n < N && kb < K && !(trb == kGemmLower && kb < n) && !(trb == kGemmUpper && kb > n);
+      rb[j] = inb ? (TB ? B[n + (long)kb * g.ldb] : B[kb + (long)n * g.ldb]) : 0.0;
     }
   };
   auto store = [&](int buf) {
@@ -84,13 +97,15 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  load(0);
-  store(0);
+  if (kbeg < kend) {
+    load(kbeg);
+    store(0);
+  }
   __syncthreads();
   int cur = 0;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += kTK) {
-    const bool more = k0 + kTK < K;
+  for (int k0 = kbeg; k0 < kend; k0 += kTK) {
+    const bool more = k0 + kTK < kend;
     if (more) load(k0 + kTK);
 #pragma unroll
     for (int kk = 0; kk < kTK; kk += 4) {

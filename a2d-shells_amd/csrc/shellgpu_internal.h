@@ -155,6 +155,12 @@ hipError_t launch_elem_kernel(int mode, int variant, const ElemArgs &args, int c
 // Batched FP64 GEMM on the matrix cores (gemm_f64.hip): C_b = alpha op(A_b)
 // op(B_b) + beta C_b, column-major, op(A) M x K, op(B) K x N; entry b's
 // matrices from the pointer arrays Ap/Bp/Cp when set, else at A + b sA etc.
+// Triangular operands: tri_a = 1 (2) takes op(A) as lower (upper)
+// triangular, tri_b likewise for op(B) -- the other triangle is not read and
+// counts as zero, and the products skip its K ranges; lower_c = 1 computes
+// only the tiles of C that hold entries on or below the diagonal (entries
+// above it in those tiles are written too: C's upper triangle is scratch).
+enum { kGemmFull = 0, kGemmLower = 1, kGemmUpper = 2 };
 struct GemmArgs {
   int M, N, K;
   double alpha, beta;
@@ -170,6 +176,7 @@ struct GemmArgs {
   long sC;
   double *const *Cp;
   int ldc;
+  int tri_a = kGemmFull, tri_b = kGemmFull, lower_c = 0;
 };
 hipError_t launch_gemm_f64(bool transA, bool transB, const GemmArgs &g, int batch, hipStream_t stream);
 

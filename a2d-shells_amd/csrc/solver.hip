@@ -163,7 +163,7 @@ struct shellgpu_factor {
   // (offsets into d_jobs, entries q = k nb + d of the round)
   int *d_noff = nullptr;
   CoupleJob *d_jobs = nullptr;
-  long job_cp = 0, job_q = 0, job_t1 = 0, job_dl = 0, job_v = 0, job_t2 = 0, job_dr = 0, job_br = 0;
+  long job_q = 0, job_t1 = 0, job_dl = 0, job_v = 0, job_t2 = 0, job_dr = 0, job_br = 0;
 };
 
 namespace {
@@ -186,8 +186,9 @@ int ferr(shellgpu_ctx *ctx, int code, const std::string &msg) {
       return ferr(ctx, 4, std::string(#call) + ": " + rocblas_status_to_string(_s)); \
   } while (0)
 
-// Dense updates of the factorisations: rocBLAS, or with SHELLGPU_GEMM=own
-// the project's FP64 matrix-core GEMM (gemm_f64.hip).
+// Dense updates of the sequential factorisation: rocBLAS, or with
+// SHELLGPU_GEMM=own the project's FP64 matrix-core GEMM (gemm_f64.hip,
+// which runs every dense update of cyclic reduction).
 bool own_gemm() {
   static const bool own = [] {
     const char *e = getenv("SHELLGPU_GEMM");
@@ -211,18 +212,18 @@ int gemm_sb(shellgpu_ctx *ctx, bool ta, bool tb, int M, int N, int K, double alp
   return 0;
 }
 
-// pointer-array batch
-int gemm_pb(shellgpu_ctx *ctx, bool ta, bool tb, int M, int N, int K, double alpha, const double *const *A, int lda,
-            const double *const *B, int ldb, double beta, double *const *C, int ldc, int batch) {
-  if (own_gemm()) {
-    GemmArgs g{M, N, K, alpha, beta, nullptr, 0, A, lda, nullptr, 0, B, ldb, nullptr, 0, C, ldc};
-    HIPCALL(ctx, launch_gemm_f64(ta, tb, g, batch, ctx->stream));
-    return 0;
-  }
-  BLASCALL(ctx, rocblas_dgemm_batched((rocblas_handle)ctx->blas,
-                                      ta ? rocblas_operation_transpose : rocblas_operation_none,
-                                      tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A,
-                                      lda, B, ldb, &beta, C, ldc, batch));
+// pointer-array batch on the matrix cores, with triangular operands and
+// lower-only results (gemm_f64.hip): the dense updates of cyclic reduction
+int mgemm(shellgpu_ctx *ctx, bool ta, bool tb, int M, int N, int K, double alpha, double *const *A, int lda,
+          int tri_a, double *const *B, int ldb, int tri_b, double beta, double *const *C, int ldc, int lower_c,
+          int batch) {
+  if (batch <= 0 || M <= 0 || N <= 0) return 0;
+  GemmArgs g{M, N, K, alpha, beta, nullptr, 0, (const double *const *)A, lda, nullptr, 0,
+             (const double *const *)B, ldb, nullptr, 0, C, ldc};
+  g.tri_a = tri_a;
+  g.tri_b = tri_b;
+  g.lower_c = lower_c;
+  HIPCALL(ctx, launch_gemm_f64(ta, tb, g, batch, ctx->stream));
   return 0;
 }
 
@@ -612,17 +613,6 @@ __global__ __launch_bounds__(256) void transpose_job_kernel(const CoupleJob *__r
   __syncthreads();
   for (int k = ty; k < 32; k += 8)
     if (c0 + tx < J.cols && r0 + k < J.rows) J.Y[(c0 + tx) + (long)(r0 + k) * ld] = tile[tx][k];
-}
-
-// Y = lower triangle of X, zeros above (rows x cols, leading dimension ld)
-// for job blockIdx.y
-__global__ __launch_bounds__(256) void copy_lower_job_kernel(const CoupleJob *__restrict__ jobs, int ld) {
-  const CoupleJob J = jobs[blockIdx.y];
-  const long n = (long)J.rows * J.cols;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
-    const long c = e / J.rows, r = e - c * J.rows;
-    J.Y[r + c * ld] = r >= c ? J.X[r + c * ld] : 0.0;
-  }
 }
 
 // dst_k = beta dst_k + alpha (E_k src_{k-1} + E_{k+1}^T src_{k+1}) over the
@@ -1291,8 +1281,10 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
 }
 
 // Cholesky-inverse node sequence of an n-block and, per node, the pointer
-// arrays (A11, A21, A22; leaves A11 only) of the blocks D[q] (lda)
-void add_chol_group(shellgpu_factor *F, std::vector<double *> &h, const std::vector<double *> &D, int n, int lda) {
+// arrays (A11, A21, A22, A12 and the scratch W[q]; leaves A11 only) of the
+// blocks D[q] (lda); W[q] holds n^2 / 4 doubles
+void add_chol_group(shellgpu_factor *F, std::vector<double *> &h, const std::vector<double *> &D,
+                    const std::vector<double *> &W, int n, int lda) {
   shellgpu_factor::CholGroup grp;
   grp.n = n;
   grp.lda = lda;
@@ -1300,19 +1292,24 @@ void add_chol_group(shellgpu_factor *F, std::vector<double *> &h, const std::vec
   chol_plan(0, n, grp.nodes);
   for (const CholNode &c : grp.nodes) {
     grp.offs.push_back((long)h.size());
-    const long a11 = (long)c.r0 * (lda + 1), a21 = a11 + c.n1, a22 = a11 + (long)c.n1 * (lda + 1);
+    const long a11 = (long)c.r0 * (lda + 1), a21 = a11 + c.n1, a22 = a11 + (long)c.n1 * (lda + 1),
+               a12 = a11 + (long)c.n1 * lda;
     for (double *p : D) h.push_back(p + a11);
     if (c.n2 > 0) {
       for (double *p : D) h.push_back(p + a21);
       for (double *p : D) h.push_back(p + a22);
+      for (double *p : D) h.push_back(p + a12);
+      for (double *p : W) h.push_back(p);
     }
   }
   F->cgroups.push_back(grp);
 }
 
 // Pointer arrays of every cyclic-reduction round, entry q = k nb + d
-// (level entry k, design d): D_i, B_i, Z_i, D_{i-s}, D_{i+s}, B_{i+s} for
-// i = s (2k + 1); the last three exist for the first nR nb entries only.
+// (level entry k, design d): D_i, B_i, Z_i, D_{i-s}, D_{i+s}, B_{i+s},
+// B_{i-1} for i = s (2k + 1); D_{i+s} and B_{i+s} exist for the first nR nb
+// entries only.  B_{i-1} and D_{i-1} (odd levels: free after the first
+// round) are the scratch of the later rounds: Y_i is stored in B_{i-1}.
 // A final group of nb entries points at D_0 (the root).
 int build_cr_pointers(shellgpu_factor *F) {
   shellgpu_ctx *ctx = F->ctx;
@@ -1322,7 +1319,7 @@ int build_cr_pointers(shellgpu_factor *F) {
   for (FactorRound &r : F->rounds) {
     r.off = (long)h.size();
     const long cnt = (long)nb * r.nE;
-    h.resize(h.size() + 6 * cnt, nullptr);
+    h.resize(h.size() + 7 * cnt, nullptr);
     double **P = h.data() + r.off;
     for (int k = 0; k < r.nE; k++) {
       const int i = r.s * (2 * k + 1);
@@ -1340,18 +1337,29 @@ int build_cr_pointers(shellgpu_factor *F) {
           P[4 * cnt + q] = P[3 * cnt + q];  // unused (entries beyond nR nb)
           P[5 * cnt + q] = P[1 * cnt + q];
         }
+        P[6 * cnt + q] = base + (long)(nl + i - 1) * M2;
       }
     }
   }
   const long root = (long)h.size();
   for (int d = 0; d < nb; d++) h.push_back(F->buf + d * PD);
+  // Cholesky-inverse scratch: B_i in the first round (its coupling products
+  // come after), D_{i-1} in the later rounds, B_0 (never used) for the root
   F->cgroups.clear();
   for (size_t g = 0; g <= F->rounds.size(); g++) {
     const bool is_root = g == F->rounds.size();
     const long cnt = is_root ? nb : (long)nb * F->rounds[g].nE;
     const long src = is_root ? root : F->rounds[g].off;  // D pointers of the group
-    std::vector<double *> D(h.begin() + src, h.begin() + src + cnt);
-    add_chol_group(F, h, D, F->M, F->M);
+    std::vector<double *> D(h.begin() + src, h.begin() + src + cnt), W(cnt);
+    for (long q = 0; q < cnt; q++) {
+      if (is_root)
+        W[q] = D[q] + (long)nl * M2;
+      else if (F->rounds[g].s == 1)
+        W[q] = h[src + cnt + q];
+      else
+        W[q] = D[q] - M2;
+    }
+    add_chol_group(F, h, D, W, F->M, F->M);
   }
   HIPCALL(ctx, hipMalloc((void **)&F->d_ptr, sizeof(double *) * h.size()));
   HIPCALL(ctx, hipMemcpy(F->d_ptr, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice));
@@ -1378,9 +1386,6 @@ int build_cr_pointers(shellgpu_factor *F) {
       }
     }
   };
-  const int M = F->M;
-  add(F->job_cp, false, [&](int, double *Di, double *, double *Zi, double *, double *, double *, const double *E,
-                            int, int, int) { return CoupleJob{Di, Zi, E, M, M, 0, 0}; });
   add(F->job_q, false, [&](int i, double *, double *Bi, double *Zi, double *, double *, double *, const double *E,
                            int mi, int ml, int) { return CoupleJob{Zi, Bi, E, mi, ml, i - 1, 0}; });
   add(F->job_t1, false, [&](int, double *Di, double *Bi, double *, double *, double *, double *, const double *E,
@@ -1625,36 +1630,37 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
 }
 
 // X = chol(A)^{-1} (lower) in place for the cnt diagonal blocks of group g:
-// leaves in LDS; internal nodes by batched BLAS on the halves,
-//   C21 = A21 X11^T,  A22 -= C21 C21^T,  (trailing half),
-//   X21 = -X22 C21 X11.
+// leaves in LDS; internal nodes on the matrix cores, out of place through
+// the block's upper triangle A12 (unused: X is lower triangular) and the
+// scratch W:
+//   A12 = X11 A21^T (= C21^T),  A22 -= A12^T A12 (lower),  (trailing half),
+//   W = X11^T A12 (= (C21 X11)^T),  A21 = -X22 W^T (= X21).
 static int cholinv(shellgpu_factor *F, size_t g, int *info) {
   shellgpu_ctx *ctx = F->ctx;
-  rocblas_handle h = (rocblas_handle)ctx->blas;
   const shellgpu_factor::CholGroup &grp = F->cgroups[g];
   const int M = grp.lda, cnt = grp.cnt;
-  const double one = 1.0, mone = -1.0;
   for (size_t k = 0; k < grp.nodes.size(); k++) {
     const CholNode &c = grp.nodes[k];
     double **P = F->d_ptr + grp.offs[k];
-    double **A11 = P, **A21 = P + cnt, **A22 = P + 2 * (long)cnt;
-    if (c.n2 == 0) {
-      hipLaunchKernelGGL(cholinv_base_kernel, dim3(cnt), dim3(kCholThreads), 0, ctx->stream, A11, c.n1, M, info);
+    double **A11 = P, **A21 = P + cnt, **A22 = P + 2 * (long)cnt, **A12 = P + 3 * (long)cnt,
+           **W = P + 4 * (long)cnt;
+    const int n1 = c.n1, n2 = c.n2;
+    int rc = 0;
+    if (n2 == 0) {
+      hipLaunchKernelGGL(cholinv_base_kernel, dim3(cnt), dim3(kCholThreads), 0, ctx->stream, A11, n1, M, info);
       HIPCALL(ctx, hipGetLastError());
     } else if (c.first) {
-      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                                          rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A11, M,
-                                          (const double *const *)A21, M, A21, M, cnt));
-      int rc = gemm_pb(ctx, false, true, c.n2, c.n2, c.n1, mone, (const double *const *)A21, M,
-                       (const double *const *)A21, M, one, A22, M, cnt);
-      if (rc) return rc;
+      if ((rc = mgemm(ctx, false, true, n1, n2, n1, 1.0, A11, M, kGemmLower, A21, M, kGemmFull, 0.0, A12, M, 0,
+                      cnt)) ||
+          (rc = mgemm(ctx, true, false, n2, n2, n1, -1.0, A12, M, kGemmFull, A12, M, kGemmFull, 1.0, A22, M, 1,
+                      cnt)))
+        return rc;
     } else {
-      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
-                                          rocblas_diagonal_non_unit, c.n2, c.n1, &one, (const double *const *)A11, M,
-                                          (const double *const *)A21, M, A21, M, cnt));
-      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                          rocblas_diagonal_non_unit, c.n2, c.n1, &mone, (const double *const *)A22, M,
-                                          (const double *const *)A21, M, A21, M, cnt));
+      if ((rc = mgemm(ctx, true, false, n1, n2, n1, 1.0, A11, M, kGemmUpper, A12, M, kGemmFull, 0.0, W, n1, 0,
+                      cnt)) ||
+          (rc = mgemm(ctx, false, true, n2, n1, n2, -1.0, A22, M, kGemmLower, W, n1, kGemmFull, 0.0, A21, M, 0,
+                      cnt)))
+        return rc;
     }
   }
   return 0;
@@ -1774,24 +1780,17 @@ static int apply_schur(shellgpu_factor *F, int nrhs) {
 // rounds run on the even levels in the dense form.
 static int factor_first_round(shellgpu_factor *F, int *info) {
   shellgpu_ctx *ctx = F->ctx;
-  rocblas_handle h = (rocblas_handle)ctx->blas;
   const FactorRound &r = F->rounds[0];
   const int nb = F->nbatch, M = F->M, cnt = nb * r.nE, cntR = nb * r.nR;
   const int maxnn = F->maxm / 6, ng = (maxnn + kCoupleNodes - 1) / kCoupleNodes;
-  const double one = 1.0;
   double **P = F->d_ptr + r.off;
   double **pD = P, **pZ = P + 2 * cnt;
   const CoupleJob *J = F->d_jobs;
   int rc = 0;
   if ((rc = cholinv(F, 0, info))) return rc;
-  // P_i = X_i^T X_i in the Z_i slot (above the diagonal of its internal
-  // nodes X_i still holds D_i: the copy takes the lower triangle)
-  hipLaunchKernelGGL(copy_lower_job_kernel, dim3(std::min(1024L, ((long)M * M + 255) / 256), cnt), dim3(256), 0,
-                     ctx->stream, J + F->job_cp, M);
-  HIPCALL(ctx, hipGetLastError());
-  BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
-                                      rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
-                                      (const double *const *)pZ, M, pZ, M, cnt));
+  // P_i = X_i^T X_i in the Z_i slot (products over k >= max(row, column))
+  if ((rc = mgemm(ctx, true, false, M, M, M, 1.0, pD, M, kGemmUpper, pD, M, kGemmLower, 0.0, pZ, M, 0, cnt)))
+    return rc;
   const dim3 gc((M + 255) / 256, ng), gt((M + 31) / 32, (M + 31) / 32);
   auto couple = [&](bool left, long off, int njobs, double alpha, double beta) -> int {
     if (njobs <= 0) return 0;
@@ -1826,15 +1825,15 @@ static int factor_first_round(shellgpu_factor *F, int *info) {
 // (after the scatter).  The first round runs on the sparse couplings
 // (factor_first_round); a later round with stride s eliminates levels
 // i = s, 3s, ... with active neighbours l = i - s and r = i + s:
-//   C_i C_i^T = D_i,  X_i = C_i^{-1},  Y_i = X_i B_i,  W_i = B_r X_i^T,
+//   C_i C_i^T = D_i,  X_i = C_i^{-1},  W_i = B_r X_i^T,  Y_i = X_i B_i,
 //   D_l -= Y_i^T Y_i,  D_r -= W_i W_i^T,  B_r <- -W_i Y_i,
-// all entries of a round (levels x designs) in one batched call each;
-// level 0 is factored last.
+// all entries of a round (levels x designs) in one batched call each, on
+// the matrix cores (triangular X_i: half the products; symmetric updates:
+// the lower tiles only); Y_i goes to the free slot B_{i-1}.  Level 0 is
+// factored last.
 static int factor_cr(shellgpu_factor *F) {
   shellgpu_ctx *ctx = F->ctx;
-  rocblas_handle h = (rocblas_handle)ctx->blas;
   const int nb = F->nbatch, nl = F->nlevels, M = F->M;
-  const double one = 1.0, mone = -1.0, zero = 0.0;
   if (*std::min_element(F->m.begin(), F->m.end()) < M) {
     hipLaunchKernelGGL(pad_identity_kernel, dim3((M + 255) / 256, nl, nb), dim3(256), 0, ctx->stream, M, (long)M * M,
                        F->d_mlev, F->buf, F->per_design);
@@ -1848,36 +1847,24 @@ static int factor_cr(shellgpu_factor *F) {
     const int cnt = nb * r.nE, cntR = nb * r.nR;
     double **P = F->d_ptr + r.off;
     double **pD = P, **pB = P + cnt, **pZ = P + 2 * cnt, **pDl = P + 3 * cnt, **pDr = P + 4 * cnt,
-           **pBr = P + 5 * cnt;
+           **pBr = P + 5 * cnt, **pY = P + 6 * cnt;
     if (r.s == 1) {
       if ((rc = factor_first_round(F, F->cr_info + ioff))) return rc;
       ioff += cnt;
       continue;
     }
-    if ((rc = cholinv(F, &r - F->rounds.data(), F->cr_info + ioff))) return rc;
-    // Y_i = X_i B_i (in place)
-    BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                        rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
-                                        (const double *const *)pB, M, pB, M, cnt));
-    if (cntR > 0)  // W_i = B_r X_i^T
-      BLASCALL(ctx, rocblas_dtrmm_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                                          rocblas_diagonal_non_unit, M, M, &one, (const double *const *)pD, M,
-                                          (const double *const *)pBr, M, pZ, M, cntR));
-    // D_l -= Y_i^T Y_i (full GEMM: rocBLAS's batched SYRK runs its
-    // diagonal tiles in a slow non-MFMA kernel at these sizes)
-    if ((rc = gemm_pb(ctx, true, false, M, M, M, mone, (const double *const *)pB, M, (const double *const *)pB, M,
-                      one, pDl, M, cnt)))
+    if ((rc = cholinv(F, &r - F->rounds.data(), F->cr_info + ioff)) ||
+        // W_i = B_r X_i^T
+        (rc = mgemm(ctx, false, true, M, M, M, 1.0, pBr, M, kGemmFull, pD, M, kGemmUpper, 0.0, pZ, M, 0, cntR)) ||
+        // Y_i = X_i B_i
+        (rc = mgemm(ctx, false, false, M, M, M, 1.0, pD, M, kGemmLower, pB, M, kGemmFull, 0.0, pY, M, 0, cnt)) ||
+        // D_l -= Y_i^T Y_i
+        (rc = mgemm(ctx, true, false, M, M, M, -1.0, pY, M, kGemmFull, pY, M, kGemmFull, 1.0, pDl, M, 1, cnt)) ||
+        // D_r -= W_i W_i^T
+        (rc = mgemm(ctx, false, true, M, M, M, -1.0, pZ, M, kGemmFull, pZ, M, kGemmFull, 1.0, pDr, M, 1, cntR)) ||
+        // B_r = -W_i Y_i (coupling of r to l)
+        (rc = mgemm(ctx, false, false, M, M, M, -1.0, pZ, M, kGemmFull, pY, M, kGemmFull, 0.0, pBr, M, 0, cntR)))
       return rc;
-    if (cntR > 0) {
-      // D_r -= W_i W_i^T
-      if ((rc = gemm_pb(ctx, false, true, M, M, M, mone, (const double *const *)pZ, M, (const double *const *)pZ, M,
-                        one, pDr, M, cntR)))
-        return rc;
-      // B_r = -W_i Y_i (coupling of r to l)
-      if ((rc = gemm_pb(ctx, false, false, M, M, M, mone, (const double *const *)pZ, M, (const double *const *)pB,
-                        M, zero, pBr, M, cntR)))
-        return rc;
-    }
     ioff += cnt;
   }
   if ((rc = cholinv(F, F->rounds.size(), F->cr_info + ioff))) return rc;
@@ -1931,8 +1918,9 @@ static int apply_cr(shellgpu_factor *F) {
     // c_i = X_i b_i
     if ((rc = gemv_batch(F, false, M, M, D0 + s * M2, M, F->vb + s * M, 1.0, 0.0, F->vz + s * M, 1, g, cnt)))
       return rc;
-    // b_l -= Y_i^T c_i
-    if ((rc = gemv_batch(F, true, M, M, B0 + s * M2, M, F->vz + s * M, -1.0, 1.0, F->vb, 0, g, cnt))) return rc;
+    // b_l -= Y_i^T c_i (Y_i in the slot B_{i-1})
+    if ((rc = gemv_batch(F, true, M, M, B0 + (s - 1) * M2, M, F->vz + s * M, -1.0, 1.0, F->vb, 0, g, cnt)))
+      return rc;
     // b_r -= W_i c_i
     if ((rc = gemv_batch(F, false, M, M, Z0 + s * M2, M, F->vz + s * M, -1.0, 1.0, F->vb + 2 * s * M, 0, g, cntR)))
       return rc;
@@ -1948,7 +1936,8 @@ static int apply_cr(shellgpu_factor *F) {
     const GemvBatch g{nb, PD, 2 * s * M2, NV, 2 * s * M, NV, 2 * s * M};
     const int cnt = nb * it->nE, cntR = nb * it->nR;
     // c_i -= Y_i x_l
-    if ((rc = gemv_batch(F, false, M, M, B0 + s * M2, M, F->vx, -1.0, 1.0, F->vz + s * M, 0, g, cnt))) return rc;
+    if ((rc = gemv_batch(F, false, M, M, B0 + (s - 1) * M2, M, F->vx, -1.0, 1.0, F->vz + s * M, 0, g, cnt)))
+      return rc;
     // c_i -= W_i^T x_r
     if ((rc = gemv_batch(F, true, M, M, Z0 + s * M2, M, F->vx + 2 * s * M, -1.0, 1.0, F->vz + s * M, 0, g, cntR)))
       return rc;
