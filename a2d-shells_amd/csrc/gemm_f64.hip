@@ -10,14 +10,16 @@
 // range where both are nonzero) and C may be computed on and below the
 // diagonal only (the symmetric updates of a factorisation).
 //
-// Work split: a workgroup of 4 wavefronts computes a 64 x 64 tile of C, each
-// wavefront a 32 x 32 quarter as 2 x 2 blocks of v_mfma_f64_16x16x4f64
-// (16 x 16 x 4 per instruction).  K advances 16 at a time through LDS:
-// every thread stages 4 doubles of the A tile (64 x 16) and 4 of the B tile
-// (16 x 64), double-buffered so the next tile's global loads are in flight
-// during the current tile's 16 matrix instructions per wavefront.  Tiles are
-// stored k-row by k-row with 80 doubles per row: the four k-rows read by one
-// matrix instruction then fall in distinct LDS banks.
+// Work split: a workgroup of 4 wavefronts computes a T x T tile of C (T =
+// 128 for large problems, 64 where 128-tiles would leave the GPU short of
+// workgroups), each wavefront a T/2 x T/2 quarter as (T/32)^2 blocks of
+// v_mfma_f64_16x16x4f64 (16 x 16 x 4 per instruction, 64 cycles).  K
+// advances 16 at a time through LDS: every thread stages T/16 doubles of the
+// A tile (T x 16) and of the B tile (16 x T), double-buffered so the next
+// tile's global loads are in flight during the current tile's (T/32)^2 x 4
+// matrix instructions per wavefront.  Tiles are stored k-row by k-row with an
+// odd row length (T + 1 doubles): the staging stores (along a row or down a
+// column) and the operand reads are free of LDS bank conflicts.
 //
 // Operand layout of v_mfma_f64_16x16x4f64 (one 64-lane wavefront), as
 // measured on gfx950: lane l supplies X[l % 16][l / 16] and Y[l / 16][l % 16]
@@ -32,131 +34,177 @@ namespace shellgpu {
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+// global-memory views of the operands: the compiler then issues global
+// loads (a generic pointer read from a struct or a pointer array gives flat
+// loads, which also count against the LDS wait counter and so serialise
+// the prefetch of the next tile with the matrix instructions)
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
 
-constexpr int kTM = 64, kTN = 64, kTK = 16;
-constexpr int kRow = 80;  // doubles per staged k-row
+constexpr int kTK = 16;
+constexpr int kGroup = 8;  // tile rows per group of the work order
 
-template <bool TA, bool TB>
+// WT: 16 x 16 blocks per wavefront and direction (tile T = 32 WT)
+template <bool TA, bool TB, int WT>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
+  constexpr int kT = 32 * WT;      // workgroup tile
+  constexpr int kRow = kT + 1;     // doubles per staged k-row: odd, so a column walk hits every bank
+  constexpr int kLd = kT / 16;     // staged doubles per thread and operand
+  constexpr int kStep = 256 / kT;  // k-rows per staging pass (lane along m or n)
   __shared__ double As[2][kTK][kRow];
   __shared__ double Bs[2][kTK][kRow];
-  const long bz = blockIdx.z;
-  const double *A = g.Ap ? g.Ap[bz] : g.A + bz * g.sA;
-  const double *B = g.Bp ? g.Bp[bz] : g.B + bz * g.sB;
-  double *C = g.Cp ? g.Cp[bz] : g.C + bz * g.sC;
+  // Work order.  The dispatcher hands consecutive workgroups to the 8 XCDs
+  // in turn, each XCD with its own L2: give XCD x the x-th contiguous eighth
+  // of the work order, which runs batch entry by batch entry and within one
+  // in groups of kGroup tile rows, column by column, so the workgroups
+  // resident on one XCD share a few panels of A and B in its L2.
+  const int tM = (g.M + kT - 1) / kT, tN = (g.N + kT - 1) / kT;
+  const unsigned G = gridDim.x, G8 = G & ~7u, gid = blockIdx.x;
+  const unsigned w = gid < G8 ? (gid & 7u) * (G8 >> 3) + (gid >> 3) : gid;
+  const long bz = w / ((unsigned)tM * tN);
+  const int wt = (int)(w - bz * tM * tN);
+  const int gm0 = wt / (kGroup * tN) * kGroup, gsz = min(tM - gm0, kGroup), tg = wt - gm0 * tN;
+  const int tile_m = gm0 + tg % gsz, tile_n = tg / gsz;
+  gcdouble *A = (gcdouble *)(g.Ap ? g.Ap[bz] : g.A + bz * g.sA);
+  gcdouble *B = (gcdouble *)(g.Bp ? g.Bp[bz] : g.B + bz * g.sB);
+  gdouble *C = (gdouble *)(g.Cp ? g.Cp[bz] : g.C + bz * g.sC);
   const int M = g.M, N = g.N, K = g.K;
-  const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
-  if (g.lower_c && n0 >= m0 + kTM) return;  // tile strictly above the diagonal
+  const int m0 = tile_m * kT, n0 = tile_n * kT;
+  if (g.lower_c && n0 >= m0 + kT) return;  // tile strictly above the diagonal
   // K range holding the tile's nonzero products of triangular operands
   int kbeg = 0, kend = K;
-  if (g.tri_a == kGemmLower) kend = min(kend, m0 + kTM);
+  if (g.tri_a == kGemmLower) kend = min(kend, m0 + kT);
   if (g.tri_a == kGemmUpper) kbeg = max(kbeg, m0);
   if (g.tri_b == kGemmLower) kbeg = max(kbeg, n0);
-  if (g.tri_b == kGemmUpper) kend = min(kend, n0 + kTN);
+  if (g.tri_b == kGemmUpper) kend = min(kend, n0 + kT);
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * (kT / 2), wn = (wave >> 1) * (kT / 2);
 
-  // staging assignment: A tile (m 0..63, k 0..15), B tile (k 0..15, n 0..63)
-  // N-op A / T-op B: 4 consecutive m (n) at one k; T-op A / N-op B: 4
-  // consecutive k at one m (n) -- the contiguous direction in memory
-  const int a_r = TA ? (t >> 2) : ((t & 15) << 2), a_k = TA ? ((t & 3) << 2) : (t >> 4);
-  const int b_c = TB ? ((t & 15) << 2) : (t >> 2), b_k = TB ? (t >> 4) : ((t & 3) << 2);
-  double ra[4], rb[4];
+  // staging assignment: A tile (m 0..T-1, k 0..15), B tile (k 0..15, n 0..T-1),
+  // consecutive lanes along the operand's contiguous direction in memory
+  // (coalesced loads, and with the odd LDS row length conflict-free stores):
+  // N-op A / T-op B: lane -> m (n) = t % T, k = t / T + kStep j;
+  // T-op A / N-op B: lane -> k = t % 16, m (n) = t / 16 + 16 j
+  const int a_r = TA ? (t >> 4) : (t % kT), a_k = TA ? (t & 15) : (t / kT);
+  const int b_c = TB ? (t % kT) : (t >> 4), b_k = TB ? (t / kT) : (t & 15);
+  double ra[kLd], rb[kLd];
 
   const int tra = g.tri_a, trb = g.tri_b;
+  // every load reads an element of the operand (indices clamped into the
+  // matrix; K >= 1 wherever the K loop runs): no branches.  Out-of-range
+  // and out-of-triangle elements are zeroed when the tile is stored to LDS,
+  // so the loaded values are first used after the current tile's matrix
+  // instructions and the loads overlap them.
+  unsigned keep = 0;
   auto load = [&](int k0) {
+    keep = 0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kLd; j++) {
       // A element (m, k) of op(A)
-      const int m = m0 + a_r + (TA ? 0 : j), k = k0 + a_k + (TA ? j : 0);
+      const int m = m0 + a_r + (TA ? 16 * j : 0), k = k0 + a_k + (TA ? 0 : kStep * j);
       const bool ina = m < M && k < K && !(tra == kGemmLower && k > m) && !(tra == kGemmUpper && k < m);
-      ra[j] = ina ? (TA ? A[k + (long)m * g.lda] : A[m + (long)k * g.lda]) : 0.0;
+      const int mc = min(m, M - 1), kc = min(k, K - 1);
+      ra[j] = TA ? A[kc + (long)mc * g.lda] : A[mc + (long)kc * g.lda];
       // B element (k, n) of op(B)
-      const int n = n0 + b_c + (TB ? j : 0), kb = k0 + b_k + (TB ? 0 : j);
+      const int n = n0 + b_c + (TB ? 0 : 16 * j), kb = k0 + b_k + (TB ? kStep * j : 0);
       const bool inb = n < N && kb < K && !(trb == kGemmLower && kb < n) && !(trb == kGemmUpper && kb > n);
-      rb[j] = inb ? (TB ? B[n + (long)kb * g.ldb] : B[kb + (long)n * g.ldb]) : 0.0;
+      const int nc = min(n, N - 1), kbc = min(kb, K - 1);
+      rb[j] = TB ? B[nc + (long)kbc * g.ldb] : B[kbc + (long)nc * g.ldb];
+      keep |= (ina ? 1u : 0u) << j | (inb ? 1u : 0u) << (16 + j);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kLd; j++) {
+      const double va = (keep >> j) & 1u ? ra[j] : 0.0, vb = (keep >> (16 + j)) & 1u ? rb[j] : 0.0;
       if (TA)
-        As[buf][a_k + j][a_r] = ra[j];
+        As[buf][a_k][a_r + 16 * j] = va;
       else
-        As[buf][a_k][a_r + j] = ra[j];
+        As[buf][a_k + kStep * j][a_r] = va;
       if (TB)
-        Bs[buf][b_k][b_c + j] = rb[j];
+        Bs[buf][b_k + kStep * j][b_c] = vb;
       else
-        Bs[buf][b_k + j][b_c] = rb[j];
+        Bs[buf][b_k][b_c + 16 * j] = vb;
     }
   };
 
-  v4d acc[2][2];
+  v4d acc[WT][WT];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < WT; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < WT; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  if (kbeg < kend) {
+  // counted loop with an unconditional prefetch of the next tile (clamped
+  // and zero-masked past the end): no branches in the loop body
+  const int nk = kbeg < kend ? (kend - kbeg + kTK - 1) / kTK : 0;
+  if (nk > 0) {
     load(kbeg);
     store(0);
   }
   __syncthreads();
-  int cur = 0;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = kbeg; k0 < kend; k0 += kTK) {
-    const bool more = k0 + kTK < kend;
-    if (more) load(k0 + kTK);
+  for (int it = 0; it < nk; it++) {
+    const int cur = it & 1;
+    load(kbeg + (it + 1) * kTK);
 #pragma unroll
     for (int kk = 0; kk < kTK; kk += 4) {
-      double a[2], b[2];
+      double a[WT], b[WT];
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
+      for (int i = 0; i < WT; i++) {
         a[i] = As[cur][kk + lk][wm + 16 * i + lr];
         b[i] = Bs[cur][kk + lk][wn + 16 * i + lr];
       }
 #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int i = 0; i < WT; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < WT; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
     }
-    if (more) store(cur ^ 1);
+    store(cur ^ 1);
     __syncthreads();
-    cur ^= 1;
   }
 
   // C = alpha acc + beta C
 #pragma unroll
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < WT; i++) {
     const int m = m0 + wm + 16 * i + lr;
     if (m >= M) continue;
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < WT; j++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const int n = n0 + wn + 16 * j + 4 * e + lk;
         if (n >= N) continue;
-        double *c = C + m + (long)n * g.ldc;
+        gdouble *c = C + m + (long)n * g.ldc;
         const double v = g.alpha * acc[i][j][e];
         *c = g.beta == 0.0 ? v : v + g.beta * *c;
       }
   }
 }
 
+template <int WT>
+hipError_t launch_tiles(bool ta, bool tb, const GemmArgs &g, long tiles, hipStream_t stream) {
+  const dim3 grid((unsigned)tiles);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<false, false, WT>), grid, dim3(256), 0, stream, g);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<false, true, WT>), grid, dim3(256), 0, stream, g);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<true, false, WT>), grid, dim3(256), 0, stream, g);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<true, true, WT>), grid, dim3(256), 0, stream, g);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_gemm_f64(bool ta, bool tb, const GemmArgs &g, int batch, hipStream_t stream) {
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
-  const dim3 grid((g.M + kTM - 1) / kTM, (g.N + kTN - 1) / kTN, batch);
-  if (!ta && !tb)
-    hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, stream, g);
-  else if (!ta && tb)
-    hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, stream, g);
-  else if (ta && !tb)
-    hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, stream, g);
-  else
-    hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, stream, g);
-  return hipGetLastError();
+  // 128-tiles while they give at least two workgroups per CU, else 64-tiles
+  const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128) * batch;
+  const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64) * batch;
+  if (t64 > 0x7fffffffL) return hipErrorInvalidValue;
+  if (t128 >= 512) return launch_tiles<4>(ta, tb, g, t128, stream);
+  return launch_tiles<2>(ta, tb, g, t64, stream);
 }
 
 }  // namespace shellgpu

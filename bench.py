@@ -86,10 +86,17 @@ def profiled_traffic(name):
     return None
 
 
-def make_context(cyl, thick, tune=True):
+# autotune each context (--no-tune keeps the default schedules: profiling
+# runs, where the tuner's trial launches would fill the kernel tables)
+TUNE = True
+
+
+def make_context(cyl, thick, tune=None):
     """A context of the cylinder with its designs; the element-kernel
     schedules and the colouring are autotuned on it (outside every timed
     region), as a TACSAssembler is set up once per structure."""
+    if tune is None:
+        tune = TUNE
     ctx = sg.Context(0)
     ctx.set_mesh(cyl.conn, cyl.Xpts)
     ctx.set_material(70e9, 0.3, 2700.0, 0.0, 10.0)
@@ -202,7 +209,8 @@ def box_beam(ncases=16, chunk=16):
     ctx.set_bcs(*b.bcs())
     ctx.set_designs(np.tile(t, (chunk, 1)))
     ctx.set_ordering_seed(b.root_nodes())
-    ctx.tune(coloring=True)
+    if TUNE:
+        ctx.tune(coloring=True)
     ctx.synchronize()
     setup_s = time.perf_counter() - t0
     lam_all, err_all, gpan, steps = [], [], [], []
@@ -456,11 +464,14 @@ def main():
     ap.add_argument("--skip-accuracy", action="store_true")
     ap.add_argument("--skip-frequency", action="store_true")
     ap.add_argument("--skip-boxbeam", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default schedules (profiling runs)")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", choices=["assembly", "calibrate"], default="assembly",
                     help="calibrate: only run a streaming kernel of known traffic (Y += a X over "
                          "3 x 36 x nnzb doubles of the workload's matrices) for counter calibration")
     args = ap.parse_args()
+    global TUNE
+    TUNE = not args.no_tune
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU only; a decomposition across GPUs is not a "
                                    "measured path of this library", "n_gpus": args.gpus}))
