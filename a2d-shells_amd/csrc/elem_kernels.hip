@@ -927,11 +927,13 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
   // lane tile: rows row0..row0+2 (one node's translations or rotations),
   // columns col0..col0+5 (all dofs of node col0 / 6)
   const int row0 = 3 * (lane >> 2), col0 = 6 * (lane & 3);
-  // the lane's 18 entries of its BSR block; the nonlinear modes read them
-  // now so that the read completes during the quadrature loop (no other
-  // element of this launch writes the block), the linear stiffness at the
-  // end (measured faster: fewer live registers through the loop)
-  constexpr bool kEarly = (MODE == kModeKNl || MODE == kModeG || MODE == kModeJacNl);
+  // the lane's 18 entries of its BSR block, read now so that the read
+  // completes during the quadrature loop (no other element of this launch
+  // writes the block): 6 % faster than reading them at the scatter for the
+  // linear stiffness of the 200 x 400 cylinder, more for the nonlinear modes
+  // (schedule bit 0 reads them at the scatter instead: fewer live registers
+  // through the loop)
+  constexpr bool kEarly = kMat && !MT::kJvp && !(VAR & 1);
   double prev[18];
   double *dst = nullptr;
   // first touch of the block in this assembly: written, not added to (the
@@ -954,8 +956,6 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
   for (int i = 0; i < 18; i++) acc[i] = 0.0;
   double racc = 0.0;
 
-  constexpr int kUnrollQ = (VAR & 1) ? 4 : 1;
-#pragma unroll kUnrollQ
   for (int iq = 0; iq < 4; iq++) {
     if (lane < 24) {
       double b[9], y[9];
@@ -1294,8 +1294,9 @@ static hipError_t launch_mode(const ElemArgs &args, int count, int nbatch, hipSt
   return hipGetLastError();
 }
 
-// the matrix modes in every schedule: variant bit 0 = VAR bit 0 (unrolled
-// quadrature loop), bit 1 = the two-wavefront register budget
+// the matrix modes in every schedule: variant bit 0 = VAR bit 0 (BSR block
+// reads at the scatter instead of before the quadrature loop), bit 1 = the
+// two-wavefront register budget
 template <int MODE>
 static hipError_t launch_tuned(const ElemArgs &args, int count, int nbatch, int variant, hipStream_t stream) {
   if (count <= 0 || nbatch <= 0) return hipSuccess;

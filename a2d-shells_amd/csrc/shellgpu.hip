@@ -1177,8 +1177,9 @@ int shellgpu_tune(shellgpu_ctx *ctx, int flags) {
              kTunedNames[c], best, 1e3 * t[0], 1e3 * t[1], 1e3 * t[2], 1e3 * t[3]);
     rep += buf;
   }
-  rep += "}, \"schedules\": [\"rolled quadrature loop\", \"unrolled quadrature loop\", "
-         "\"rolled, 2 wavefronts/SIMD register budget\", \"unrolled, 2 wavefronts/SIMD register budget\"]}";
+  rep += "}, \"schedules\": [\"block reads before the quadrature loop\", \"block reads at the scatter\", "
+         "\"reads before the loop, 2 wavefronts/SIMD register budget\", "
+         "\"reads at the scatter, 2 wavefronts/SIMD register budget\"]}";
   ctx->schedule = rep;
 out:
   ctx->timing = timing;

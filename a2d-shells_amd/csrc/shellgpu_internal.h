@@ -146,8 +146,9 @@ struct XptArgs {
 hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream_t stream);
 
 // variant: the compiled schedule of the matrix modes (kModeKLin, kModeKNl,
-// kModeG, kModeJac*): bit 0 unrolled quadrature loop, bit 1 two-wavefront
-// register budget; other modes ignore it
+// kModeG, kModeJac*): bit 0 reads the BSR blocks at the scatter (default:
+// before the quadrature loop), bit 1 two-wavefront register budget; other
+// modes ignore it
 constexpr int kElemVariants = 4;
 hipError_t launch_elem_kernel(int mode, int variant, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream);

@@ -385,8 +385,9 @@ int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double
  * has one schedule, TACSAssembler_thread.cpp:27-252) ---------------------- */
 /* Autotuner: times, on the context's mesh and designs, every compiled
  * schedule of the matrix-assembly element kernels (stiffness, tangent,
- * geometric stiffness, Jacobian of either model: rolled or unrolled
- * quadrature loop, one- or two-wavefront register budget) and, with flags
+ * geometric stiffness, Jacobian of either model: BSR block reads before
+ * the quadrature loop or at the scatter, one- or two-wavefront register
+ * budget) and, with flags
  * bit 0, the two element colourings (greedy first fit in element order, or
  * the elements of most neighbours first), and keeps the fastest of each for
  * the context.  Results are the same arithmetic under every schedule. */

@@ -1,6 +1,6 @@
 """The autotuner's schedules: every compiled schedule of the matrix-assembly
-kernels (rolled or unrolled quadrature loop, one- or two-wavefront register
-budget) and both element colourings assemble the same matrices and
+kernels (BSR block reads before the quadrature loop or at the scatter, one-
+or two-wavefront register budget) and both element colourings assemble the same matrices and
 residuals (the same arithmetic; within round-off of the default schedule,
 which the oracle tests pin), and shellgpu_tune reports and keeps valid
 choices."""
