@@ -465,6 +465,8 @@ def main():
     ap.add_argument("--skip-frequency", action="store_true")
     ap.add_argument("--skip-boxbeam", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default schedules (profiling runs)")
+    ap.add_argument("--skip-spmv-hbm", action="store_true",
+                    help="no HBM-resident SpMV leg (its 64-design assembly runs the same element kernel)")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", choices=["assembly", "calibrate"], default="assembly",
                     help="calibrate: only run a streaming kernel of known traffic (Y += a X over "
@@ -573,7 +575,7 @@ def main():
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
         },
     }
-    legs = [("bsr_spmv_hbm", True, spmv_hbm),
+    legs = [("bsr_spmv_hbm", not args.skip_spmv_hbm, spmv_hbm),
             ("buckling_pipeline", not args.skip_pipeline, buckling_pipeline),
             ("frequency", not args.skip_frequency, frequency_analysis),
             ("box_beam", not args.skip_boxbeam, box_beam),
