@@ -709,15 +709,25 @@ __global__ void couple_bwd_kernel(int nnodes, const int *__restrict__ rowp, cons
 // transposed product are reduced over the lanes by a halving exchange (32
 // values in 32 shuffles).
 constexpr int kSymvTile = 128;
+// Entry q of a batch of symmetric products (blockIdx.y, or act[blockIdx.y]):
+// level entry k = q / nd, design q % nd; matrix, vector and result offsets
+// (q % nd) s*d + k s*k (the sequential form: nd = entries, one stride).
+struct SymvBatch {
+  int nd;
+  long sAd, sAk, sxd, sxk, syd, syk;
+  __device__ long a(int q) const { return (q % nd) * sAd + (q / nd) * sAk; }
+  __device__ long x(int q) const { return (q % nd) * sxd + (q / nd) * sxk; }
+  __device__ long y(int q) const { return (q % nd) * syd + (q / nd) * syk; }
+};
 __global__ __launch_bounds__(256) void symv_pass1_kernel(int m, int nT, const double *__restrict__ A, int lda,
-                                                         long sAd, const double *__restrict__ x, long sxd,
+                                                         const double *__restrict__ x, SymvBatch sb,
                                                          double *__restrict__ part, const int *__restrict__ act) {
   __shared__ double2 red[4][64];
   int I, J;
   tri_tile(blockIdx.x, I, J);
   const int d = act ? act[blockIdx.y] : blockIdx.y;
-  A += d * sAd;
-  x += d * sxd;
+  A += sb.a(d);
+  x += sb.x(d);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int row = I * kSymvTile + 2 * lane;
   const bool rv = row < m;  // m is even: row + 1 < m as well
@@ -840,7 +850,7 @@ __global__ __launch_bounds__(256) void symv_multi_pass1_kernel(int m, int nT, co
 template <int G>
 __global__ __launch_bounds__(64 * G) void symv_pass2_kernel(int m, int nT, int ns, const double *__restrict__ part,
                                                             double alpha, const double *__restrict__ c, double beta,
-                                                            double *__restrict__ y, long svd,
+                                                            double *__restrict__ y, SymvBatch sb,
                                                             const int *__restrict__ act) {
   __shared__ double red[G][64];
   const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -856,7 +866,7 @@ __global__ __launch_bounds__(64 * G) void symv_pass2_kernel(int m, int nT, int n
     if (g != 0) return;
     for (int k = 1; k < G; k++) s += red[k][lane];
   }
-  if (row < m) y[(long)d * svd + row] = (beta == 0.0 ? 0.0 : beta * c[(long)d * svd + row]) + alpha * s;
+  if (row < m) y[sb.y(d) + row] = (beta == 0.0 ? 0.0 : beta * c[sb.y(d) + row]) + alpha * s;
 }
 
 // identity on the padded diagonal entries of the cyclic-reduction levels
@@ -1448,7 +1458,10 @@ int factor_create(shellgpu_ctx *ctx, int nbatch, shellgpu_factor **out) {
     // GEMV partial sums of cyclic reduction; symmetric-product tile sums of
     // the sequential factorisation (64 x 64 tiles)
     const long nT = (F->maxm + kSymvTile - 1) / kSymvTile;
-    const long parts = F->cr ? (long)F->maxm * 32 * nb * F->nkmax : nT * nT * kSymvTile * nb;
+    // (cyclic reduction: GEMV splits of any round, and the first round's
+    // symmetric products of its nE entries per design)
+    const long parts = F->cr ? std::max((long)F->maxm * 32 * nb * F->nkmax, nT * nT * kSymvTile * nb * F->rounds[0].nE)
+                             : nT * nT * kSymvTile * nb;
     e = hipMalloc((void **)&F->part, sizeof(double) * (size_t)parts);
   }
   if (e == hipSuccess) e = hipMalloc((void **)&F->cr_info, sizeof(int) * 2 * nb * (F->nlevels + 1));
@@ -1580,6 +1593,8 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
   const int *act = (nrhs == 1 && F->nact >= 0) ? F->d_act : nullptr;
   // column slabs per tile when the tiles alone would not fill the GPU
   const int cs = (nrhs > 1 && (long)nT * nT * nb < 512) ? kMaxSplit : 1;
+  // one stride: entry e (design, or vector r nb + d) at e ps / e nv
+  const SymvBatch sb{1 << 30, ps, 0, nv, 0, nv, 0};
   switch (nrhs * 8 + (cs > 1)) {
 #define SYMV_MULTI(R)                                                                                               \
   case R * 8:                                                                                                       \
@@ -1592,7 +1607,7 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
     break;
     case 8:
       hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, act ? F->nact : nb), dim3(256), 0, ctx->stream,
-                         m, nT, P, m, ps, x, nv, F->part, act);
+                         m, nT, P, m, x, sb, F->part, act);
       break;
       SYMV_MULTI(2)
       SYMV_MULTI(3)
@@ -1612,19 +1627,42 @@ static int symv(shellgpu_factor *F, int l, const double *x, double alpha, const 
   const dim3 g2((m + 63) / 64, act ? F->nact : nrhs * nb);
   if (ns >= 64)
     hipLaunchKernelGGL(symv_pass2_kernel<16>, g2, dim3(1024), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y,
-                       nv, act);
+                       sb, act);
   else if (ns >= 32)
-    hipLaunchKernelGGL(symv_pass2_kernel<8>, g2, dim3(512), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+    hipLaunchKernelGGL(symv_pass2_kernel<8>, g2, dim3(512), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, sb,
                        act);
   else if (ns >= 16)
-    hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+    hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, sb,
                        act);
   else if (ns >= 8)
-    hipLaunchKernelGGL(symv_pass2_kernel<2>, g2, dim3(128), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+    hipLaunchKernelGGL(symv_pass2_kernel<2>, g2, dim3(128), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, sb,
                        act);
   else
-    hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, nv,
+    hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, m, nT, ns, F->part, alpha, c, beta, y, sb,
                        act);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
+// y_q = beta c_q + alpha P_q x_q for the first round of cyclic reduction:
+// entries q = k nb + d (odd level 2k + 1, design d), P_q symmetric in the
+// Z slot of the level (P_i = D_i^{-1}), half of it read
+static int symv_cr(shellgpu_factor *F, const double *x, double alpha, const double *c, double beta, double *y) {
+  shellgpu_ctx *ctx = F->ctx;
+  const int M = F->M, nT = (M + kSymvTile - 1) / kSymvTile, nb = F->nbatch, cnt = nb * F->rounds[0].nE;
+  const long M2 = (long)M * M, PD = F->per_design, NV = F->nv;
+  const double *P = F->buf + 2L * F->nlevels * M2 + M2;  // Z slot of level 1
+  const SymvBatch sb{nb, PD, 2 * M2, NV, 2L * M, NV, 2L * M};
+  hipLaunchKernelGGL(symv_pass1_kernel, dim3(nT * (nT + 1) / 2, cnt), dim3(256), 0, ctx->stream, M, nT, P, M, x, sb,
+                     F->part, nullptr);
+  HIPCALL(ctx, hipGetLastError());
+  const dim3 g2((M + 63) / 64, cnt);
+  if (nT >= 16)
+    hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, M, nT, nT, F->part, alpha, c, beta, y, sb,
+                       nullptr);
+  else
+    hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, M, nT, nT, F->part, alpha, c, beta, y, sb,
+                       nullptr);
   HIPCALL(ctx, hipGetLastError());
   return 0;
 }
@@ -1897,8 +1935,7 @@ static int apply_cr(shellgpu_factor *F) {
   const long M2 = (long)M * M, PD = F->per_design, NV = F->nv;
   double *D0 = F->buf, *B0 = F->buf + (long)nl * M2, *Z0 = F->buf + 2L * nl * M2;
   const double *E = F->buf + F->E_off;
-  const int gx = (F->maxm + 255) / 256, cnt1 = nb * F->rounds[0].nE;
-  const GemvBatch g1{nb, PD, 2 * M2, NV, 2 * M, NV, 2 * M};
+  const int gx = (F->maxm + 255) / 256;
   auto gather = [&](int k0, const double *src, double *dst, double alpha, double beta) -> int {
     const int nk = (nl - k0 + 1) / 2;
     hipLaunchKernelGGL(cr_gather_kernel, dim3(gx, nk, nb), dim3(256), 0, ctx->stream, k0, nl, M, F->d_noff,
@@ -1908,7 +1945,8 @@ static int apply_cr(shellgpu_factor *F) {
     return 0;
   };
   int rc = 0;
-  if ((rc = gemv_batch(F, false, M, M, Z0 + M2, M, F->vb + M, 1.0, 0.0, F->vx + M, 0, g1, cnt1))) return rc;
+  // t_i = P_i b_i (symmetric P_i: half of it read)
+  if ((rc = symv_cr(F, F->vb + M, 1.0, nullptr, 0.0, F->vx + M))) return rc;
   if ((rc = gather(0, F->vx, F->vb, -1.0, 1.0))) return rc;
   for (const FactorRound &r : F->rounds) {
     const long s = r.s;
@@ -1946,7 +1984,7 @@ static int apply_cr(shellgpu_factor *F) {
       return rc;
   }
   if ((rc = gather(1, F->vx, F->vb, 1.0, 0.0))) return rc;
-  return gemv_batch(F, false, M, M, Z0 + M2, M, F->vb + M, -1.0, 1.0, F->vx + M, 0, g1, cnt1);
+  return symv_cr(F, F->vb + M, -1.0, F->vx + M, 1.0, F->vx + M);
 }
 
 }  // extern "C"
