@@ -100,3 +100,15 @@ def test_rigid_translation_in_nullspace_of_cylinder_K():
         r[d::6] = 1.0
         y = O.bsr_matvec(rowp, cols, vals, r)
         assert np.abs(y).max() <= 1e-8 * np.abs(vals).max()
+
+
+def test_first_appearance_known_answer():
+    # TACSCreator.cpp:1150-1202 on one partition: nodes numbered in order
+    # of first appearance, elements in order, nodes within an element in
+    # order; the result maps old -> new (unreferenced nodes: -1)
+    conn = np.array([[5, 2, 7, 1], [2, 9, 1, 3], [0, 5, 3, 9]], np.int32)
+    out = O.first_appearance(10, conn)
+    expect = np.full(10, -1)
+    for old, new in {5: 0, 2: 1, 7: 2, 1: 3, 9: 4, 3: 5, 0: 6}.items():
+        expect[old] = new
+    assert np.array_equal(out, expect)

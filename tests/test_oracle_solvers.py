@@ -88,3 +88,68 @@ def test_buckling_with_prescribed_displacements_and_given_path():
     w0 = np.real(w0[np.isfinite(w0)])
     w0 = w0[np.argsort(np.abs(w0))][:4]
     assert np.abs(lam0 - w0).max() <= 1e-9 * np.abs(w0).max()
+
+
+def _bandwidth(rowp, cols, perm):
+    """Largest |pos(i) - pos(j)| over the pattern for the order perm
+    (perm[k] = node at position k)."""
+    pos = np.empty(perm.size, np.int64)
+    pos[perm] = np.arange(perm.size)
+    bw = 0
+    for i in range(rowp.size - 1):
+        for j in cols[rowp[i]:rowp[i + 1]]:
+            bw = max(bw, abs(int(pos[i]) - int(pos[j])))
+    return bw
+
+
+def test_rcm_is_a_permutation_of_small_bandwidth():
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    # a path graph with shuffled labels: any Cuthill-McKee order from an
+    # end walks the path, bandwidth 1
+    n = 23
+    rng = np.random.default_rng(7)
+    lab = rng.permutation(n)
+    edges = [(lab[k], lab[k + 1]) for k in range(n - 1)]
+    adj = [{i} for i in range(n)]
+    for a, b in edges:
+        adj[a].add(b)
+        adj[b].add(a)
+    rowp = np.zeros(n + 1, np.int32)
+    cols = []
+    for i in range(n):
+        cols += sorted(adj[i])
+        rowp[i + 1] = len(cols)
+    cols = np.array(cols, np.int32)
+    perm = O.rcm(rowp, cols)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert _bandwidth(rowp, cols, perm) == 1
+    # the cylinder: level sets are rings of nc nodes (bandwidth about one
+    # ring plus the periodic seam), and no worse than scipy's RCM
+    for na, nc in [(6, 10), (12, 8)]:
+        cyl = Cylinder(na, nc)
+        rowp, cols = O.node_csr(cyl.nnodes, cyl.conn)
+        perm = O.rcm(rowp, cols)
+        assert np.array_equal(np.sort(perm), np.arange(cyl.nnodes))
+        bw = _bandwidth(rowp, cols, perm)
+        A = sp.csr_matrix((np.ones(cols.size), cols, rowp), shape=(cyl.nnodes, cyl.nnodes))
+        ref = _bandwidth(rowp, cols, np.asarray(reverse_cuthill_mckee(A, symmetric_mode=True)))
+        assert bw <= ref and bw <= 2 * nc, (bw, ref)
+
+
+def test_band_solve_matches_dense_solve():
+    # a nonsymmetric, block-diagonally dominant matrix on a mesh pattern:
+    # the band LU without pivoting is stable on it
+    cyl = Cylinder(5, 6)
+    rowp, cols = O.node_csr(cyl.nnodes, cyl.conn)
+    rng = np.random.default_rng(12)
+    vals = rng.uniform(-1.0, 1.0, (cols.size, 6, 6))
+    for i in range(cyl.nnodes):
+        for p in range(rowp[i], rowp[i + 1]):
+            if cols[p] == i:
+                vals[p] += 60.0 * np.eye(6)
+    A = O.bsr_to_dense(rowp, cols, vals.reshape(-1))
+    b = rng.standard_normal(6 * cyl.nnodes)
+    x = O.band_solve(rowp, cols, vals.reshape(-1), b)
+    xd = np.linalg.solve(A, b)
+    assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max()
