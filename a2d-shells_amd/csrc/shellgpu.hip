@@ -520,6 +520,22 @@ int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn
   return 0;
 }
 
+int shellgpu_set_nodes(shellgpu_ctx *ctx, const double *Xpts) {
+  if (!ctx || !Xpts) return set_err(ctx, 1, "invalid node arguments");
+  if (ctx->nnodes <= 0 || !ctx->d_X) return set_err(ctx, 1, "set the mesh before the nodes");
+  HIPCALL(ctx, hipMemcpyAsync(ctx->d_X, Xpts, sizeof(double) * 3 * (size_t)ctx->nnodes, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  return 0;
+}
+
+int shellgpu_get_nodes(shellgpu_ctx *ctx, double *Xpts) {
+  if (!ctx || !Xpts) return set_err(ctx, 1, "invalid node arguments");
+  if (ctx->nnodes <= 0 || !ctx->d_X) return set_err(ctx, 1, "no mesh");
+  HIPCALL(ctx, hipMemcpyAsync(Xpts, ctx->d_X, sizeof(double) * 3 * (size_t)ctx->nnodes, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  return 0;
+}
+
 int shellgpu_set_material(shellgpu_ctx *ctx, double E, double nu, double rho, double cte, double k_drill) {
   if (!ctx) return 1;
   if (!(E > 0.0) || !(nu > -1.0 && nu < 0.5)) return set_err(ctx, 1, "invalid material");
@@ -1017,12 +1033,13 @@ int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *psi
 }
 
 // per-design coefficient quadruples (c0, c1, c2, 0) into ctx->d_coef
-static int set_coef4(shellgpu_ctx *ctx, double c0, double c1, double c2) {
+static int set_coef4(shellgpu_ctx *ctx, double c0, double c1, double c2, double c3 = 0.0) {
   std::vector<double> h(4 * ctx->nbatch, 0.0);
   for (int d = 0; d < ctx->nbatch; d++) {
     h[4 * d] = c0;
     h[4 * d + 1] = c1;
     h[4 * d + 2] = c2;
+    h[4 * d + 3] = c3;
   }
   HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
   HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
@@ -1034,11 +1051,14 @@ int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, doub
   int rc = check_batch(ctx, nullptr);
   if (rc) return rc;
   if (!phi || !dfdX) return set_err(ctx, 1, "phi and dfdX are required");
-  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC)
-    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS or SHELLGPU_GEOMETRIC");
+  if (mattype != SHELLGPU_STIFFNESS && mattype != SHELLGPU_GEOMETRIC && mattype != SHELLGPU_MASS)
+    return set_err(ctx, 1, "matrix type must be SHELLGPU_STIFFNESS, SHELLGPU_GEOMETRIC or SHELLGPU_MASS");
   const bool geo = mattype == SHELLGPU_GEOMETRIC;
+  const int slot = mattype == SHELLGPU_STIFFNESS ? 0 : (geo ? 1 : 3);
   return polarized(ctx, psi, phi, [&](const double *w, double c) {
-    int r = set_coef4(ctx, geo ? 0.0 : c * scale, geo ? c * scale : 0.0, 0.0);
+    double cc[4] = {0.0, 0.0, 0.0, 0.0};
+    cc[slot] = c * scale;
+    int r = set_coef4(ctx, cc[0], cc[1], cc[2], cc[3]);
     if (r) return r;
     return run_xpt(ctx, w, geo ? u : nullptr, nullptr, ctx->d_coef, 0, dfdX);
   });

@@ -127,7 +127,8 @@ __host__ __device__ inline Section make_section_dt(const Material &m, double t) 
 
 // coordinate-derivative kernel: dfdX[3 node + x] += d/dX of
 // coef[4d] phi^T K_e phi + coef[4d+1] phi^T G_e(u) phi + coef[4d+2] psi^T R_e(u)
-// (residual of the nonlinear model when nonlin != 0); phi, u, psi may be null
+// + coef[4d+3] phi^T M_e phi (residual of the nonlinear model when
+// nonlin != 0); phi, u, psi may be null
 struct XptArgs {
   const int *elems;
   const int *conn;

@@ -3533,16 +3533,16 @@ int shellgpu_frequency_info(shellgpu_frequency *F, int *nsteps, double *sigma_us
   return 0;
 }
 
-int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef, double *dfdt) {
+int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef, double *dfdt, double *dfdX) {
   if (!F || !coef) return 1;
   shellgpu_ctx *ctx = F->ctx;
   if (!F->solved) return ferr(ctx, 9, "frequency problem not solved");
-  if (!dfdt) return 0;
+  if (!dfdt && !dfdX) return 0;
   const int nb = ctx->nbatch, n = 6 * ctx->nnodes, nev = F->nev;
   int rc = 0;
   VecOps vo;
   if ((rc = vo.init(ctx, n, nb))) return rc;
-  std::vector<double> hc(2 * nb, 0.0);
+  std::vector<double> hc(4 * nb, 0.0);
   for (int j = 0; j < nev; j++) {
     bool any = false;
     for (int d = 0; d < nb; d++) any = any || coef[(size_t)d * nev + j] != 0.0;
@@ -3552,6 +3552,20 @@ int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef,
     if ((rc = shellgpu_mat_mult(F->M, F->ph, F->work))) return rc;
     std::vector<double> pmp;
     if ((rc = vo.dot(F->ph, F->work, pmp))) return rc;
+    if (dfdX) {
+      // coordinates: (c, 0, 0, -c lambda) per design, K and M in one pass
+      for (int d = 0; d < nb; d++) {
+        const double f = coef[(size_t)d * nev + j] / pmp[d];
+        hc[4 * d] = f;
+        hc[4 * d + 1] = hc[4 * d + 2] = 0.0;
+        hc[4 * d + 3] = -f * F->lam[d][j];
+      }
+      HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, hc.data(), sizeof(double) * 4 * nb, hipMemcpyHostToDevice,
+                                  ctx->stream));
+      if ((rc = run_xpt(ctx, F->ph, nullptr, nullptr, ctx->d_coef, 0, dfdX))) return rc;
+      HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (!dfdt) continue;
     // stiffness term: (c, 0) pairs of the thickness-adjoint kernel
     for (int d = 0; d < nb; d++) {
       hc[2 * d] = coef[(size_t)d * nev + j] / pmp[d];

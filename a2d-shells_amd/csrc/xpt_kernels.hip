@@ -15,6 +15,14 @@
 //   s_G = sum_qp det [2 Q(u, phi)^T C e1(phi) + (e1(u) - e_th)^T C Q(phi, phi)]
 //   s_R = sum_qp det e1(psi)^T C (e1(u) - e_th)                  (linear model)
 //       = sum_qp det (e1(psi) + Q(u, psi))^T C (e(u) - e_th)     (nonlinear)
+//   s_M = phi^T M_e phi = sum_qp det (m0 |u0(phi)|^2 + m2 |d0(phi)|^2)
+//
+// The last is the consistent mass of the linearized-rotation shell
+// (TACSShellElement.h:613-648 with the moments of
+// TACSIsoShellConstitutive::evalMassMoments, m0 = rho t, m2 = rho t^3/12):
+// its rotational block -[t_i]x S_ij [t_j]x contracts with phi to
+// S_ij (q_i x t_i).(q_j x t_j), and the interpolated director is
+// d0 = sum N_m (q_m x t_m).
 //
 // with the strain evaluation of the element kernels: node normals and
 // drilling strains (TACSShellUtilities.h:301-343, 649-707), MITC tying
@@ -87,7 +95,7 @@ constexpr int kPairB[4] = {0, 0, 2, 1};
 
 template <typename S>
 __device__ void element_scalars(const S X[12], const double q[3][24], const Section &sec, double eth, bool nonlin,
-                                const double c[3], S &out) {
+                                const double c[4], double m0, double m2, S &out) {
   // ---- node normals ----
   S fn[12];
   for (int n = 0; n < 4; n++) {
@@ -292,6 +300,19 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
         acc = acc + c[1] * (2.0 * dot9(eq[0], y) + dot9(eu, z));
       }
     }
+    if (c[3] != 0.0) {
+      S u0[3] = {0.0, 0.0, 0.0}, d0[3] = {0.0, 0.0, 0.0};
+      for (int m = 0; m < 4; m++) {
+        const S qr[3] = {q[0][6 * m + 3], q[0][6 * m + 4], q[0][6 * m + 5]};
+        S dm[3];
+        cross3(qr, &fn[3 * m], dm);
+        for (int d = 0; d < 3; d++) {
+          u0[d] = u0[d] + N[m] * q[0][6 * m + d];
+          d0[d] = d0[d] + N[m] * dm[d];
+        }
+      }
+      acc = acc + c[3] * det * (m0 * dot3(u0, u0) + m2 * dot3(d0, d0));
+    }
     if (c[2] != 0.0) {
       S ep[9];
       for (int i = 0; i < 9; i++) ep[i] = e1[2][i];
@@ -325,9 +346,11 @@ __global__ __launch_bounds__(kThreads) void xpt_kernel(XptArgs a, int nel) {
   const double thick = a.thick[design * a.nelems + e];
   const double temp = a.temp ? a.temp[design * a.nelems + e] : 0.0;
   const Section sec = make_section(a.mat, thick);
-  const double c[3] = {a.coef[4 * design], a.coef[4 * design + 1], a.coef[4 * design + 2]};
+  const double c[4] = {a.coef[4 * design], a.coef[4 * design + 1], a.coef[4 * design + 2], a.coef[4 * design + 3]};
+  // mass moments of TACSIsoShellConstitutive with tOffset = 0
+  const double m0 = a.mat.rho * thick, m2 = a.mat.rho * thick * thick * thick / 12.0;
   Dual s;
-  element_scalars(X, q, sec, a.mat.cte * temp, a.nonlin != 0, c, s);
+  element_scalars(X, q, sec, a.mat.cte * temp, a.nonlin != 0, c, m0, m2, s);
   a.dfdX[design * a.dX_stride + 3 * cn[dir / 3] + dir % 3] += s.d;
 }
 

@@ -3,11 +3,12 @@
 // shell per component, aluminium) followed by TACSFrequencyAnalysis with the
 // Lanczos constructor (src/TACSBuckling.cpp:614-702), as a TACS user would
 // write it.  The reference ships no frequency example; this one exercises
-// TACSFrequencyAnalysis::solve / extractEigenvalue / evalEigenDVSens.
+// TACSFrequencyAnalysis::solve / extractEigenvalue / evalEigenDVSens /
+// evalEigenXptSens.
 //
 // Usage: mech_frequency [file.bdf] [thickness] [sigma] [num_eigvals] [max_lanczos]
 // Prints lambda[i] = omega_i^2, the frequency in Hz and the Ritz residual,
-// and the sum over elements of d(lambda_0)/dt.
+// the sum over elements of d(lambda_0)/dt and X . d(lambda_0)/dX.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -77,6 +78,17 @@ int main(int argc, char *argv[]) {
   double sum = 0.0;
   for (double v : g) sum += v;
   printf("sum d(lambda_0)/dt = %.16e\n", sum);
+
+  // shape sensitivity: X . d(lambda_0)/dX is the derivative under a uniform
+  // scaling of the mesh about the origin
+  TACSBVec *X = assembler->createNodeVec(), *dfdX = assembler->createNodeVec();
+  X->incref();
+  dfdX->incref();
+  assembler->getNodes(X);
+  freq->evalEigenXptSens(0, dfdX);
+  printf("X . d(lambda_0)/dX = %.16e\n", X->dot(dfdX));
+  X->decref();
+  dfdX->decref();
 
   dfdx->decref();
   freq->decref();

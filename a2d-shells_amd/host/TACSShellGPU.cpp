@@ -368,6 +368,22 @@ int TACSAssembler::syncDesigns() {
 
 TACSBVec *TACSAssembler::createVec() { return new TACSBVec(this, 6 * (size_t)num_nodes); }
 TACSBVec *TACSAssembler::createNodeVec() { return new TACSBVec(this, 3 * (size_t)num_nodes); }
+
+void TACSAssembler::setNodes(TACSBVec *X) {
+  if (!X || X->getSize() < 3 * (size_t)num_nodes) {
+    fprintf(stderr, "TACSAssembler::setNodes: a node vector of 3 x %d values is required\n", num_nodes);
+    return;
+  }
+  check(shellgpu_set_nodes(ctx, X->getDeviceArray()), "TACSAssembler::setNodes");
+  std::vector<TacsScalar> x;
+  X->getValues(x);
+  X_h.assign(x.begin(), x.begin() + 3 * (size_t)num_nodes);
+}
+
+void TACSAssembler::getNodes(TACSBVec *X) {
+  if (!X || X->getSize() < 3 * (size_t)num_nodes) return;
+  check(shellgpu_get_nodes(ctx, X->getDeviceArray()), "TACSAssembler::getNodes");
+}
 TACSBVec *TACSAssembler::createDesignVec() { return new TACSBVec(this, (size_t)num_elements); }
 TACSSchurMat *TACSAssembler::createSchurMat() { return new TACSSchurMat(this); }
 
@@ -509,11 +525,9 @@ void TACSAssembler::addMatDVSensInnerProduct(TacsScalar scale, ElementMatrixType
 void TACSAssembler::addMatXptSensInnerProduct(TacsScalar scale, ElementMatrixType matType, TACSBVec *psi,
                                               TACSBVec *phi, TACSBVec *dfdX) {
   syncDesigns();
-  if (matType == TACS_MASS_MATRIX) {
-    fprintf(stderr, "TACSAssembler::addMatXptSensInnerProduct: no coordinate derivative of the mass matrix\n");
-    return;
-  }
-  const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC : SHELLGPU_STIFFNESS;
+  const int kind = matType == TACS_GEOMETRIC_STIFFNESS_MATRIX ? SHELLGPU_GEOMETRIC
+                   : matType == TACS_MASS_MATRIX              ? SHELLGPU_MASS
+                                                              : SHELLGPU_STIFFNESS;
   check(shellgpu_add_mat_xpt_sens_inner_product(ctx, kind, scale, psi->getDeviceArray(), phi->getDeviceArray(),
                                                 vars->getDeviceArray(), dfdX->getDeviceArray()),
         "TACSAssembler::addMatXptSensInnerProduct");
@@ -759,8 +773,17 @@ void TACSFrequencyAnalysis::evalEigenDVSens(int n, TACSBVec *dfdx) {
   if (n < 0 || n >= num_eigvals || !freq) return;
   std::vector<double> coef(num_eigvals, 0.0);
   coef[n] = 1.0;
-  report(assembler->getContext(), shellgpu_frequency_add_eigen_sens(freq, coef.data(), dfdx->getDeviceArray()),
-         "evalEigenDVSens");
+  report(assembler->getContext(),
+         shellgpu_frequency_add_eigen_sens(freq, coef.data(), dfdx->getDeviceArray(), NULL), "evalEigenDVSens");
+}
+
+void TACSFrequencyAnalysis::evalEigenXptSens(int n, TACSBVec *dfdX) {
+  dfdX->zeroEntries();
+  if (n < 0 || n >= num_eigvals || !freq) return;
+  std::vector<double> coef(num_eigvals, 0.0);
+  coef[n] = 1.0;
+  report(assembler->getContext(),
+         shellgpu_frequency_add_eigen_sens(freq, coef.data(), NULL, dfdX->getDeviceArray()), "evalEigenXptSens");
 }
 
 void TACSFrequencyAnalysis::checkEigenvector(int n) {

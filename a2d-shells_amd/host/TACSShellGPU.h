@@ -345,6 +345,10 @@ class TACSAssembler : public TACSObject {
   TACSBVec *createVec();
   TACSBVec *createNodeVec();
   TACSBVec *createDesignVec();
+  // nodal coordinates (TACSAssembler.h:156-157): moving the nodes keeps the
+  // connectivity, the matrix pattern and the colouring
+  void setNodes(TACSBVec *X);
+  void getNodes(TACSBVec *X);
   TACSSchurMat *createSchurMat();
   TACSMat *createMat() { return createSchurMat(); }
 
@@ -462,9 +466,10 @@ class TACSLinearBuckling : public TACSObject {
 // by shift-invert Lanczos on (K - sigma M)^{-1} M in the M inner product.
 // The solver must be bound to kmat, as in the reference.  solve() leaves
 // mmat = M and kmat = K - sigma M with BCs, as the reference does; the GPU
-// object factors its own copy of the shifted operator.  evalEigenDVSens is
-// exact (the reference's version, commented out there, takes element
-// central differences).
+// object factors its own copy of the shifted operator.  evalEigenDVSens and
+// evalEigenXptSens (declared at src/TACSBuckling.h:153-154) are exact (the
+// reference's versions, commented out there, take element central
+// differences).
 class TACSFrequencyAnalysis : public TACSObject {
  public:
   TACSFrequencyAnalysis(TACSAssembler *assembler, TacsScalar sigma, TACSMat *mmat, TACSMat *kmat, TACSKsm *solver,
@@ -476,6 +481,7 @@ class TACSFrequencyAnalysis : public TACSObject {
   TacsScalar extractEigenvalue(int n, TacsScalar *error);
   TacsScalar extractEigenvector(int n, TACSBVec *ans, TacsScalar *error);
   void evalEigenDVSens(int n, TACSBVec *dfdx);
+  void evalEigenXptSens(int n, TACSBVec *dfdX);
   void checkEigenvector(int n);
   int getNumEigs() const { return num_eigvals; }
 

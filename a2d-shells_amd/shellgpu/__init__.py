@@ -52,6 +52,8 @@ def load_library(path=LIB_PATH):
         "shellgpu_memcpy_d2d": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
         "shellgpu_memset": (C.c_int, [_VP, _VP, C.c_int, C.c_size_t]),
         "shellgpu_set_mesh": (C.c_int, [_VP, C.c_int, C.c_int, _IP, _DP]),
+        "shellgpu_set_nodes": (C.c_int, [_VP, _VP]),
+        "shellgpu_get_nodes": (C.c_int, [_VP, _VP]),
         "shellgpu_set_material": (C.c_int, [_VP] + [C.c_double] * 5),
         "shellgpu_set_bcs": (C.c_int, [_VP, C.c_int, _IP, _IP, _DP]),
         "shellgpu_set_designs": (C.c_int, [_VP, C.c_int, _DP, _DP]),
@@ -108,7 +110,7 @@ def load_library(path=LIB_PATH):
         "shellgpu_frequency_solve": (C.c_int, [_VP]),
         "shellgpu_frequency_extract": (C.c_int, [_VP, C.c_int, _DP, _VP, _DP]),
         "shellgpu_frequency_info": (C.c_int, [_VP, _IP, _DP]),
-        "shellgpu_frequency_add_eigen_sens": (C.c_int, [_VP, _DP, _VP]),
+        "shellgpu_frequency_add_eigen_sens": (C.c_int, [_VP, _DP, _VP, _VP]),
         "shellgpu_ks_aggregate": (C.c_int, [C.c_int, C.c_double, _DP, _DP, _DP]),
         "shellgpu_newton_solve": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_double, _VP, _IP, _DP]),
         "shellgpu_device_info": (C.c_int, [_VP, C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int, _IP,
@@ -327,15 +329,23 @@ class Frequency:
         self.ctx._check(self.ctx.lib.shellgpu_frequency_info(self.h, C.byref(steps), _dp(sig)))
         return {"steps": steps.value, "sigma": sig}
 
-    def eigen_sens(self, coef):
-        """sum_j coef[d, j] d(lambda_j)/dt per design, [nbatch, nelems]."""
+    def eigen_sens(self, coef, xpts=False):
+        """sum_j coef[d, j] d(lambda_j)/dt per design, [nbatch, nelems]; with
+        xpts also the coordinate derivative, [nbatch, nnodes, 3]."""
         ctx = self.ctx
         nb = ctx.nbatch
         c = np.ascontiguousarray(np.broadcast_to(np.asarray(coef, dtype=np.float64), (nb, self.nev)))
         dt = DeviceArray(ctx, nb * ctx.nelems)
         dt.zero()
-        ctx._check(ctx.lib.shellgpu_frequency_add_eigen_sens(self.h, _dp(c), dt.ptr))
-        return dt.download().reshape(nb, ctx.nelems)
+        dX = None
+        if xpts:
+            dX = DeviceArray(ctx, nb * 3 * ctx.nnodes)
+            dX.zero()
+        ctx._check(ctx.lib.shellgpu_frequency_add_eigen_sens(self.h, _dp(c), dt.ptr, dX.ptr if dX else None))
+        g = dt.download().reshape(nb, ctx.nelems)
+        if xpts:
+            return g, dX.download().reshape(nb, ctx.nnodes, 3)
+        return g
 
     def destroy(self):
         if self.h and self.ctx.h:
@@ -495,6 +505,21 @@ class Context:
         Xpts = np.ascontiguousarray(Xpts, dtype=np.float64)
         self._check(self.lib.shellgpu_set_mesh(self.h, Xpts.size // 3, conn.size // 4, _ip(conn), _dp(Xpts)))
         self._refresh()
+
+    def set_nodes(self, Xpts):
+        """Move the mesh: new nodal coordinates [nnodes, 3], same connectivity."""
+        Xpts = np.ascontiguousarray(Xpts, dtype=np.float64)
+        if Xpts.size != 3 * self.nnodes:
+            raise ShellGPUError("Xpts must hold 3 coordinates per node")
+        d = DeviceArray(self, Xpts.size)
+        d.upload(Xpts.ravel())
+        self._check(self.lib.shellgpu_set_nodes(self.h, d.ptr))
+        self.synchronize()
+
+    def get_nodes(self):
+        d = DeviceArray(self, 3 * self.nnodes)
+        self._check(self.lib.shellgpu_get_nodes(self.h, d.ptr))
+        return d.download().reshape(-1, 3)
 
     def set_material(self, E, nu, rho=2700.0, cte=0.0, k_drill=10.0):
         self._check(self.lib.shellgpu_set_material(self.h, E, nu, rho, cte, k_drill))

@@ -62,6 +62,13 @@ int shellgpu_memset(shellgpu_ctx *ctx, void *dst, int value, size_t bytes);
  * colouring. */
 int shellgpu_set_mesh(shellgpu_ctx *ctx, int nnodes, int nelems, const int *conn,
                       const double *Xpts);
+/* TACSAssembler::setNodes / getNodes(TACSBVec *) (src/TACSAssembler.h:
+ * 156-157, src/TACSAssembler.cpp:912-929): replace or read the nodal
+ * coordinates of the mesh, device arrays of 3*nnodes (x, y, z per node).
+ * The connectivity, pattern, colouring and factorisation plans are kept;
+ * matrices and factors are assembled again by the caller. */
+int shellgpu_set_nodes(shellgpu_ctx *ctx, const double *Xpts);
+int shellgpu_get_nodes(shellgpu_ctx *ctx, double *Xpts);
 /* TACSMaterialProperties(rho, 0, E, nu, ys, cte, 0) + TACSIsoShellConstitutive
  * (tOffset = 0) + TACSShellConstitutive::setDrillingRegularization */
 int shellgpu_set_material(shellgpu_ctx *ctx, double E, double nu, double rho,
@@ -191,8 +198,9 @@ int shellgpu_eval_mat_sv_sens_inner_product(shellgpu_ctx *ctx, const double *psi
 /* TACSAssembler::addMatXptSensInnerProduct(scale, matType, psi, phi, dfdX)
  * (element default src/elements/TACSElement.cpp:434-507, central
  * differences there, exact forward-mode derivatives here): dfdX += scale
- * d(psi^T A phi)/dX with A = K (SHELLGPU_STIFFNESS, u ignored) or G(u);
- * psi NULL means psi = phi (psi != phi by polarization, as above) */
+ * d(psi^T A phi)/dX with A = K (SHELLGPU_STIFFNESS, u ignored), G(u)
+ * (SHELLGPU_GEOMETRIC) or M (SHELLGPU_MASS, u ignored); psi NULL means
+ * psi = phi (psi != phi by polarization, as above) */
 int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, double scale,
                                             const double *psi, const double *phi, const double *u,
                                             double *dfdX);
@@ -373,8 +381,13 @@ int shellgpu_frequency_info(shellgpu_frequency *F, int *nsteps, double *sigma_us
 /* d(sum_n coef[n] lambda_n)/dt added into dfdt (device, nbatch*nelems) with
  * d lambda / dt = phi^T (dK/dt - lambda dM/dt) phi / (phi^T M phi)
  * (TACSFrequencyAnalysis::evalEigenDVSens, src/TACSBuckling.cpp:950-995, where
- * the element products are central differences; exact here) */
-int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef, double *dfdt);
+ * the element products are central differences; exact here); dfdX (device,
+ * nbatch*3*nnodes) receives the coordinate sensitivities by the same formula
+ * (TACSFrequencyAnalysis::evalEigenXptSens, src/TACSBuckling.cpp:997-1022,
+ * commented out in the reference).  coef is host, nbatch*nev; either output
+ * may be NULL. */
+int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef, double *dfdt,
+                                      double *dfdX);
 
 /* KS aggregate of f[0..n) (ksAggregation / ksAggregationSens,
  * src/utils/TacsUtilities.cpp:971-1011): ks = m + log(sum exp(rho (f_i -

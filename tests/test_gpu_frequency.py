@@ -2,9 +2,10 @@
 path) on the GPU against the oracle: the assembled mass matrix within 1e-12
 (row-floor relative metric), the first 5 eigenvalues omega^2 within 1e-8 of a
 dense generalised eigensolve of the oracle's K and M, M-normalised
-eigenvectors with small residuals, the thickness gradient of a weighted sum
-of eigenvalues against central differences of the dense eigenvalues, a shift
-above the first eigenvalue, and batched designs against single designs."""
+eigenvectors with small residuals, the thickness and nodal-coordinate
+gradients of a weighted sum of eigenvalues against central differences of the
+dense eigenvalues, a shift above the first eigenvalue, and batched designs
+against single designs."""
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -127,6 +128,54 @@ def test_frequency_thickness_gradient_matches_differences():
         an = grad @ p
         print("directional", an, fd)
         assert abs(an - fd) <= 1e-6 * abs(fd)
+    fr.destroy()
+    ctx.close()
+
+
+def test_frequency_coordinate_gradient_matches_differences():
+    # d(lambda)/dX = phi^T (dK/dX - lambda dM/dX) phi / (phi^T M phi), the
+    # formula of TACSFrequencyAnalysis::evalEigenXptSens; checked along random
+    # nodal-coordinate directions against Richardson-extrapolated central
+    # differences of the dense eigenvalues of the moved mesh
+    cyl = Cylinder(8, 16)
+    rng = np.random.default_rng(7)
+    thick = rng.uniform(0.0015, 0.0025, cyl.nelems)
+    ctx = make_ctx(cyl.conn, cyl.Xpts, thick, cyl.bcs(), cyl.ring_nodes(0))
+    rowp, cols = ctx.pattern()
+    nev = 6
+    fr = ctx.frequency(sigma=0.0, nev=nev, max_lanczos=150, tol=1e-12)
+    fr.solve()
+    coef = np.array([1.0, 0.5, 0.25, 0.0, 0.0, 0.0])
+    gt, gX = fr.eigen_sens(coef, xpts=True)
+    # the thickness part of the same call is the thickness-only result
+    assert np.abs(gt[0] - fr.eigen_sens(coef)[0]).max() <= 1e-12 * np.abs(gt[0]).max()
+    ref = dense_frequencies(cyl.conn, cyl.Xpts, thick, cyl.bcs(), rowp, cols, nev)
+    assert (np.diff(ref[:4]) / ref[:3]).min() > 1e-4
+    for k in range(3):
+        p = rng.standard_normal(cyl.Xpts.shape)
+        h = 1e-5
+
+        def f(s):
+            return coef[:3] @ dense_frequencies(cyl.conn, cyl.Xpts + s * p, thick, cyl.bcs(), rowp, cols, 3)
+
+        fd1 = (f(h) - f(-h)) / (2 * h)
+        fd2 = (f(2 * h) - f(-2 * h)) / (4 * h)
+        fd = (4 * fd1 - fd2) / 3
+        an = np.sum(gX[0] * p)
+        print("directional", an, fd)
+        assert abs(an - fd) <= 1e-6 * abs(fd)
+    fr.destroy()
+    # moving the nodes of the same context (TACSAssembler::setNodes): the
+    # eigenvalues of the moved mesh, and its coordinates read back exactly
+    Xm = cyl.Xpts + 1e-3 * rng.standard_normal(cyl.Xpts.shape)
+    ctx.set_nodes(Xm)
+    assert np.array_equal(ctx.get_nodes(), Xm)
+    fr = ctx.frequency(sigma=0.0, nev=nev, max_lanczos=150, tol=1e-12)
+    fr.solve()
+    lam, _ = fr.eigenvalues()
+    refm = dense_frequencies(cyl.conn, Xm, thick, cyl.bcs(), rowp, cols, nev)
+    assert np.abs(lam[0][:5] - refm[:5]).max() <= 1e-8 * np.abs(refm[:5]).max()
+    assert np.abs(refm[0] - ref[0]) > 1e-6 * ref[0]  # the move changed the answer
     fr.destroy()
     ctx.close()
 

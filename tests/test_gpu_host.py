@@ -63,7 +63,8 @@ def test_mech_frequency_driver_matches_dense_oracle(tmp_path):
     both end rings clamped: omega^2 against a dense solve of the oracle's K
     and M (M x = mu K x on the free dofs, lambda = 1 / mu), and the element
     sum of d(lambda_0)/dt against a central difference of the dense value
-    under a uniform thickness change."""
+    under a uniform thickness change, and X . d(lambda_0)/dX against one
+    under a uniform scaling of the mesh."""
     import scipy.linalg as sla
 
     cyl = Cylinder(8, 16, R=0.2, L=0.4)
@@ -88,10 +89,10 @@ def test_mech_frequency_driver_matches_dense_oracle(tmp_path):
     for n in np.concatenate([ring0, ring1]):
         free[6 * n:6 * n + 6] = False
 
-    def dense(tt):
+    def dense(tt, X=Xf):
         th = np.full(cyl.nelems, tt)
-        K = O.bsr_to_dense(rowp, cols, O.assemble_mat(O.KIND_K, 0, m, Xf, cyl.conn, th, rowp, cols))
-        M = O.bsr_to_dense(rowp, cols, O.assemble_mat(O.KIND_M, 0, m, Xf, cyl.conn, th, rowp, cols))
+        K = O.bsr_to_dense(rowp, cols, O.assemble_mat(O.KIND_K, 0, m, X, cyl.conn, th, rowp, cols))
+        M = O.bsr_to_dense(rowp, cols, O.assemble_mat(O.KIND_M, 0, m, X, cyl.conn, th, rowp, cols))
         mu = sla.eigh(M[np.ix_(free, free)], K[np.ix_(free, free)], eigvals_only=True)
         return np.sort(1.0 / mu[mu > 0])[:nev]
 
@@ -104,6 +105,13 @@ def test_mech_frequency_driver_matches_dense_oracle(tmp_path):
     h = 1e-6 * t
     fd = (dense(t + h)[0] - dense(t - h)[0]) / (2 * h)
     assert abs(dsum - fd) <= 1e-6 * abs(fd)
+    # X . d(lambda_0)/dX: the derivative under a uniform scaling of the mesh,
+    # which also keeps the pair together
+    dscale = float(re.search(r"X \. d\(lambda_0\)/dX = (\S+)", out.stdout).group(1))
+    s = 1e-6
+    fd = (dense(t, Xf * (1 + s))[0] - dense(t, Xf * (1 - s))[0]) / (2 * s)
+    print("scaling derivative", dscale, fd)
+    assert abs(dscale - fd) <= 1e-6 * abs(fd)
 
 
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")

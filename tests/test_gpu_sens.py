@@ -7,8 +7,8 @@ gradient d(phi^T G phi)/du against differences of oracle G matrices (G is
 affine in the path), and psi^T dR/dt against the oracle's restatement of
 addAdjResProduct, both element models.
 
-Coordinates: d/dX of phi^T K phi, phi^T G(u) phi and psi^T R(u) (both
-models) summed over the mesh against five-point differences of the
+Coordinates: d/dX of phi^T K phi, phi^T G(u) phi, phi^T M phi, psi^T M phi
+and psi^T R(u) (both models) summed over the mesh against five-point differences of the
 oracle's element matrices and residuals in every nodal coordinate.
 
 Analysis level: d(lambda_j)/dt and d(lambda_j)/dX of the linear buckling
@@ -119,10 +119,14 @@ def test_coordinate_products_match_oracle(thermal):
     ctx = _mesh_ctx(conn, X, thick, temp)
     dphi, du, dpsi = ctx.vector(phi), ctx.vector(u), ctx.vector(psi)
     out = {}
-    for name in ("K", "G", "R0", "R1"):
+    for name in ("K", "G", "M", "Mpsi", "R0", "R1"):
         g = ctx.coord_vector()
         if name == "K":
             ctx.add_mat_xpt_sens_inner_product(g, sg.STIFFNESS, dphi)
+        elif name == "M":
+            ctx.add_mat_xpt_sens_inner_product(g, sg.MASS, dphi)
+        elif name == "Mpsi":
+            ctx.add_mat_xpt_sens_inner_product(g, sg.MASS, dphi, psi=dpsi)
         elif name == "G":
             ctx.add_mat_xpt_sens_inner_product(g, sg.GEOMETRIC, dphi, du)
         else:
@@ -138,6 +142,10 @@ def test_coordinate_products_match_oracle(thermal):
             t = thick[0, e]
             if name == "K":
                 s += phi[idx] @ O.elem_stiffness(0, mat, t, T, Xe, np.zeros(24)) @ phi[idx]
+            elif name == "M":
+                s += phi[idx] @ O.elem_mass(mat, t, Xe) @ phi[idx]
+            elif name == "Mpsi":
+                s += psi[idx] @ O.elem_mass(mat, t, Xe) @ phi[idx]
             elif name == "G":
                 s += phi[idx] @ O.elem_geometric_step(mat, t, T, Xe, u[idx], 1.0) @ phi[idx]
             else:
