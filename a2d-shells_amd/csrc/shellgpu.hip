@@ -346,7 +346,7 @@ int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a) {
 }
 
 int run_xpt(shellgpu_ctx *ctx, const double *phi, const double *u, const double *psi, const double *coef,
-            int nonlin, double *dfdX) {
+            int nonlin, int terms, double *dfdX) {
   XptArgs a{};
   a.conn = ctx->d_conn;
   a.Xpts = ctx->d_X;
@@ -362,6 +362,7 @@ int run_xpt(shellgpu_ctx *ctx, const double *phi, const double *u, const double 
   a.nonlin = nonlin;
   a.dfdX = dfdX;
   a.dX_stride = 3L * ctx->nnodes;
+  a.terms = terms;
   for (int c = 0; c < ctx->ncolors; c++) {
     a.elems = ctx->d_color_elems + ctx->h_color_ptr[c];
     const int nb = ctx->h_color_ptr[c + 1] - ctx->h_color_ptr[c];
@@ -1060,7 +1061,8 @@ int shellgpu_add_mat_xpt_sens_inner_product(shellgpu_ctx *ctx, int mattype, doub
     cc[slot] = c * scale;
     int r = set_coef4(ctx, cc[0], cc[1], cc[2], cc[3]);
     if (r) return r;
-    return run_xpt(ctx, w, geo ? u : nullptr, nullptr, ctx->d_coef, 0, dfdX);
+    return run_xpt(ctx, w, geo ? u : nullptr, nullptr, ctx->d_coef, 0, slot == 0 ? kXptK : (geo ? kXptG : kXptM),
+                   dfdX);
   });
 }
 
@@ -1070,7 +1072,7 @@ int shellgpu_add_adj_res_xpt_product(shellgpu_ctx *ctx, int model, double scale,
   if (rc) return rc;
   if (!psi || !dfdX) return set_err(ctx, 1, "psi and dfdX are required");
   if ((rc = set_coef4(ctx, 0.0, 0.0, scale))) return rc;
-  return run_xpt(ctx, nullptr, u, psi, ctx->d_coef, model == SHELLGPU_NONLINEAR, dfdX);
+  return run_xpt(ctx, nullptr, u, psi, ctx->d_coef, model == SHELLGPU_NONLINEAR, kXptR, dfdX);
 }
 
 int shellgpu_vec_zero_bcs(shellgpu_ctx *ctx, double *x) {

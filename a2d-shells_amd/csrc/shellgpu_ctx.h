@@ -76,5 +76,5 @@ int run_colored(shellgpu_ctx *ctx, int mode, ElemArgs a);
 // coordinate-derivative kernel over all colours; coef (device, 4 per
 // design), vectors with stride 6*nnodes, dfdX with stride 3*nnodes
 int run_xpt(shellgpu_ctx *ctx, const double *phi, const double *u, const double *psi, const double *coef,
-            int nonlin, double *dfdX);
+            int nonlin, int terms, double *dfdX);
 }  // namespace shellgpu

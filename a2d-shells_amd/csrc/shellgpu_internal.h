@@ -128,7 +128,10 @@ __host__ __device__ inline Section make_section_dt(const Material &m, double t) 
 // coordinate-derivative kernel: dfdX[3 node + x] += d/dX of
 // coef[4d] phi^T K_e phi + coef[4d+1] phi^T G_e(u) phi + coef[4d+2] psi^T R_e(u)
 // + coef[4d+3] phi^T M_e phi (residual of the nonlinear model when
-// nonlin != 0); phi, u, psi may be null
+// nonlin != 0); phi, u, psi may be null.  terms (kXpt* bits) names the
+// scalars whose coefficients may be non-zero: the launch evaluates only the
+// vectors and bilinear pairs those need.
+enum { kXptK = 1, kXptG = 2, kXptR = 4, kXptM = 8 };
 struct XptArgs {
   const int *elems;
   const int *conn;
@@ -143,6 +146,7 @@ struct XptArgs {
   int nonlin;
   double *dfdX;
   long dX_stride;
+  int terms;
 };
 hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream_t stream);
 

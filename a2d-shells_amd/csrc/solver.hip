@@ -3280,7 +3280,7 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
     a.dv = dfdt;
     a.res = B->gv;
     if ((rc = run_colored(ctx, kModeAdjDV, a))) return rc;
-    if (dfdX && (rc = run_xpt(ctx, B->ph, B->u, nullptr, c4, 0, dfdX))) return rc;
+    if (dfdX && (rc = run_xpt(ctx, B->ph, B->u, nullptr, c4, 0, kXptK | kXptG, dfdX))) return rc;
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   }
   // path adjoint: dR/du psi = sum_j f_j lambda_j d(phi_j^T G phi_j)/du, with
@@ -3328,7 +3328,7 @@ int shellgpu_buckling_add_eigen_sens(shellgpu_buckling *B, const double *coef, d
     a.dv = dfdt;
     if ((rc = run_colored(ctx, nonlin ? kModeAdjResNl : kModeAdjResLin, a))) return rc;
   }
-  if (dfdX && (rc = run_xpt(ctx, nullptr, B->u, B->psi, c4, nonlin, dfdX))) return rc;
+  if (dfdX && (rc = run_xpt(ctx, nullptr, B->u, B->psi, c4, nonlin, kXptR, dfdX))) return rc;
   HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -3562,7 +3562,7 @@ int shellgpu_frequency_add_eigen_sens(shellgpu_frequency *F, const double *coef,
       }
       HIPCALL(ctx, hipMemcpyAsync(ctx->d_coef, hc.data(), sizeof(double) * 4 * nb, hipMemcpyHostToDevice,
                                   ctx->stream));
-      if ((rc = run_xpt(ctx, F->ph, nullptr, nullptr, ctx->d_coef, 0, dfdX))) return rc;
+      if ((rc = run_xpt(ctx, F->ph, nullptr, nullptr, ctx->d_coef, 0, kXptK | kXptM, dfdX))) return rc;
       HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
     }
     if (!dfdt) continue;

@@ -93,9 +93,57 @@ __device__ __forceinline__ S tying_quad(int f, const S Ua[6], const S da[3], con
 constexpr int kPairA[4] = {1, 0, 1, 1};
 constexpr int kPairB[4] = {0, 0, 2, 1};
 
-template <typename S>
+// tying strains of the needed vectors and pairs at tying point t
+// (TACSShellElementModel.h:32-77 linear, :643-697 quadratic)
+template <typename S, int WM, int PM>
+__device__ __forceinline__ void tying_at(int t, const S X[12], const S fn[12], const double q[3][24], S tl[3],
+                                         S tq[4]) {
+  double u, v, N[4], Na[4], Nb[4];
+  tying_point(t, u, v);
+  shape4(u, v, N, Na, Nb);
+  S Xxi[6], n0[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < 6; i++) Xxi[i] = 0.0;
+  for (int m = 0; m < 4; m++)
+    for (int d = 0; d < 3; d++) {
+      Xxi[2 * d] = Xxi[2 * d] + Na[m] * X[3 * m + d];
+      Xxi[2 * d + 1] = Xxi[2 * d + 1] + Nb[m] * X[3 * m + d];
+      n0[d] = n0[d] + N[m] * fn[3 * m + d];
+    }
+  S U[3][6], d0[3][3];
+  for (int w = 0; w < 3; w++) {
+    if (!((WM >> w) & 1)) continue;
+    for (int i = 0; i < 6; i++) U[w][i] = 0.0;
+    for (int i = 0; i < 3; i++) d0[w][i] = 0.0;
+    for (int m = 0; m < 4; m++) {
+      const S qr[3] = {q[w][6 * m + 3], q[w][6 * m + 4], q[w][6 * m + 5]};
+      S dm[3];
+      cross3(qr, &fn[3 * m], dm);
+      for (int d = 0; d < 3; d++) {
+        d0[w][d] = d0[w][d] + N[m] * dm[d];
+        U[w][2 * d] = U[w][2 * d] + Na[m] * q[w][6 * m + d];
+        U[w][2 * d + 1] = U[w][2 * d + 1] + Nb[m] * q[w][6 * m + d];
+      }
+    }
+  }
+  const int f = tying_field(t);
+  for (int w = 0; w < 3; w++)
+    if ((WM >> w) & 1) tl[w] = tying_lin(f, U[w], d0[w], Xxi, n0);
+  for (int p = 0; p < 4; p++)
+    if ((PM >> p) & 1) tq[p] = tying_quad(f, U[kPairA[p]], d0[kPairA[p]], U[kPairB[p]], d0[kPairB[p]]);
+}
+
+// WM: the vectors a variant evaluates (bit w), PM: the pairs (bit p).  Each
+// scalar needs: s_K and s_M phi; s_G phi, u and the pairs (u,phi),
+// (phi,phi); s_R u, psi and, for the nonlinear model, (u,psi), (u,u).
+// Vectors and pairs outside the masks are never formed, which keeps the
+// dual-number strains of a variant in registers.
+template <typename S, int WM, int PM>
 __device__ void element_scalars(const S X[12], const double q[3][24], const Section &sec, double eth, bool nonlin,
                                 const double c[4], double m0, double m2, S &out) {
+  constexpr bool kPhi = WM & 1, kU = (WM >> 1) & 1, kPsi = (WM >> 2) & 1;
+  constexpr bool kGeo = kPhi && kU && (PM & 3) == 3;
+  constexpr bool kRes = kU && kPsi;
+  constexpr bool kResNl = kRes && (PM & 12) == 12;
   // ---- node normals ----
   S fn[12];
   for (int n = 0; n < 4; n++) {
@@ -154,48 +202,31 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
     const S rot[3] = {dC[5] - dC[7], dC[6] - dC[2], dC[1] - dC[3]};
     const S g[6] = {du0d[0], du0d[1], du0d[3], du0d[4], du0d[6], du0d[7]};
     for (int w = 0; w < 3; w++) {
+      if (!((WM >> w) & 1)) continue;
       S et = rot[0] * q[w][6 * n + 3] + rot[1] * q[w][6 * n + 4] + rot[2] * q[w][6 * n + 5];
       for (int m = 0; m < 4; m++)
         for (int d = 0; d < 3; d++) et = et + (Na[m] * g[2 * d] + Nb[m] * g[2 * d + 1]) * q[w][6 * m + d];
       etn[w][n] = et;
     }
   }
-  // ---- tying strains: linear parts of the three vectors and the bilinear
-  // quadratic terms of the four pairs ----
-  S tyl[3][9], tyq[4][9];
-  for (int t = 0; t < 9; t++) {
-    double u, v, N[4], Na[4], Nb[4];
-    tying_point(t, u, v);
-    shape4(u, v, N, Na, Nb);
-    S Xxi[6], n0[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < 6; i++) Xxi[i] = 0.0;
-    for (int m = 0; m < 4; m++)
-      for (int d = 0; d < 3; d++) {
-        Xxi[2 * d] = Xxi[2 * d] + Na[m] * X[3 * m + d];
-        Xxi[2 * d + 1] = Xxi[2 * d + 1] + Nb[m] * X[3 * m + d];
-        n0[d] = n0[d] + N[m] * fn[3 * m + d];
-      }
-    S U[3][6], d0[3][3];
-    for (int w = 0; w < 3; w++) {
-      for (int i = 0; i < 6; i++) U[w][i] = 0.0;
-      for (int i = 0; i < 3; i++) d0[w][i] = 0.0;
-      for (int m = 0; m < 4; m++) {
-        const S qr[3] = {q[w][6 * m + 3], q[w][6 * m + 4], q[w][6 * m + 5]};
-        S dm[3];
-        cross3(qr, &fn[3 * m], dm);
-        for (int d = 0; d < 3; d++) {
-          d0[w][d] = d0[w][d] + N[m] * dm[d];
-          U[w][2 * d] = U[w][2 * d] + Na[m] * q[w][6 * m + d];
-          U[w][2 * d + 1] = U[w][2 * d + 1] + Nb[m] * q[w][6 * m + d];
-        }
-      }
+  // ---- tying strains: linear parts of the vectors and the bilinear
+  // quadratic terms of the pairs.  Variants with pairs evaluate them again at
+  // every quadrature point instead of holding 9 points' worth of dual
+  // numbers through the quadrature loop (which spilled to scratch memory) ----
+  constexpr bool kLate = PM != 0;
+  S tyl[3][kLate ? 1 : 9], tyq[4][kLate ? 1 : 9];
+  if constexpr (!kLate) {
+    for (int t = 0; t < 9; t++) {
+      S tl[3], tq[4];
+      tying_at<S, WM, PM>(t, X, fn, q, tl, tq);
+      for (int w = 0; w < 3; w++)
+        if ((WM >> w) & 1) tyl[w][t] = tl[w];
     }
-    const int f = tying_field(t);
-    for (int w = 0; w < 3; w++) tyl[w][t] = tying_lin(f, U[w], d0[w], Xxi, n0);
-    for (int p = 0; p < 4; p++) tyq[p][t] = tying_quad(f, U[kPairA[p]], d0[kPairA[p]], U[kPairB[p]], d0[kPairB[p]]);
   }
   // ---- quadrature points ----
   S acc = 0.0;
+  // rolled: an unrolled loop keeps four points' dual numbers live at once
+#pragma unroll 1
   for (int iq = 0; iq < 4; iq++) {
     double u, v, N[4], Na[4], Nb[4];
     quad_point(iq, u, v);
@@ -222,6 +253,7 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
     // displacement gradients u0x, u1x of the three vectors
     S U0[3][9], U1[3][9];
     for (int w = 0; w < 3; w++) {
+      if (!((WM >> w) & 1)) continue;
       S d0[3] = {0.0, 0.0, 0.0}, d0xi[6], u0xi[6];
       for (int i = 0; i < 6; i++) d0xi[i] = u0xi[i] = 0.0;
       for (int m = 0; m < 4; m++) {
@@ -253,31 +285,48 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
     const int kk[5] = {0, 3, 1, 4, 2};
     const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
     const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
+    S al[5][3], aq[5][4];
     for (int r = 0; r < 5; r++) {
-      const int ka = pa[kk[r]], kb = pb[kk[r]];
-      S al[3], aq[4];
-      for (int w = 0; w < 3; w++) al[w] = 0.0;
-      for (int p = 0; p < 4; p++) aq[p] = 0.0;
-      for (int t = 0; t < 9; t++) {
-        const int sl = field_slot(tying_field(t));
-        const int cc = pa[sl], dd = pb[sl];
+      for (int w = 0; w < 3; w++) al[r][w] = 0.0;
+      for (int p = 0; p < 4; p++) aq[r][p] = 0.0;
+    }
+    for (int t = 0; t < 9; t++) {
+      S tl[3], tq[4];
+      if constexpr (kLate) {
+        tying_at<S, WM, PM>(t, X, fn, q, tl, tq);
+      } else {
+        for (int w = 0; w < 3; w++)
+          if ((WM >> w) & 1) tl[w] = tyl[w][t];
+      }
+      const int sl = field_slot(tying_field(t));
+      const int cc = pa[sl], dd = pb[sl];
+      for (int r = 0; r < 5; r++) {
+        const int ka = pa[kk[r]], kb = pb[kk[r]];
         const S M = (cc == dd) ? XT[3 * cc + ka] * XT[3 * cc + kb]
                                : XT[3 * cc + ka] * XT[3 * dd + kb] + XT[3 * dd + ka] * XT[3 * cc + kb];
         const S W = (ff[r] * Nt[t]) * M;
-        for (int w = 0; w < 3; w++) al[w] = al[w] + W * tyl[w][t];
-        for (int p = 0; p < 4; p++) aq[p] = aq[p] + W * tyq[p][t];
+        for (int w = 0; w < 3; w++)
+          if ((WM >> w) & 1) al[r][w] = al[r][w] + W * tl[w];
+        for (int p = 0; p < 4; p++)
+          if ((PM >> p) & 1) aq[r][p] = aq[r][p] + W * tq[p];
       }
+    }
+    for (int r = 0; r < 5; r++) {
       const int row = r < 3 ? r : (r == 3 ? 6 : 7);
-      for (int w = 0; w < 3; w++) e1[w][row] = al[w];
-      for (int p = 0; p < 4; p++) eq[p][row] = aq[p];
+      for (int w = 0; w < 3; w++)
+        if ((WM >> w) & 1) e1[w][row] = al[r][w];
+      for (int p = 0; p < 4; p++)
+        if ((PM >> p) & 1) eq[p][row] = aq[r][p];
     }
     for (int w = 0; w < 3; w++) {
+      if (!((WM >> w) & 1)) continue;
       e1[w][3] = U1[w][0];
       e1[w][4] = U1[w][4];
       e1[w][5] = U1[w][1] + U1[w][3];
       e1[w][8] = N[0] * etn[w][0] + N[1] * etn[w][1] + N[2] * etn[w][2] + N[3] * etn[w][3];
     }
     for (int p = 0; p < 4; p++) {
+      if (!((PM >> p) & 1)) continue;
       const S *A0 = U0[kPairA[p]], *A1 = U1[kPairA[p]], *B0 = U0[kPairB[p]], *B1 = U1[kPairB[p]];
       eq[p][3] = A0[0] * B1[0] + A0[3] * B1[3] + A0[6] * B1[6] + B0[0] * A1[0] + B0[3] * A1[3] + B0[6] * A1[6];
       eq[p][4] = A0[1] * B1[1] + A0[4] * B1[4] + A0[7] * B1[7] + B0[1] * A1[1] + B0[4] * A1[4] + B0[7] * A1[7];
@@ -285,22 +334,29 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
                  B0[0] * A1[1] + B0[3] * A1[4] + B0[6] * A1[7] + B1[0] * A0[1] + B1[3] * A0[4] + B1[6] * A0[7];
       eq[p][8] = 0.0;
     }
-    // mechanical strain of the path
+    // mechanical strain of the path (u = 0 where u is not evaluated)
     S eu[9];
-    for (int i = 0; i < 9; i++) eu[i] = e1[1][i];
+    for (int i = 0; i < 9; i++) {
+      if constexpr (kU)
+        eu[i] = e1[1][i];
+      else
+        eu[i] = 0.0;
+    }
     eu[0] = eu[0] - eth;
     eu[1] = eu[1] - eth;
     S y[9];
-    if (c[0] != 0.0 || c[1] != 0.0) {
+    if (kPhi && (c[0] != 0.0 || c[1] != 0.0)) {
       apply_cs(sec, det, e1[0], y);  // det C e1(phi)
       if (c[0] != 0.0) acc = acc + c[0] * dot9(e1[0], y);
-      if (c[1] != 0.0) {
-        S z[9];
-        apply_cs(sec, det, eq[1], z);  // det C Q(phi, phi)
-        acc = acc + c[1] * (2.0 * dot9(eq[0], y) + dot9(eu, z));
+      if constexpr (kGeo) {
+        if (c[1] != 0.0) {
+          S z[9];
+          apply_cs(sec, det, eq[1], z);  // det C Q(phi, phi)
+          acc = acc + c[1] * (2.0 * dot9(eq[0], y) + dot9(eu, z));
+        }
       }
     }
-    if (c[3] != 0.0) {
+    if (kPhi && c[3] != 0.0) {
       S u0[3] = {0.0, 0.0, 0.0}, d0[3] = {0.0, 0.0, 0.0};
       for (int m = 0; m < 4; m++) {
         const S qr[3] = {q[0][6 * m + 3], q[0][6 * m + 4], q[0][6 * m + 5]};
@@ -313,36 +369,46 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
       }
       acc = acc + c[3] * det * (m0 * dot3(u0, u0) + m2 * dot3(d0, d0));
     }
-    if (c[2] != 0.0) {
-      S ep[9];
-      for (int i = 0; i < 9; i++) ep[i] = e1[2][i];
-      if (nonlin) {
-        for (int i = 0; i < 9; i++) {
-          ep[i] = ep[i] + eq[2][i];
-          eu[i] = eu[i] + 0.5 * eq[3][i];
+    if constexpr (kRes) {
+      if (c[2] != 0.0) {
+        S ep[9];
+        for (int i = 0; i < 9; i++) ep[i] = e1[2][i];
+        if constexpr (kResNl) {
+          if (nonlin) {
+            for (int i = 0; i < 9; i++) {
+              ep[i] = ep[i] + eq[2][i];
+              eu[i] = eu[i] + 0.5 * eq[3][i];
+            }
+          }
         }
+        apply_cs(sec, det, eu, y);
+        acc = acc + c[2] * dot9(ep, y);
       }
-      apply_cs(sec, det, eu, y);
-      acc = acc + c[2] * dot9(ep, y);
     }
   }
   out = acc;
 }
 
+template <int WM, int PM>
 __global__ __launch_bounds__(kThreads) void xpt_kernel(XptArgs a, int nel) {
+  // the element's three vectors, shared by its 12 threads (read-only)
+  __shared__ double qs[kThreads / 12][3][24];
   const int tid = blockIdx.x * kThreads + threadIdx.x;
-  const int le = tid / 12, dir = tid % 12;
-  if (le >= nel) return;
-  const int e = a.elems[le];
+  const int le = tid / 12, dir = tid % 12, ll = threadIdx.x / 12;
+  const bool active = le < nel;
+  const int e = a.elems[active ? le : nel - 1];
   const long design = blockIdx.y;
   const int *cn = a.conn + 4 * e;
+  const double *src[3] = {a.phi, a.u, a.psi};
+  for (int k = dir; k < 72; k += 12) {
+    const int w = k / 24, i = k % 24;
+    qs[ll][w][i] = ((WM >> w) & 1) && src[w] ? src[w][design * a.stride + 6 * cn[i / 6] + i % 6] : 0.0;
+  }
+  __syncthreads();
+  if (!active) return;
+  const double(*q)[24] = qs[ll];
   Dual X[12];
   for (int i = 0; i < 12; i++) X[i] = Dual(a.Xpts[3 * cn[i / 3] + i % 3], i == dir ? 1.0 : 0.0);
-  double q[3][24];
-  const double *src[3] = {a.phi, a.u, a.psi};
-  for (int w = 0; w < 3; w++)
-    for (int i = 0; i < 24; i++)
-      q[w][i] = src[w] ? src[w][design * a.stride + 6 * cn[i / 6] + i % 6] : 0.0;
   const double thick = a.thick[design * a.nelems + e];
   const double temp = a.temp ? a.temp[design * a.nelems + e] : 0.0;
   const Section sec = make_section(a.mat, thick);
@@ -350,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void xpt_kernel(XptArgs a, int nel) {
   // mass moments of TACSIsoShellConstitutive with tOffset = 0
   const double m0 = a.mat.rho * thick, m2 = a.mat.rho * thick * thick * thick / 12.0;
   Dual s;
-  element_scalars(X, q, sec, a.mat.cte * temp, a.nonlin != 0, c, m0, m2, s);
+  element_scalars<Dual, WM, PM>(X, q, sec, a.mat.cte * temp, a.nonlin != 0, c, m0, m2, s);
   a.dfdX[design * a.dX_stride + 3 * cn[dir / 3] + dir % 3] += s.d;
 }
 
@@ -359,7 +425,19 @@ __global__ __launch_bounds__(kThreads) void xpt_kernel(XptArgs a, int nel) {
 hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream_t stream) {
   if (nel <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((12L * nel + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(xpt_kernel, dim3(blocks, nbatch), dim3(kThreads), 0, stream, args, nel);
+  const dim3 grid(blocks, nbatch);
+  const int t = args.terms;
+  if ((t & kXptR) && (t & (kXptK | kXptG | kXptM)))  // every vector and pair
+    hipLaunchKernelGGL((xpt_kernel<7, 15>), grid, dim3(kThreads), 0, stream, args, nel);
+  else if (t & kXptR)  // u, psi; (u, psi) and (u, u) for the nonlinear model
+    if (args.nonlin)
+      hipLaunchKernelGGL((xpt_kernel<6, 12>), grid, dim3(kThreads), 0, stream, args, nel);
+    else
+      hipLaunchKernelGGL((xpt_kernel<6, 0>), grid, dim3(kThreads), 0, stream, args, nel);
+  else if (t & kXptG)  // phi, u; (u, phi) and (phi, phi)
+    hipLaunchKernelGGL((xpt_kernel<3, 3>), grid, dim3(kThreads), 0, stream, args, nel);
+  else  // phi alone: s_K, s_M
+    hipLaunchKernelGGL((xpt_kernel<1, 0>), grid, dim3(kThreads), 0, stream, args, nel);
   return hipGetLastError();
 }
 
