@@ -283,19 +283,20 @@ __global__ __launch_bounds__(256) void gemv_n_pass1(int m, int n, const double *
   if (row < m) {
     if (LOWER) c1 = min(c1, row + 2);
     int c = c0 + wave;
-    for (; c + 12 < c1; c += 16) {
-      const double2 v0 = load_stream2(A + row + (long)c * lda);
-      const double2 v1 = load_stream2(A + row + (long)(c + 4) * lda);
-      const double2 v2 = load_stream2(A + row + (long)(c + 8) * lda);
-      const double2 v3 = load_stream2(A + row + (long)(c + 12) * lda);
-      const double x0 = x[c], x1 = x[c + 4], x2 = x[c + 8], x3 = x[c + 12];
-      if (LOWER) {
-        a0 += (c <= row ? v0.x : 0.0) * x0 + (c + 4 <= row ? v1.x : 0.0) * x1 + (c + 8 <= row ? v2.x : 0.0) * x2 +
-              (c + 12 <= row ? v3.x : 0.0) * x3;
-      } else {
-        a0 += v0.x * x0 + v1.x * x1 + v2.x * x2 + v3.x * x3;
+    // 8 columns' loads in flight per lane (about 10 % more bandwidth than 4)
+    constexpr int kU = 8;
+    for (; c + 4 * (kU - 1) < c1; c += 4 * kU) {
+      double2 v[kU];
+      double xc[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) v[u] = load_stream2(A + row + (long)(c + 4 * u) * lda);
+#pragma unroll
+      for (int u = 0; u < kU; u++) xc[u] = x[c + 4 * u];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        a0 += ((!LOWER || c + 4 * u <= row) ? v[u].x : 0.0) * xc[u];
+        a1 += v[u].y * xc[u];
       }
-      a1 += v0.y * x0 + v1.y * x1 + v2.y * x2 + v3.y * x3;
     }
     for (; c < c1; c += 4) {
       const double2 v = load_stream2(A + row + (long)c * lda);
@@ -352,19 +353,21 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, const double 
   const int i0 = LOWER ? (j & ~1) : 0;  // even start keeps the 16-byte alignment
   double a0 = 0.0, a1 = 0.0;
   int i = i0 + 2 * lane;
-  for (; i + 384 < m; i += 512) {
-    const double2 v0 = load_stream2(col + i);
-    const double2 v1 = load_stream2(col + i + 128);
-    const double2 v2 = load_stream2(col + i + 256);
-    const double2 v3 = load_stream2(col + i + 384);
-    const double2 x0 = *reinterpret_cast<const double2 *>(xx + i);
-    const double2 x1 = *reinterpret_cast<const double2 *>(xx + i + 128);
-    const double2 x2 = *reinterpret_cast<const double2 *>(xx + i + 256);
-    const double2 x3 = *reinterpret_cast<const double2 *>(xx + i + 384);
+  // 8 loads in flight per lane (about 8 % more bandwidth than 4)
+  constexpr int kU = 8;
+  for (; i + 128 * (kU - 1) < m; i += 128 * kU) {
+    double2 v[kU], xv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) v[u] = load_stream2(col + i + 128 * u);
+#pragma unroll
+    for (int u = 0; u < kU; u++) xv[u] = *reinterpret_cast<const double2 *>(xx + i + 128 * u);
     // in a LOWER column only the first pair can hold a row above the
     // diagonal (row j - 1 when j is odd)
-    a0 += ((LOWER && i < j) ? 0.0 : v0.x * x0.x) + v1.x * x1.x + v2.x * x2.x + v3.x * x3.x;
-    a1 += v0.y * x0.y + v1.y * x1.y + v2.y * x2.y + v3.y * x3.y;
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      a0 += (LOWER && u == 0 && i < j) ? 0.0 : v[u].x * xv[u].x;
+      a1 += v[u].y * xv[u].y;
+    }
   }
   for (; i < m; i += 128) {
     const double2 v = load_stream2(col + i);
@@ -1552,7 +1555,10 @@ static int gemv_batch(shellgpu_factor *F, bool trans, int m, int n, const double
   shellgpu_ctx *ctx = F->ctx;
   if (nq <= 0) return 0;
   if (!trans) {
-    const int nsplit = std::max(1, std::min(32, n / 64));
+    // column chunks per row block: enough workgroups to fill the GPU
+    // (about 2,048), and as few as that allows (longer column loops)
+    const long rb = (m + 127) / 128;
+    const int nsplit = (int)std::max<long>(1, std::min<long>({32, n / 64, (2048 + rb * nq - 1) / (rb * nq)}));
     const int chunk = (n + nsplit - 1) / nsplit;
     const dim3 grid((m + 127) / 128, nsplit, nq);
     if (lower)

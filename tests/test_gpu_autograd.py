@@ -28,8 +28,16 @@ def test_autograd_ks_gradients_match_differences():
     def f(tv, Xv):
         return float((w.numpy() * prob.evaluate(tv, Xv)[0]).sum())
 
-    h = 1e-5
-    fd_t = (f(t0 + h * dt, cyl.Xpts) - f(t0 - h * dt, cyl.Xpts)) / (2 * h)
+    # the eigenvalues carry round-off of ~1e-13 relative, which puts ~1e-8 of
+    # noise on a central difference at h = 1e-5; h = 1e-4 with Richardson
+    # extrapolation keeps both the noise and the truncation error below
+    # 1e-9 of the derivative
+    h = 1e-4
+
+    def cd(hh):
+        return (f(t0 + hh * dt, cyl.Xpts) - f(t0 - hh * dt, cyl.Xpts)) / (2 * hh)
+
+    fd_t = (4.0 * cd(h) - cd(2 * h)) / 3.0
     ad_t = float((thick.grad.numpy() * dt).sum())
     print("thickness", ad_t, fd_t)
     assert abs(ad_t - fd_t) <= 1e-5 * max(abs(fd_t), 1e-3)
