@@ -2909,6 +2909,9 @@ struct shellgpu_buckling {
   // FK == F and K is refactored where needed
   shellgpu_factor *FK = nullptr;
   bool fk_valid = false;
+  // FK is the caller's (shellgpu_buckling_set_stiffness_factor): kept, not
+  // refactored, not destroyed
+  bool fk_ext = false;
   // static-path GMRES settings
   int ksm_m = 10, ksm_restarts = 15;
   double ksm_rtol = 1e-12, ksm_atol = 1e-30;
@@ -2917,7 +2920,7 @@ struct shellgpu_buckling {
 namespace {
 
 int buckling_free(shellgpu_buckling *B) {
-  if (B->FK && B->FK != B->F) shellgpu_factor_destroy(B->FK);
+  if (B->FK && B->FK != B->F && !B->fk_ext) shellgpu_factor_destroy(B->FK);
   B->FK = nullptr;
   if (B->own_ops) {
     if (B->K) shellgpu_mat_destroy(B->K);
@@ -3016,12 +3019,27 @@ void shellgpu_buckling_destroy(shellgpu_buckling *B) {
   delete B;
 }
 
+int shellgpu_buckling_set_stiffness_factor(shellgpu_buckling *B, shellgpu_factor *F) {
+  if (!B) return 1;
+  shellgpu_ctx *ctx = B->ctx;
+  if (!F || F->ctx != ctx) return ferr(ctx, 1, "a factor of this context is required");
+  if (F == B->F) return ferr(ctx, 1, "the stiffness factor must differ from the factor of A");
+  if (!F->factored) return ferr(ctx, 9, "the stiffness factor holds no factorisation");
+  if (F->nbatch != ctx->nbatch) return ferr(ctx, 1, "factor batch size does not match the context's designs");
+  if (B->FK && B->FK != B->F && !B->fk_ext) shellgpu_factor_destroy(B->FK);
+  B->FK = F;
+  B->fk_ext = true;
+  B->fk_valid = true;
+  return 0;
+}
+
 int shellgpu_buckling_set_operators(shellgpu_buckling *B, shellgpu_mat *K, shellgpu_mat *G, shellgpu_mat *A,
                                     shellgpu_factor *F) {
   if (!B) return 1;
   shellgpu_ctx *ctx = B->ctx;
   if (!K || !G || !A || !F) return ferr(ctx, 1, "K, G, A and the factor are required");
   if (K == G || K == A || G == A) return ferr(ctx, 1, "K, G and A must be distinct matrices");
+  if (B->fk_ext && F == B->FK) return ferr(ctx, 1, "the factor of A must differ from the stiffness factor");
   const int nb = ctx->nbatch;
   if (K->nbatch != nb || G->nbatch != nb || A->nbatch != nb)
     return ferr(ctx, 1, "matrix batch size does not match the context's designs");
@@ -3070,17 +3088,19 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
   // identical designs (load cases of one structure) share one factor of K;
   // the designs may have changed since the factor was made
   const int fk_nb = ctx->same_designs ? 1 : nb;
-  if (B->FK && B->FK != B->F && B->FK->nbatch != fk_nb) {
-    shellgpu_factor_destroy(B->FK);
-    B->FK = nullptr;
+  if (!B->fk_ext) {
+    if (B->FK && B->FK != B->F && B->FK->nbatch != fk_nb) {
+      shellgpu_factor_destroy(B->FK);
+      B->FK = nullptr;
+    }
+    if (!B->FK && factor_create(ctx, fk_nb, &B->FK) != 0) {
+      B->FK = B->F;  // no room for a second factor
+      ctx->err.clear();
+      (void)hipGetLastError();  // drop the failed allocation's error state
+    }
+    B->fk_valid = false;
   }
-  if (!B->FK && factor_create(ctx, fk_nb, &B->FK) != 0) {
-    B->FK = B->F;  // no room for a second factor
-    ctx->err.clear();
-    (void)hipGetLastError();  // drop the failed allocation's error state
-  }
-  B->fk_valid = false;
-  if (!u0 || B->sigma <= 0.0) {
+  if (!B->fk_valid && (!u0 || B->sigma <= 0.0)) {
     if ((rc = shellgpu_factor_compute(B->FK, B->K))) return rc;
     B->fk_valid = true;
   }
@@ -3617,8 +3637,23 @@ int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double
   return 0;
 }
 
+static int newton_body(shellgpu_ctx *ctx, shellgpu_factor *Fext, const double *force, int nsteps, int max_iters,
+                       double rtol, double *u, int *iters, double *relres);
+
 int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, int max_iters, double rtol, double *u,
                           int *iters, double *relres) {
+  return newton_body(ctx, nullptr, force, nsteps, max_iters, rtol, u, iters, relres);
+}
+
+int shellgpu_newton_solve_factor(shellgpu_ctx *ctx, shellgpu_factor *F, const double *force, int nsteps,
+                                 int max_iters, double rtol, double *u, int *iters, double *relres) {
+  if (!F || F->ctx != ctx) return ferr(ctx, 1, "a factor of this context is required");
+  if (F->nbatch != ctx->nbatch) return ferr(ctx, 1, "factor batch size does not match the context's designs");
+  return newton_body(ctx, F, force, nsteps, max_iters, rtol, u, iters, relres);
+}
+
+static int newton_body(shellgpu_ctx *ctx, shellgpu_factor *Fext, const double *force, int nsteps, int max_iters,
+                       double rtol, double *u, int *iters, double *relres) {
   if (!ctx || !force || !u || nsteps <= 0 || max_iters <= 0 || !(rtol > 0.0))
     return ferr(ctx, 1, "invalid Newton arguments");
   if (!ctx->d_thick) return ferr(ctx, 3, "designs not set (shellgpu_set_designs)");
@@ -3633,7 +3668,10 @@ int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, in
   if ((rc = vo.init(ctx, n, nb))) return rc;
   if ((rc = ensure_blas(ctx))) return rc;
   if ((rc = shellgpu_mat_create(ctx, nb, &A))) goto out;
-  if ((rc = shellgpu_factor_create(ctx, &F))) goto out;
+  if (Fext)
+    F = Fext;
+  else if ((rc = shellgpu_factor_create(ctx, &F)))
+    goto out;
   if (hipMalloc((void **)&R, sizeof(double) * (size_t)n * nb) != hipSuccess ||
       hipMalloc((void **)&f0, sizeof(double) * (size_t)n * nb) != hipSuccess ||
       hipMalloc((void **)&du, sizeof(double) * (size_t)n * nb) != hipSuccess) {
@@ -3694,7 +3732,7 @@ int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, in
     for (int d = 0; d < nb; d++) relres[d] = rn[d];
 out:
   if (A) shellgpu_mat_destroy(A);
-  if (F) shellgpu_factor_destroy(F);
+  if (F && F != Fext) shellgpu_factor_destroy(F);
   double *bufs[] = {R, f0, du};
   for (double *b : bufs)
     if (b) (void)hipFree(b);

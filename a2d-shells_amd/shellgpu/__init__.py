@@ -104,6 +104,8 @@ def load_library(path=LIB_PATH):
         "shellgpu_buckling_extract": (C.c_int, [_VP, C.c_int, _DP, _VP, _DP]),
         "shellgpu_buckling_info": (C.c_int, [_VP, _IP, _DP, _VP]),
         "shellgpu_buckling_add_eigen_sens": (C.c_int, [_VP, _DP, _VP, _VP]),
+        "shellgpu_buckling_set_stiffness_factor": (C.c_int, [_VP, _VP]),
+        "shellgpu_newton_solve_factor": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_double, _VP, _IP, _DP]),
         "shellgpu_element_output": (C.c_int, [_VP, C.c_int, _VP, C.c_double, _VP]),
         "shellgpu_frequency_create": (C.c_int, [_VP, C.c_double, C.c_int, C.c_int, C.c_double, C.POINTER(_VP)]),
         "shellgpu_frequency_destroy": (None, [_VP]),
@@ -370,6 +372,12 @@ class Buckling:
         ctx._check(ctx.lib.shellgpu_buckling_create(ctx.h, LINEAR if model is None else model, float(sigma), nev,
                                                     max_lanczos, tol, C.byref(h)))
         self.h = h.value
+
+    def set_stiffness_factor(self, F):
+        """Share the caller's factor of K (zero state) for the shift
+        estimate and the path solves; F must stay alive and current."""
+        self.ctx._check(self.ctx.lib.shellgpu_buckling_set_stiffness_factor(self.h, F.h))
+        self._fk = F
 
     def solve(self, force=None, u0=None):
         """Static path (or u0), G, shift, Lanczos; force/u0 are host arrays
@@ -669,15 +677,21 @@ class Context:
         self._check(self.lib.shellgpu_eval_mat_sv_sens_inner_product(self.h, psi.ptr if psi else None, phi.ptr,
                                                                      dfdu.ptr))
 
-    def newton_solve(self, force, nsteps=5, max_iters=20, rtol=1e-10, u0=None):
+    def newton_solve(self, force, nsteps=5, max_iters=20, rtol=1e-10, u0=None, factor=None):
         """Load-stepped Newton on the nonlinear model: returns (u device
-        array [nbatch*6n], Newton iterations, relative residuals [nbatch])."""
+        array [nbatch*6n], Newton iterations, relative residuals [nbatch]).
+        With a Factor, the tangent factors go into it and the last stays
+        there (from u0 = 0 at zero temperature: the stiffness K)."""
         nb, n = self.nbatch, 6 * self.nnodes
         f = force if isinstance(force, DeviceArray) else self.vector(np.broadcast_to(force, (nb, n)))
         u = self.vector(None if u0 is None else np.broadcast_to(u0, (nb, n)))
         its, rel = C.c_int(), np.zeros(nb)
-        self._check(self.lib.shellgpu_newton_solve(self.h, f.ptr, nsteps, max_iters, rtol, u.ptr, C.byref(its),
-                                                   _dp(rel)))
+        if factor is None:
+            self._check(self.lib.shellgpu_newton_solve(self.h, f.ptr, nsteps, max_iters, rtol, u.ptr,
+                                                       C.byref(its), _dp(rel)))
+        else:
+            self._check(self.lib.shellgpu_newton_solve_factor(self.h, factor.h, f.ptr, nsteps, max_iters, rtol,
+                                                              u.ptr, C.byref(its), _dp(rel)))
         return u, its.value, rel
 
     def buckling(self, sigma=0.0, nev=5, max_lanczos=100, tol=1e-12, model=None):

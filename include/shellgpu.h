@@ -280,6 +280,15 @@ int shellgpu_gmres_noalloc(shellgpu_ctx *ctx, shellgpu_mat *A, shellgpu_factor *
  * relres (host, nbatch, may be NULL) = final relative residuals. */
 int shellgpu_newton_solve(shellgpu_ctx *ctx, const double *force, int nsteps, int max_iters,
                           double rtol, double *u, int *iters, double *relres);
+/* shellgpu_newton_solve with a factor object of the caller's
+ * (shellgpu_factor_create on this context): Newton factors its tangents into
+ * F and leaves the last one there.  From a zero initial state at zero
+ * temperature, with no refactorisation, that is the tangent at zero state,
+ * i.e. the stiffness K, which shellgpu_buckling_set_stiffness_factor can
+ * share.  (No reference counterpart, as for shellgpu_newton_solve.) */
+int shellgpu_newton_solve_factor(shellgpu_ctx *ctx, shellgpu_factor *F, const double *force,
+                                 int nsteps, int max_iters, double rtol, double *u, int *iters,
+                                 double *relres);
 
 /* ---- linear buckling (TACSLinearBuckling, src/TACSBuckling.h:43-107) ---- */
 /* TACSLinearBuckling::solve(rhs = force, u0 = NULL) (src/TACSBuckling.cpp:
@@ -318,6 +327,19 @@ void shellgpu_buckling_destroy(shellgpu_buckling *B);
  * own. */
 int shellgpu_buckling_set_operators(shellgpu_buckling *B, shellgpu_mat *K, shellgpu_mat *G,
                                     shellgpu_mat *A, shellgpu_factor *F);
+/* A factor of the caller's holding the stiffness K at zero state (BCs
+ * applied) for this context's designs, such as the one
+ * shellgpu_newton_solve_factor leaves.  The automatic shift estimate and
+ * the GMRES of the static path and of the path adjoint use it as K^-1 (or
+ * preconditioner) instead of the object factoring K.  The reference's
+ * TACSLinearBuckling::solve(rhs, u0) factors only K + sigma G when u0 is
+ * given and takes sigma from the user (src/TACSBuckling.cpp:238-275); the
+ * automatic shift here needs K^-1, and this call shares the caller's factor
+ * as TACS drivers share one preconditioner between analyses.  F stays the
+ * caller's and must stay current (same designs).  A factor of a nearby
+ * matrix only loosens the shift estimate: a shift above the first load is
+ * caught by the Cholesky of K + sigma G and lowered. */
+int shellgpu_buckling_set_stiffness_factor(shellgpu_buckling *B, shellgpu_factor *F);
 /* GMRES settings of the static path solve (GMRES constructor and
  * setTolerances, src/bpmat/KSM.cpp:547-560, 727-730; default 10, 15, 1e-12,
  * 1e-30) */

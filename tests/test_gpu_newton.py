@@ -87,3 +87,44 @@ def test_buckling_about_nonlinear_state_and_gradient():
         fd = (lp[0] - lm[0]) / (2 * h)
         print(k, g @ dirn, fd)
         assert abs(g @ dirn - fd) <= 1e-6 * abs(lam[0]), k
+
+
+def test_shared_newton_factor_matches_own_factors():
+    # Newton from the zero state factors its tangent at zero state (= K) into
+    # the caller's factor; the buckling analysis then uses it for the shift
+    # estimate and the path adjoint instead of factoring K itself.  Loads,
+    # shift-independent, and the gradient agree with the unshared run.
+    cyl = Cylinder(10, 20, imperfection=(0.0002, 2, 3))
+    thick = seeded_thickness(cyl.nelems, 73)
+    f = cyl.edge_load(0.2 * cyl.classical_load(0.002, 70e9, 0.3))
+    nev = 3
+    out = []
+    for shared in (False, True):
+        ctx = _ctx(cyl, thick)
+        F = ctx.factor() if shared else None
+        u, _, rel = ctx.newton_solve(f, nsteps=5, max_iters=20, rtol=1e-12, factor=F)
+        B = ctx.buckling(sigma=0.0, nev=nev, max_lanczos=80, tol=1e-12, model=sg.NONLINEAR)
+        if shared:
+            B.set_stiffness_factor(F)
+        B.solve(u0=u)
+        lam, err = B.eigenvalues()
+        g = B.eigen_sens(np.eye(nev)[0])[0][0]
+        out.append((lam[0], g, u.download()))
+        B.destroy()
+        if F is not None:
+            F.destroy()
+        ctx.close()
+    (l0, g0, u0), (l1, g1, u1) = out
+    print("own", l0, "shared", l1)
+    assert np.abs(u1 - u0).max() <= 1e-12 * np.abs(u0).max()
+    assert np.abs(l1 - l0).max() <= 1e-10 * np.abs(l0).max()
+    assert np.abs(g1 - g0).max() <= 1e-8 * np.abs(g0).max()
+    # a factor that holds no factorisation is refused
+    ctx = _ctx(cyl, thick)
+    B = ctx.buckling(sigma=0.0, nev=nev, max_lanczos=80, tol=1e-12, model=sg.NONLINEAR)
+    F = ctx.factor()
+    with pytest.raises(sg.ShellGPUError):
+        B.set_stiffness_factor(F)  # holds no factorisation yet
+    B.destroy()
+    F.destroy()
+    ctx.close()
