@@ -2,6 +2,7 @@
 // matrix objects, assembly launches, boundary conditions and the 6x6-block
 // SpMV.  Declarations and reference citations are in include/shellgpu.h.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1143,7 +1144,25 @@ int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch
     if (!name[0] && hipDeviceGetName(name, name_len, ctx->device) != hipSuccess) name[0] = 0;
   }
   if (arch && arch_len > 0) snprintf(arch, arch_len, "%s", prop.gcnArchName);
-  if (pci_bus_id && pci_len > 0) HIPCALL(ctx, hipDeviceGetPCIBusId(pci_bus_id, pci_len, ctx->device));
+  char pci[64] = {0};
+  HIPCALL(ctx, hipDeviceGetPCIBusId(pci, sizeof(pci), ctx->device));
+  if (pci_bus_id && pci_len > 0) snprintf(pci_bus_id, pci_len, "%s", pci);
+  if (name && name_len > 0 && !name[0]) {
+    // without the libdrm marketing-name table the runtime reports none: the
+    // kernel driver's product name of this PCI device
+    std::string path = "/sys/bus/pci/devices/";
+    for (const char *c = pci; *c; c++) path += (char)tolower((unsigned char)*c);
+    path += "/product_name";
+    if (FILE *f = fopen(path.c_str(), "r")) {
+      if (fgets(name, name_len, f)) {
+        size_t k = strlen(name);
+        while (k > 0 && (name[k - 1] == '\n' || name[k - 1] == ' ')) name[--k] = 0;
+      } else {
+        name[0] = 0;
+      }
+      fclose(f);
+    }
+  }
   if (compute_units) *compute_units = prop.multiProcessorCount;
   if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
   return 0;

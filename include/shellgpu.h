@@ -435,8 +435,10 @@ int shellgpu_set_elem_schedule(shellgpu_ctx *ctx, int kernel_class, int variant)
 
 /* ---- device identification (no reference counterpart: measurement
  * plumbing) ---------------------------------------------------------------- */
-/* the context's GPU: marketing name and architecture (e.g. "gfx950"), PCI bus
- * id ("0000:05:00.0"), compute units and HBM bytes; any output may be NULL */
+/* the context's GPU: marketing name (the runtime's, else the kernel
+ * driver's product name of the PCI device) and architecture (e.g.
+ * "gfx950"), PCI bus id ("0000:05:00.0"), compute units and HBM bytes; any
+ * output may be NULL */
 int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch, int arch_len,
                          char *pci_bus_id, int pci_len, int *compute_units, size_t *hbm_bytes);
 
