@@ -199,11 +199,13 @@ hipError_t launch_tiles(bool ta, bool tb, const GemmArgs &g, long tiles, hipStre
 
 hipError_t launch_gemm_f64(bool ta, bool tb, const GemmArgs &g, int batch, hipStream_t stream) {
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
-  // 128-tiles while they give at least two workgroups per CU, else 64-tiles
+  // 128-tiles while they give at least two workgroups per CU and both
+  // dimensions exceed 64 (a 128-tile on a narrow product, such as a
+  // product with 16 vectors, would compute mostly padding), else 64-tiles
   const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128) * batch;
   const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64) * batch;
   if (t64 > 0x7fffffffL) return hipErrorInvalidValue;
-  if (t128 >= 512) return launch_tiles<4>(ta, tb, g, t128, stream);
+  if (t128 >= 512 && g.M > 64 && g.N > 64) return launch_tiles<4>(ta, tb, g, t128, stream);
   return launch_tiles<2>(ta, tb, g, t64, stream);
 }
 

@@ -133,6 +133,13 @@ struct shellgpu_factor {
   // nact) when nact >= 0 (Lanczos stops solving for converged designs)
   int *d_act = nullptr;
   int nact = -1;
+  // several vectors through a cyclic-reduction factor of one design: work
+  // vectors of crm_cap vectors each (level-pair stride), the symmetric
+  // products' partial sums, and the (vector, odd level) entries they
+  // reduce
+  int crm_cap = 0;
+  double *crm_b = nullptr, *crm_z = nullptr, *crm_x = nullptr, *crm_part = nullptr;
+  int *d_crm_act = nullptr;
   bool factored = false;
   long nv = 0;               // level-ordered vector length per design
   // block cyclic reduction over uniform (padded) levels of M dofs: per
@@ -897,6 +904,20 @@ __global__ void permute_kernel(long nv, long n, const int *__restrict__ pos2dof,
   }
 }
 
+// level order <-> dof order for nrhs vectors (blockIdx.y) of one design,
+// level-ordered vectors at a stride of their own
+__global__ void permute_rhs_kernel(long nv, long n, long stride, const int *__restrict__ pos2dof,
+                                   const double *__restrict__ src, double *__restrict__ dst, int to_level) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int p = pos2dof[i];
+  const long v = blockIdx.y;
+  if (to_level)
+    dst[v * stride + i] = p >= 0 ? src[v * n + p] : 0.0;
+  else if (p >= 0)
+    dst[v * n + p] = src[v * stride + i];
+}
+
 // x[b] *= s[b]  (per design scale)
 __global__ void scale_batched_kernel(int n, double *x, const double *s) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1034,10 +1055,11 @@ int setup_plan(shellgpu_factor *F) {
 
   // Block cyclic reduction when the padded factor (three M x M blocks per
   // level and design) fits in 85 % of the free device memory, padding the
-  // levels to M adds at most half the unknowns, and few vectors are solved
-  // at a time (the context's designs: a shared factor solves one per
-  // design); otherwise the sequential factorisation.  SHELLGPU_FACTOR=thomas
-  // / =cr selects either (cr only where it fits).
+  // levels to M adds at most half the unknowns, and the factor holds few
+  // designs; a factor of one design shared by the context's designs (load
+  // cases) solves their vectors together (up to 8, or 16) and reads
+  // itself once per solve.  Otherwise the sequential factorisation.
+  // SHELLGPU_FACTOR=thomas / =cr selects either (cr only where it fits).
   {
     const long M = F->maxm;
     size_t freeb = 0, totalb = 0;
@@ -1051,7 +1073,9 @@ int setup_plan(shellgpu_factor *F) {
     // with many vectors per solve the sequential factorisation is parallel
     // enough and reads fewer bytes per solve (half of one inverse per
     // level, where cyclic reduction reads three padded blocks)
-    F->cr = fits && uniform && (long)ctx->nbatch * M <= 8192;
+    const int per = ctx->nbatch / F->nbatch;  // vectors per design of one solve
+    const bool multi_ok = per == 1 || (F->nbatch == 1 && (per <= 8 || per == 16));
+    F->cr = fits && uniform && (long)F->nbatch * M <= 8192 && multi_ok;
     if (env && strcmp(env, "thomas") == 0) F->cr = false;
     if (env && strcmp(env, "cr") == 0) F->cr = fits;
   }
@@ -1497,7 +1521,8 @@ void shellgpu_factor_destroy(shellgpu_factor *F) {
   void *bufs[] = {F->buf,      F->tmp,       F->info,      F->d_bcflag,  F->d_pos2dof, F->d_blk_base,
                   F->d_blk_ld, F->vb,        F->vz,        F->vx,        F->part,      F->d_mlev,
                   F->d_ptr,    F->cr_info,   F->d_cp_rowp, F->d_cp_col,  F->d_ct_rowp, F->d_ct_src,
-                  F->d_ct_blk, F->wbuf,      F->d_act,     F->d_noff,    F->d_jobs};
+                  F->d_ct_blk, F->wbuf,      F->d_act,     F->d_noff,    F->d_jobs,    F->crm_b,
+                  F->crm_z,    F->crm_x,     F->crm_part,  F->d_crm_act};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete F;
@@ -1993,6 +2018,143 @@ static int apply_cr(shellgpu_factor *F) {
   return symv_cr(F, F->vb + M, -1.0, F->vx + M, 1.0, F->vx + M);
 }
 
+// x = A^{-1} b for R vectors through a cyclic-reduction factor of one
+// design, reading the factor once for all of them: the first round's
+// symmetric inverses by the multi-vector symmetric product (read in full),
+// the dense rounds' products as matrix products with R columns, the
+// gathers over the vectors.  The level-ordered vectors lie at a stride of
+// whole level pairs (NVR), so vector r of odd level 2k + 1 starts at
+// r NVR + (2k + 1) M.  b, x: R vectors of n dofs.
+static int apply_cr_multi(shellgpu_factor *F, const double *b, double *x, int R, bool may_alloc) {
+  shellgpu_ctx *ctx = F->ctx;
+  const int nl = F->nlevels, M = F->M, nbf = (nl + 1) / 2, nE1 = F->rounds[0].nE;
+  const long M2 = (long)M * M, NV = F->nv, NVR = 2L * nbf * M, n = F->n;
+  const int nT = (M + kSymvTile - 1) / kSymvTile;
+  if (R > F->crm_cap) {
+    if (!may_alloc) return ferr(ctx, 1, "the factor's vector buffers would have to grow");
+    HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
+    void **bufs[] = {(void **)&F->crm_b, (void **)&F->crm_z, (void **)&F->crm_x, (void **)&F->crm_part,
+                     (void **)&F->d_crm_act};
+    for (void **p : bufs) {
+      if (*p) HIPCALL(ctx, hipFree(*p));
+      *p = nullptr;
+    }
+    for (double **p : {&F->crm_b, &F->crm_z, &F->crm_x})
+      HIPCALL(ctx, hipMalloc((void **)p, sizeof(double) * (size_t)NVR * R));
+    HIPCALL(ctx, hipMalloc((void **)&F->crm_part, sizeof(double) * (size_t)R * nbf * nT * nT * kSymvTile));
+    std::vector<int> act;
+    for (int r = 0; r < R; r++)
+      for (int k = 0; k < nE1; k++) act.push_back(r * nbf + k);
+    HIPCALL(ctx, hipMalloc((void **)&F->d_crm_act, sizeof(int) * act.size()));
+    HIPCALL(ctx, hipMemcpy(F->d_crm_act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice));
+    F->crm_cap = R;
+  }
+  double *vb = F->crm_b, *vz = F->crm_z, *vx = F->crm_x;
+  double *D0 = F->buf, *B0 = F->buf + (long)nl * M2, *Z0 = F->buf + 2L * nl * M2;
+  const double *E = F->buf + F->E_off;
+  const dim3 pg((unsigned)((NV + 255) / 256), R);
+  hipLaunchKernelGGL(permute_rhs_kernel, pg, dim3(256), 0, ctx->stream, NV, n, NVR, F->d_pos2dof, b, vb, 1);
+  HIPCALL(ctx, hipGetLastError());
+  // y = beta c + alpha P_i x over the odd levels and the R vectors
+  auto symv1 = [&](const double *xin, double alpha, const double *c, double beta, double *y) -> int {
+    const double *P = Z0 + M2;  // Z slot of level 1, odd levels 2 M2 apart
+    switch (R) {
+#define SYMV_CRM(RR)                                                                                               \
+  case RR:                                                                                                         \
+    hipLaunchKernelGGL((symv_multi_pass1_kernel<RR, 1>), dim3(nT * nT, nE1), dim3(256), 0, ctx->stream, M, nT, P,  \
+                       M, 2 * M2, xin, 2L * M, nbf, F->crm_part);                                                  \
+    break;
+      SYMV_CRM(2)
+      SYMV_CRM(3)
+      SYMV_CRM(4)
+      SYMV_CRM(5)
+      SYMV_CRM(6)
+      SYMV_CRM(7)
+      SYMV_CRM(8)
+      SYMV_CRM(16)
+#undef SYMV_CRM
+      default:
+        return ferr(ctx, 1, "unsupported number of vectors for the cyclic-reduction solve");
+    }
+    HIPCALL(ctx, hipGetLastError());
+    const SymvBatch sb{nbf, 0, 0, 0, 0, 2L * M, NVR};
+    const dim3 g2((M + 63) / 64, R * nE1);
+    if (nT >= 64)
+      hipLaunchKernelGGL(symv_pass2_kernel<16>, g2, dim3(1024), 0, ctx->stream, M, nT, nT, F->crm_part, alpha, c,
+                         beta, y, sb, F->d_crm_act);
+    else if (nT >= 32)
+      hipLaunchKernelGGL(symv_pass2_kernel<8>, g2, dim3(512), 0, ctx->stream, M, nT, nT, F->crm_part, alpha, c, beta,
+                         y, sb, F->d_crm_act);
+    else if (nT >= 16)
+      hipLaunchKernelGGL(symv_pass2_kernel<4>, g2, dim3(256), 0, ctx->stream, M, nT, nT, F->crm_part, alpha, c, beta,
+                         y, sb, F->d_crm_act);
+    else if (nT >= 8)
+      hipLaunchKernelGGL(symv_pass2_kernel<2>, g2, dim3(128), 0, ctx->stream, M, nT, nT, F->crm_part, alpha, c, beta,
+                         y, sb, F->d_crm_act);
+    else
+      hipLaunchKernelGGL(symv_pass2_kernel<1>, g2, dim3(64), 0, ctx->stream, M, nT, nT, F->crm_part, alpha, c, beta,
+                         y, sb, F->d_crm_act);
+    HIPCALL(ctx, hipGetLastError());
+    return 0;
+  };
+  const int gx = (F->maxm + 255) / 256;
+  auto gather = [&](int k0, const double *src, double *dst, double alpha, double beta) -> int {
+    const int nk = (nl - k0 + 1) / 2;
+    hipLaunchKernelGGL(cr_gather_kernel, dim3(gx, nk, R), dim3(256), 0, ctx->stream, k0, nl, M, F->d_noff,
+                       F->d_cp_rowp, F->d_cp_col, F->d_ct_rowp, F->d_ct_src, F->d_ct_blk, E, 0L, src, dst, NVR,
+                       alpha, beta);
+    HIPCALL(ctx, hipGetLastError());
+    return 0;
+  };
+  // C = beta C + alpha op(A) X over cnt levels (matrices sA apart, vector
+  // segments sv apart), X and C with R columns NVR apart; tri: triangle of
+  // op(A)
+  auto prod = [&](bool ta, const double *A, long sA, int tri, const double *X, double alpha, double beta, double *C,
+                  long sv, int cnt) -> int {
+    if (cnt <= 0) return 0;
+    GemmArgs g{M, R, M, alpha, beta, A, sA, nullptr, M, X, sv, nullptr, (int)NVR, C, sv, nullptr, (int)NVR};
+    g.tri_a = tri;
+    HIPCALL(ctx, launch_gemm_f64(ta, false, g, cnt, ctx->stream));
+    return 0;
+  };
+  if (NVR > 0x7fffffffL) return ferr(ctx, 1, "vectors too long for the cyclic-reduction multi-vector solve");
+  int rc = 0;
+  // t_i = P_i b_i
+  if ((rc = symv1(vb + M, 1.0, nullptr, 0.0, vx + M))) return rc;
+  if ((rc = gather(0, vx, vb, -1.0, 1.0))) return rc;
+  for (const FactorRound &r : F->rounds) {
+    const long s = r.s;
+    if (s == 1) continue;
+    // c_i = X_i b_i; b_l -= Y_i^T c_i (Y_i in the slot B_{i-1}); b_r -= W_i c_i
+    if ((rc = prod(false, D0 + s * M2, 2 * s * M2, kGemmLower, vb + s * M, 1.0, 0.0, vz + s * M, 2 * s * M, r.nE)))
+      return rc;
+    if ((rc = prod(true, B0 + (s - 1) * M2, 2 * s * M2, kGemmFull, vz + s * M, -1.0, 1.0, vb, 2 * s * M, r.nE)))
+      return rc;
+    if ((rc = prod(false, Z0 + s * M2, 2 * s * M2, kGemmFull, vz + s * M, -1.0, 1.0, vb + 2 * s * M, 2 * s * M,
+                   r.nR)))
+      return rc;
+  }
+  if ((rc = prod(false, D0, 0, kGemmLower, vb, 1.0, 0.0, vz, 0, 1))) return rc;
+  if ((rc = prod(true, D0, 0, kGemmUpper, vz, 1.0, 0.0, vx, 0, 1))) return rc;
+  for (auto it = F->rounds.rbegin(); it != F->rounds.rend(); ++it) {
+    const long s = it->s;
+    if (s == 1) continue;
+    // c_i -= Y_i x_l; c_i -= W_i^T x_r; x_i = X_i^T c_i
+    if ((rc = prod(false, B0 + (s - 1) * M2, 2 * s * M2, kGemmFull, vx, -1.0, 1.0, vz + s * M, 2 * s * M, it->nE)))
+      return rc;
+    if ((rc = prod(true, Z0 + s * M2, 2 * s * M2, kGemmFull, vx + 2 * s * M, -1.0, 1.0, vz + s * M, 2 * s * M,
+                   it->nR)))
+      return rc;
+    if ((rc = prod(true, D0 + s * M2, 2 * s * M2, kGemmUpper, vz + s * M, 1.0, 0.0, vx + s * M, 2 * s * M, it->nE)))
+      return rc;
+  }
+  if ((rc = gather(1, vx, vb, 1.0, 0.0))) return rc;
+  if ((rc = symv1(vb + M, -1.0, vx + M, 1.0, vx + M))) return rc;
+  hipLaunchKernelGGL(permute_rhs_kernel, pg, dim3(256), 0, ctx->stream, NV, n, NVR, F->d_pos2dof, vx, x, 0);
+  HIPCALL(ctx, hipGetLastError());
+  return 0;
+}
+
 }  // extern "C"
 
 namespace {
@@ -2050,6 +2212,7 @@ int apply_block(shellgpu_factor *F, const double *b, double *x, int nrhs, bool m
   if (act && F->nact == 0) return 0;
   const dim3 pgrid((unsigned)((nv + 255) / 256), act ? F->nact : nb);
   int rc = 0;
+  if (F->cr && nb == 1 && nrhs > 1 && (nrhs <= 8 || nrhs == kMaxRhs)) return apply_cr_multi(F, b, x, nrhs, may_alloc);
   if (F->cr || nrhs == 1) {
     for (int r = 0; r < nrhs; r++) {
       const double *br = b + (size_t)r * nb * n;

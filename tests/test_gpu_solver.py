@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 def set_factor(monkeypatch, factor):
     """cr: block cyclic reduction; thomas: the sequential level factorisation;
-    thomas-block: the latter with block Lanczos (blocks of 4 vectors)"""
+    -block: with block Lanczos (blocks of 4 vectors; cyclic reduction then
+    solves the 4 vectors of a block together)"""
     monkeypatch.setenv("SHELLGPU_FACTOR", factor.split("-")[0])
     if factor.endswith("-block"):
         monkeypatch.setenv("SHELLGPU_LANCZOS_BLOCK", "4")
@@ -66,7 +67,7 @@ def _subspace_angle(V, W, rowp, cols, Kvals):
     return float(np.arccos(min(1.0, s.min())))
 
 
-@pytest.mark.parametrize("factor", ["cr", "thomas", "thomas-block"])
+@pytest.mark.parametrize("factor", ["cr", "thomas", "thomas-block", "cr-block"])
 @pytest.mark.parametrize("na,nc", [(20, 40)])
 def test_buckling_matches_oracle(na, nc, factor, monkeypatch):
     set_factor(monkeypatch, factor)
