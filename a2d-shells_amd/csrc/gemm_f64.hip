@@ -181,6 +181,94 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
   }
 }
 
+// Products with at most 16 columns, op(B) = B (a block of vectors): C =
+// alpha op(A) B + beta C in 64 x 16 workgroup tiles with the four
+// wavefronts stacked along M (16 rows each, one v_mfma_f64_16x16x4f64
+// block per k-step), where the square tiles above would spend three
+// quarters of their matrix instructions on padding columns.  Such a product
+// is bound by the reads of A: the A tile streams through LDS 16 k-rows at a
+// time, double-buffered and prefetched as above, and triangular operands
+// skip their zero K range.
+template <bool TA>
+__global__ __launch_bounds__(256) void gemm_narrow_kernel(GemmArgs g) {
+  constexpr int kTM = 64, kRowA = kTM + 1, kRowB = 17;
+  __shared__ double As[2][kTK][kRowA];
+  __shared__ double Bs[2][kTK][kRowB];
+  const int tM = (g.M + kTM - 1) / kTM;
+  const long bz = blockIdx.x / tM;
+  const int tile_m = (int)(blockIdx.x - bz * tM);
+  gcdouble *A = (gcdouble *)(g.Ap ? g.Ap[bz] : g.A + bz * g.sA);
+  gcdouble *B = (gcdouble *)(g.Bp ? g.Bp[bz] : g.B + bz * g.sB);
+  gdouble *C = (gdouble *)(g.Cp ? g.Cp[bz] : g.C + bz * g.sC);
+  const int M = g.M, N = g.N, K = g.K, tra = g.tri_a;
+  const int m0 = tile_m * kTM;
+  int kbeg = 0, kend = K;
+  if (tra == kGemmLower) kend = min(kend, m0 + kTM);
+  if (tra == kGemmUpper) kbeg = max(kbeg, m0);
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  // staging, lanes along each operand's contiguous direction: A tile
+  // (64 m x 16 k, 4 per thread): N-op m = t % 64, k = t / 64 + 4 j; T-op
+  // k = t % 16, m = t / 16 + 16 j; B tile (16 k x 16 n): k = t % 16, n = t / 16
+  const int a_r = TA ? (t >> 4) : (t & 63), a_k = TA ? (t & 15) : (t >> 6);
+  const int b_k = t & 15, b_n = t >> 4;
+  double ra[4], rb;
+  unsigned keep = 0;
+  auto load = [&](int k0) {
+    keep = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int m = m0 + a_r + (TA ? 16 * j : 0), k = k0 + a_k + (TA ? 0 : 4 * j);
+      const bool ina = m < M && k < K && !(tra == kGemmLower && k > m) && !(tra == kGemmUpper && k < m);
+      const int mc = min(m, M - 1), kc = min(k, K - 1);
+      ra[j] = TA ? A[kc + (long)mc * g.lda] : A[mc + (long)kc * g.lda];
+      keep |= (ina ? 1u : 0u) << j;
+    }
+    const int kb = k0 + b_k;
+    rb = B[min(kb, K - 1) + (long)min(b_n, N - 1) * g.ldb];
+    keep |= (b_n < N && kb < K ? 1u : 0u) << 4;
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double va = (keep >> j) & 1u ? ra[j] : 0.0;
+      if (TA)
+        As[buf][a_k][a_r + 16 * j] = va;
+      else
+        As[buf][a_k + 4 * j][a_r] = va;
+    }
+    Bs[buf][b_k][b_n] = (keep >> 4) & 1u ? rb : 0.0;
+  };
+  v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+  const int nk = kbeg < kend ? (kend - kbeg + kTK - 1) / kTK : 0;
+  if (nk > 0) {
+    load(kbeg);
+    store(0);
+  }
+  __syncthreads();
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int it = 0; it < nk; it++) {
+    const int cur = it & 1;
+    load(kbeg + (it + 1) * kTK);
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[cur][kk + lk][lr], As[cur][kk + lk][16 * wave + lr], acc, 0, 0,
+                                                 0);
+    store(cur ^ 1);
+    __syncthreads();
+  }
+  // the transposed block: lane l holds C[16 wave + l % 16][4 e + l / 16]
+  const int m = m0 + 16 * wave + lr;
+  if (m >= M) return;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int n = 4 * e + lk;
+    if (n >= N) continue;
+    gdouble *c = C + m + (long)n * g.ldc;
+    const double v = g.alpha * acc[e];
+    *c = g.beta == 0.0 ? v : v + g.beta * *c;
+  }
+}
+
 template <int WT>
 hipError_t launch_tiles(bool ta, bool tb, const GemmArgs &g, long tiles, hipStream_t stream) {
   const dim3 grid((unsigned)tiles);
@@ -199,6 +287,15 @@ hipError_t launch_tiles(bool ta, bool tb, const GemmArgs &g, long tiles, hipStre
 
 hipError_t launch_gemm_f64(bool ta, bool tb, const GemmArgs &g, int batch, hipStream_t stream) {
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
+  if (g.N <= 16 && !tb && g.tri_b == kGemmFull && !g.lower_c) {
+    const long tiles = (long)((g.M + 63) / 64) * batch;
+    if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
+    if (ta)
+      hipLaunchKernelGGL(gemm_narrow_kernel<true>, dim3((unsigned)tiles), dim3(256), 0, stream, g);
+    else
+      hipLaunchKernelGGL(gemm_narrow_kernel<false>, dim3((unsigned)tiles), dim3(256), 0, stream, g);
+    return hipGetLastError();
+  }
   // 128-tiles while they give at least two workgroups per CU and both
   // dimensions exceed 64 (a 128-tile on a narrow product, such as a
   // product with 16 vectors, would compute mostly padding), else 64-tiles
