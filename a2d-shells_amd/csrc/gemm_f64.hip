@@ -27,6 +27,8 @@
 // i = 0..3.  The kernel computes the transposed block, X from B and Y from A
 // (D = (A B)^T), so lanes 0-15 of a quarter hold 16 consecutive rows of one
 // column of C and the stores coalesce.
+#include <algorithm>
+
 #include "shellgpu_internal.h"
 
 namespace shellgpu {
@@ -189,14 +191,19 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g) {
 // is bound by the reads of A: the A tile streams through LDS 16 k-rows at a
 // time, double-buffered and prefetched as above, and triangular operands
 // skip their zero K range.
+// With ksplit > 1 (a launch of few tiles) each tile's K range is cut into
+// ksplit chunks of whole k-steps, one workgroup each, whose unscaled sums go
+// to the workspace and are added in chunk order by gemm_narrow_reduce_kernel.
 template <bool TA>
-__global__ __launch_bounds__(256) void gemm_narrow_kernel(GemmArgs g) {
+__global__ __launch_bounds__(256) void gemm_narrow_kernel(GemmArgs g, int ksplit) {
   constexpr int kTM = 64, kRowA = kTM + 1, kRowB = 17;
   __shared__ double As[2][kTK][kRowA];
   __shared__ double Bs[2][kTK][kRowB];
   const int tM = (g.M + kTM - 1) / kTM;
-  const long bz = blockIdx.x / tM;
-  const int tile_m = (int)(blockIdx.x - bz * tM);
+  const long tix = blockIdx.x / ksplit;
+  const int ks = (int)(blockIdx.x - tix * ksplit);
+  const long bz = tix / tM;
+  const int tile_m = (int)(tix - bz * tM);
   gcdouble *A = (gcdouble *)(g.Ap ? g.Ap[bz] : g.A + bz * g.sA);
   gcdouble *B = (gcdouble *)(g.Bp ? g.Bp[bz] : g.B + bz * g.sB);
   gdouble *C = (gdouble *)(g.Cp ? g.Cp[bz] : g.C + bz * g.sC);
@@ -205,6 +212,12 @@ __global__ __launch_bounds__(256) void gemm_narrow_kernel(GemmArgs g) {
   int kbeg = 0, kend = K;
   if (tra == kGemmLower) kend = min(kend, m0 + kTM);
   if (tra == kGemmUpper) kbeg = max(kbeg, m0);
+  if (ksplit > 1 && kbeg < kend) {
+    const int steps = (kend - kbeg + kTK - 1) / kTK, per = (steps + ksplit - 1) / ksplit;
+    const int b0 = kbeg + ks * per * kTK;
+    kend = min(kend, b0 + per * kTK);
+    kbeg = b0;
+  }
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   // staging, lanes along each operand's contiguous direction: A tile
   // (64 m x 16 k, 4 per thread): N-op m = t % 64, k = t / 64 + 4 j; T-op
@@ -263,10 +276,29 @@ __global__ __launch_bounds__(256) void gemm_narrow_kernel(GemmArgs g) {
   for (int e = 0; e < 4; e++) {
     const int n = 4 * e + lk;
     if (n >= N) continue;
+    if (ksplit > 1) {
+      g.ws[((bz * ksplit + ks) * N + n) * (long)M + m] = acc[e];
+      continue;
+    }
     gdouble *c = C + m + (long)n * g.ldc;
     const double v = g.alpha * acc[e];
     *c = g.beta == 0.0 ? v : v + g.beta * *c;
   }
+}
+
+// C = beta C + alpha (sum of the ksplit K chunks, in chunk order)
+__global__ void gemm_narrow_reduce_kernel(GemmArgs g, int ksplit) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long MN = (long)g.M * g.N;
+  if (i >= MN) return;
+  const long bz = blockIdx.y;
+  const int n = (int)(i / g.M), m = (int)(i - (long)n * g.M);
+  double s = 0.0;
+  for (int ks = 0; ks < ksplit; ks++) s += g.ws[(bz * ksplit + ks) * MN + i];
+  double *C = g.Cp ? g.Cp[bz] : g.C + bz * g.sC;
+  double *c = C + m + (long)n * g.ldc;
+  const double v = g.alpha * s;
+  *c = g.beta == 0.0 ? v : v + g.beta * *c;
 }
 
 template <int WT>
@@ -289,11 +321,28 @@ hipError_t launch_gemm_f64(bool ta, bool tb, const GemmArgs &g, int batch, hipSt
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
   if (g.N <= 16 && !tb && g.tri_b == kGemmFull && !g.lower_c) {
     const long tiles = (long)((g.M + 63) / 64) * batch;
-    if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
+    // split K when the tiles would not fill the GPU with workgroups (each
+    // then runs the whole K chain): about 2,048 workgroups, chunks of at
+    // least 4 k-steps, within the workspace
+    int ksplit = 1;
+    if (g.ws && tiles < 1024) {
+      const long steps = (g.K + kTK - 1) / kTK;
+      ksplit = (int)std::min<long>((2048 + tiles - 1) / tiles, std::max<long>(1, steps / 4));
+      const long per = (long)g.M * g.N * batch;
+      if (per > 0) ksplit = (int)std::min<long>(ksplit, g.ws_len / per);
+      if (ksplit < 2) ksplit = 1;
+    }
+    if (tiles * ksplit > 0x7fffffffL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(tiles * ksplit));
     if (ta)
-      hipLaunchKernelGGL(gemm_narrow_kernel<true>, dim3((unsigned)tiles), dim3(256), 0, stream, g);
+      hipLaunchKernelGGL(gemm_narrow_kernel<true>, grid, dim3(256), 0, stream, g, ksplit);
     else
-      hipLaunchKernelGGL(gemm_narrow_kernel<false>, dim3((unsigned)tiles), dim3(256), 0, stream, g);
+      hipLaunchKernelGGL(gemm_narrow_kernel<false>, grid, dim3(256), 0, stream, g, ksplit);
+    if (ksplit > 1) {
+      const long MN = (long)g.M * g.N;
+      hipLaunchKernelGGL(gemm_narrow_reduce_kernel, dim3((unsigned)((MN + 255) / 256), batch), dim3(256), 0, stream,
+                         g, ksplit);
+    }
     return hipGetLastError();
   }
   // 128-tiles while they give at least two workgroups per CU and both

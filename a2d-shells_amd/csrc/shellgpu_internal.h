@@ -183,6 +183,11 @@ struct GemmArgs {
   double *const *Cp;
   int ldc;
   int tri_a = kGemmFull, tri_b = kGemmFull, lower_c = 0;
+  // optional workspace (ws_len doubles) for products with at most 16
+  // columns: a launch of few tiles splits K over workgroups into it and
+  // adds the parts in a fixed order
+  double *ws = nullptr;
+  long ws_len = 0;
 };
 hipError_t launch_gemm_f64(bool transA, bool transB, const GemmArgs &g, int batch, hipStream_t stream);
 

@@ -2114,6 +2114,9 @@ static int apply_cr_multi(shellgpu_factor *F, const double *b, double *x, int R,
     if (cnt <= 0) return 0;
     GemmArgs g{M, R, M, alpha, beta, A, sA, nullptr, M, X, sv, nullptr, (int)NVR, C, sv, nullptr, (int)NVR};
     g.tri_a = tri;
+    // the symmetric products' partial sums are free between them
+    g.ws = F->crm_part;
+    g.ws_len = (long)R * nbf * nT * nT * kSymvTile;
     HIPCALL(ctx, launch_gemm_f64(ta, false, g, cnt, ctx->stream));
     return 0;
   };
