@@ -3022,14 +3022,15 @@ int lanczos_block(shellgpu_ctx *ctx, shellgpu_factor *F, shellgpu_mat *K, shellg
 }
 
 // Vectors per Lanczos block: 1 (the single-vector recurrence) unless
-// SHELLGPU_LANCZOS_BLOCK=b (2 ... 8) asks for blocks, which only the
-// sequential factorisation solves together (one pass over its level
-// inverses per block).  Measured on 64 designs of the 100 x 200 cylinder,
-// blocks of 4 read the level inverses 2.5 times less per vector but need a
-// basis as large as the single-vector run's, and the block Gram-Schmidt
-// products (tall, skinny, transposed GEMMs) cost more than the saving.
+// SHELLGPU_LANCZOS_BLOCK=b (2 ... 8) asks for blocks, which the sequential
+// factorisation and a cyclic-reduction factor of one design solve together
+// (one pass over the factor per block).  Measured on 64 designs of the
+// 100 x 200 cylinder, blocks of 4 read the level inverses 2.5 times less
+// per vector but need a basis as large as the single-vector run's, and the
+// block Gram-Schmidt products (tall, skinny, transposed GEMMs) cost more
+// than the saving.
 int lanczos_block_size(const shellgpu_factor *F) {
-  if (F->cr || F->nbatch != F->ctx->nbatch) return 1;
+  if (F->nbatch != F->ctx->nbatch || (F->cr && F->nbatch != 1)) return 1;
   int bs = 1;
   if (const char *env = getenv("SHELLGPU_LANCZOS_BLOCK")) bs = atoi(env);
   return std::max(1, std::min(8, bs));
