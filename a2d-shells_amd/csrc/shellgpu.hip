@@ -165,6 +165,47 @@ __global__ __launch_bounds__(192) void bsr_spmv6_kernel(int nrows, const int *__
   y[6L * row + r] = acc;
 }
 
+// y_v = A x_v for the vectors of every design when all designs' copies of A
+// hold the same values (load cases of one structure): one lane per scalar
+// row reads the copy of design 0 once for kV vectors (grid.y counts groups
+// of kV), against once per vector above.  x and y are 16-byte aligned, the
+// sums run in block order and each has the expression of the kernel above,
+// so y agrees with it bit for bit.
+template <int kV>
+__global__ __launch_bounds__(192) void bsr_spmv6_same_kernel(int nrows, const int *__restrict__ rowp,
+                                                             const int *__restrict__ cols,
+                                                             const double *__restrict__ vals, int nv,
+                                                             const double *__restrict__ x,
+                                                             double *__restrict__ y, long xstride) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(tid / 6), r = (int)(tid % 6);
+  if (row >= nrows) return;
+  const int v0 = blockIdx.y * kV;
+  const int nvl = min(kV, nv - v0);
+  x += (long)v0 * xstride;
+  y += (long)v0 * xstride;
+  double acc[kV];
+#pragma unroll
+  for (int v = 0; v < kV; v++) acc[v] = 0.0;
+  const int p1 = rowp[row + 1];
+  for (int p = rowp[row]; p < p1; p++) {
+    const double2 *ap = reinterpret_cast<const double2 *>(vals + 36L * p + 6 * r);
+    const double2 a0 = ap[0], a1 = ap[1], a2 = ap[2];
+    const long c = 6L * cols[p];
+#pragma unroll
+    for (int v = 0; v < kV; v++) {
+      if (v < nvl) {
+        const double2 *xp = reinterpret_cast<const double2 *>(x + v * xstride + c);
+        const double2 x0 = xp[0], x1 = xp[1], x2 = xp[2];
+        acc[v] += a0.x * x0.x + a0.y * x0.y + a1.x * x1.x + a1.y * x1.y + a2.x * x2.x + a2.y * x2.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < kV; v++)
+    if (v < nvl) y[v * xstride + 6L * row + r] = acc[v];
+}
+
 // y = A^T x, 6x6 blocks: row j of A^T gathers block (i, j) of A, transposed,
 // for every block (j, i) of row j (the pattern is structurally symmetric)
 __global__ __launch_bounds__(192) void bsr_spmv6_t_kernel(int nrows, const int *__restrict__ rowp,
@@ -660,6 +701,7 @@ int shellgpu_mat_create(shellgpu_ctx *ctx, int nbatch, shellgpu_mat **A) {
     delete m;
     return set_err(ctx, 2, "hipMemset matrix");
   }
+  m->same = true;
   *A = m;
   return 0;
 }
@@ -671,7 +713,11 @@ void shellgpu_mat_destroy(shellgpu_mat *A) {
   delete A;
 }
 
-double *shellgpu_mat_values(shellgpu_mat *A) { return A ? A->vals : nullptr; }
+double *shellgpu_mat_values(shellgpu_mat *A) {
+  if (!A) return nullptr;
+  A->same = false;  // the caller may write design by design
+  return A->vals;
+}
 
 int shellgpu_mat_get_values(shellgpu_mat *A, double *h) {
   shellgpu_ctx *ctx = A->ctx;
@@ -683,6 +729,7 @@ int shellgpu_mat_get_values(shellgpu_mat *A, double *h) {
 
 int shellgpu_mat_set_values(shellgpu_mat *A, const double *h) {
   shellgpu_ctx *ctx = A->ctx;
+  A->same = false;
   HIPCALL(ctx, hipMemcpyAsync(A->vals, h, sizeof(double) * 36L * ctx->nnzb * A->nbatch, hipMemcpyHostToDevice,
                               ctx->stream));
   HIPCALL(ctx, hipStreamSynchronize(ctx->stream));
@@ -692,6 +739,7 @@ int shellgpu_mat_set_values(shellgpu_mat *A, const double *h) {
 int shellgpu_mat_zero(shellgpu_mat *A) {
   shellgpu_ctx *ctx = A->ctx;
   HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  A->same = true;
   return 0;
 }
 
@@ -700,6 +748,7 @@ int shellgpu_mat_copy(shellgpu_mat *dst, const shellgpu_mat *src) {
   if (dst->nbatch != src->nbatch || dst->ctx != src->ctx) return set_err(ctx, 1, "matrix shapes differ");
   HIPCALL(ctx, hipMemcpyAsync(dst->vals, src->vals, sizeof(double) * 36L * ctx->nnzb * src->nbatch,
                               hipMemcpyDeviceToDevice, ctx->stream));
+  dst->same = src->same;
   return 0;
 }
 
@@ -709,6 +758,7 @@ int shellgpu_mat_axpy(shellgpu_mat *Y, double alpha, const shellgpu_mat *X) {
   const long n = 36L * ctx->nnzb * Y->nbatch;
   hipLaunchKernelGGL(axpy_kernel, dim3(launch_grid(n)), dim3(256), 0, ctx->stream, n, alpha, X->vals, Y->vals);
   HIPCALL(ctx, hipGetLastError());
+  Y->same = Y->same && X->same;
   return 0;
 }
 
@@ -776,6 +826,7 @@ int shellgpu_mat_add_blocks(shellgpu_mat *A, int design, int nblk, const int *br
     if (e == hipSuccess) e = hipMemcpyAsync(dp, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(db, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
+      A->same = false;
       hipLaunchKernelGGL(add_blocks_kernel, dim3((unsigned)((36L * nseg + 255) / 256)), dim3(256), 0, ctx->stream,
                          nseg, dp, db, dv, A->vals + (long)design * 36L * ctx->nnzb);
       e = hipGetLastError();
@@ -819,7 +870,12 @@ int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y) {
   int rc = record_begin(ctx, ctx->spmv_ev);
   if (rc) return rc;
   // 16-byte loads need x on a 16-byte boundary (the matrix values are)
-  if ((reinterpret_cast<uintptr_t>(x) & 15) == 0)
+  const bool aligned = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+  if (aligned && A->same && A->nbatch > 1) {
+    constexpr int kV = 8;
+    hipLaunchKernelGGL(bsr_spmv6_same_kernel<kV>, dim3(blocks, (A->nbatch + kV - 1) / kV), dim3(192), 0, ctx->stream,
+                       ctx->nnodes, ctx->d_rowp, ctx->d_cols, A->vals, A->nbatch, x, y, 6L * ctx->nnodes);
+  } else if (aligned)
     hipLaunchKernelGGL(bsr_spmv6_kernel<true>, dim3(blocks, A->nbatch), dim3(192), 0, ctx->stream, ctx->nnodes,
                        ctx->d_rowp, ctx->d_cols, A->vals, 36L * ctx->nnzb, x, y, 6L * ctx->nnodes);
   else
@@ -845,6 +901,9 @@ static int assemble_type(shellgpu_ctx *ctx, int mattype, const double *u, shellg
   a.first = (accumulate || mattype == SHELLGPU_MASS) ? nullptr : ctx->d_first;
   if (!accumulate && !a.first)
     HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  // identical designs give identical copies unless a per-design state enters
+  const bool state_free = !u || mattype == SHELLGPU_STIFFNESS || mattype == SHELLGPU_MASS;
+  A->same = (accumulate ? A->same : true) && ctx->same_designs && state_free;
   if (mattype == SHELLGPU_STIFFNESS) {
     if ((rc = run_colored(ctx, kModeKLin, a))) return rc;
   } else if (mattype == SHELLGPU_GEOMETRIC) {
@@ -876,6 +935,7 @@ int shellgpu_assemble_mat_combo(shellgpu_ctx *ctx, int nmats, const int *mattype
   if (rc) return rc;
   if (nmats < 0 || (nmats > 0 && (!mattypes || !scale))) return set_err(ctx, 1, "invalid matrix combination");
   HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
+  A->same = true;
   for (int k = 0; k < nmats; k++)
     if ((rc = assemble_type(ctx, mattypes[k], u, A, scale[k], true))) return rc;
   return shellgpu_mat_apply_bcs(A, 1.0);
@@ -951,6 +1011,8 @@ int shellgpu_assemble_jacobian(shellgpu_ctx *ctx, int model, double alpha, doubl
   if (!a.first)
     HIPCALL(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * 36L * ctx->nnzb * A->nbatch, ctx->stream));
   const bool nonlin = model == SHELLGPU_NONLINEAR;
+  // u enters the matrix only through the nonlinear tangent; u'' only R
+  A->same = ctx->same_designs && (!nonlin || !u);
   if (!R && gamma == 0.0) {
     // alpha K (or alpha K_t) alone
     if ((rc = run_colored(ctx, nonlin ? kModeKNl : kModeKLin, a))) return rc;

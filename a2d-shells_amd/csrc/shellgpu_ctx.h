@@ -64,6 +64,10 @@ struct shellgpu_mat {
   shellgpu_ctx *ctx = nullptr;
   int nbatch = 0;
   double *vals = nullptr;
+  // every design's values are equal (zeroed, or assembled from identical
+  // designs without a per-design state, and changed only uniformly since):
+  // a product then reads one copy for all designs' vectors
+  bool same = false;
 };
 
 namespace shellgpu {

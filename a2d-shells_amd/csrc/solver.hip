@@ -1554,7 +1554,7 @@ int shellgpu_factor_compute(shellgpu_factor *F, shellgpu_mat *A) {
   {
     const long nthreads = 36L * ctx->nnzb;
     hipLaunchKernelGGL(scatter_dense_kernel, dim3((unsigned)((nthreads + 255) / 256), nb), dim3(256), 0, ctx->stream,
-                       ctx->nnzb, shellgpu_mat_values(A), 36L * ctx->nnzb, F->d_blk_base, F->d_blk_ld, F->d_bcflag,
+                       ctx->nnzb, A->vals, 36L * ctx->nnzb, F->d_blk_base, F->d_blk_ld, F->d_bcflag,
                        F->buf, ps);
     HIPCALL(ctx, hipGetLastError());
   }
@@ -3355,7 +3355,8 @@ int shellgpu_buckling_solve(shellgpu_buckling *B, const double *force, const dou
       {
         const long nv = 36L * ctx->nnzb;
         hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
-                           shellgpu_mat_values(B->K), shellgpu_mat_values(B->G), B->dsig, shellgpu_mat_values(B->A));
+                           B->K->vals, B->G->vals, B->dsig, B->A->vals);
+        B->A->same = false;
         HIPCALL(ctx, hipGetLastError());
       }
       if ((rc = shellgpu_mat_apply_bcs(B->A, 1.0))) return rc;
@@ -3651,7 +3652,8 @@ int shellgpu_frequency_solve(shellgpu_frequency *F) {
     {
       const long nv = 36L * ctx->nnzb;
       hipLaunchKernelGGL(shifted_kernel, dim3((unsigned)((nv + 255) / 256), nb), dim3(256), 0, ctx->stream, nv,
-                         shellgpu_mat_values(F->K), shellgpu_mat_values(F->M), F->dsig, shellgpu_mat_values(F->A));
+                         F->K->vals, F->M->vals, F->dsig, F->A->vals);
+      F->A->same = false;
       HIPCALL(ctx, hipGetLastError());
     }
     HIPCALL(ctx, hipStreamSynchronize(ctx->stream));  // ms is released next iteration

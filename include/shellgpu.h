@@ -94,6 +94,8 @@ int shellgpu_get_colors(shellgpu_ctx *ctx, int *elem_color);
  * nbatch*nnzb*36 doubles, 6x6 blocks row-major (BCSRMat, BCSRMatImpl.h) */
 int shellgpu_mat_create(shellgpu_ctx *ctx, int nbatch, shellgpu_mat **A);
 void shellgpu_mat_destroy(shellgpu_mat *A);
+/* device pointer to the values; the caller may write through it, so the
+ * matrix no longer counts as holding equal copies for all designs (below) */
 double *shellgpu_mat_values(shellgpu_mat *A);
 int shellgpu_mat_get_values(shellgpu_mat *A, double *host_vals);
 int shellgpu_mat_set_values(shellgpu_mat *A, const double *host_vals);
@@ -105,7 +107,12 @@ int shellgpu_mat_axpy(shellgpu_mat *Y, double alpha, const shellgpu_mat *X);
  * rows zeroed, diagonal of masked dofs set to diag (1.0 in the reference) */
 int shellgpu_mat_apply_bcs(shellgpu_mat *A, double diag);
 /* TACSMat::mult -> BCSRMatVecMult6 (BCSRMatMult6.cpp:82-125): y = A x for
- * every matrix of the batch; x, y device, nbatch*6*nnodes */
+ * every matrix of the batch; x, y device, nbatch*6*nnodes.  A matrix whose
+ * copies are known equal (zeroed, or assembled from identical designs with
+ * no per-design state: K, M, or any type at u = null; changed since only by
+ * the uniform operations scale / axpy of such a matrix / add_diag /
+ * apply_bcs / copy) is read once for all designs' vectors, with the same
+ * result bit for bit */
 int shellgpu_mat_mult(shellgpu_mat *A, const double *x, double *y);
 /* TACSMat::multTranspose (src/bpmat/KSM.h:176): y = A^T x for every
  * matrix of the batch (the node pattern is structurally symmetric) */

@@ -272,3 +272,55 @@ def test_inner_products_with_psi_not_phi():
         w = rng.standard_normal(6 * nn)
         ref = pGp(w) - base
         assert abs(gu @ w - ref) <= 1e-9 * max(abs(ref), 1e-30), k
+
+
+def test_mult_identical_designs():
+    """Load cases of one structure: products with K, M and the linear-model
+    Jacobian read one copy for every design's vector; the result equals the
+    per-design product bit for bit, over a group of eight vectors and a
+    ragged remainder, and a per-design change (addValues) or a state-
+    dependent type (G(u), the nonlinear tangent) turns the shared read off."""
+    conn, X = random_mesh(5, 4, seed=71)
+    nn = X.shape[0]
+    rng = np.random.default_rng(72)
+    nb = 11
+    thick = np.tile(rng.uniform(0.0015, 0.0025, conn.shape[0]), (nb, 1))
+    with _ctx(conn, X, thick, _bcs()) as ctx:
+        A, P = ctx.matrix(), ctx.matrix()
+        x = rng.standard_normal((nb, 6 * nn))
+        u = 1e-3 * rng.standard_normal((nb, 6 * nn))
+        dx, dy, dz, du = ctx.vector(x), ctx.vector(), ctx.vector(), ctx.vector(u)
+        R = ctx.vector()
+
+        def check(exact=True):
+            A.mult(dx, dy)
+            P.set_values(A.values())  # the same values, read per design
+            P.mult(dx, dz)
+            y, z = dy.download(), dz.download()
+            if exact:
+                assert np.array_equal(y, z)
+            return y.reshape(nb, -1), A.values()
+
+        for mattype in (sg.STIFFNESS, sg.MASS):
+            ctx.assemble_mattype(mattype, A, du)
+            check()
+        ctx.assemble_mattype(sg.GEOMETRIC, A, du)
+        check()
+        ctx.assemble_mattype(sg.TANGENT_NONLINEAR, A, du)
+        check()
+        ctx.assemble_jacobian(A, R, du, model=sg.LINEAR, alpha=1.5, gamma=-0.25)
+        check()
+        ctx.assemble_mat_combo(A, [sg.STIFFNESS, sg.MASS], [1.0, 2.0], du)
+        check()
+        A.scale(-2.0)
+        A.add_diag(3.0)
+        check()
+        ctx.assemble_mattype(sg.STIFFNESS, A)
+        y0, _ = check()
+        rowp, cols = ctx.pattern()
+        A.add_blocks([0], [0], np.ones((1, 6, 6)), design=4)
+        y1, v1 = check()
+        for d in (0, 4):
+            D = O.bsr_to_dense(rowp, cols, v1[d])
+            assert np.abs(y1[d] - D @ x[d]).max() <= 1e-13 * np.abs(D).sum(axis=1).max() * np.abs(x).max()
+        assert np.array_equal(y1[0], y0[0]) and not np.array_equal(y1[4], y0[4])
