@@ -25,10 +25,13 @@
 // with B1 = dB/d(eps) along u and sig(u) = w detXd C (B0 u - e_thermal).
 //
 // Work split: each half-wavefront (32 lanes) owns one element and its LDS.
-// Geometry per node / quadrature point / tying point runs on lanes 0-16,
-// strain-Jacobian columns on lanes 0-23 (B1 after B), and the 24 x 24 product
-// as one 3 x 6 register tile per lane.  Packing two elements per wavefront
-// fills the lanes that the per-element geometry phases leave idle.  Elements
+// Strain-Jacobian columns run on lanes 0-23 (B1 after B) and the 24 x 24
+// product as one 3 x 6 register tile per lane.  The geometry per node /
+// quadrature point / tying point (4 to 9 tasks per element) is spread over
+// the workgroup by (element, point): with two elements per workgroup (one
+// wavefront) it fills lanes the per-element phases would leave idle; the
+// wide schedules (8 elements, 4 wavefronts) give one wavefront the frames
+// of all 8 elements while two others compute their tying points.  Elements
 // of one colour share no node, so each half adds its 16 blocks into the BSR
 // values without atomics.
 #include "elem_common.h"
@@ -322,25 +325,53 @@ __device__ __forceinline__ double mass_entry(const MassTile &m, int i, int j) {
   return (j < 3) ? 0.0 : m.m2 * m.S * ((j - 3 == i ? m.tt : 0.0) - m.tj[i] * m.ti[j - 3]);
 }
 
-// Two elements per wavefront: lanes 0-31 work on element 2 blockIdx.x of the
-// launch list, lanes 32-63 on the next one, each half with its own LDS.  An
-// odd list leaves the last upper half computing a copy of the lower half's
-// element with its writes suppressed.
+// EPW elements per workgroup: lanes 32 h .. 32 h + 31 work on element
+// EPW blockIdx.x + h of the launch list, each with its own LDS.  Slots past
+// the end of the list compute a copy of its last element with their writes
+// suppressed.
 //
 // VAR selects a compiled schedule of the same arithmetic (chosen per mode
 // and mesh by shellgpu_tune): bit 0 unrolls the quadrature loop of the
 // matrix tile (the shape values become constants, the code 4 times longer).
-template <int MODE, int VAR>
+// Synchronisation of one element's 32 lanes once the geometry phases are
+// done (those share data across elements and take the workgroup barrier).
+// With one wavefront per workgroup: the workgroup barrier.  With wide
+// workgroups the 32 lanes are half of one wavefront, whose LDS operations
+// complete in issue order: waiting for its own LDS operations and keeping
+// the compiler from moving memory operations across suffices, and the
+// workgroup's wavefronts then run their quadrature loops independently.
+template <int EPW>
+__device__ __forceinline__ void elem_sync() {
+  if constexpr (EPW == 2) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+template <int MODE, int VAR, int EPW>
 __device__ __forceinline__ void elem_body(const ElemArgs &a) {
   using MT = ModeTraits<MODE>;
   constexpr bool kTwo = MT::kTwo;
   constexpr int kNB = ElemSmem<MODE>::kNB;
-  __shared__ ElemSmem<MODE> smem[2];
+  static_assert(EPW == 2 || EPW == 4 || EPW == 8, "2, 4 or 8 elements per workgroup");
+  __shared__ ElemSmem<MODE> smem[EPW];
   const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
   ElemSmem<MODE> &sm = smem[half];
-  const int slot = 2 * blockIdx.x + half;
+  const int slot = EPW * blockIdx.x + half;
   const bool valid = slot < a.count;
-  const int e = a.elems[valid ? slot : slot - 1];
+  const int e = a.elems[valid ? slot : a.count - 1];
+  // The geometry phases (P1-P2) run per node, quadrature point or tying
+  // point: their tasks go to the workgroup's threads by (element, point),
+  // not by each element's own 32 lanes, so with 8 elements per workgroup
+  // one wavefront computes the frames of all 8 while two others compute the
+  // tying points.  Task classes of one phase start at thread 0 and kG2.
+  const int gt = threadIdx.x;
+  constexpr int kG2 = EPW >= 4 ? 64 : 16;
+  static_assert(kG2 >= 8 * EPW && kG2 + 9 * EPW <= 32 * EPW, "geometry task classes overlap");
   const long design = blockIdx.y;
   // output modes: the element's 4 x 31 output values
   [[maybe_unused]] double *obase = a.out ? a.out + (design * a.nelems + e) * 124 : nullptr;
@@ -385,8 +416,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
 
   // ---- P1: node normals; tying-point X,xi ----
   // (TacsShellComputeNodeNormals, TACSShellUtilities.h:301-343)
-  if (lane < 4) {
-    const int n = lane;
+  auto p1_normal = [&](ElemSmem<MODE> &sm, int n) {
     double u, v, N[4], Na[4], Nb[4];
     node_point(n, u, v);
     shape4(u, v, N, Na, Nb);
@@ -411,8 +441,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
     sm.fn[3 * n] = f[0];
     sm.fn[3 * n + 1] = f[1];
     sm.fn[3 * n + 2] = f[2];
-  } else if (lane >= 8 && lane < 17) {
-    const int t = lane - 8;
+  };
+  auto p1_tying = [&](ElemSmem<MODE> &sm, int t) {
     double u, v, N[4], Na[4], Nb[4];
     tying_point(t, u, v);
     shape4(u, v, N, Na, Nb);
@@ -426,17 +456,21 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
       }
 #pragma unroll
     for (int i = 0; i < 6; i++) sm.f.tXxi[6 * t + i] = Xxi[i];
-  }
+  };
+  if (gt < 4 * EPW)
+    p1_normal(smem[gt >> 2], gt & 3);
+  else if (gt >= kG2 && gt < kG2 + 9 * EPW)
+    p1_tying(smem[(gt - kG2) / 9], (gt - kG2) % 9);
   __syncthreads();
 
-  // ---- P1b: frames at the nodes (lanes 0-3: the drilling-strain frames) and
-  // at the evaluation points (lanes 4-7: the quadrature points, or the nodes
+  // ---- P1b: frames at the nodes (points 0-3: the drilling-strain frames)
+  // and at the evaluation points (4-7: the quadrature points, or the nodes
   // in the output modes) in one pass of the same code; quadrature-point
   // kinematics (TacsShellComputeDispGrad, TACSShellUtilities.h:361-422) and
   // tying weights; tying-point state and derivative coefficients ----
-  if (lane < 8) {
-    const bool at_node = lane < 4;
-    const int iq = lane & 3;
+  auto p1b_frame = [&](ElemSmem<MODE> &sm, int p) {
+    const bool at_node = p < 4;
+    const int iq = p & 3;
     double u, v, N[4], Na[4], Nb[4];
     if (at_node || MT::kOut)
       node_point(iq, u, v);
@@ -539,8 +573,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
         }
       }
     }
-  } else if (lane >= 8 && lane < 17) {
-    const int t = lane - 8;
+  };
+  auto p1b_tying = [&](ElemSmem<MODE> &sm, int t) {
     double u, v, N[4], Na[4], Nb[4];
     tying_point(t, u, v);
     shape4(u, v, N, Na, Nb);
@@ -628,14 +662,17 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
 #pragma unroll
       for (int i = 0; i < 3; i++) cd[3 * t + i] = dd[i];
     }
-  }
+  };
+  if (gt < 8 * EPW)
+    p1b_frame(smem[gt >> 3], gt & 7);
+  else if (gt >= kG2 && gt < kG2 + 9 * EPW)
+    p1b_tying(smem[(gt - kG2) / 9], (gt - kG2) % 9);
   __syncthreads();
 
   // ---- P2: drilling rows at the nodes ----
-  if (lane < 4) {
+  auto p2_drill = [&](ElemSmem<MODE> &sm, int n) {
     // TacsShellAddDrillStrainSens map (TACSShellUtilities.h:787-835) with
     // TACSLinearizedRotation: et = 0.5 (Ct[3] + u0x[3] - Ct[1] - u0x[1])
-    const int n = lane;
     const double *Tn = &sm.f.Tn[9 * n], *XTn = &sm.f.XTn[9 * n];
     const double dA[9] = {0, -0.5, 0, 0.5, 0, 0, 0, 0, 0};
     double tmp[9], dC[9], du0d[9];
@@ -673,7 +710,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
         et += val * sm.s[6 * m + d];
       }
     sm.etn[n] = et;
-  }
+  };
+  if (gt < 4 * EPW) p2_drill(smem[gt >> 2], gt & 3);
   __syncthreads();
 
   // ---- P3: tying-strain Jacobian rows; strains and stresses at the
@@ -792,7 +830,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
       }
     }
   }
-  __syncthreads();
+  elem_sync<EPW>();
 
   if (MT::kOut) return;
 
@@ -834,7 +872,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
 #pragma unroll
         for (int i = 0; i < 9; i++) sm.c.B[1][24 * i + lane] = b[i];
       }
-      __syncthreads();
+      elem_sync<EPW>();
       // E = [e1(phi), e1(u), Q(phi, u), Q(phi, phi)]
       for (int it = lane; it < 36; it += 32) {
         const int v = it / 9, i = it % 9;
@@ -845,7 +883,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
         for (int k = 0; k < 24; k++) acc += M[k] * x[k];
         E[it] = acc;
       }
-      __syncthreads();
+      elem_sync<EPW>();
       if (lane == 0) {
         const double w = sm.wdet[iq];
         const double *e1p = &E[0], *e1u = &E[9], *e2up = &E[18], *e2pp = &E[27];
@@ -871,13 +909,13 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
           cv[9 + i] = cb * c2[i];
         }
       }
-      __syncthreads();
+      elem_sync<EPW>();
       // d/du: Bq(phi)^T (2 w C e1(phi)) + B0^T (w C Q(phi, phi))
       if (lane < 24) {
 #pragma unroll
         for (int i = 0; i < 9; i++) gu += sm.c.B[1][24 * i + lane] * cv[i] + sm.c.B[0][24 * i + lane] * cv[9 + i];
       }
-      __syncthreads();
+      elem_sync<EPW>();
     }
     if (valid && lane == 0 && a.dv) a.dv[design * a.nelems + e] += gdv;
     if (valid && lane < 24 && a.res) a.res[design * a.res_stride + 6 * cn[lane / 6] + lane % 6] += gu;
@@ -901,14 +939,14 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
 #pragma unroll
         for (int i = 0; i < 9; i++) sm.c.B[0][24 * i + lane] = b[i];
       }
-      __syncthreads();
+      elem_sync<EPW>();
       if (lane < 9) {  // dR/dq^T psi strain: B(u) psi
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < 24; k++) acc += sm.c.B[0][24 * lane + k] * sp[k];
         E[lane] = acc;
       }
-      __syncthreads();
+      elem_sync<EPW>();
       if (lane == 0) {
         double t2[9];
         apply_c(dsec, sm.wdet[iq], &sm.sig[9 * iq], t2);  // sig holds e(u) - e_th
@@ -917,7 +955,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
         for (int i = 0; i < 9; i++) d += E[i] * t2[i];
         g += d;
       }
-      __syncthreads();
+      elem_sync<EPW>();
     }
     if (valid && lane == 0) a.dv[design * a.nelems + e] += a.coef[design] * g;
     return;
@@ -987,7 +1025,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
       }
     }
     if (kMat) {
-      __syncthreads();
+      elem_sync<EPW>();
 #pragma unroll
       for (int s = 0; s < 9; s++) {
         double b[3], y[6];
@@ -1016,7 +1054,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
             for (int j = 0; j < 6; j++) acc[6 * i + j] += b[i] * y[j];
         }
       }
-      __syncthreads();
+      elem_sync<EPW>();
     }
   }
 
@@ -1149,17 +1187,17 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
   }
 }
 
-template <int MODE, int VAR>
-__global__ __launch_bounds__(64) void elem_kernel(ElemArgs a) {
-  elem_body<MODE, VAR>(a);
+template <int MODE, int VAR, int EPW>
+__global__ __launch_bounds__(32 * EPW) void elem_kernel(ElemArgs a) {
+  elem_body<MODE, VAR, EPW>(a);
 }
 
 // the same body within the register budget of two wavefronts per SIMD (the
 // compiler spills to scratch memory instead of holding one wavefront with
 // accumulation registers)
-template <int MODE, int VAR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void elem_kernel_w2(ElemArgs a) {
-  elem_body<MODE, VAR>(a);
+template <int MODE, int VAR, int EPW>
+__global__ __launch_bounds__(32 * EPW) __attribute__((amdgpu_waves_per_eu(2))) void elem_kernel_w2(ElemArgs a) {
+  elem_body<MODE, VAR, EPW>(a);
 }
 
 // ---- mass matrix ----
@@ -1290,31 +1328,49 @@ static hipError_t launch_mode(const ElemArgs &args, int count, int nbatch, hipSt
   if (count <= 0 || nbatch <= 0) return hipSuccess;
   ElemArgs a = args;
   a.count = count;
-  hipLaunchKernelGGL((elem_kernel<MODE, 0>), dim3((count + 1) / 2, nbatch), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL((elem_kernel<MODE, 0, 2>), dim3((count + 1) / 2, nbatch), dim3(64), 0, stream, a);
   return hipGetLastError();
+}
+
+// elements per workgroup of the wide schedules: 8 while their LDS stays
+// within 52 KB (two workgroups per CU), else 4
+template <int MODE>
+constexpr int wide_epw() {
+  return sizeof(ElemSmem<MODE>) * 8 <= 52 * 1024 ? 8 : 4;
+}
+
+template <int MODE, int EPW>
+static void launch_var(int variant, const dim3 &grid, hipStream_t stream, const ElemArgs &a) {
+  const dim3 block(32 * EPW);
+  switch (variant & 3) {
+    case 1:
+      hipLaunchKernelGGL((elem_kernel<MODE, 1, EPW>), grid, block, 0, stream, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((elem_kernel_w2<MODE, 0, EPW>), grid, block, 0, stream, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((elem_kernel_w2<MODE, 1, EPW>), grid, block, 0, stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((elem_kernel<MODE, 0, EPW>), grid, block, 0, stream, a);
+  }
 }
 
 // the matrix modes in every schedule: variant bit 0 = VAR bit 0 (BSR block
 // reads at the scatter instead of before the quadrature loop), bit 1 = the
-// two-wavefront register budget
+// two-wavefront register budget, bit 2 = wide workgroups (wide_epw elements,
+// the geometry tasks spread over the workgroup) instead of two elements
 template <int MODE>
 static hipError_t launch_tuned(const ElemArgs &args, int count, int nbatch, int variant, hipStream_t stream) {
   if (count <= 0 || nbatch <= 0) return hipSuccess;
   ElemArgs a = args;
   a.count = count;
-  const dim3 grid((count + 1) / 2, nbatch), block(64);
-  switch (variant & 3) {
-    case 1:
-      hipLaunchKernelGGL((elem_kernel<MODE, 1>), grid, block, 0, stream, a);
-      break;
-    case 2:
-      hipLaunchKernelGGL((elem_kernel_w2<MODE, 0>), grid, block, 0, stream, a);
-      break;
-    case 3:
-      hipLaunchKernelGGL((elem_kernel_w2<MODE, 1>), grid, block, 0, stream, a);
-      break;
-    default:
-      hipLaunchKernelGGL((elem_kernel<MODE, 0>), grid, block, 0, stream, a);
+  if (variant & 4) {
+    constexpr int W = wide_epw<MODE>();
+    launch_var<MODE, W>(variant, dim3((count + W - 1) / W, nbatch), stream, a);
+  } else {
+    launch_var<MODE, 2>(variant, dim3((count + 1) / 2, nbatch), stream, a);
   }
   return hipGetLastError();
 }

@@ -1314,14 +1314,20 @@ int shellgpu_tune(shellgpu_ctx *ctx, int flags) {
       if (t[v] < t[best]) best = v;
     }
     ctx->elem_variant[mode] = best;
-    char buf[256];
-    snprintf(buf, sizeof(buf), "%s\"%s\": {\"schedule\": %d, \"us\": [%.1f, %.1f, %.1f, %.1f]}", c ? ", " : "",
-             kTunedNames[c], best, 1e3 * t[0], 1e3 * t[1], 1e3 * t[2], 1e3 * t[3]);
+    char buf[512];
+    int len = snprintf(buf, sizeof(buf), "%s\"%s\": {\"schedule\": %d, \"us\": [", c ? ", " : "", kTunedNames[c],
+                       best);
+    for (int v = 0; v < kElemVariants; v++)
+      len += snprintf(buf + len, sizeof(buf) - len, "%s%.1f", v ? ", " : "", 1e3 * t[v]);
+    snprintf(buf + len, sizeof(buf) - len, "]}");
     rep += buf;
   }
   rep += "}, \"schedules\": [\"block reads before the quadrature loop\", \"block reads at the scatter\", "
          "\"reads before the loop, 2 wavefronts/SIMD register budget\", "
-         "\"reads at the scatter, 2 wavefronts/SIMD register budget\"]}";
+         "\"reads at the scatter, 2 wavefronts/SIMD register budget\", "
+         "\"wide workgroups, reads before the loop\", \"wide workgroups, reads at the scatter\", "
+         "\"wide workgroups, reads before the loop, 2 wavefronts/SIMD\", "
+         "\"wide workgroups, reads at the scatter, 2 wavefronts/SIMD\"]}";
   ctx->schedule = rep;
 out:
   ctx->timing = timing;

@@ -460,6 +460,16 @@ def cpu_analysis_throughput():
             "host": host_cpu()}
 
 
+def stiffness_kernel_label(schedule):
+    """The stiffness kernel the timed assembly ran: its tuned schedule."""
+    elem = schedule.get("elem", {}).get("stiffness")
+    if elem is None:
+        return "elem_kernel<0> (stiffness), default schedule 0: " + "two elements per workgroup"
+    v = elem["schedule"]
+    return f"elem_kernel<0> (stiffness), tuned schedule {v}: {schedule['schedules'][v]}" + (
+        "" if v & 4 else ", two elements per workgroup")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -578,7 +588,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": profiled_traffic("elem_stiffness_cylinder200x400.md"),
-            "kernel": "elem_kernel<0> (stiffness, two elements per wavefront)",
+            "kernel": stiffness_kernel_label(schedule),
             "avg_launch_ms": elem_ms / max(elem_launches, 1),
             "flops_per_element": FLOPS_PER_ELEMENT["stiffness_linear"],
         },

@@ -1,6 +1,8 @@
 """The autotuner's schedules: every compiled schedule of the matrix-assembly
 kernels (BSR block reads before the quadrature loop or at the scatter, one-
-or two-wavefront register budget) and both element colourings assemble the same matrices and
+or two-wavefront register budget, two elements per workgroup or wide
+workgroups of 8 or 4 with the geometry spread over them, on colours of
+ragged length) and both element colourings assemble the same matrices and
 residuals (the same arithmetic; within round-off of the default schedule,
 which the oracle tests pin), and shellgpu_tune reports and keeps valid
 choices."""
@@ -50,7 +52,7 @@ def test_every_schedule_assembles_the_same_matrices():
     with _ctx(conn, X, thick, bcs) as ctx:
         u = ctx.vector(1e-3 * rng.standard_normal(2 * 6 * X.shape[0]))
         ref = _assemble_all(ctx, u, X.shape[0])
-        for v in range(1, 4):
+        for v in range(1, 8):
             for c in range(5):
                 ctx.set_elem_schedule(c, v)
             got = _assemble_all(ctx, u, X.shape[0])
@@ -67,7 +69,7 @@ def test_tune_keeps_valid_choices_and_results():
         ref = _assemble_all(ctx, u, cyl.nnodes)
         rep = ctx.tune(coloring=True)
         assert set(rep["elem"]) == {"stiffness", "tangent", "geometric", "jacobian_linear", "jacobian_nonlinear"}
-        assert all(0 <= r["schedule"] < 4 and len(r["us"]) == 4 for r in rep["elem"].values())
+        assert all(0 <= r["schedule"] < 8 and len(r["us"]) == 8 for r in rep["elem"].values())
         assert rep["coloring"]["colors"] == ctx.ncolors
         # the colouring in force: no two elements of a colour share a node
         col = ctx.colors()
