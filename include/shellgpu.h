@@ -429,15 +429,18 @@ int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double
  * schedule of the matrix-assembly element kernels (stiffness, tangent,
  * geometric stiffness, Jacobian of either model: BSR block reads before
  * the quadrature loop or at the scatter, one- or two-wavefront register
- * budget) and, with flags
+ * budget, two elements per workgroup or wide workgroups of 8 (4 for the
+ * larger-LDS kernels) with the geometry spread over them) and, with flags
  * bit 0, the two element colourings (greedy first fit in element order, or
  * the elements of most neighbours first), and keeps the fastest of each for
  * the context.  Results are the same arithmetic under every schedule. */
 int shellgpu_tune(shellgpu_ctx *ctx, int flags);
 /* the tuner's choices and timings as a JSON object ("{}" before tuning) */
 int shellgpu_get_schedule(shellgpu_ctx *ctx, char *buf, int len);
-/* force a schedule (0-3) of one kernel class (0 stiffness, 1 tangent,
- * 2 geometric, 3 Jacobian linear, 4 Jacobian nonlinear) */
+/* force a schedule (0-7: bit 0 block reads at the scatter, bit 1 the
+ * two-wavefront budget, bit 2 wide workgroups) of one kernel class
+ * (0 stiffness, 1 tangent, 2 geometric, 3 Jacobian linear, 4 Jacobian
+ * nonlinear) */
 int shellgpu_set_elem_schedule(shellgpu_ctx *ctx, int kernel_class, int variant);
 
 /* ---- device identification (no reference counterpart: measurement
