@@ -465,9 +465,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
 
   // ---- P1b: frames at the nodes (points 0-3: the drilling-strain frames)
   // and at the evaluation points (4-7: the quadrature points, or the nodes
-  // in the output modes) in one pass of the same code; quadrature-point
-  // kinematics (TacsShellComputeDispGrad, TACSShellUtilities.h:361-422) and
-  // tying weights; tying-point state and derivative coefficients ----
+  // in the output modes) in one pass of the same code; tying-point state
+  // and derivative coefficients ----
   auto p1b_frame = [&](ElemSmem<MODE> &sm, int p) {
     const bool at_node = p < 4;
     const int iq = p & 3;
@@ -520,58 +519,6 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
         sm.XzT[9 * iq + i] = XzT[i];
       }
       sm.wdet[iq] = det;  // Gauss weight 1
-
-      // tying field -> strain weights: e0ty = XT^T gty XT, gty the tying
-      // fields interpolated at the point (mat3x3SymmTransformTranspose,
-      // TACSElementAlgebra.h:1094); the interpolation weights are applied
-      // where the fields are formed (tying_fields)
-      const int kk[5] = {0, 3, 1, 4, 2};
-      const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
-      const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
-  #pragma unroll
-      for (int r = 0; r < 5; r++) {
-        const int ka = pa[kk[r]], kb = pb[kk[r]];
-  #pragma unroll
-        for (int f = 0; f < 5; f++) {
-          const int sl = field_slot(f);
-          const int c = pa[sl], d = pb[sl];
-          const double M = (c == d) ? XT[3 * c + ka] * XT[3 * c + kb]
-                                    : XT[3 * c + ka] * XT[3 * d + kb] + XT[3 * d + ka] * XT[3 * c + kb];
-          sm.Wf[25 * iq + 5 * r + f] = ff[r] * M;
-        }
-      }
-      if (kState) {
-        // displacement-gradient state at the point: u0x, u1x
-        double d0[3] = {0, 0, 0}, d0xi[6] = {0, 0, 0, 0, 0, 0}, u0xi[6] = {0, 0, 0, 0, 0, 0};
-  #pragma unroll
-        for (int m = 0; m < 4; m++) {
-          double dm[3];
-          cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
-  #pragma unroll
-          for (int d = 0; d < 3; d++) {
-            d0[d] += N[m] * dm[d];
-            d0xi[2 * d] += Na[m] * dm[d];
-            d0xi[2 * d + 1] += Nb[m] * dm[d];
-            u0xi[2 * d] += Na[m] * sm.s[6 * m + d];
-            u0xi[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
-          }
-        }
-        double u0d[9], u1d[9], tmp[9], tmp2[9], o0[9], o1[9];
-        frame3(u0xi, d0, u0d);
-        frame3z(d0xi, u1d);
-        mm3(u1d, XT, tmp);
-        mm3(u0d, XzT, tmp2);
-  #pragma unroll
-        for (int i = 0; i < 9; i++) tmp[i] += tmp2[i];
-        mtm3(T, tmp, o1);
-        mm3(u0d, XT, tmp);
-        mtm3(T, tmp, o0);
-  #pragma unroll
-        for (int i = 0; i < 9; i++) {
-          sm.u0x[9 * iq + i] = o0[i];
-          sm.u1x[9 * iq + i] = o1[i];
-        }
-      }
     }
   };
   auto p1b_tying = [&](ElemSmem<MODE> &sm, int t) {
@@ -669,7 +616,77 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
     p1b_tying(smem[(gt - kG2) / 9], (gt - kG2) % 9);
   __syncthreads();
 
-  // ---- P2: drilling rows at the nodes ----
+  // ---- P2: tying-field weights and the state's displacement gradients at
+  // the evaluation points (TacsShellComputeDispGrad, TACSShellUtilities.h:
+  // 361-422) from the frames P1b stored -- a task of its own, so that it
+  // runs beside the drilling rows rather than on the frame wavefront's
+  // critical path -- and the drilling rows at the nodes ----
+  auto p2_point = [&](ElemSmem<MODE> &sm, int iq) {
+    double u, v, N[4], Na[4], Nb[4];
+    if (MT::kOut)
+      node_point(iq, u, v);
+    else
+      quad_point(iq, u, v);
+    shape4(u, v, N, Na, Nb);
+    double T[9], XT[9], XzT[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      T[i] = sm.T[9 * iq + i];
+      XT[i] = sm.XT[9 * iq + i];
+      XzT[i] = sm.XzT[9 * iq + i];
+    }
+    // tying field -> strain weights: e0ty = XT^T gty XT, gty the tying
+    // fields interpolated at the point (mat3x3SymmTransformTranspose,
+    // TACSElementAlgebra.h:1094); the interpolation weights are applied
+    // where the fields are formed (tying_fields)
+    const int kk[5] = {0, 3, 1, 4, 2};
+    const double ff[5] = {1.0, 1.0, 2.0, 2.0, 2.0};
+    const int pa[5] = {0, 0, 0, 1, 1}, pb[5] = {0, 1, 2, 1, 2};
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+      const int ka = pa[kk[r]], kb = pb[kk[r]];
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        const int sl = field_slot(f);
+        const int c = pa[sl], d = pb[sl];
+        const double M = (c == d) ? XT[3 * c + ka] * XT[3 * c + kb]
+                                  : XT[3 * c + ka] * XT[3 * d + kb] + XT[3 * d + ka] * XT[3 * c + kb];
+        sm.Wf[25 * iq + 5 * r + f] = ff[r] * M;
+      }
+    }
+    if (kState) {
+      // displacement-gradient state at the point: u0x, u1x
+      double d0[3] = {0, 0, 0}, d0xi[6] = {0, 0, 0, 0, 0, 0}, u0xi[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        double dm[3];
+        cross3(&sm.s[6 * m + 3], &sm.fn[3 * m], dm);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          d0[d] += N[m] * dm[d];
+          d0xi[2 * d] += Na[m] * dm[d];
+          d0xi[2 * d + 1] += Nb[m] * dm[d];
+          u0xi[2 * d] += Na[m] * sm.s[6 * m + d];
+          u0xi[2 * d + 1] += Nb[m] * sm.s[6 * m + d];
+        }
+      }
+      double u0d[9], u1d[9], tmp[9], tmp2[9], o0[9], o1[9];
+      frame3(u0xi, d0, u0d);
+      frame3z(d0xi, u1d);
+      mm3(u1d, XT, tmp);
+      mm3(u0d, XzT, tmp2);
+#pragma unroll
+      for (int i = 0; i < 9; i++) tmp[i] += tmp2[i];
+      mtm3(T, tmp, o1);
+      mm3(u0d, XT, tmp);
+      mtm3(T, tmp, o0);
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        sm.u0x[9 * iq + i] = o0[i];
+        sm.u1x[9 * iq + i] = o1[i];
+      }
+    }
+  };
   auto p2_drill = [&](ElemSmem<MODE> &sm, int n) {
     // TacsShellAddDrillStrainSens map (TACSShellUtilities.h:787-835) with
     // TACSLinearizedRotation: et = 0.5 (Ct[3] + u0x[3] - Ct[1] - u0x[1])
@@ -711,7 +728,10 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
       }
     sm.etn[n] = et;
   };
-  if (gt < 4 * EPW) p2_drill(smem[gt >> 2], gt & 3);
+  if (gt < 4 * EPW)
+    p2_drill(smem[gt >> 2], gt & 3);
+  else if (gt >= kG2 && gt < kG2 + 4 * EPW)
+    p2_point(smem[(gt - kG2) >> 2], (gt - kG2) & 3);
   __syncthreads();
 
   // ---- P3: tying-strain Jacobian rows; strains and stresses at the

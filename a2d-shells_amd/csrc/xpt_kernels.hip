@@ -214,7 +214,7 @@ __device__ void element_scalars(const S X[12], const double q[3][24], const Sect
   // every quadrature point instead of holding 9 points' worth of dual
   // numbers through the quadrature loop (which spilled to scratch memory) ----
   constexpr bool kLate = PM != 0;
-  S tyl[3][kLate ? 1 : 9], tyq[4][kLate ? 1 : 9];
+  S tyl[3][kLate ? 1 : 9];
   if constexpr (!kLate) {
     for (int t = 0; t < 9; t++) {
       S tl[3], tq[4];
