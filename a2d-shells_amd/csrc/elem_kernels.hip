@@ -30,8 +30,8 @@
 // quadrature point / tying point (4 to 9 tasks per element) is spread over
 // the workgroup by (element, point): with two elements per workgroup (one
 // wavefront) it fills lanes the per-element phases would leave idle; the
-// wide schedules (8 elements, 4 wavefronts) give one wavefront the frames
-// of all 8 elements while two others compute their tying points.  Elements
+// wide schedules (4 elements, 2 wavefronts) give one wavefront the frames
+// of all 4 elements while the other computes their tying points.  Elements
 // of one colour share no node, so each half adds its 16 blocks into the BSR
 // values without atomics.
 #include "elem_common.h"
@@ -366,9 +366,10 @@ __device__ __forceinline__ void elem_body(const ElemArgs &a) {
   const int e = a.elems[valid ? slot : a.count - 1];
   // The geometry phases (P1-P2) run per node, quadrature point or tying
   // point: their tasks go to the workgroup's threads by (element, point),
-  // not by each element's own 32 lanes, so with 8 elements per workgroup
-  // one wavefront computes the frames of all 8 while two others compute the
-  // tying points.  Task classes of one phase start at thread 0 and kG2.
+  // not by each element's own 32 lanes, so with 4 elements per workgroup
+  // one wavefront computes the frames of all 4 while the other computes
+  // their tying points (with 8, two others).  Task classes of one phase
+  // start at thread 0 and kG2.
   const int gt = threadIdx.x;
   constexpr int kG2 = EPW >= 4 ? 64 : 16;
   static_assert(kG2 >= 8 * EPW && kG2 + 9 * EPW <= 32 * EPW, "geometry task classes overlap");
@@ -1352,11 +1353,14 @@ static hipError_t launch_mode(const ElemArgs &args, int count, int nbatch, hipSt
   return hipGetLastError();
 }
 
-// elements per workgroup of the wide schedules: 8 while their LDS stays
-// within 52 KB (two workgroups per CU), else 4
+// elements per workgroup of the wide schedules: 4 (two wavefronts).  On the
+// 200x400 cylinder the stiffness took 90.2 us per colour launch with 4
+// against 91.9 with 8 (four wavefronts, 51 KB of LDS); the tangent, G and
+// the Jacobians, whose LDS for 8 would not fit twice per CU, take 4 either
+// way.
 template <int MODE>
 constexpr int wide_epw() {
-  return sizeof(ElemSmem<MODE>) * 8 <= 52 * 1024 ? 8 : 4;
+  return 4;
 }
 
 template <int MODE, int EPW>

@@ -153,8 +153,8 @@ hipError_t launch_xpt_kernel(const XptArgs &args, int nel, int nbatch, hipStream
 // variant: the compiled schedule of the matrix modes (kModeKLin, kModeKNl,
 // kModeG, kModeJac*): bit 0 reads the BSR blocks at the scatter (default:
 // before the quadrature loop), bit 1 two-wavefront register budget, bit 2
-// wide workgroups (8 elements, or 4 where their LDS would not fit twice per
-// CU, instead of 2); other modes ignore it
+// wide workgroups (4 elements, the geometry spread over them, instead of
+// 2); other modes ignore it
 constexpr int kElemVariants = 8;
 hipError_t launch_elem_kernel(int mode, int variant, const ElemArgs &args, int count, int nbatch,
                               hipStream_t stream);

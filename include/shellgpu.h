@@ -429,8 +429,8 @@ int shellgpu_ks_aggregate(int n, double rho, const double *f, double *ks, double
  * schedule of the matrix-assembly element kernels (stiffness, tangent,
  * geometric stiffness, Jacobian of either model: BSR block reads before
  * the quadrature loop or at the scatter, one- or two-wavefront register
- * budget, two elements per workgroup or wide workgroups of 8 (4 for the
- * larger-LDS kernels) with the geometry spread over them) and, with flags
+ * budget, two elements per workgroup or wide workgroups of 4 with the
+ * geometry spread over them) and, with flags
  * bit 0, the two element colourings (greedy first fit in element order, or
  * the elements of most neighbours first), and keeps the fastest of each for
  * the context.  Results are the same arithmetic under every schedule. */

@@ -1,7 +1,7 @@
 """The autotuner's schedules: every compiled schedule of the matrix-assembly
 kernels (BSR block reads before the quadrature loop or at the scatter, one-
 or two-wavefront register budget, two elements per workgroup or wide
-workgroups of 8 or 4 with the geometry spread over them, on colours of
+workgroups of 4 with the geometry spread over them, on colours of
 ragged length) and both element colourings assemble the same matrices and
 residuals (the same arithmetic; within round-off of the default schedule,
 which the oracle tests pin), and shellgpu_tune reports and keeps valid
