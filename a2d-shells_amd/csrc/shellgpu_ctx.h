@@ -48,6 +48,7 @@ struct shellgpu_ctx {
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> elem_ev, spmv_ev;
+  int elem_launches = 0;  // element-kernel launches inside the elem_ev pairs
   hipEvent_t t0 = nullptr, t1 = nullptr;
 
   // the device's copy of the library's static scratch array (mapped at

@@ -454,9 +454,10 @@ int shellgpu_device_info(shellgpu_ctx *ctx, char *name, int name_len, char *arch
 
 /* ---- timing (no reference counterpart: measurement plumbing) ---------- */
 /* With timing enabled, HIP events on the context's stream bracket every
- * element-kernel launch and every SpMV launch; shellgpu_get_timing waits for
- * the stream and returns the summed kernel durations (ms) and launch counts
- * since timing was enabled, then resets them. */
+ * element-kernel pass (its colour launches, which run back to back) and
+ * every SpMV launch; shellgpu_get_timing waits for the stream and returns
+ * the summed durations (ms) and the launch counts since timing was
+ * enabled, then resets them. */
 int shellgpu_set_timing(shellgpu_ctx *ctx, int enable);
 int shellgpu_get_timing(shellgpu_ctx *ctx, double *elem_ms, int *elem_launches,
                         double *spmv_ms, int *spmv_launches);
