@@ -306,11 +306,11 @@ def spmv_hbm(nb=64, reps=20):
     y = ctx.vector()
     for _ in range(3):
         K.mult(x, y)
-    ctx.set_timing(True)
+    ctx.synchronize()
+    ctx.timer_start()
     for _ in range(reps):
         K.mult(x, y)
-    _, _, ms, n = ctx.get_timing()
-    ctx.set_timing(False)
+    ms, n = ctx.timer_stop(), reps
     per = spmv_bytes(ctx.nnzb, ctx.nnodes) * nb
     out = {"workload": f"cylinder 100x200, {nb} stiffness matrices (one batched launch)",
            "bytes_per_launch": per, "avg_launch_ms": ms / n, "GBps": per / (ms / n * 1e-3) / 1e9,
@@ -544,13 +544,15 @@ def main():
     y = ctx.vector()
     for _ in range(3):
         K.mult(x, y)
-    ctx.set_timing(True)
+    # one pair of stream events around the back-to-back products (an event
+    # pair around each ~35 us product adds a few us to it)
     nspmv = 50
+    ctx.synchronize()
+    ctx.timer_start()
     for _ in range(nspmv):
         K.mult(x, y)
-    _, _, spmv_ms, spmv_n = ctx.get_timing()
-    ctx.set_timing(False)
-    spmv_gbps = spmv_bytes(ctx.nnzb, ctx.nnodes) * spmv_n / (spmv_ms * 1e-3) / 1e9
+    spmv_ms = ctx.timer_stop()
+    spmv_gbps = spmv_bytes(ctx.nnzb, ctx.nnodes) * nspmv / (spmv_ms * 1e-3) / 1e9
     nnzb, nnodes, ncolors = ctx.nnzb, ctx.nnodes, ctx.ncolors
     device = ctx.device_info()
     schedule = ctx.schedule()
