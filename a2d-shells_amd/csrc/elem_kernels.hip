@@ -325,14 +325,6 @@ __device__ __forceinline__ double mass_entry(const MassTile &m, int i, int j) {
   return (j < 3) ? 0.0 : m.m2 * m.S * ((j - 3 == i ? m.tt : 0.0) - m.tj[i] * m.ti[j - 3]);
 }
 
-// EPW elements per workgroup: lanes 32 h .. 32 h + 31 work on element
-// EPW blockIdx.x + h of the launch list, each with its own LDS.  Slots past
-// the end of the list compute a copy of its last element with their writes
-// suppressed.
-//
-// VAR selects a compiled schedule of the same arithmetic (chosen per mode
-// and mesh by shellgpu_tune): bit 0 unrolls the quadrature loop of the
-// matrix tile (the shape values become constants, the code 4 times longer).
 // Synchronisation of one element's 32 lanes once the geometry phases are
 // done (those share data across elements and take the workgroup barrier).
 // With one wavefront per workgroup: the workgroup barrier.  With wide
@@ -352,6 +344,15 @@ __device__ __forceinline__ void elem_sync() {
   }
 }
 
+// EPW elements per workgroup: lanes 32 h .. 32 h + 31 work on element
+// EPW blockIdx.x + h of the launch list, each with its own LDS.  Slots past
+// the end of the list compute a copy of its last element with their writes
+// suppressed.
+//
+// VAR selects a compiled schedule of the same arithmetic (chosen per mode
+// and mesh by shellgpu_tune): bit 0 reads the lane's BSR block entries at
+// the scatter instead of before the quadrature loop (fewer live registers
+// through the loop).
 template <int MODE, int VAR, int EPW>
 __device__ __forceinline__ void elem_body(const ElemArgs &a) {
   using MT = ModeTraits<MODE>;
