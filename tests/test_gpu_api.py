@@ -274,16 +274,16 @@ def test_inner_products_with_psi_not_phi():
         assert abs(gu @ w - ref) <= 1e-9 * max(abs(ref), 1e-30), k
 
 
-def test_mult_identical_designs():
+@pytest.mark.parametrize("nb", [11, 16])
+def test_mult_identical_designs(nb):
     """Load cases of one structure: products with K, M and the linear-model
     Jacobian read one copy for every design's vector; the result equals the
-    per-design product bit for bit, over a group of eight vectors and a
-    ragged remainder, and a per-design change (addValues) or a state-
+    per-design product bit for bit, over groups of eight vectors with and
+    without a ragged remainder, and a per-design change (addValues) or a state-
     dependent type (G(u), the nonlinear tangent) turns the shared read off."""
     conn, X = random_mesh(5, 4, seed=71)
     nn = X.shape[0]
     rng = np.random.default_rng(72)
-    nb = 11
     thick = np.tile(rng.uniform(0.0015, 0.0025, conn.shape[0]), (nb, 1))
     with _ctx(conn, X, thick, _bcs()) as ctx:
         A, P = ctx.matrix(), ctx.matrix()

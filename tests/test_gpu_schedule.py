@@ -96,3 +96,24 @@ def test_most_neighbours_first_coloring_is_valid_on_a_junction_mesh():
         got = _assemble_all(ctx, u, b.nnodes)
         for k in ref:
             assert _close(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("nx, ny", [(1, 1), (3, 1)])
+def test_schedules_on_colours_shorter_than_a_workgroup(nx, ny):
+    """Colours of one or two elements: the wide schedules' padding slots
+    compute a copy of the colour's last element and write nothing, so every
+    schedule assembles what the default one does."""
+    conn, X = random_mesh(nx, ny, seed=84)
+    rng = np.random.default_rng(85)
+    thick = rng.uniform(0.0015, 0.0025, (1, conn.shape[0]))
+    bcs = (np.array([0], np.int32), np.array([0x3F], np.int32))
+    with _ctx(conn, X, thick, bcs) as ctx:
+        assert ctx.ncolors == (1 if conn.shape[0] == 1 else 2)
+        u = ctx.vector(1e-3 * rng.standard_normal(6 * X.shape[0]))
+        ref = _assemble_all(ctx, u, X.shape[0])
+        for v in range(1, 8):
+            for c in range(5):
+                ctx.set_elem_schedule(c, v)
+            got = _assemble_all(ctx, u, X.shape[0])
+            for k in ref:
+                assert _close(got[k], ref[k]), (v, k)
