@@ -5,7 +5,7 @@ workgroups of 4 with the geometry spread over them, on colours of
 ragged length) and both element colourings assemble the same matrices and
 residuals (the same arithmetic; within round-off of the default schedule,
 which the oracle tests pin), and shellgpu_tune reports and keeps valid
-choices."""
+choices, and that the timing counters count every colour launch."""
 import numpy as np
 import pytest
 
@@ -117,3 +117,26 @@ def test_schedules_on_colours_shorter_than_a_workgroup(nx, ny):
             got = _assemble_all(ctx, u, X.shape[0])
             for k in ref:
                 assert _close(got[k], ref[k]), (v, k)
+
+
+def test_timing_counts_colour_launches():
+    """With timing on, each assembly pass is one event pair around its colour
+    launches; get_timing reports every colour launch and the SpMVs, then
+    resets."""
+    cyl = Cylinder(8, 16)
+    thick = np.full((1, cyl.nelems), 0.002)
+    with _ctx(cyl.conn, cyl.Xpts, thick, cyl.bcs()) as ctx:
+        A = ctx.matrix()
+        x, y = ctx.vector(np.ones(6 * cyl.nnodes)), ctx.vector()
+        ctx.set_timing(True)
+        for _ in range(3):
+            ctx.assemble_mattype(sg.STIFFNESS, A)
+        A.mult(x, y)
+        elem_ms, elem_n, spmv_ms, spmv_n = ctx.get_timing()
+        assert elem_n == 3 * ctx.ncolors and elem_ms > 0.0
+        assert spmv_n == 1 and spmv_ms > 0.0
+        assert ctx.get_timing()[1] == 0
+        ctx.set_timing(False)
+        ctx.assemble_mattype(sg.STIFFNESS, A)
+        assert ctx.get_timing()[1] == 0
+        A.destroy()
